@@ -1,0 +1,200 @@
+"""Benchmark: 3D uniform-grid split sweeps of the perfect-gas Euler equations.
+
+Workload (the 256^3 uniform-grid configuration): one 256^3 patch on [0,1]^3
+with transmissive boundaries, MUSCL-Hancock (MinMod) + HLLEM (Larrouturou),
+X-Y-Z Godunov splitting with direction reversal on odd cycles, CFL 0.8, FP64.
+Initial state: rho = 1 + 0.2 * (8 seeded Fourier modes, |k| <= 4),
+p = 1 + 0.1 * (same form), velocities N(0, 0.1) smoothed (synthetic data).
+
+A step is one full time step: the CFL reduction over the state plus three
+fused split sweeps (each with its own CFL check).  `value` is interior
+cell-updates per second summed over the sweeps (Gcell-updates/s), with the
+state resident in HBM before the timed region starts.
+
+Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--precision fp64]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+BYTES_PER_CELL_UPDATE_3D = 80  # 5 conservative FP64 read + write
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_kernel_launches(fn, reps: int, torch) -> float:
+    """Average duration (ms) of fn() measured with HIP events on the stream the
+    library launches on (torch's current stream, handed to the library)."""
+    stream = torch.cuda.current_stream()
+    fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        fn()
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def cpu_baseline(n: int, steps: int, precision: str) -> dict:
+    """Times the FP64 CPU oracle (this project's restatement of the reference
+    algorithm; the reference itself cannot be built without its libraries)
+    on a bounded sample of the same workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    import finitevolumesolver_amd as fvs
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d
+    from finitevolumesolver_amd.patch import make_desc
+    threads = int(os.environ.get("FVS_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    threads = O.load().oracle_set_threads(threads)
+    cons, dx = fourier_modes_3d(n)
+    desc = make_desc(fvs.PerfectGas(3))
+    t0 = time.perf_counter()
+    _, dts, _ = O.level_run(desc, cons, dx, steps)
+    el = time.perf_counter() - t0
+    updates = n ** 3 * 3 * len(dts)
+    return {"value": updates / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle level driver, {n}^3 patch, {len(dts)} steps x 3 sweeps "
+                      f"({updates / 1e6:.1f} M cell-updates, {el:.1f} s, {threads} threads)"}
+
+
+def l1_vs_oracle(precision: str, torch) -> dict:
+    """L1 / Linf error of the GPU path against the FP64 oracle on a 48^3
+    sample of the same workload (4 steps)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+    import oracle_lib as O
+    import finitevolumesolver_amd as fvs
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d
+    n = 48
+    cons, dx = fourier_modes_3d(n)
+    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=precision)
+    lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx)
+    lvl.set_state(torch.from_numpy(cons).cuda())
+    lvl.advance(4, batch=4)
+    got = lvl.get_state().cpu().numpy()
+    fm64 = fvs.FluxMethod(fvs.PerfectGas(3))
+    ref, _, _ = O.level_run(fm64._h.desc, cons, dx, 4)
+    d = np.abs(got[0] - ref[0])
+    return {"l1_density": float(d.mean()), "linf_density": float(d.max()),
+            "bitexact": bool(np.array_equal(got, ref)), "sample": "48^3, 4 steps"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the patch shards as z-slabs with 2-plane halos; see DESIGN.md).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    import finitevolumesolver_amd as fvs
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d
+
+    n = args.size
+    log(f"[bench] generating {n}^3 initial state")
+    cons, dx = fourier_modes_3d(n)
+    dev = torch.device("cuda:0")
+    t_cons = torch.from_numpy(cons).to(dev)
+    del cons
+    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=args.precision)
+    lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
+    lvl.set_state(t_cons)
+    log("[bench] warmup")
+    lvl.advance(args.warmup, batch=max(args.warmup, 1))
+    torch.cuda.synchronize()
+
+    log(f"[bench] timing {args.steps} steps")
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    done, fails = lvl.advance(args.steps, batch=args.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    updates = n ** 3 * 3 * done
+    value = updates / (ms * 1e-3) / 1e9
+    log(f"[bench] {done} steps ({fails} CFL restarts) in {ms:.2f} ms -> {value:.3f} Gcell-updates/s")
+
+    # Per-kernel timing of each fused sweep direction and of the CFL pass.
+    t_out = torch.empty_like(t_cons)
+    sdt = torch.empty(1, dtype=torch.float64, device=dev)
+    state = lvl.get_state()
+    dt = 0.5 * 0.8 / n
+    per = {}
+    for d, name in enumerate("XYZ"):
+        per[name] = time_kernel_launches(
+            lambda d=d: fvs.split_sweep(fm, state, t_out, dt, dx[d], d, sdt), args.kernel_reps, torch)
+    per["CFL"] = time_kernel_launches(lambda: fvs.patch_stable_dt(fm, state, dx),
+                                      args.kernel_reps, torch)
+    dominant = max(("X", "Y", "Z"), key=lambda k: per[k])
+    alg_bytes = BYTES_PER_CELL_UPDATE_3D * n ** 3
+    achieved = alg_bytes / (per[dominant] * 1e-3) / 1e9
+    log("[bench] per-launch ms: " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
+
+    result = {
+        "metric": "Gcell-updates/s per split sweep, % HBM roofline, L1 err vs FP64 CPU oracle",
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / max(done, 1), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-flux/f64-update",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D uniform grid {n}^3 (one patch), MUSCL-Hancock(MinMod)+HLLEM "
+                        f"(Larrouturou), X-Y-Z Godunov split, CFL 0.8, transmissive, "
+                        f"{args.precision}",
+            "cells": n ** 3, "sweeps_per_step": 3, "steps_completed": done,
+            "cfl_restarts": fails, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": f"fused sweep {dominant}",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_3D,
+            "per_launch_ms": {k: round(v, 4) for k, v in per.items()},
+        },
+    }
+    try:
+        result["accuracy"] = l1_vs_oracle(args.precision, torch)
+    except Exception as e:  # accuracy is reported, never required for timing
+        result["accuracy"] = {"error": str(e)}
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle)")
+        result["cpu_baseline"] = cpu_baseline(128, 12, args.precision)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,824 @@
+// C ABI of libfvs_hip.so: handles, the reference's patch-level operators on
+// strided views, and the single-patch level driver built on the fused
+// sweeps.
+#include "fvs_internal.hpp"
+#include "fvs_physics.hpp"
+
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace fvs {
+
+namespace {
+thread_local std::string g_last_error = "no error";
+}
+
+void SetError(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct DevView {
+  double* data;
+  int64_t lo[3];
+  int64_t ext[3];
+  int64_t stride[3];
+  int64_t comp_stride;
+  __device__ __forceinline__ double* at(int64_t i, int64_t j, int64_t k, int c) const {
+    return data + (i - lo[0]) * stride[0] + (j - lo[1]) * stride[1] +
+           (k - lo[2]) * stride[2] + c * comp_stride;
+  }
+  __device__ __forceinline__ int64_t count() const { return ext[0] * ext[1] * ext[2]; }
+  __device__ __forceinline__ void index(int64_t q, int64_t* idx) const {
+    idx[0] = lo[0] + q % ext[0];
+    idx[1] = lo[1] + (q / ext[0]) % ext[1];
+    idx[2] = lo[2] + q / (ext[0] * ext[1]);
+  }
+};
+
+int MakeDevView(const fvs_view* v, int min_comp, DevView* out) {
+  if (!v || !v->data) {
+    SetError("null view or data pointer");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (v->ncomp < min_comp) {
+    SetError("view has %d components, need %d", v->ncomp, min_comp);
+    return FVS_BAD_ARGUMENT;
+  }
+  out->data = v->data;
+  for (int d = 0; d < 3; ++d) {
+    if (v->hi[d] < v->lo[d]) {
+      SetError("empty box in dimension %d", d);
+      return FVS_BAD_ARGUMENT;
+    }
+    out->lo[d] = v->lo[d];
+    out->ext[d] = v->hi[d] - v->lo[d] + 1;
+    out->stride[d] = v->stride[d];
+  }
+  out->comp_stride = v->comp_stride;
+  return FVS_OK;
+}
+
+bool ContainsBox(const fvs_view* v, const int64_t lo[3], const int64_t hi[3]) {
+  for (int d = 0; d < 3; ++d)
+    if (lo[d] < v->lo[d] || hi[d] > v->hi[d]) return false;
+  return true;
+}
+
+int CheckDesc(const fvs_method_desc* d) {
+  if (!d) {
+    SetError("null method description");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (d->dim != 2 && d->dim != 3) {
+    SetError("dim must be 2 or 3 (got %d)", d->dim);
+    return FVS_BAD_ARGUMENT;
+  }
+  if (!(d->gamma > 1.0)) {
+    SetError("gamma must exceed 1");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (d->base < 0 || d->base > FVS_BASE_GODUNOV_EXACT || d->reconstruction < 0 ||
+      d->reconstruction > FVS_RECON_MUSCL_HANCOCK || d->limiter < 0 ||
+      d->limiter > FVS_LIMITER_NO_GRADIENT || d->precision < 0 ||
+      d->precision > FVS_FP32_FLUX) {
+    SetError("method enum out of range");
+    return FVS_BAD_ARGUMENT;
+  }
+  return FVS_OK;
+}
+
+int CheckDir(const fvs_handle* h, int dir) {
+  if (dir < 0 || dir >= h->desc.dim) {
+    SetError("direction %d out of range for dim %d", dir, h->desc.dim);
+    return FVS_BAD_ARGUMENT;
+  }
+  return FVS_OK;
+}
+
+Gas<double> GasOf(const fvs_method_desc& d) {
+  Gas<double> g;
+  g.gamma = d.gamma;
+  g.gm1inv = 1.0 / (d.gamma - 1.0);
+  g.gm1 = d.gamma - 1.0;
+  return g;
+}
+
+// ---- view kernels ----------------------------------------------------------
+
+template <int D>
+__device__ __forceinline__ void LoadComplete(const DevView& v, const int64_t* idx, double* q) {
+  for (int c = 0; c < D + 4; ++c) q[c] = *v.at(idx[0], idx[1], idx[2], c);
+}
+
+// FluxMethod::ComputeNumericFluxes on one face per thread.  For MUSCL-Hancock
+// the stencil is the four cells face-2 .. face+1; rec_[0] is the right
+// half-step state of cell face-1 and rec_[1] the left one of cell face.
+template <int D, int DIR, bool LARR>
+__global__ __launch_bounds__(kBlock) void ViewFluxKernel(DevView fl, DevView st, double dt,
+                                                         double dx, Gas<double> g,
+                                                         int limiter, int muscl, int quirk) {
+  constexpr int NC = D + 2, NQ = D + 4;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= fl.count()) return;
+  int64_t f[3];
+  fl.index(q, f);
+  double L[NQ], R[NQ];
+  if (muscl) {
+    double u[4][NQ];
+    for (int s = 0; s < 4; ++s) {
+      int64_t c[3] = {f[0], f[1], f[2]};
+      c[DIR] += s - 2;
+      LoadComplete<D>(st, c, u[s]);
+    }
+    const double lh = 0.5 * dt / dx;
+    double dummy[NQ];
+    switch (limiter) {
+    case kMinMod:
+      ReconstructCell<D, DIR, kMinMod>(u[0], u[1], u[2], lh, g, dummy, L);
+      ReconstructCell<D, DIR, kMinMod>(u[1], u[2], u[3], lh, g, R, dummy);
+      break;
+    case kVanLeer:
+      ReconstructCell<D, DIR, kVanLeer>(u[0], u[1], u[2], lh, g, dummy, L);
+      ReconstructCell<D, DIR, kVanLeer>(u[1], u[2], u[3], lh, g, R, dummy);
+      break;
+    case kNoLimiter:
+      ReconstructCell<D, DIR, kNoLimiter>(u[0], u[1], u[2], lh, g, dummy, L);
+      ReconstructCell<D, DIR, kNoLimiter>(u[1], u[2], u[3], lh, g, R, dummy);
+      break;
+    default:
+      ReconstructCell<D, DIR, kNoGradient>(u[0], u[1], u[2], lh, g, dummy, L);
+      ReconstructCell<D, DIR, kNoGradient>(u[1], u[2], u[3], lh, g, R, dummy);
+      break;
+    }
+  } else {
+    int64_t c[3] = {f[0], f[1], f[2]};
+    c[DIR] -= 1;
+    LoadComplete<D>(st, c, L);
+    LoadComplete<D>(st, f, R);
+  }
+  double out[NC];
+  if constexpr (D == 3) {
+    if (quirk) HllemFlux<D, DIR, LARR, true>(out, L, R, g);
+    else HllemFlux<D, DIR, LARR, false>(out, L, R, g);
+  } else {
+    HllemFlux<D, DIR, LARR, false>(out, L, R, g);
+  }
+  for (int c = 0; c < NC; ++c) *fl.at(f[0], f[1], f[2], c) = out[c];
+}
+
+__device__ __forceinline__ unsigned long long WaveMin(unsigned long long k) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(k, off, 64);
+    k = o < k ? o : k;
+  }
+  return k;
+}
+
+__device__ __forceinline__ void BlockAtomicMin(unsigned long long key,
+                                               unsigned long long* dst) {
+  __shared__ unsigned long long s[kBlock / 64];
+  key = WaveMin(key);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long k = s[0];
+    for (int w = 1; w < kBlock / 64; ++w) k = s[w] < k ? s[w] : k;
+    atomicMin(dst, k);
+  }
+}
+
+// FluxMethod::ComputeStableDt: `faces` enumerates the first stencil cell of
+// every stencil inside the states box; the base method sees the two middle
+// cells (MUSCL) or both cells (first order).
+template <int D, int DIR>
+__global__ __launch_bounds__(kBlock) void ViewStableDtKernel(DevView st, DevView faces,
+                                                             int offset, double dx,
+                                                             Gas<double> g,
+                                                             unsigned long long* key) {
+  constexpr int NQ = D + 4;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  unsigned long long k = 0x7ff0000000000000ull;
+  if (q < faces.count()) {
+    int64_t c[3];
+    faces.index(q, c);
+    c[DIR] += offset;
+    double L[NQ], R[NQ];
+    LoadComplete<D>(st, c, L);
+    c[DIR] += 1;
+    LoadComplete<D>(st, c, R);
+    k = DtKey(HllemStableDt<D, DIR>(L, R, dx, g));
+  }
+  BlockAtomicMin(k, key);
+}
+
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void ViewUpdateKernel(DevView nx, DevView pv, DevView fl,
+                                                           double dt_dx, int ncomp) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= nx.count()) return;
+  int64_t c[3];
+  nx.index(q, c);
+  int64_t r[3] = {c[0], c[1], c[2]};
+  r[DIR] += 1;
+  for (int k = 0; k < ncomp; ++k) {
+    const double U = *pv.at(c[0], c[1], c[2], k);
+    const double fl_ = *fl.at(c[0], c[1], c[2], k);
+    const double fr_ = *fl.at(r[0], r[1], r[2], k);
+    *nx.at(c[0], c[1], c[2], k) = U + dt_dx * (fl_ - fr_);
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void ViewCompleteKernel(DevView cv, DevView uv,
+                                                             Gas<double> g) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= cv.count()) return;
+  int64_t c[3];
+  cv.index(q, c);
+  double s[D + 4];
+  for (int k = 0; k < D + 2; ++k) s[k] = *uv.at(c[0], c[1], c[2], k);
+  CompleteFromCons<D>(s, g);
+  for (int k = 0; k < D + 4; ++k) *cv.at(c[0], c[1], c[2], k) = s[k];
+}
+
+__global__ __launch_bounds__(kBlock) void ViewFillKernel(DevView v, int ncomp, int dir,
+                                                         int64_t dlo, int64_t dhi) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= v.count()) return;
+  int64_t c[3];
+  v.index(q, c);
+  int64_t s[3] = {c[0], c[1], c[2]};
+  if (c[dir] < dlo) {
+    s[dir] = dlo;
+  } else if (c[dir] > dhi) {
+    s[dir] = dhi;
+  } else {
+    return;
+  }
+  for (int k = 0; k < ncomp; ++k) *v.at(c[0], c[1], c[2], k) = *v.at(s[0], s[1], s[2], k);
+}
+
+unsigned Blocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+int64_t Count(const DevView& v) { return v.ext[0] * v.ext[1] * v.ext[2]; }
+
+} // namespace
+} // namespace fvs
+
+using namespace fvs;
+
+extern "C" {
+
+const char* fvs_last_error(void) { return g_last_error.c_str(); }
+
+const char* fvs_version(void) { return "fvs-mi355x 0.1 (gfx950)"; }
+
+int fvs_create(const fvs_method_desc* desc, void* hip_stream, fvs_handle** out) {
+  if (!out) {
+    SetError("null output handle pointer");
+    return FVS_BAD_ARGUMENT;
+  }
+  *out = nullptr;
+  if (int rc = CheckDesc(desc)) return rc;
+  fvs_handle* h = new (std::nothrow) fvs_handle();
+  if (!h) {
+    SetError("out of host memory");
+    return FVS_BAD_ARGUMENT;
+  }
+  h->desc = *desc;
+  if (h->desc.max_newton_iterations <= 0) h->desc.max_newton_iterations = 5000;
+  if (!(h->desc.newton_tolerance > 0.0)) h->desc.newton_tolerance = 1e-7;
+  h->stream = static_cast<hipStream_t>(hip_stream);
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    SetError("hipGetDevice failed: no usable HIP device");
+    return FVS_HIP_ERROR;
+  }
+  void* mem = nullptr;
+  if (hipMalloc(&mem, 256) != hipSuccess) {
+    delete h;
+    SetError("hipMalloc of the handle scratch failed");
+    return FVS_HIP_ERROR;
+  }
+  h->scratch_dt = static_cast<double*>(mem);
+  h->scratch_keys = reinterpret_cast<unsigned long long*>(static_cast<char*>(mem) + 64);
+  h->scratch_counter = reinterpret_cast<unsigned int*>(static_cast<char*>(mem) + 192);
+  if (int rc = LaunchResetKeys(h->stream, h->scratch_keys, 8, h->scratch_counter, 4)) {
+    (void)hipFree(mem);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return FVS_OK;
+}
+
+int fvs_destroy(fvs_handle* h) {
+  if (!h) return FVS_OK;
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(h->scratch_dt);
+  delete h;
+  return FVS_OK;
+}
+
+int fvs_stable_dt(fvs_handle* h, const fvs_view* states, double dx, int dir,
+                  double* dt_device) {
+  if (!h || !dt_device) {
+    SetError("null handle or output");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    SetError("exact Godunov stable dt is provided by fvs_godunov_stable_dt");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = h->desc.dim;
+  DevView st;
+  if (int rc = MakeDevView(states, D + 4, &st)) return rc;
+  const int S = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK ? 4 : 2;
+  DevView faces = st;
+  faces.ext[dir] -= (S - 1);
+  if (faces.ext[dir] <= 0) {
+    SetError("states box too small for the stencil");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int offset = S == 4 ? 1 : 0;
+  const Gas<double> g = GasOf(h->desc);
+  const unsigned nb = Blocks(Count(faces));
+  hipStream_t s = h->stream;
+  unsigned long long* key = h->scratch_keys;
+  if (D == 2) {
+    if (dir == 0) hipLaunchKernelGGL((ViewStableDtKernel<2, 0>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+    else hipLaunchKernelGGL((ViewStableDtKernel<2, 1>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+  } else {
+    if (dir == 0) hipLaunchKernelGGL((ViewStableDtKernel<3, 0>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+    else if (dir == 1) hipLaunchKernelGGL((ViewStableDtKernel<3, 1>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+    else hipLaunchKernelGGL((ViewStableDtKernel<3, 2>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+  }
+  FVS_LAUNCH_CHECK();
+  return LaunchKeysToDt(s, key, 1, dt_device);
+}
+
+int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* states,
+                       double dt, double dx, int dir) {
+  if (!h) {
+    SetError("null handle");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    SetError("exact Godunov fluxes are provided by fvs_godunov_fluxes");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = h->desc.dim;
+  DevView fl, st;
+  if (int rc = MakeDevView(fluxes, D + 2, &fl)) return rc;
+  if (int rc = MakeDevView(states, D + 4, &st)) return rc;
+  const int muscl = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK;
+  const int w = muscl ? 2 : 1;
+  int64_t need_lo[3], need_hi[3];
+  for (int d = 0; d < 3; ++d) {
+    need_lo[d] = fluxes->lo[d];
+    need_hi[d] = fluxes->hi[d];
+  }
+  need_lo[dir] -= w;
+  need_hi[dir] += w - 1;
+  if (!ContainsBox(states, need_lo, need_hi)) {
+    SetError("states box does not cover the flux stencils");
+    return FVS_BAD_ARGUMENT;
+  }
+  const Gas<double> g = GasOf(h->desc);
+  const bool larr = h->desc.base == FVS_BASE_HLLEM_LARROUTUROU;
+  const unsigned nb = Blocks(Count(fl));
+  hipStream_t s = h->stream;
+  const int lim = h->desc.limiter, q = h->desc.quirks;
+#define FVS_VF(DD, DIR_, LA)                                                              \
+  hipLaunchKernelGGL((ViewFluxKernel<DD, DIR_, LA>), dim3(nb), dim3(kBlock), 0, s, fl, st, \
+                     dt, dx, g, lim, muscl, q)
+  if (D == 2) {
+    if (dir == 0) { if (larr) FVS_VF(2, 0, true); else FVS_VF(2, 0, false); }
+    else { if (larr) FVS_VF(2, 1, true); else FVS_VF(2, 1, false); }
+  } else {
+    if (dir == 0) { if (larr) FVS_VF(3, 0, true); else FVS_VF(3, 0, false); }
+    else if (dir == 1) { if (larr) FVS_VF(3, 1, true); else FVS_VF(3, 1, false); }
+    else { if (larr) FVS_VF(3, 2, true); else FVS_VF(3, 2, false); }
+  }
+#undef FVS_VF
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int fvs_update_conservatively(fvs_handle* h, const fvs_view* next, const fvs_view* prev,
+                              const fvs_view* fluxes, double dt, double dx, int dir) {
+  if (!h) {
+    SetError("null handle");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  DevView nv, pv, fv;
+  if (int rc = MakeDevView(next, 1, &nv)) return rc;
+  if (int rc = MakeDevView(prev, next->ncomp, &pv)) return rc;
+  if (int rc = MakeDevView(fluxes, next->ncomp, &fv)) return rc;
+  int64_t flo[3], fhi[3];
+  for (int d = 0; d < 3; ++d) {
+    flo[d] = next->lo[d];
+    fhi[d] = next->hi[d];
+  }
+  fhi[dir] += 1;
+  if (!ContainsBox(fluxes, flo, fhi) || !ContainsBox(prev, next->lo, next->hi)) {
+    SetError("flux or prev box does not cover the update box");
+    return FVS_BAD_ARGUMENT;
+  }
+  const double dt_dx = dt / dx;
+  const unsigned nb = Blocks(Count(nv));
+  hipStream_t s = h->stream;
+  if (dir == 0) hipLaunchKernelGGL(ViewUpdateKernel<0>, dim3(nb), dim3(kBlock), 0, s, nv, pv, fv, dt_dx, next->ncomp);
+  else if (dir == 1) hipLaunchKernelGGL(ViewUpdateKernel<1>, dim3(nb), dim3(kBlock), 0, s, nv, pv, fv, dt_dx, next->ncomp);
+  else hipLaunchKernelGGL(ViewUpdateKernel<2>, dim3(nb), dim3(kBlock), 0, s, nv, pv, fv, dt_dx, next->ncomp);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int fvs_complete_from_cons(fvs_handle* h, const fvs_view* complete, const fvs_view* cons) {
+  if (!h) {
+    SetError("null handle");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = h->desc.dim;
+  DevView cv, uv;
+  if (int rc = MakeDevView(complete, D + 4, &cv)) return rc;
+  if (int rc = MakeDevView(cons, D + 2, &uv)) return rc;
+  if (!ContainsBox(cons, complete->lo, complete->hi)) {
+    SetError("cons box does not cover the complete box");
+    return FVS_BAD_ARGUMENT;
+  }
+  const Gas<double> g = GasOf(h->desc);
+  const unsigned nb = Blocks(Count(cv));
+  if (D == 2) hipLaunchKernelGGL(ViewCompleteKernel<2>, dim3(nb), dim3(kBlock), 0, h->stream, cv, uv, g);
+  else hipLaunchKernelGGL(ViewCompleteKernel<3>, dim3(nb), dim3(kBlock), 0, h->stream, cv, uv, g);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int fvs_transmissive_fill(fvs_handle* h, const fvs_view* states, const int64_t domain_lo[3],
+                          const int64_t domain_hi[3], int dir) {
+  if (!h || !domain_lo || !domain_hi) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  DevView v;
+  if (int rc = MakeDevView(states, 1, &v)) return rc;
+  if (domain_lo[dir] < states->lo[dir] || domain_hi[dir] > states->hi[dir] ||
+      domain_hi[dir] < domain_lo[dir]) {
+    SetError("domain boundary cells lie outside the view");
+    return FVS_BAD_ARGUMENT;
+  }
+  hipLaunchKernelGGL(ViewFillKernel, dim3(Blocks(Count(v))), dim3(kBlock), 0, h->stream, v,
+                     states->ncomp, dir, domain_lo[dir], domain_hi[dir]);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+// ---- fused single-patch operators ------------------------------------------
+
+static int FillSweepArgs(fvs_handle* h, const double* const* in, double* const* out,
+                         const int64_t n[3], double dx, SweepArgs* a) {
+  const int D = h->desc.dim;
+  std::memset(a, 0, sizeof *a);
+  for (int c = 0; c < D + 2; ++c) {
+    if (!in || !in[c] || (out && !out[c])) {
+      SetError("null component pointer %d", c);
+      return FVS_BAD_ARGUMENT;
+    }
+    a->in[c] = in[c];
+    if (out) a->out[c] = out[c];
+  }
+  for (int d = 0; d < 3; ++d) a->n[d] = (d < D) ? n[d] : 1;
+  for (int d = 0; d < D; ++d) {
+    if (n[d] < 1) {
+      SetError("patch extent %d must be positive", d);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  a->dx = dx;
+  a->gamma = h->desc.gamma;
+  a->gm1inv = 1.0 / (h->desc.gamma - 1.0);
+  a->limiter = h->desc.limiter;
+  a->muscl = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK;
+  a->quirk3d = h->desc.quirks;
+  a->seg = 32;
+  return FVS_OK;
+}
+
+int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* cons_out,
+                    const int64_t n[3], double dt, double dx, int dir,
+                    double* stable_dt_device) {
+  if (!h || !n) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  SweepArgs a;
+  if (int rc = FillSweepArgs(h, cons_in, cons_out, n, dx, &a)) return rc;
+  for (int c = 0; c < h->desc.dim + 2; ++c) {
+    if (cons_in[c] == cons_out[c]) {
+      SetError("fused sweep needs distinct input and output arrays");
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  a.dt_val = dt;
+  a.check = stable_dt_device != nullptr;
+  a.check_key = h->scratch_keys;
+  if (int rc = LaunchSweep(h->desc, h->stream, dir, a)) return rc;
+  if (stable_dt_device) return LaunchKeysToDt(h->stream, h->scratch_keys, 1, stable_dt_device);
+  return FVS_OK;
+}
+
+int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t n[3],
+                        const double dx[3], double* dt_dir_device) {
+  if (!h || !cons || !n || !dx || !dt_dir_device) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = h->desc.dim;
+  for (int c = 0; c < D + 2; ++c) {
+    if (!cons[c]) {
+      SetError("null component pointer %d", c);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  int64_t nn[3] = {n[0], n[1], D == 3 ? n[2] : 1};
+  if (int rc = LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, h->scratch_keys,
+                              h->scratch_counter, nullptr, nullptr, 0, 0))
+    return rc;
+  return LaunchKeysToDt(h->stream, h->scratch_keys, D, dt_dir_device);
+}
+
+} // extern "C"
+
+// ---- level driver ----------------------------------------------------------
+
+struct fvs_level {
+  fvs_handle* h;
+  fvs_patch_desc patch;
+  int64_t n[3];
+  int64_t ncell;
+  double* block;          // 3 state buffers
+  double* buf[3][5];
+  Ctl* ctl;               // device control block
+  int cur;                // buffer holding the current state
+};
+
+extern "C" {
+
+int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out) {
+  if (!h || !patch || !out) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  *out = nullptr;
+  const int D = h->desc.dim;
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    SetError("the fused level driver runs HLLEM methods");
+    return FVS_BAD_ARGUMENT;
+  }
+  int64_t n[3] = {patch->n[0], patch->n[1], D == 3 ? patch->n[2] : 1};
+  for (int d = 0; d < D; ++d) {
+    if (n[d] < 1 || !(patch->dx[d] > 0)) {
+      SetError("bad patch extent or cell width in dimension %d", d);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  if (patch->splitting != FVS_SPLIT_GODUNOV && patch->splitting != FVS_SPLIT_STRANG) {
+    SetError("unknown splitting %d", patch->splitting);
+    return FVS_BAD_ARGUMENT;
+  }
+  fvs_level* L = new (std::nothrow) fvs_level();
+  if (!L) {
+    SetError("out of host memory");
+    return FVS_BAD_ARGUMENT;
+  }
+  L->h = h;
+  L->patch = *patch;
+  for (int d = 0; d < 3; ++d) L->n[d] = n[d];
+  L->ncell = n[0] * n[1] * n[2];
+  const size_t bytes = static_cast<size_t>(L->ncell) * (D + 2) * 3 * sizeof(double);
+  if (hipMalloc(&L->block, bytes) != hipSuccess) {
+    delete L;
+    SetError("hipMalloc of %zu bytes for the level state failed", bytes);
+    return FVS_HIP_ERROR;
+  }
+  if (hipMalloc(&L->ctl, sizeof(Ctl)) != hipSuccess) {
+    (void)hipFree(L->block);
+    delete L;
+    SetError("hipMalloc of the control block failed");
+    return FVS_HIP_ERROR;
+  }
+  for (int b = 0; b < 3; ++b)
+    for (int c = 0; c < D + 2; ++c) L->buf[b][c] = L->block + (static_cast<int64_t>(b) * (D + 2) + c) * L->ncell;
+  L->cur = 0;
+  (void)hipMemsetAsync(L->block, 0, bytes, h->stream);
+  Ctl init{};
+  init.cfl = patch->cfl;
+  init.cfl_key[0] = init.cfl_key[1] = init.cfl_key[2] = 0x7ff0000000000000ull;
+  init.check_key = 0x7ff0000000000000ull;
+  (void)hipMemcpyAsync(L->ctl, &init, sizeof init, hipMemcpyHostToDevice, h->stream);
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    SetError("level initialisation failed");
+    return FVS_HIP_ERROR;
+  }
+  *out = L;
+  return FVS_OK;
+}
+
+int fvs_level_destroy(fvs_level* L) {
+  if (!L) return FVS_OK;
+  (void)hipStreamSynchronize(L->h->stream);
+  (void)hipFree(L->block);
+  (void)hipFree(L->ctl);
+  delete L;
+  return FVS_OK;
+}
+
+double* fvs_level_state(fvs_level* L, int c) {
+  if (!L || c < 0 || c >= L->h->desc.dim + 2) return nullptr;
+  return L->buf[L->cur][c];
+}
+
+int fvs_level_set_state(fvs_level* L, const double* const* comps, double t0) {
+  if (!L || !comps) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = L->h->desc.dim;
+  for (int c = 0; c < D + 2; ++c) {
+    if (!comps[c]) {
+      SetError("null component pointer %d", c);
+      return FVS_BAD_ARGUMENT;
+    }
+    FVS_HIP_CHECK(hipMemcpyAsync(L->buf[L->cur][c], comps[c], L->ncell * sizeof(double),
+                                 hipMemcpyDeviceToDevice, L->h->stream));
+  }
+  Ctl init{};
+  init.t = t0;
+  init.cfl = L->patch.cfl;
+  init.cfl_key[0] = init.cfl_key[1] = init.cfl_key[2] = 0x7ff0000000000000ull;
+  init.check_key = 0x7ff0000000000000ull;
+  FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &init, sizeof init, hipMemcpyHostToDevice, L->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
+  return FVS_OK;
+}
+
+int fvs_level_get_state(fvs_level* L, double* const* comps) {
+  if (!L || !comps) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = L->h->desc.dim;
+  for (int c = 0; c < D + 2; ++c) {
+    if (!comps[c]) {
+      SetError("null component pointer %d", c);
+      return FVS_BAD_ARGUMENT;
+    }
+    FVS_HIP_CHECK(hipMemcpyAsync(comps[c], L->buf[L->cur][c], L->ncell * sizeof(double),
+                                 hipMemcpyDeviceToDevice, L->h->stream));
+  }
+  return FVS_OK;
+}
+
+int fvs_level_time(fvs_level* L, double* t, int64_t* cycles) {
+  if (!L) {
+    SetError("null level");
+    return FVS_BAD_ARGUMENT;
+  }
+  Ctl c;
+  FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, L->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
+  if (t) *t = c.t;
+  if (cycles) *cycles = c.cycle;
+  return FVS_OK;
+}
+
+int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
+                      double* dt_log, int* steps_done, int* failures) {
+  if (!L || max_steps < 0 || batch < 1) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  fvs_handle* h = L->h;
+  hipStream_t s = h->stream;
+  const int D = h->desc.dim;
+  const bool strang = L->patch.splitting == FVS_SPLIT_STRANG;
+  Ctl c;
+  FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
+  FVS_HIP_CHECK(hipStreamSynchronize(s));
+  c.final_time = final_time;
+  c.cfl = L->patch.cfl;
+  c.halt = 0;
+  FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+  const int64_t cycle0 = c.cycle;
+  int64_t cycle = c.cycle;
+  int nfail = 0;
+  double* dlog = nullptr;
+  if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
+  int rc = FVS_OK;
+  std::vector<int> start_buf;
+  while (cycle - cycle0 < max_steps) {
+    const int k = static_cast<int>(std::min<int64_t>(batch, max_steps - (cycle - cycle0)));
+    start_buf.assign(k + 1, 0);
+    int cur = L->cur;
+    for (int sidx = 0; sidx < k && rc == FVS_OK; ++sidx) {
+      start_buf[sidx] = cur;
+      const int64_t cyc = cycle + sidx;
+      // MakeSplitDirections (SplittingMethod.hpp:115-126), subcycle {0, 1}.
+      const int is_odd = static_cast<int>(cyc % 2);
+      int dirs[3];
+      for (int i = 0; i < D; ++i) dirs[i] = is_odd ? (D - 1 - i) : i;
+      int seq[6];
+      int ns = 0;
+      for (int i = 0; i < D; ++i) seq[ns++] = dirs[i];
+      if (strang)
+        for (int i = D - 1; i >= 0; --i) seq[ns++] = dirs[i];
+      const double* cin[5];
+      for (int q = 0; q < D + 2; ++q) cin[q] = L->buf[cur][q];
+      rc = LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->ctl->cfl_key,
+                          &L->ctl->cfl_counter, L->ctl, &L->ctl->halt, seq[0], strang);
+      if (rc) break;
+      const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
+      int src = b0, dst = b1;
+      for (int w = 0; w < ns; ++w) {
+        SweepArgs a;
+        double* outp[5];
+        for (int q = 0; q < D + 2; ++q) outp[q] = L->buf[dst][q];
+        const double* inp[5];
+        for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
+        rc = FillSweepArgs(h, inp, outp, L->n, L->patch.dx[seq[w]], &a);
+        if (rc) break;
+        a.dt_ptr = &L->ctl->split_dt;
+        a.halt = &L->ctl->halt;
+        a.check = (w > 0);
+        a.check_key = &L->ctl->check_key;
+        a.ctl = L->ctl;
+        rc = LaunchSweep(h->desc, s, seq[w], a);
+        if (rc) break;
+        src = dst;
+        dst = (dst == b1) ? b2 : b1;
+      }
+      if (rc) break;
+      cur = src;
+      rc = LaunchStepEnd(s, L->ctl, dlog, cycle0, max_steps);
+    }
+    if (rc) break;
+    FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
+    FVS_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t completed = c.cycle - cycle;
+    if (c.halt == 0) {
+      L->cur = cur;
+      cycle = c.cycle;
+      continue;
+    }
+    // The step `completed` of this batch stopped; its start buffer holds the
+    // state after the last completed step.
+    L->cur = start_buf[completed];
+    cycle = c.cycle;
+    if (c.halt == 2) break;
+    // TimeStepTooLarge: restart the step with cfl * failure_dt
+    // (RunSimulation.hpp:141-151).
+    ++nfail;
+    c.halt = 0;
+    c.have_failure = 1;
+    FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+    if (nfail > 64) {
+      SetError("more than 64 consecutive CFL failures");
+      rc = FVS_TIME_STEP_TOO_LARGE;
+      break;
+    }
+  }
+  if (dlog) {
+    if (rc == FVS_OK) {
+      const int64_t nd = cycle - cycle0;
+      if (nd > 0) {
+        (void)hipMemcpyAsync(dt_log, dlog, nd * sizeof(double), hipMemcpyDeviceToHost, s);
+      }
+    }
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dlog);
+  }
+  if (steps_done) *steps_done = static_cast<int>(cycle - cycle0);
+  if (failures) *failures = nfail;
+  return rc;
+}
+
+} // extern "C"

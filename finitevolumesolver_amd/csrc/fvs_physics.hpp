@@ -1,0 +1,392 @@
+// Device arithmetic of the perfect-gas split patch update (gfx950).
+//
+// Every FP64 expression below keeps the operation order of the reference's
+// 8-lane Eigen array path so that the FP64 kernels reproduce the reference
+// bit for bit when compiled with -ffp-contract=off:
+//   PerfectGas::Flux / CompleteFromCons   src/equations/PerfectGas.cpp:52-144
+//   MinMod / VanLeer / NoLimiter          include/fub/flux_method/MusclHancockMethod.hpp:31-187
+//   MusclHancock::ComputeNumericFlux      include/fub/flux_method/MusclHancockMethod.hpp:419-485
+//   Hllem<Eq, Larrouturou>::ComputeNumericFlux  src/equations/perfect_gas/HllemMethod.hpp:711-1072
+//   Hllem::ComputeStableDt                src/equations/perfect_gas/HllemMethod.hpp:1152-1216
+// Simplifications used here are exact for finite inputs: multiplications by
+// MaSq() == 1 and Minv == 1 are dropped, terms that the Larrouturou variant
+// multiplies by zero are dropped (they can only change the sign of a zero
+// result, never a non-zero value), and per-cell reconstructions are shared
+// by the two faces of a cell instead of being recomputed per face.
+//
+// min/max follow std::min/std::max argument semantics ((b < a) ? b : a),
+// which is what Eigen 3.3's scalar and packet min/max reproduce.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fvs {
+
+template <class T> __device__ __forceinline__ T rmin(T a, T b) { return (b < a) ? b : a; }
+template <class T> __device__ __forceinline__ T rmax(T a, T b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double rsqrt_ieee(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float rsqrt_ieee(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double rabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float rabs(float x) { return __builtin_fabsf(x); }
+
+template <class T> struct Gas {
+  T gamma;  // 1.4
+  T gm1inv; // 1 / (gamma - 1), evaluated in double on the host
+  T gm1;    // gamma - 1 (stable-dt formula)
+};
+
+// Component layout of a Complete state in registers: rho, mom[0..D-1], E, p, a.
+template <int D> struct Idx {
+  static constexpr int kRho = 0;
+  static constexpr int kMom = 1;
+  static constexpr int kE = D + 1;
+  static constexpr int kP = D + 2;
+  static constexpr int kA = D + 3;
+  static constexpr int kCons = D + 2;
+  static constexpr int kComp = D + 4;
+};
+
+// Squared momentum norm in the order of the reference's array path
+// (Eigen column reduction): 2D m0^2 + m1^2, 3D m0^2 + (m1^2 + m2^2).
+template <int D, class T> __device__ __forceinline__ T SqNormArray(const T* m) {
+  if constexpr (D == 2) {
+    return m[0] * m[0] + m[1] * m[1];
+  } else {
+    return m[0] * m[0] + (m[1] * m[1] + m[2] * m[2]);
+  }
+}
+
+// PerfectGas::CompleteFromCons (array form): fills p and a of q.
+template <int D, class T>
+__device__ __forceinline__ void CompleteFromCons(T* q, const Gas<T>& g) {
+  using I = Idx<D>;
+  const T e_kin = T(0.5) * SqNormArray<D>(q + I::kMom) / q[I::kRho];
+  const T e_int = q[I::kE] - e_kin;
+  q[I::kP] = e_int / g.gm1inv;
+  q[I::kA] = rsqrt_ieee(g.gamma * q[I::kP] / q[I::kRho]);
+}
+
+// PerfectGas::Flux in direction DIR of a complete state.
+template <int D, int DIR, class T>
+__device__ __forceinline__ void PhysFlux(T* f, const T* q) {
+  using I = Idx<D>;
+  const T velocity = q[I::kMom + DIR] / q[I::kRho];
+  f[I::kRho] = q[I::kMom + DIR];
+#pragma unroll
+  for (int d = 0; d < D; ++d) f[I::kMom + d] = velocity * q[I::kMom + d];
+  f[I::kMom + DIR] += q[I::kP];
+  f[I::kE] = velocity * (q[I::kE] + q[I::kP]);
+}
+
+enum { kMinMod = 0, kVanLeer = 1, kNoLimiter = 2, kNoGradient = 3 };
+
+template <int LIM, class T>
+__device__ __forceinline__ T LimitedSlope(T qL, T qM, T qR) {
+  const T sL = qM - qL;
+  const T sR = qR - qM;
+  if constexpr (LIM == kMinMod) {
+    return (sR > T(0)) ? rmax(T(0), rmin(sL, sR)) : rmin(T(0), rmax(sL, sR));
+  } else if constexpr (LIM == kVanLeer) {
+    const T r = T(2) * sL * sR;
+    return (r > T(0)) ? r / (sL + sR) : T(0);
+  } else if constexpr (LIM == kNoLimiter) {
+    return T(0.5) * (sL + sR);
+  } else {
+    return T(0);
+  }
+}
+
+// MUSCL-Hancock reconstruction of one cell from its conservative neighbours
+// um (left), u0 (centre), up (right).  Produces the complete half-step
+// states at the cell's left face (recL = rec_[1] of that face) and right
+// face (recR = rec_[0] of that face).  lh = 0.5 * dt / dx.
+template <int D, int DIR, int LIM, class T>
+__device__ __forceinline__ void ReconstructCell(const T* um, const T* u0, const T* up,
+                                                T lh, const Gas<T>& g, T* recL,
+                                                T* recR) {
+  using I = Idx<D>;
+  T qL[I::kComp], qR[I::kComp];
+#pragma unroll
+  for (int k = 0; k < I::kCons; ++k) {
+    const T s = LimitedSlope<LIM>(um[k], u0[k], up[k]);
+    qL[k] = u0[k] - T(0.5) * s;
+    qR[k] = u0[k] + T(0.5) * s;
+  }
+  CompleteFromCons<D>(qL, g);
+  CompleteFromCons<D>(qR, g);
+  T fL[I::kCons], fR[I::kCons];
+  PhysFlux<D, DIR>(fL, qL);
+  PhysFlux<D, DIR>(fR, qR);
+#pragma unroll
+  for (int k = 0; k < I::kCons; ++k) {
+    const T dq = lh * (fL[k] - fR[k]);
+    recR[k] = qR[k] + dq;
+    recL[k] = qL[k] + dq;
+  }
+  CompleteFromCons<D>(recR, g);
+  CompleteFromCons<D>(recL, g);
+}
+
+// Reconstruction of a ghost cell whose three stencil cells are copies of the
+// same boundary state u (transmissive boundary): every limiter returns a zero
+// slope, qL = qR = u, the half-step correction vanishes and the result is the
+// complete state of u (equal in value to the general formula).
+template <int D, class T>
+__device__ __forceinline__ void GhostRec(const T* u, const Gas<T>& g, T* rec) {
+#pragma unroll
+  for (int k = 0; k < D + 2; ++k) rec[k] = u[k];
+  CompleteFromCons<D>(rec, g);
+}
+
+// Hllem<PerfectGas<D>, Larrouturou> numerical flux in direction DIR between
+// complete states L and R; writes the D+2 flux components.
+template <int D, int DIR, bool LARR, bool QUIRK, class T>
+__device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
+                                          const Gas<T>& g) {
+  using I = Idx<D>;
+  constexpr int M = I::kMom;
+  const T rhoL = L[I::kRho];
+  const T rhoR = R[I::kRho];
+  const T aL = L[I::kA];
+  const T aR = R[I::kA];
+  const T pL = L[I::kP];
+  const T pR = R[I::kP];
+  const T sqRhoL = rsqrt_ieee(rhoL);
+  const T sqRhoR = rsqrt_ieee(rhoR);
+  const T sqRhoSum = sqRhoL + sqRhoR;
+  const T sqRhoL_over_Sum = sqRhoL / sqRhoSum;
+  const T sqRhoR_over_Sum = sqRhoR / sqRhoSum;
+  const T roeGamma = (sqRhoL * g.gamma + sqRhoR * g.gamma) / sqRhoSum;
+  const T gm1 = roeGamma - T(1);
+  if constexpr (LARR) {
+    // Transverse velocities and kinetic energies (HllemMethod.hpp:729-745);
+    // the state's transverse momentum becomes zero.
+    T vL[D], vR[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      vL[i] = L[M + i] / rhoL;
+      vR[i] = R[M + i] / rhoR;
+    }
+    T eKinRL = T(0), eKinRR = T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      if (i != DIR) {
+        eKinRL += vL[i] * vL[i];
+        eKinRR += vR[i] * vR[i];
+      }
+    }
+    eKinRL *= T(0.5);
+    eKinRR *= T(0.5);
+    const T EL = L[I::kE] - rhoL * eKinRL;
+    const T ER = R[I::kE] - rhoR * eKinRR;
+    const T mL = L[M + DIR];
+    const T mR = R[M + DIR];
+    // Physical fluxes of the modified states (only the components used).
+    const T uLd = vL[DIR]; // == mL / rhoL
+    const T uRd = vR[DIR];
+    const T fL_rho = mL;
+    const T fR_rho = mR;
+    const T fL_m = uLd * mL + pL;
+    const T fR_m = uRd * mR + pR;
+    const T fL_E = uLd * (EL + pL);
+    const T fR_E = uRd * (ER + pR);
+    const T hL = (EL + pL) / rhoL;
+    const T hR = (ER + pR) / rhoR;
+    const T roeU = sqRhoL_over_Sum * uLd + sqRhoR_over_Sum * uRd;
+    const T roeH = sqRhoL_over_Sum * hL + sqRhoR_over_Sum * hR;
+    const T squaredNormRoeU = roeU * roeU;
+    const T squaredNormRoeU_half = T(0.5) * squaredNormRoeU;
+    const T roeA2 = gm1 * (roeH - squaredNormRoeU_half);
+    const T roeA = rsqrt_ieee(roeA2);
+    const T maxA = rmax(aL, aR);
+    const T sL1 = uLd - maxA;
+    const T sL2 = roeU - roeA;
+    const T sR1 = roeU + roeA;
+    const T sR2 = uRd + maxA;
+    const T sL = rmin(sL1, sL2);
+    const T sR = rmax(sR1, sR2);
+    const T bL = rmin(sL, T(0));
+    const T bR = rmax(sR, T(0));
+    const T bLbR = bL * bR;
+    const T db = bR - bL;
+    const T db_positive = (db > T(0)) ? db : T(1);
+    const T hlle_rho = (bR * fL_rho - bL * fR_rho + bLbR * (rhoR - rhoL)) / db_positive;
+    const T hlle_m = (bR * fL_m - bL * fR_m + bLbR * (mR - mL)) / db_positive;
+    const T hlle_E = (bR * fL_E - bL * fR_E + bLbR * (ER - EL)) / db_positive;
+    const T u_bar = T(0.5) * (sR + sL);
+    const T delta = roeA / (roeA + rabs(u_bar));
+    const T b = bLbR / db_positive;
+    const T deltaRho = rhoR - rhoL;
+    const T deltaRhoU = mR - mL;
+    const T deltaRhoE = ER - EL;
+    const T gm1_over_roeA2 = gm1 / roeA2;
+    const T l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
+    const T lUd = gm1_over_roeA2 * roeU;
+    const T lE = -gm1_over_roeA2;
+    // alpha_2 summed in component order; the transverse terms are zero.
+    T alpha_2;
+    if constexpr (DIR == 0) {
+      alpha_2 = l21 * deltaRho + lUd * deltaRhoU;
+    } else {
+      alpha_2 = l21 * deltaRho + lUd * deltaRhoU;
+    }
+    alpha_2 = alpha_2 + lE * deltaRhoE;
+    const T b_delta_alpha_2 = b * delta * alpha_2;
+    const T frho = hlle_rho - b_delta_alpha_2;
+    flux[I::kRho] = frho;
+    if constexpr (D == 3 && QUIRK) {
+      // HllemMethod.hpp:990-992 uses roeU.row(iy), which the Larrouturou
+      // variant has zeroed: the normal momentum flux keeps the HLLE value.
+      flux[M + DIR] = hlle_m;
+    } else {
+      flux[M + DIR] = hlle_m - b_delta_alpha_2 * roeU;
+    }
+    const bool upwind = frho > T(0);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      if (i != DIR) flux[M + i] = frho * (upwind ? vL[i] : vR[i]);
+    }
+    const T f_rhoE = frho * (upwind ? eKinRL : eKinRR);
+    flux[I::kE] = hlle_E - b_delta_alpha_2 * squaredNormRoeU_half + f_rhoE;
+  } else {
+    // Plain HLLEM (Larrouturou = false): every component.
+    T fL[I::kCons], fR[I::kCons];
+    PhysFlux<D, DIR>(fL, L);
+    PhysFlux<D, DIR>(fR, R);
+    const T hL = (L[I::kE] + pL) / rhoL;
+    const T hR = (R[I::kE] + pR) / rhoR;
+    T uL[D], uR[D], roeU[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      uL[i] = L[M + i] / rhoL;
+      uR[i] = R[M + i] / rhoR;
+      roeU[i] = sqRhoL_over_Sum * uL[i] + sqRhoR_over_Sum * uR[i];
+    }
+    const T roeH = sqRhoL_over_Sum * hL + sqRhoR_over_Sum * hR;
+    T squaredNormRoeU = roeU[0] * roeU[0];
+#pragma unroll
+    for (int i = 1; i < D; ++i) squaredNormRoeU += roeU[i] * roeU[i];
+    const T squaredNormRoeU_half = T(0.5) * squaredNormRoeU;
+    const T roeA2 = gm1 * (roeH - squaredNormRoeU_half);
+    const T roeA = rsqrt_ieee(roeA2);
+    const T maxA = rmax(aL, aR);
+    const T sL = rmin(uL[DIR] - maxA, roeU[DIR] - roeA);
+    const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
+    const T bL = rmin(sL, T(0));
+    const T bR = rmax(sR, T(0));
+    const T bLbR = bL * bR;
+    const T db = bR - bL;
+    const T db_positive = (db > T(0)) ? db : T(1);
+    T hlle[I::kCons];
+#pragma unroll
+    for (int k = 0; k < I::kCons; ++k)
+      hlle[k] = (bR * fL[k] - bL * fR[k] + bLbR * (R[k] - L[k])) / db_positive;
+    const T u_bar = T(0.5) * (sR + sL);
+    const T delta = roeA / (roeA + rabs(u_bar));
+    const T b = bLbR / db_positive;
+    const T deltaRho = rhoR - rhoL;
+    T deltaRhoU[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) deltaRhoU[i] = R[M + i] - L[M + i];
+    const T deltaRhoE = R[I::kE] - L[I::kE];
+    const T gm1_over_roeA2 = gm1 / roeA2;
+    const T l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
+    T alpha_2 = l21 * deltaRho;
+#pragma unroll
+    for (int i = 0; i < D; ++i) alpha_2 = alpha_2 + (gm1_over_roeA2 * roeU[i]) * deltaRhoU[i];
+    alpha_2 = alpha_2 + (-gm1_over_roeA2) * deltaRhoE;
+    const T b_delta = b * delta;
+    const T b_delta_alpha_2 = b_delta * alpha_2;
+    flux[I::kRho] = hlle[I::kRho] - b_delta_alpha_2;
+    constexpr int IY = (D == 2) ? (DIR == 0 ? 1 : 0) : (DIR + 1) % 3;
+    constexpr int IZ = (IY + 1) % 3;
+    if constexpr (D == 3 && QUIRK) {
+      flux[M + DIR] = hlle[M + DIR] - b_delta_alpha_2 * roeU[IY];
+    } else {
+      flux[M + DIR] = hlle[M + DIR] - b_delta_alpha_2 * roeU[DIR];
+    }
+    const T alpha_3 = (-roeU[IY]) * deltaRho + deltaRhoU[IY];
+    const T b_delta_alpha_3 = b_delta * alpha_3;
+    flux[M + IY] = hlle[M + IY] - b_delta_alpha_2 * roeU[IY] - b_delta_alpha_3;
+    if constexpr (D == 3) {
+      const T alpha_4 = (-roeU[IZ]) * deltaRho + deltaRhoU[IZ];
+      const T b_delta_alpha_4 = b_delta * alpha_4;
+      flux[M + IZ] = hlle[M + IZ] - b_delta_alpha_2 * roeU[IZ] - b_delta_alpha_4;
+      flux[I::kE] = hlle[I::kE] - b_delta_alpha_2 * squaredNormRoeU_half -
+                    b_delta_alpha_3 * roeU[IY] - b_delta_alpha_4 * roeU[IZ];
+    } else {
+      flux[I::kE] = hlle[I::kE] - b_delta_alpha_2 * squaredNormRoeU_half -
+                    b_delta_alpha_3 * roeU[IY];
+    }
+  }
+}
+
+// Hllem::ComputeStableDt (array form) between complete states L and R.
+template <int D, int DIR, class T>
+__device__ __forceinline__ T HllemStableDt(const T* L, const T* R, T dx,
+                                           const Gas<T>& g) {
+  using I = Idx<D>;
+  constexpr int M = I::kMom;
+  const T rhoL = L[I::kRho];
+  const T rhoR = R[I::kRho];
+  const T hL = (L[I::kE] + L[I::kP]) / rhoL;
+  const T hR = (R[I::kE] + R[I::kP]) / rhoR;
+  const T sqRhoL = rsqrt_ieee(rhoL);
+  const T sqRhoR = rsqrt_ieee(rhoR);
+  const T sqRhoSum = sqRhoL + sqRhoR;
+  const T wL = sqRhoL / sqRhoSum;
+  const T wR = sqRhoR / sqRhoSum;
+  T uL[D], uR[D], roeU[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    uL[i] = L[M + i] / rhoL;
+    uR[i] = R[M + i] / rhoR;
+    roeU[i] = wL * uL[i] + wR * uR[i];
+  }
+  const T roeH = wL * hL + wR * hR;
+  T sq = roeU[0] * roeU[0];
+#pragma unroll
+  for (int i = 1; i < D; ++i) sq += roeU[i] * roeU[i];
+  const T roeA2 = g.gm1 * (roeH - T(0.5) * sq);
+  const T roeA = rsqrt_ieee(roeA2);
+  const T maxA = rmax(L[I::kA], R[I::kA]);
+  const T sL = rmin(uL[DIR] - maxA, roeU[DIR] - roeA);
+  const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
+  const T maxS = rmax(rabs(sL), rabs(sR));
+  return dx / maxS;
+}
+
+// Stable dt of a face whose two states are the same state Q (boundary face
+// between a cell and its transmissive ghost copy).  With L == R the Roe
+// weights are exactly 1/2 and every Roe average equals the state's own value,
+// so this equals HllemStableDt(Q, Q) bit for bit.
+template <int D, int DIR, class T>
+__device__ __forceinline__ T HllemStableDtUniform(const T* Q, T dx, const Gas<T>& g) {
+  using I = Idx<D>;
+  constexpr int M = I::kMom;
+  const T rho = Q[I::kRho];
+  const T h = (Q[I::kE] + Q[I::kP]) / rho;
+  T u[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) u[i] = Q[M + i] / rho;
+  T sq = u[0] * u[0];
+#pragma unroll
+  for (int i = 1; i < D; ++i) sq += u[i] * u[i];
+  const T roeA2 = g.gm1 * (h - T(0.5) * sq);
+  const T roeA = rsqrt_ieee(roeA2);
+  const T a = Q[I::kA];
+  const T sL = rmin(u[DIR] - a, u[DIR] - roeA);
+  const T sR = rmax(u[DIR] + roeA, u[DIR] + a);
+  const T maxS = rmax(rabs(sL), rabs(sR));
+  return dx / maxS;
+}
+
+// Order-preserving integer key of a non-negative double (for atomic min).
+__device__ __forceinline__ unsigned long long DtKey(double dt) {
+  // Stable dts are positive (or +inf); NaN maps above +inf so it never wins.
+  if (dt != dt) return 0x7ff8000000000000ull;
+  return static_cast<unsigned long long>(__double_as_longlong(dt));
+}
+
+} // namespace fvs

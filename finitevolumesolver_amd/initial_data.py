@@ -1,0 +1,161 @@
+"""Initial data of the benchmark workloads (host, numpy FP64).
+
+Random numbers come from SplitMix64 (Steele, Lea & Flood, "Fast splittable
+pseudorandom number generators", OOPSLA 2014), normals from Box-Muller, so
+the same seed gives the same field on every machine.  States are returned as
+conservative SoA arrays of shape (D+2, nz, ny, nx); pressure enters through
+E = p / (gamma - 1) + 0.5 |rho u|^2 / rho (PerfectGas::CompleteFromPrim,
+src/equations/PerfectGas.cpp:146-158).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self, n: int) -> np.ndarray:
+        out = np.empty(n, dtype=np.uint64)
+        s = self.state
+        for i in range(n):
+            s = (s + 0x9E3779B97F4A7C15) & MASK64
+            z = s
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+            out[i] = z ^ (z >> 31)
+        self.state = s
+        return out
+
+    def next_u64_fast(self, n: int) -> np.ndarray:
+        """Vectorised stream: element i is the (i+1)-th output of next_u64."""
+        with np.errstate(over="ignore"):
+            idx = np.arange(1, n + 1, dtype=np.uint64)
+            s = np.uint64(self.state) + idx * np.uint64(0x9E3779B97F4A7C15)
+            z = s
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.state = int(s[-1]) if n else self.state
+        return z
+
+    def uniform(self, n: int) -> np.ndarray:
+        """Uniform doubles in [0, 1) from the top 53 bits."""
+        return (self.next_u64_fast(n) >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+    def normal(self, n: int) -> np.ndarray:
+        m = (n + 1) // 2
+        u1 = self.uniform(m)
+        u2 = self.uniform(m)
+        r = np.sqrt(-2.0 * np.log(1.0 - u1))
+        z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return z[:n]
+
+
+def cons_from_prim(rho, vel, p, gamma=1.4) -> np.ndarray:
+    """rho, p: arrays of the grid shape; vel: list of D arrays."""
+    D = len(vel)
+    gm1inv = 1.0 / (gamma - 1.0)
+    mom = [rho * v for v in vel]
+    sq = mom[0] * mom[0]
+    if D == 2:
+        sq = mom[0] * mom[0] + mom[1] * mom[1]
+    elif D == 3:
+        sq = (mom[0] * mom[0] + mom[1] * mom[1]) + mom[2] * mom[2]
+    e_kin = 0.5 * sq / rho
+    E = e_kin + p * gm1inv
+    out = np.stack([rho, *mom, E])
+    if D == 2:
+        out = out[:, None, :, :]
+    return np.ascontiguousarray(out)
+
+
+def cell_centers(n: int, lo: float, hi: float) -> np.ndarray:
+    dx = (hi - lo) / n
+    return lo + (np.arange(n) + 0.5) * dx
+
+
+def circular_riemann_2d(n: int = 256, gamma: float = 1.4):
+    """2D circular Riemann problem on [-1, 1]^2: r < 0.4 rho=1, p=1, else
+    rho=0.125, p=0.1, fluid at rest.  Returns (cons, dx)."""
+    x = cell_centers(n, -1.0, 1.0)
+    X, Y = np.meshgrid(x, x, indexing="xy")  # Y rows, X columns -> [ny, nx]
+    r = np.sqrt(X * X + Y * Y)
+    inside = r < 0.4
+    rho = np.where(inside, 1.0, 0.125)
+    p = np.where(inside, 1.0, 0.1)
+    zero = np.zeros_like(rho)
+    dx = 2.0 / n
+    return cons_from_prim(rho, [zero, zero], p, gamma), (dx, dx)
+
+
+def _fourier_field(coords, seed: int, n_modes: int = 8, kmax: int = 4) -> np.ndarray:
+    """sum_m a_m cos(2 pi k_m . x + phi_m), sum |a_m| = 1, 1 <= |k|_inf <= kmax."""
+    rng = SplitMix64(seed)
+    D = len(coords)
+    field = np.zeros(np.broadcast_shapes(*[c.shape for c in coords]))
+    ks, amps, phis = [], [], []
+    while len(ks) < n_modes:
+        k = (rng.uniform(D) * (2 * kmax + 1)).astype(np.int64) - kmax
+        if np.max(np.abs(k)) < 1:
+            continue
+        ks.append(k)
+        amps.append(rng.uniform(1)[0] * 2.0 - 1.0)
+        phis.append(rng.uniform(1)[0] * 2.0 * np.pi)
+    amps = np.asarray(amps)
+    amps = amps / np.sum(np.abs(amps))
+    for k, a, ph in zip(ks, amps, phis):
+        arg = ph + 2.0 * np.pi * sum(float(k[d]) * coords[d] for d in range(D))
+        field = field + a * np.cos(arg)
+    return field
+
+
+def _box_smooth(f: np.ndarray, passes: int = 3) -> np.ndarray:
+    """5-point box filter per axis (periodic wrap), `passes` times."""
+    for _ in range(passes):
+        for ax in range(f.ndim):
+            f = (np.roll(f, -2, ax) + np.roll(f, -1, ax) + f + np.roll(f, 1, ax)
+                 + np.roll(f, 2, ax)) / 5.0
+    return f
+
+
+def fourier_modes_3d(n: int = 256, gamma: float = 1.4, seed0: int = 1):
+    """3D smooth random field on [0, 1]^3: rho = 1 + 0.2 F_1, p = 1 + 0.1 F_2,
+    velocities N(0, 0.1) smoothed by 3 box-filter passes per axis.
+    Returns (cons (5, n, n, n), dx)."""
+    x = cell_centers(n, 0.0, 1.0)
+    X = x[None, None, :]
+    Y = x[None, :, None]
+    Z = x[:, None, None]
+    rho = 1.0 + 0.2 * _fourier_field([X, Y, Z], seed0)
+    p = 1.0 + 0.1 * _fourier_field([X, Y, Z], seed0 + 1)
+    rho = np.broadcast_to(rho, (n, n, n)).copy()
+    p = np.broadcast_to(p, (n, n, n)).copy()
+    vel = []
+    for s in range(3):
+        v = SplitMix64(seed0 + 2 + s).normal(n ** 3).reshape(n, n, n) * 0.1
+        vel.append(_box_smooth(v))
+    dx = 1.0 / n
+    return cons_from_prim(rho, vel, p, gamma), (dx, dx, dx)
+
+
+def random_states(n: int, dim: int, seed: int, gamma: float = 1.4,
+                  rho_range=(0.1, 10.0), p_range=(0.1, 10.0), u_scale=1.0):
+    """n random conservative states (D+2, n) with log-uniform rho, p."""
+    rng = SplitMix64(seed)
+    lr = np.log(rho_range)
+    lp = np.log(p_range)
+    rho = np.exp(lr[0] + (lr[1] - lr[0]) * rng.uniform(n))
+    p = np.exp(lp[0] + (lp[1] - lp[0]) * rng.uniform(n))
+    vel = [u_scale * (2.0 * rng.uniform(n) - 1.0) for _ in range(dim)]
+    gm1inv = 1.0 / (gamma - 1.0)
+    mom = [rho * v for v in vel]
+    if dim == 2:
+        sq = mom[0] * mom[0] + mom[1] * mom[1]
+    else:
+        sq = (mom[0] * mom[0] + mom[1] * mom[1]) + mom[2] * mom[2]
+    E = 0.5 * sq / rho + p * gm1inv
+    return np.stack([rho, *mom, E])

@@ -1,0 +1,229 @@
+/*
+ * fvs.h -- C ABI of the MI355X (gfx950) split patch update for the
+ * perfect-gas Euler equations.
+ *
+ * Every entry point is stream-ordered on the HIP stream given to fvs_create,
+ * takes plain pointers to device memory that the caller owns, never frees a
+ * caller buffer and never throws.  Status codes mirror the reference's error
+ * conventions:
+ *
+ *   FVS_OK                   success
+ *   FVS_TIME_STEP_TOO_LARGE  CFL check failed; mirrors TimeStepTooLarge
+ *                            (include/fub/TimeStepError.hpp:71-74)
+ *   FVS_NEWTON_NOT_CONVERGED Newton iteration hit its iteration cap; mirrors
+ *                            the throw in include/fub/NewtonIteration.hpp:44-50
+ *   FVS_BAD_ARGUMENT         shape / enum / pointer error (nothing launched)
+ *   FVS_HIP_ERROR            a HIP runtime call failed
+ *
+ * Layout.  States are component-major structure-of-arrays with x fastest, as
+ * the reference's PatchDataView with layout_left over an AMReX FArrayBox
+ * (include/fub/PatchDataView.hpp:184-356).  A Complete state of a D-dimensional
+ * perfect gas has D+4 components (rho, rho*u_1..rho*u_D, rho*E, p, a), a
+ * Conservative state D+2 (include/fub/equations/PerfectGas.hpp:132-161).
+ * Directions are 0 = X, 1 = Y, 2 = Z.
+ */
+#ifndef FVS_H
+#define FVS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fvs_status {
+  FVS_OK = 0,
+  FVS_TIME_STEP_TOO_LARGE = 1,
+  FVS_NEWTON_NOT_CONVERGED = 2,
+  FVS_BAD_ARGUMENT = 3,
+  FVS_HIP_ERROR = 4
+};
+
+/* Base (two-state) numerical flux.
+ * FVS_BASE_HLLEM_LARROUTUROU: perfect_gas::Hllem<Eq, true>, the default
+ *   (include/fub/equations/perfect_gas/HllemMethod.hpp:30,92;
+ *    src/equations/perfect_gas/HllemMethod.hpp:711-1072).
+ * FVS_BASE_HLLEM: perfect_gas::Hllem<Eq, false>.
+ * FVS_BASE_GODUNOV_EXACT: GodunovMethod<Eq, ExactRiemannSolver<Eq>>
+ *   (include/fub/flux_method/GodunovMethod.hpp:130-199,
+ *    src/equations/perfect_gas/ExactRiemannSolver.hpp:34-353). */
+enum fvs_base_method {
+  FVS_BASE_HLLEM_LARROUTUROU = 0,
+  FVS_BASE_HLLEM = 1,
+  FVS_BASE_GODUNOV_EXACT = 2
+};
+
+/* Reconstruction wrapped around the base flux.
+ * FVS_RECON_NONE: FluxMethod<Base> (stencil width 1).
+ * FVS_RECON_MUSCL_HANCOCK: MusclHancockMethod<Eq, Base, Limiter>
+ *   (include/fub/flux_method/MusclHancockMethod.hpp:277-571, stencil width 2). */
+enum fvs_reconstruction { FVS_RECON_NONE = 0, FVS_RECON_MUSCL_HANCOCK = 1 };
+
+/* Slope limiters, include/fub/flux_method/MusclHancockMethod.hpp:31-187. */
+enum fvs_limiter {
+  FVS_LIMITER_MINMOD = 0,
+  FVS_LIMITER_VANLEER = 1,
+  FVS_LIMITER_NONE = 2, /* NoLimiter: central slope 0.5 (sL + sR) */
+  FVS_LIMITER_NO_GRADIENT = 3
+};
+
+/* Arithmetic of reconstruction and flux.  The conservative update, Newton
+ * iterations and cut-cell divisions are FP64 in every mode. */
+enum fvs_precision { FVS_FP64 = 0, FVS_FP32_FLUX = 1 };
+
+/* Newton initial guess of the exact Riemann solver. */
+enum fvs_newton_guess {
+  FVS_GUESS_MEAN_PRESSURE = 0, /* 0.5 (pL + pR), ExactRiemannSolver.hpp:91 */
+  FVS_GUESS_LEARNED = 1        /* small MLP, see fvs_mlp_* below */
+};
+
+typedef struct fvs_method_desc {
+  int dim;            /* 2 or 3 */
+  double gamma;       /* ratio of specific heats, 1.4 in the reference */
+  int base;           /* enum fvs_base_method */
+  int reconstruction; /* enum fvs_reconstruction */
+  int limiter;        /* enum fvs_limiter */
+  int precision;      /* enum fvs_precision */
+  int quirks;         /* 1: reproduce the reference's 3D HLLEM normal-momentum
+                         anti-diffusion with roeU[iy] (HllemMethod.hpp:990-992);
+                         0: use roeU[ix] as the 2D branch does */
+  int newton_guess;   /* enum fvs_newton_guess */
+  int max_newton_iterations; /* 5000 in the reference */
+  double newton_tolerance;   /* 1e-7 in the reference (absolute |f|) */
+} fvs_method_desc;
+
+/* Strided view over a box of cells or faces, the analogue of
+ * PatchDataView<double, R+1, layout_stride>.  Element (i, j, k, c) lives at
+ * data + (i-lo[0])*stride[0] + (j-lo[1])*stride[1] + (k-lo[2])*stride[2]
+ * + c*comp_stride.  Boxes are inclusive [lo, hi]; unused dimensions have
+ * lo = hi = 0. */
+typedef struct fvs_view {
+  double* data;
+  int ncomp;
+  int64_t lo[3];
+  int64_t hi[3];
+  int64_t stride[3];
+  int64_t comp_stride;
+} fvs_view;
+
+typedef struct fvs_handle fvs_handle;
+
+/* Creates a flux-method handle bound to a HIP stream (NULL = default). */
+int fvs_create(const fvs_method_desc* desc, void* hip_stream, fvs_handle** out);
+int fvs_destroy(fvs_handle* handle);
+/* Text of the last error on this thread (never NULL). */
+const char* fvs_last_error(void);
+/* Version string of the library build. */
+const char* fvs_version(void);
+
+/* ---- patch-level operators (reference API surface) -------------------- */
+
+/* FluxMethod<Base>::ComputeStableDt(simd, states, dx, dir)
+ * (include/fub/flux_method/FluxMethod.hpp:341-382): the minimum over all
+ * faces of the states box (shrunk by the stencil in dir) of the base
+ * method's stable dt.  The result is written to *dt_device (device memory). */
+int fvs_stable_dt(fvs_handle* h, const fvs_view* states, double dx, int dir,
+                  double* dt_device);
+
+/* FluxMethod<Base>::ComputeNumericFluxes(simd, fluxes, states, dt, dx, dir)
+ * (include/fub/flux_method/FluxMethod.hpp:246-293).  fluxes has D+2
+ * components, states D+4; every face of fluxes' box is computed from the
+ * stencil of 2*width cells around it. */
+int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes,
+                       const fvs_view* states, double dt, double dx, int dir);
+
+/* HyperbolicPatchIntegrator<Tag>::UpdateConservatively
+ * (src/HyperbolicPatchIntegrator.cpp:35-154):
+ * next = prev + dt/dx (F_left - F_right) over next's box; next and prev may
+ * alias.  Components: the first ncomp of next. */
+int fvs_update_conservatively(fvs_handle* h, const fvs_view* next,
+                              const fvs_view* prev, const fvs_view* fluxes,
+                              double dt, double dx, int dir);
+
+/* CompleteFromConsFn (include/fub/CompleteFromCons.hpp:141-204) with
+ * PerfectGas::CompleteFromCons (src/equations/PerfectGas.cpp:112-124):
+ * copies the D+2 conservative components of cons into complete and
+ * recomputes pressure and speed of sound.  complete and cons may alias. */
+int fvs_complete_from_cons(fvs_handle* h, const fvs_view* complete,
+                           const fvs_view* cons);
+
+/* amrex::TransmissiveBoundary::FillBoundary
+ * (src/AMReX/boundary_condition/TransmissiveBoundary.cpp:31-93): every cell
+ * of the view's box that lies outside [domain_lo, domain_hi] in direction dir
+ * gets the value of the domain's boundary cell of its row, all components. */
+int fvs_transmissive_fill(fvs_handle* h, const fvs_view* states,
+                          const int64_t domain_lo[3], const int64_t domain_hi[3],
+                          int dir);
+
+/* ---- fused single-patch level integrator -------------------------------- */
+
+/* Splitting methods (src/split_method/GodunovSplitting.cpp:25-34,
+ * src/split_method/StrangSplitting.cpp:25-69). */
+enum fvs_splitting { FVS_SPLIT_GODUNOV = 0, FVS_SPLIT_STRANG = 1 };
+
+/* A single patch of n[0] x n[1] x n[2] cells (n[2] = 1 in 2D) with
+ * transmissive boundaries on every side, stored as D+2 conservative
+ * component arrays without ghost cells (x fastest).  Pressure and speed of
+ * sound are recomputed from the conservative variables wherever the
+ * reference reads its Complete state; this is bit-identical because the
+ * reference derives them with CompleteFromCons after every update. */
+typedef struct fvs_patch_desc {
+  int64_t n[3];
+  double dx[3];
+  int splitting; /* enum fvs_splitting */
+  double cfl;    /* RunOptions::cfl, 0.8 in the reference */
+} fvs_patch_desc;
+
+typedef struct fvs_level fvs_level;
+
+/* Allocates the level's three state buffers and its device control block and
+ * uploads nothing.  The state starts zeroed. */
+int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch,
+                     fvs_level** out);
+int fvs_level_destroy(fvs_level* level);
+/* Device pointer of component c of the current state (valid until the next
+ * advance call). */
+double* fvs_level_state(fvs_level* level, int c);
+/* Copies D+2 component arrays (device memory, ncell each) into the current
+ * state; resets time and cycle count to (t0, 0). */
+int fvs_level_set_state(fvs_level* level, const double* const* comps, double t0);
+/* Copies the current state out into D+2 device arrays. */
+int fvs_level_get_state(fvs_level* level, double* const* comps);
+
+/* RunSimulation (include/fub/RunSimulation.hpp:86-191) restricted to one
+ * patch: up to max_steps cycles, stopping at final_time, each step with
+ * dt = min(final_time - t, cfl * min_dir stable_dt), the split sweeps in the
+ * direction order of MakeSplitDirections (SplittingMethod.hpp:115-126), the
+ * per-sweep CFL check of DimensionalSplitLevelIntegrator (:240-245) and
+ * restart of a failed step from the untouched step-start state with
+ * cfl * failure_dt.  Steps are enqueued `batch` at a time without host
+ * synchronisation; the host syncs once per batch.  dt_log (host memory,
+ * max_steps doubles, may be NULL) receives the dt of each completed cycle. */
+int fvs_level_advance(fvs_level* level, int max_steps, double final_time,
+                      int batch, double* dt_log, int* steps_done,
+                      int* failures);
+/* Time point and cycle count of the current state. */
+int fvs_level_time(fvs_level* level, double* t, int64_t* cycles);
+
+/* One fused split sweep (flux + update + completion + CFL check of the
+ * input) from cons_in to cons_out (distinct arrays, D+2 components of
+ * n[0]*n[1]*n[2] each, x fastest, transmissive boundaries).  Writes the
+ * minimum face stable dt of the input in dir to *stable_dt_device.  This is
+ * the operator one split step of AdvanceLevelNonRecursively applies
+ * (DimensionalSplitLevelIntegrator.hpp:227-264). */
+int fvs_split_sweep(fvs_handle* h, const double* const* cons_in,
+                    double* const* cons_out, const int64_t n[3], double dt,
+                    double dx, int dir, double* stable_dt_device);
+
+/* Minimum over directions and faces of the stable dt of a patch with
+ * transmissive boundaries (DimensionalSplitLevelIntegrator::ComputeStableDt,
+ * :204-219); per-direction minima go to dt_dir_device[0..dim-1]. */
+int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons,
+                        const int64_t n[3], const double dx[3],
+                        double* dt_dir_device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVS_H */

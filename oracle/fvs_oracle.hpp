@@ -1,0 +1,652 @@
+// TEST INFRASTRUCTURE ONLY.  This is the FP64 CPU oracle of the split patch
+// update: a restatement of FiniteVolumeSolver's perfect-gas flux methods,
+// written as scalar code that follows the reference's 8-lane (SIMD, Eigen
+// Array1d) semantics lane by lane, because the reference's production
+// adapters run the SIMD paths (include/fub/AMReX/FluxMethodAdapter.hpp:54-56).
+// Only tests, the smoke check and the CPU-baseline leg of the benchmark may
+// load it.  The product library never includes, links or calls it.
+//
+// Arithmetic is plain IEEE binary64; the build pins -ffp-contract=off so no
+// multiply-add is fused, and every expression keeps the reference's
+// operation order (left-to-right, as the Eigen expression templates
+// evaluate lane-wise).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <limits>
+
+namespace fvs_oracle {
+
+// PerfectGas<Dim> parameters (include/fub/equations/PerfectGas.hpp:216-222):
+// gamma = 1.4, gamma_minus_1_inv = 1 / (gamma - 1) evaluated at run time.
+struct Gas {
+  double gamma;
+  double gm1inv;
+  explicit Gas(double g = 1.4) : gamma(g), gm1inv(1.0 / (g - 1.0)) {}
+};
+
+template <int D> struct Cons {
+  double rho;
+  double mom[D];
+  double E;
+};
+
+// Complete<PerfectGas<D>>: rho, momentum, energy, pressure, speed_of_sound.
+template <int D> struct State {
+  double rho;
+  double mom[D];
+  double E;
+  double p;
+  double a;
+};
+
+// Squared momentum norm of the array path: Eigen 3.3 reduces a column block
+// of 2 or 3 rows (no packet access for a 3-row block) with its split-half
+// unroller, a0 + (a1 + a2) (PerfectGas.cpp:33-38).
+template <int D> inline double SquaredNormArray(const double* m) {
+  if constexpr (D == 1) {
+    return m[0] * m[0];
+  } else if constexpr (D == 2) {
+    return m[0] * m[0] + m[1] * m[1];
+  } else {
+    return m[0] * m[0] + (m[1] * m[1] + m[2] * m[2]);
+  }
+}
+
+// Squared norm of a fixed Vector<D> in the scalar path: Eigen's linear
+// vectorised reduction with SSE2 packets of two, (a0 + a1) + a2
+// (PerfectGas.cpp:25-29, ExactRiemannSolver.hpp:28-31).
+template <int D> inline double SquaredNormScalar(const double* m) {
+  if constexpr (D == 1) {
+    return m[0] * m[0];
+  } else if constexpr (D == 2) {
+    return m[0] * m[0] + m[1] * m[1];
+  } else {
+    return (m[0] * m[0] + m[1] * m[1]) + m[2] * m[2];
+  }
+}
+
+// PerfectGas::CompleteFromCons, array form (src/equations/PerfectGas.cpp:112-124).
+template <int D>
+inline void CompleteFromConsArray(const Gas& g, State<D>& q) {
+  const double e_kin = 0.5 * SquaredNormArray<D>(q.mom) / q.rho;
+  const double e_int = q.E - e_kin;
+  q.p = e_int / g.gm1inv;
+  q.a = std::sqrt(g.gamma * q.p / q.rho);
+}
+
+// PerfectGas::CompleteFromCons, scalar form (src/equations/PerfectGas.cpp:96-110).
+template <int D>
+inline void CompleteFromConsScalar(const Gas& g, State<D>& q) {
+  const double e_kin = 0.5 * SquaredNormScalar<D>(q.mom) / q.rho;
+  const double e_int = q.E - e_kin;
+  q.p = e_int / g.gm1inv;
+  q.a = std::sqrt(g.gamma * q.p / q.rho);
+}
+
+// PerfectGas::CompleteFromCons, masked array form
+// (src/equations/PerfectGas.cpp:126-144): invalid lanes (mask false or
+// rho <= 0) become the zero state.
+template <int D>
+inline void CompleteFromConsMasked(const Gas& g, State<D>& q, bool mask) {
+  mask = mask && (q.rho > 0.0);
+  const double sq = SquaredNormArray<D>(q.mom);
+  const double safe_rho_ke = mask ? q.rho : 1.0;
+  const double e_kin = 0.5 * sq / safe_rho_ke;
+  const double e_int = q.E - e_kin;
+  State<D> out{};
+  out.rho = mask ? q.rho : 0.0;
+  for (int d = 0; d < D; ++d) out.mom[d] = mask ? q.mom[d] : 0.0;
+  out.E = mask ? q.E : 0.0;
+  out.p = mask ? e_int / g.gm1inv : 0.0;
+  const double safe_rho = mask ? out.rho : 1.0;
+  out.a = mask ? std::sqrt(g.gamma * out.p / safe_rho) : 0.0;
+  q = out;
+}
+
+// PerfectGas::CompleteFromPrim, scalar (src/equations/PerfectGas.cpp:146-158).
+template <int D>
+inline State<D> CompleteFromPrim(const Gas& g, double rho, const double* v,
+                                 double p) {
+  State<D> q{};
+  q.rho = rho;
+  for (int d = 0; d < D; ++d) q.mom[d] = rho * v[d];
+  q.p = p;
+  const double e_kin = 0.5 * SquaredNormScalar<D>(q.mom) / q.rho;
+  q.E = e_kin + p * g.gm1inv;
+  q.a = std::sqrt(g.gamma * q.p / q.rho);
+  return q;
+}
+
+// PerfectGas::Flux (src/equations/PerfectGas.cpp:52-78); array and scalar
+// forms are the same expression.
+template <int D>
+inline void Flux(Cons<D>& f, const State<D>& q, int dir) {
+  const double velocity = q.mom[dir] / q.rho;
+  f.rho = q.mom[dir];
+  for (int d = 0; d < D; ++d) f.mom[d] = velocity * q.mom[d];
+  f.mom[dir] += q.p;
+  f.E = velocity * (q.E + q.p);
+}
+
+// Masked array flux (src/equations/PerfectGas.cpp:80-93).
+template <int D>
+inline void FluxMasked(Cons<D>& f, const State<D>& q, bool mask, int dir) {
+  mask = mask && (q.rho > 0.0);
+  const double density = mask ? q.rho : 1.0;
+  const double velocity = q.mom[dir] / density;
+  f.rho = q.mom[dir];
+  for (int d = 0; d < D; ++d) f.mom[d] = velocity * q.mom[d];
+  f.mom[dir] += q.p;
+  f.E = velocity * (q.E + q.p);
+}
+
+enum Limiter { kMinMod = 0, kVanLeer = 1, kNoLimiter = 2, kNoGradient = 3 };
+
+// Limited slope of one component (include/fub/flux_method/MusclHancockMethod.hpp:31-187),
+// array forms.
+inline double LimitedSlope(int limiter, double qL, double qM, double qR) {
+  const double sL = qM - qL;
+  const double sR = qR - qM;
+  switch (limiter) {
+  case kMinMod:
+    // (sR > 0).select(zero.max(sL.min(sR)), zero.min(sL.max(sR)))
+    return (sR > 0) ? std::max(0.0, std::min(sL, sR))
+                    : std::min(0.0, std::max(sL, sR));
+  case kVanLeer: {
+    const double r = 2.0 * sL * sR;
+    return (r > 0.0) ? r / (sL + sR) : 0.0;
+  }
+  case kNoLimiter:
+    return 0.5 * (sL + sR);
+  default:
+    return 0.0;
+  }
+}
+
+template <int D> inline double& C(Cons<D>& c, int k) {
+  return (&c.rho)[k];
+}
+template <int D> inline double C(const Cons<D>& c, int k) {
+  return (&c.rho)[k];
+}
+template <int D> inline double& C(State<D>& c, int k) {
+  return (&c.rho)[k];
+}
+template <int D> inline double C(const State<D>& c, int k) {
+  return (&c.rho)[k];
+}
+
+// perfect_gas::Hllem<PerfectGas<D>, Larrouturou>::ComputeNumericFlux, array
+// form (src/equations/perfect_gas/HllemMethod.hpp:711-1072).  Returns p*.
+// quirk3d: the 3D branch subtracts b*delta*alpha_2*roeU[iy] from the normal
+// momentum flux (:990-992); with quirk3d = false it uses roeU[ix].
+template <int D>
+inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
+                        const State<D>& stR, int dir, bool larrouturou,
+                        bool quirk3d) {
+  State<D> L = stL;
+  State<D> R = stR;
+  const double Msq = 1.0; // euler::MaSq(PerfectGas) == 1 (EulerEquation.hpp:53-62)
+  const double Minv = 1.0 / std::sqrt(Msq);
+  const double rhoL = L.rho;
+  const double rhoR = R.rho;
+  double vL_[D > 0 ? D : 1];
+  double vR_[D > 0 ? D : 1];
+  double eKinRL = 0.0;
+  double eKinRR = 0.0;
+  if (larrouturou) {
+    for (int i = 0; i < D; ++i) {
+      const int dim_mask = (i == dir);
+      vL_[i] = L.mom[i] / rhoL;
+      vR_[i] = R.mom[i] / rhoR;
+      eKinRL += static_cast<double>(1 - dim_mask) * vL_[i] * vL_[i];
+      eKinRR += static_cast<double>(1 - dim_mask) * vR_[i] * vR_[i];
+      L.mom[i] *= static_cast<double>(dim_mask);
+      R.mom[i] *= static_cast<double>(dim_mask);
+    }
+    eKinRL *= 0.5 * Msq;
+    eKinRR *= 0.5 * Msq;
+    L.E -= rhoL * eKinRL;
+    R.E -= rhoR * eKinRR;
+  }
+  Cons<D> fluxL, fluxR;
+  Flux<D>(fluxL, L, dir);
+  Flux<D>(fluxR, R, dir);
+
+  const double aL = L.a;
+  const double aR = R.a;
+  const double rhoEL = L.E;
+  const double rhoER = R.E;
+  const double pL = L.p;
+  const double pR = R.p;
+  const double hL = (rhoEL + pL) / rhoL;
+  const double hR = (rhoER + pR) / rhoR;
+  const double sqRhoL = std::sqrt(rhoL);
+  const double sqRhoR = std::sqrt(rhoR);
+  const double sqRhoSum = sqRhoL + sqRhoR;
+  const double sqRhoL_over_Sum = sqRhoL / sqRhoSum;
+  const double sqRhoR_over_Sum = sqRhoR / sqRhoSum;
+
+  double uL[D], uR[D], roeU[D];
+  for (int i = 0; i < D; ++i) {
+    uL[i] = L.mom[i] / rhoL;
+    uR[i] = R.mom[i] / rhoR;
+    roeU[i] = sqRhoL_over_Sum * uL[i] + sqRhoR_over_Sum * uR[i];
+  }
+  const double roeH = sqRhoL_over_Sum * hL + sqRhoR_over_Sum * hR;
+  double squaredNormRoeU = 0.0;
+  for (int i = 0; i < D; ++i) squaredNormRoeU += roeU[i] * roeU[i];
+  squaredNormRoeU *= Msq;
+  const double squaredNormRoeU_half = 0.5 * squaredNormRoeU;
+
+  const double gammaL = g.gamma;
+  const double gammaR = g.gamma;
+  const double roeGamma = (sqRhoL * gammaL + sqRhoR * gammaR) / sqRhoSum;
+  const double gm1 = roeGamma - 1.0;
+
+  const double roeA2 = gm1 * (roeH - squaredNormRoeU_half);
+  const double roeA = std::sqrt(roeA2);
+  const double maxA = std::max(aL, aR);
+  const double sL1 = uL[dir] - maxA * Minv;
+  const double sL2 = roeU[dir] - roeA * Minv;
+  const double sR1 = roeU[dir] + roeA * Minv;
+  const double sR2 = uR[dir] + maxA * Minv;
+  const double sL = std::min(sL1, sL2);
+  const double sR = std::max(sR1, sR2);
+
+  const double bL = std::min(sL, 0.0);
+  const double bR = std::max(sR, 0.0);
+  const double bLbR = bL * bR;
+  const double db = bR - bL;
+  const double db_positive = (db > 0.0) ? db : 1.0;
+
+  Cons<D> hlle;
+  for (int k = 0; k < D + 2; ++k) {
+    C(hlle, k) = (bR * C(fluxL, k) - bL * C(fluxR, k) +
+                  bLbR * (C(R, k) - C(L, k))) /
+                 db_positive;
+  }
+
+  const double u_bar = 0.5 * (sR + sL);
+  const double u_bar_abs = std::abs(u_bar);
+  const double delta = roeA / (roeA + u_bar_abs);
+  const double b = bLbR / db_positive;
+
+  const double deltaRho = rhoR - rhoL;
+  double deltaRhoU[D];
+  for (int i = 0; i < D; ++i) deltaRhoU[i] = R.mom[i] - L.mom[i];
+  const double deltaRhoE = rhoER - rhoEL;
+  const double gm1_over_roeA2 = gm1 / roeA2;
+
+  if constexpr (D == 1) {
+    const double l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
+    const double l22 = gm1_over_roeA2 * Msq * roeU[0];
+    const double l23 = gm1_over_roeA2 * (-1.0);
+    const double alpha_2 = l21 * deltaRho + l22 * deltaRhoU[0] + l23 * deltaRhoE;
+    const double b_delta_alpha_2 = b * delta * alpha_2;
+    flux.rho = hlle.rho - b_delta_alpha_2;
+    flux.mom[0] = hlle.mom[0] - b_delta_alpha_2 * roeU[0];
+    flux.E = hlle.E - b_delta_alpha_2 * squaredNormRoeU_half;
+  } else if constexpr (D == 2) {
+    const int ix = dir;
+    const int iy = (ix == 0);
+    const double l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
+    const double l22 = gm1_over_roeA2 * Msq * roeU[0];
+    const double l23 = gm1_over_roeA2 * roeU[1];
+    const double l24 = -gm1_over_roeA2;
+    const double alpha_2 = l21 * deltaRho + l22 * deltaRhoU[0] +
+                           l23 * deltaRhoU[1] + l24 * deltaRhoE;
+    const double b_delta = b * delta;
+    const double b_delta_alpha_2 = b_delta * alpha_2;
+    flux.rho = hlle.rho - b_delta_alpha_2;
+    flux.mom[ix] = hlle.mom[ix] - b_delta_alpha_2 * roeU[ix];
+    if (larrouturou) {
+      const bool upwind = flux.rho > 0.0;
+      flux.mom[iy] = flux.rho * (upwind ? vL_[iy] : vR_[iy]);
+      const double f_rhoE = flux.rho * (upwind ? eKinRL : eKinRR);
+      flux.E = hlle.E - b_delta_alpha_2 * squaredNormRoeU_half + f_rhoE;
+    } else {
+      const double l31 = -roeU[iy];
+      const double alpha_3 = l31 * deltaRho + deltaRhoU[iy];
+      const double b_delta_alpha_3 = b_delta * alpha_3;
+      flux.mom[iy] = hlle.mom[iy] - b_delta_alpha_2 * roeU[iy] - b_delta_alpha_3;
+      flux.E = hlle.E - b_delta_alpha_2 * squaredNormRoeU_half -
+               b_delta_alpha_3 * roeU[iy];
+    }
+  } else {
+    const int ix = dir;
+    const int iy = (ix + 1) % 3;
+    const int iz = (iy + 1) % 3;
+    const double l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
+    const double l22 = gm1_over_roeA2 * Msq * roeU[0];
+    const double l23 = gm1_over_roeA2 * roeU[1];
+    const double l24 = gm1_over_roeA2 * roeU[2];
+    const double l25 = -gm1_over_roeA2;
+    const double alpha_2 = l21 * deltaRho + l22 * deltaRhoU[0] +
+                           l23 * deltaRhoU[1] + l24 * deltaRhoU[2] +
+                           l25 * deltaRhoE;
+    const double b_delta = b * delta;
+    const double b_delta_alpha_2 = b_delta * alpha_2;
+    flux.rho = hlle.rho - b_delta_alpha_2;
+    flux.mom[ix] =
+        hlle.mom[ix] - b_delta_alpha_2 * roeU[quirk3d ? iy : ix];
+    if (larrouturou) {
+      const bool upwind = flux.rho > 0.0;
+      flux.mom[iy] = flux.rho * (upwind ? vL_[iy] : vR_[iy]);
+      flux.mom[iz] = flux.rho * (upwind ? vL_[iz] : vR_[iz]);
+      const double f_rhoE = flux.rho * (upwind ? eKinRL : eKinRR);
+      flux.E = hlle.E - b_delta_alpha_2 * squaredNormRoeU_half + f_rhoE;
+    } else {
+      const double l31 = -roeU[iy];
+      const double l41 = -roeU[iz];
+      const double alpha_3 = l31 * deltaRho + deltaRhoU[iy];
+      const double alpha_4 = l41 * deltaRho + deltaRhoU[iz];
+      const double b_delta_alpha_3 = b_delta * alpha_3;
+      const double b_delta_alpha_4 = b_delta * alpha_4;
+      flux.mom[iy] = hlle.mom[iy] - b_delta_alpha_2 * roeU[iy] - b_delta_alpha_3;
+      flux.mom[iz] = hlle.mom[iz] - b_delta_alpha_2 * roeU[iz] - b_delta_alpha_4;
+      flux.E = hlle.E - b_delta_alpha_2 * squaredNormRoeU_half -
+               b_delta_alpha_3 * roeU[iy] - b_delta_alpha_4 * roeU[iz];
+    }
+  }
+  return (bR * pL - bL * pR) / db_positive;
+}
+
+// Hllem::ComputeStableDt, array form (src/equations/perfect_gas/HllemMethod.hpp:1152-1216).
+template <int D>
+inline double HllemStableDt(const Gas& g, const State<D>& L, const State<D>& R,
+                            double dx, int dir) {
+  const double gm1 = g.gamma - 1.0;
+  const double Msq = 1.0;
+  const double Minv = 1.0 / std::sqrt(Msq);
+  const double rhoL = L.rho;
+  const double rhoR = R.rho;
+  const double aL = L.a;
+  const double aR = R.a;
+  const double hL = (L.E + L.p) / rhoL;
+  const double hR = (R.E + R.p) / rhoR;
+  const double sqRhoL = std::sqrt(rhoL);
+  const double sqRhoR = std::sqrt(rhoR);
+  const double sqRhoSum = sqRhoL + sqRhoR;
+  const double sqRhoL_over_Sum = sqRhoL / sqRhoSum;
+  const double sqRhoR_over_Sum = sqRhoR / sqRhoSum;
+  double uL[D], uR[D], roeU[D];
+  for (int i = 0; i < D; ++i) {
+    uL[i] = L.mom[i] / rhoL;
+    uR[i] = R.mom[i] / rhoR;
+    roeU[i] = sqRhoL_over_Sum * uL[i] + sqRhoR_over_Sum * uR[i];
+  }
+  const double roeH = sqRhoL_over_Sum * hL + sqRhoR_over_Sum * hR;
+  double squaredNormRoeU = 0.0;
+  for (int i = 0; i < D; ++i) squaredNormRoeU += roeU[i] * roeU[i];
+  squaredNormRoeU *= Msq;
+  const double squaredNormRoeU_half = 0.5 * squaredNormRoeU;
+  const double roeA2 = gm1 * (roeH - squaredNormRoeU_half);
+  const double roeA = std::sqrt(roeA2);
+  const double maxA = std::max(aL, aR);
+  const double sL1 = uL[dir] - maxA * Minv;
+  const double sL2 = roeU[dir] - roeA * Minv;
+  const double sR1 = roeU[dir] + roeA * Minv;
+  const double sR2 = uR[dir] + maxA * Minv;
+  const double sL = std::min(sL1, sL2);
+  const double sR = std::max(sR1, sR2);
+  const double maxS = std::max(std::abs(sL), std::abs(sR));
+  return dx / maxS;
+}
+
+// ---------------------------------------------------------------------------
+// Exact Riemann solver (src/equations/perfect_gas/ExactRiemannSolver.hpp).
+
+struct NewtonResult {
+  double x;
+  int iterations;
+  bool converged;
+};
+
+// NewtonIteration (include/fub/NewtonIteration.hpp:28-52).
+template <typename F, typename DF>
+inline NewtonResult NewtonIteration(F&& f, DF&& Df, double x0,
+                                    double tolerance = 1e-7,
+                                    int max_iterations = 5000) {
+  double x = x0;
+  double fx = f(x);
+  int counter = 0;
+  double err = std::abs(fx);
+  while (err > tolerance && counter < max_iterations) {
+    const double dfx = Df(x);
+    const double offset = fx / dfx;
+    const double next = x - offset;
+    x = ((next - next) != (next - next) || next < 0.0) ? 0.5 * x : next;
+    fx = f(x);
+    err = std::abs(fx);
+    counter += 1;
+  }
+  const bool failed = (counter == max_iterations && fx > tolerance);
+  return {x, counter, !failed};
+}
+
+struct MiddleState {
+  double pM;
+  double uM;
+  int iterations;
+  bool converged;
+};
+
+// ExactRiemannSolver::ComputeMiddleState (ExactRiemannSolver.hpp:34-97).
+// x0 < 0 selects the reference's initial guess 0.5 * (pL + pR).
+template <int D>
+inline MiddleState ComputeMiddleState(const Gas& eq, const State<D>& left,
+                                      const State<D>& right, int dir,
+                                      double x0 = -1.0, double tol = 1e-7,
+                                      int max_it = 5000) {
+  const double g = eq.gamma;
+  const double gp1 = g + 1.0;
+  const double gm1 = g - 1.0;
+  const double rhoL = left.rho;
+  const double pL = left.p;
+  const double aL = left.a;
+  const double A_l = 2.0 / (gp1 * rhoL);
+  const double B_l = gm1 / gp1 * pL;
+  const double exponent = 0.5 * gm1 / g;
+  const double d_exponent = -0.5 * gp1 / g;
+  auto f_left = [=](double p) {
+    if (p > pL) {
+      return (p - pL) * std::sqrt(A_l / (p + B_l));
+    } else {
+      return 2.0 * aL / gm1 * (std::pow(p / pL, exponent) - 1.0);
+    }
+  };
+  auto df_left = [=](double p) {
+    if (p > pL) {
+      return std::sqrt(A_l / (B_l + p)) * (1.0 - 0.5 * (p - pL) / (B_l + p));
+    } else {
+      return std::pow(p / pL, d_exponent) / (rhoL * aL);
+    }
+  };
+  const double rhoR = right.rho;
+  const double pR = right.p;
+  const double aR = right.a;
+  const double A_r = 2.0 / (gp1 * rhoR);
+  const double B_r = gm1 / gp1 * pR;
+  auto f_right = [=](double p) {
+    if (p > pR) {
+      return (p - pR) * std::sqrt(A_r / (p + B_r));
+    } else {
+      return 2 * aR / gm1 * (std::pow(p / pR, exponent) - 1.0);
+    }
+  };
+  auto df_right = [=](double p) {
+    if (p > pR) {
+      return std::sqrt(A_r / (B_r + p)) * (1.0 - 0.5 * (p - pR) / (B_r + p));
+    } else {
+      return std::pow(p / pR, d_exponent) / (rhoR * aR);
+    }
+  };
+  const double uL = left.mom[dir] / left.rho;
+  const double uR = right.mom[dir] / right.rho;
+  const double du = uR - uL;
+  auto f = [=](double h) { return f_right(h) + f_left(h) + du; };
+  auto df = [=](double h) { return df_right(h) + df_left(h); };
+  const double guess = x0 < 0.0 ? 0.5 * (pL + pR) : x0;
+  const NewtonResult r = NewtonIteration(f, df, guess, tol, max_it);
+  const double pM = r.x;
+  const double fL = f_left(pM);
+  const double fR = f_right(pM);
+  const double uM = 0.5 * (uL + uR) + 0.5 * (fR - fL);
+  return {pM, uM, r.iterations, r.converged};
+}
+
+// ExactRiemannSolver::ComputeSignals, scalar (ExactRiemannSolver.hpp:105-148).
+template <int D>
+inline void ComputeSignals(const Gas& eq, const State<D>& left,
+                           const State<D>& right, int dir, double s[2],
+                           MiddleState* ms_out = nullptr, double x0 = -1.0,
+                           double tol = 1e-7, int max_it = 5000) {
+  const MiddleState ms = ComputeMiddleState<D>(eq, left, right, dir, x0, tol, max_it);
+  if (ms_out) *ms_out = ms;
+  const double pM = ms.pM;
+  const double g = eq.gamma;
+  const double gp1 = g + 1.0;
+  const double gm1 = g - 1.0;
+  const double uL = left.mom[dir] / left.rho;
+  const double aL = left.a;
+  const double pL = left.p;
+  const double pL_ratio = pM / pL;
+  if (pM > pL) {
+    s[0] = uL - aL * std::sqrt((gp1 * pL_ratio + gm1) / (2.0 * g));
+  } else {
+    s[0] = uL - aL;
+  }
+  const double uR = right.mom[dir] / right.rho;
+  const double aR = right.a;
+  const double pR = right.p;
+  const double pR_ratio = pM / pR;
+  if (pM > pR) {
+    s[1] = uR + aR * std::sqrt((gp1 * pR_ratio + gm1) / (2.0 * g));
+  } else {
+    s[1] = uR + aR;
+  }
+}
+
+// ExactRiemannSolver::SolveRiemannProblem, scalar (ExactRiemannSolver.hpp:204-353).
+// The state at x/t = 0 of the exact solution.
+template <int D>
+inline MiddleState SolveRiemannProblem(const Gas& eq, State<D>& state,
+                                       const State<D>& left,
+                                       const State<D>& right, int dir,
+                                       double x0 = -1.0, double tol = 1e-7,
+                                       int max_it = 5000) {
+  auto velocity = [dir](const State<D>& s) { return s.mom[dir] / s.rho; };
+  auto set_momentum = [dir](State<D>& s, double momentum, const State<D>& side) {
+    for (int d = 0; d < D; ++d) s.mom[d] = s.rho * (side.mom[d] / side.rho);
+    s.mom[dir] = momentum;
+  };
+  auto from_prim = [&eq](State<D>& s) {
+    const double rhoE_kin = 0.5 * SquaredNormScalar<D>(s.mom) / s.rho;
+    const double rhoE_internal = s.p * eq.gm1inv;
+    s.E = rhoE_internal + rhoE_kin;
+    s.a = std::sqrt(eq.gamma * s.p / s.rho);
+  };
+  MiddleState ms{0.0, 0.0, 0, true};
+  if (0.0 < left.rho && 0.0 < right.rho) {
+    ms = ComputeMiddleState<D>(eq, left, right, dir, x0, tol, max_it);
+    const double pM = ms.pM;
+    const double uM = ms.uM;
+    const double g = eq.gamma;
+    const double gp1 = g + 1.0;
+    const double gm1 = g - 1.0;
+    if (0.0 <= uM) {
+      const double uL = velocity(left);
+      const double aL = left.a;
+      const double pL = left.p;
+      const double p_ratio = pM / pL;
+      const double g_ratio = gm1 / gp1;
+      if (pM > pL) {
+        const double shock_speed =
+            uL - aL * std::sqrt((gp1 * p_ratio + gm1) / (2.0 * g));
+        if (shock_speed < 0.0) {
+          state.rho = left.rho * (p_ratio + g_ratio) / (g_ratio * p_ratio + 1.0);
+          set_momentum(state, state.rho * uM, left);
+          state.p = pM;
+          from_prim(state);
+        } else {
+          state = left;
+        }
+      } else {
+        const double aM = aL * std::pow(p_ratio, gm1 / (2.0 * g));
+        const double shock_speed_TL = uM - aM;
+        const double shock_speed_HL = uL - aL;
+        if (0.0 < shock_speed_HL) {
+          state = left;
+        } else if (shock_speed_TL < 0.0) {
+          state.rho = left.rho * std::pow(p_ratio, 1 / g);
+          set_momentum(state, state.rho * uM, left);
+          state.p = pM;
+          from_prim(state);
+        } else {
+          state.rho = left.rho * std::pow(2.0 / gp1 + g_ratio * uL / aL, 2.0 / gm1);
+          const double u = 2.0 / gp1 * (aL + 0.5 * gm1 * uL);
+          set_momentum(state, state.rho * u, left);
+          state.p = pL * std::pow(2.0 / gp1 + g_ratio * uL / aL, 2.0 * g / gm1);
+          from_prim(state);
+        }
+      }
+    } else {
+      const double uR = velocity(right);
+      const double aR = right.a;
+      const double pR = right.p;
+      const double p_ratio = pM / pR;
+      const double g_ratio = gm1 / gp1;
+      if (pM > pR) {
+        const double shock_speed =
+            uR + aR * std::sqrt((gp1 * p_ratio + gm1) / (2.0 * g));
+        if (0.0 < shock_speed) {
+          state.rho = right.rho * (p_ratio + g_ratio) / (g_ratio * p_ratio + 1.0);
+          set_momentum(state, state.rho * uM, right);
+          state.p = pM;
+          from_prim(state);
+        } else {
+          state = right;
+        }
+      } else {
+        const double aM = aR * std::pow(p_ratio, gm1 / (2.0 * g));
+        const double shock_speed_TR = uM + aM;
+        const double shock_speed_HR = uR + aR;
+        if (shock_speed_HR <= 0.0) {
+          state = right;
+        } else if (shock_speed_TR >= 0.0) {
+          state.rho = right.rho * std::pow(p_ratio, 1 / g);
+          set_momentum(state, state.rho * uM, right);
+          state.p = pM;
+          from_prim(state);
+        } else {
+          state.rho = right.rho * std::pow(2.0 / gp1 - g_ratio * uR / aR, 2.0 / gm1);
+          const double u = 2.0 / gp1 * (-aR + 0.5 * gm1 * uR);
+          set_momentum(state, state.rho * u, right);
+          state.p = pR * std::pow(2.0 / gp1 - g_ratio * uR / aR, 2.0 * g / gm1);
+          from_prim(state);
+        }
+      }
+    }
+  }
+  return ms;
+}
+
+// Godunov<Eq, ExactRiemannSolver>::ComputeStableDt, scalar
+// (include/fub/flux_method/GodunovMethod.hpp:163-170).
+template <int D>
+inline double GodunovStableDt(const Gas& eq, const State<D>& L,
+                              const State<D>& R, double dx, int dir,
+                              double tol = 1e-7, int max_it = 5000) {
+  double s[2];
+  ComputeSignals<D>(eq, L, R, dir, s, nullptr, -1.0, tol, max_it);
+  double s_max = 0.0;
+  s_max = std::max(s_max, std::abs(s[0]));
+  s_max = std::max(s_max, std::abs(s[1]));
+  return dx / s_max;
+}
+
+} // namespace fvs_oracle

@@ -1,0 +1,174 @@
+"""ctypes access to the CPU oracle (oracle/libfvs_oracle.so) for tests.
+
+Test infrastructure only: the oracle is the checker, never the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from finitevolumesolver_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_PATH = ROOT / "oracle" / "libfvs_oracle.so"
+
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int)
+_oracle = None
+
+
+def load():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_PATH.exists():
+            raise FileNotFoundError(f"{ORACLE_PATH} missing; run make -C oracle")
+        lib = C.CDLL(str(ORACLE_PATH))
+        I, D, I64, P = C.c_int, C.c_double, C.c_int64, C.c_void_p
+        sig = {
+            "oracle_set_threads": (I, [I]),
+            "oracle_complete_from_cons": (I, [I, D, I64, P, P]),
+            "oracle_complete_from_prim": (I, [I, D, I64, P, P, P, P]),
+            "oracle_physical_flux": (I, [I, I, I64, P, P]),
+            "oracle_limited_slope": (I, [I, I64, P, P, P, P]),
+            "oracle_hllem_flux": (I, [I, D, I, I, I, I64, P, P, P, P]),
+            "oracle_hllem_stable_dt": (I, [I, D, I, D, I64, P, P, P]),
+            "oracle_face_flux": (I, [C.POINTER(L.MethodDesc), I, D, D, I64, P, P]),
+            "oracle_exact_middle_state": (I, [I, D, I, I64, P, P, P, D, I, P, P, P, P]),
+            "oracle_exact_riemann_state": (I, [I, D, I, I64, P, P, P]),
+            "oracle_exact_signals": (I, [I, D, I, I64, P, P, P, P]),
+            "oracle_stable_dt": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.View), D, I, _PD]),
+            "oracle_numeric_fluxes": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.View),
+                                          C.POINTER(L.View), D, D, I]),
+            "oracle_update_conservatively": (I, [C.POINTER(L.View), C.POINTER(L.View),
+                                                 C.POINTER(L.View), D, D, I]),
+            "oracle_complete_view": (I, [I, D, C.POINTER(L.View), C.POINTER(L.View)]),
+            "oracle_transmissive_fill": (I, [C.POINTER(L.View), C.POINTER(C.c_int64),
+                                             C.POINTER(C.c_int64), I]),
+            "oracle_level_run": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.PatchDesc), P, D, I,
+                                     D, _PD, _PI, _PI]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _oracle = lib
+    return _oracle
+
+
+def ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def host_view(arr: np.ndarray, lo=(0, 0, 0)) -> L.View:
+    """fvs_view over a numpy array of shape (ncomp, nz, ny, nx)."""
+    assert arr.dtype == np.float64 and arr.ndim == 4
+    st = [s // 8 for s in arr.strides]
+    v = L.View()
+    v.data = arr.ctypes.data
+    v.ncomp = arr.shape[0]
+    lo = tuple(lo) + (0,) * (3 - len(lo))
+    v.lo = L.i64x3(lo)
+    v.hi = L.i64x3((lo[0] + arr.shape[3] - 1, lo[1] + arr.shape[2] - 1, lo[2] + arr.shape[1] - 1))
+    v.stride = L.i64x3((st[3], st[2], st[1]))
+    v.comp_stride = st[0]
+    return v
+
+
+def complete_from_cons(cons: np.ndarray, dim: int, gamma: float = 1.4) -> np.ndarray:
+    """cons: (D+2, n) -> complete (D+4, n), array-path arithmetic."""
+    cons = np.ascontiguousarray(cons, dtype=np.float64)
+    n = cons.shape[1]
+    out = np.empty((dim + 4, n))
+    load().oracle_complete_from_cons(dim, gamma, n, ptr(cons), ptr(out))
+    return out
+
+
+def hllem_flux(left, right, dim, dir, larrouturou=True, quirk=True, gamma=1.4):
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    n = left.shape[1]
+    f = np.empty((dim + 2, n))
+    ps = np.empty(n)
+    load().oracle_hllem_flux(dim, gamma, dir, int(larrouturou), int(quirk), n, ptr(left),
+                             ptr(right), ptr(f), ptr(ps))
+    return f, ps
+
+
+def hllem_stable_dt(left, right, dim, dir, dx, gamma=1.4):
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    n = left.shape[1]
+    dt = np.empty(n)
+    load().oracle_hllem_stable_dt(dim, gamma, dir, dx, n, ptr(left), ptr(right), ptr(dt))
+    return dt
+
+
+def physical_flux(complete, dim, dir):
+    complete = np.ascontiguousarray(complete)
+    n = complete.shape[1]
+    f = np.empty((dim + 2, n))
+    load().oracle_physical_flux(dim, dir, n, ptr(complete), ptr(f))
+    return f
+
+
+def face_flux(desc: L.MethodDesc, stencil: np.ndarray, dir, dt, dx):
+    """stencil: (S, D+4, n)."""
+    stencil = np.ascontiguousarray(stencil)
+    n = stencil.shape[2]
+    f = np.empty((desc.dim + 2, n))
+    rc = load().oracle_face_flux(C.byref(desc), dir, dt, dx, n, ptr(stencil), ptr(f))
+    return f, rc
+
+
+def exact_middle_state(left, right, dim, dir, x0=None, tol=1e-7, max_it=5000, gamma=1.4):
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    n = left.shape[1]
+    pM, uM = np.empty(n), np.empty(n)
+    it = np.empty(n, dtype=np.int32)
+    conv = np.empty(n, dtype=np.int32)
+    x0p = ptr(np.ascontiguousarray(x0, dtype=np.float64)) if x0 is not None else None
+    load().oracle_exact_middle_state(dim, gamma, dir, n, ptr(left), ptr(right), x0p, tol, max_it,
+                                     ptr(pM), ptr(uM), ptr(it), ptr(conv))
+    return pM, uM, it, conv
+
+
+def exact_riemann_state(left, right, dim, dir, gamma=1.4):
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    n = left.shape[1]
+    s = np.empty((dim + 4, n))
+    load().oracle_exact_riemann_state(dim, gamma, dir, n, ptr(left), ptr(right), ptr(s))
+    return s
+
+
+def exact_signals(left, right, dim, dir, gamma=1.4):
+    left = np.ascontiguousarray(left)
+    right = np.ascontiguousarray(right)
+    n = left.shape[1]
+    sL, sR = np.empty(n), np.empty(n)
+    load().oracle_exact_signals(dim, gamma, dir, n, ptr(left), ptr(right), ptr(sL), ptr(sR))
+    return sL, sR
+
+
+def level_run(desc: L.MethodDesc, cons: np.ndarray, dx, n_steps, splitting=L.SPLIT_GODUNOV,
+              cfl=0.8, final_time=1e300, t0=0.0):
+    """Runs the oracle's single-patch level driver; returns (cons, dts, failures)."""
+    D = desc.dim
+    cons = np.ascontiguousarray(cons.copy())
+    pd = L.PatchDesc()
+    shape = cons.shape  # (D+2, nz, ny, nx)
+    pd.n = L.i64x3((shape[3], shape[2], shape[1] if D == 3 else 1))
+    pd.dx = (C.c_double * 3)(*(list(dx) + [1.0] * (3 - len(dx))))
+    pd.splitting = splitting
+    pd.cfl = cfl
+    dts = np.zeros(max(n_steps, 1))
+    done = C.c_int()
+    fails = C.c_int()
+    rc = load().oracle_level_run(C.byref(desc), C.byref(pd), ptr(cons), t0, n_steps, final_time,
+                                 dts.ctypes.data_as(_PD), C.byref(done), C.byref(fails))
+    assert rc == 0, rc
+    return cons, dts[:done.value], fails.value
