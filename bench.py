@@ -94,6 +94,40 @@ def l1_vs_oracle(precision: str, torch) -> dict:
             "bitexact": bool(np.array_equal(got, ref)), "sample": "48^3, 4 steps"}
 
 
+def kernels_only(args, torch, fvs) -> int:
+    """Each fused kernel (X, Y, Z sweep without and with the CFL check, the
+    CFL pass) --kernel-reps times on the 256^3 state, then one device copy of
+    a 1 GiB buffer: the copy's FETCH_SIZE/WRITE_SIZE calibrate the counters
+    (it moves exactly 1 GiB each way)."""
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d
+    n = args.size
+    cons, dx = fourier_modes_3d(n)
+    state = torch.from_numpy(cons).cuda()
+    out = torch.empty_like(state)
+    sdt = torch.empty(1, dtype=torch.float64, device="cuda")
+    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=args.precision)
+    dt = 0.5 * 0.8 / n
+    for d, name in enumerate("XYZ"):
+        for check in (False, True):
+            log(f"[kernels] sweep {name} check={check}")
+            for _ in range(args.kernel_reps):
+                fvs.split_sweep(fm, state, out, dt, dx[d], d, sdt if check else None)
+            torch.cuda.synchronize()
+    log("[kernels] CFL")
+    for _ in range(args.kernel_reps):
+        fvs.patch_stable_dt(fm, state, dx)
+    torch.cuda.synchronize()
+    log("[kernels] copy 1 GiB")
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("[kernels] done")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +137,9 @@ def main() -> int:
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="launch each fused kernel --kernel-reps times (and a plain "
+                         "copy of known size) for hardware-counter runs; no timing line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -115,6 +152,8 @@ def main() -> int:
     from finitevolumesolver_amd.initial_data import fourier_modes_3d
 
     n = args.size
+    if args.kernels_only:
+        return kernels_only(args, torch, fvs)
     log(f"[bench] generating {n}^3 initial state")
     cons, dx = fourier_modes_3d(n)
     dev = torch.device("cuda:0")
@@ -191,7 +230,7 @@ def main() -> int:
         result["accuracy"] = {"error": str(e)}
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
-        result["cpu_baseline"] = cpu_baseline(128, 12, args.precision)
+        result["cpu_baseline"] = cpu_baseline(192, 16, args.precision)
     print(json.dumps(result), flush=True)
     return 0
 

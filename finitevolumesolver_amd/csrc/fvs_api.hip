@@ -4,6 +4,7 @@
 #include "fvs_internal.hpp"
 #include "fvs_physics.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -312,6 +313,8 @@ int fvs_create(const fvs_method_desc* desc, void* hip_stream, fvs_handle** out) 
     SetError("hipMalloc of the handle scratch failed");
     return FVS_HIP_ERROR;
   }
+  h->partials = nullptr;
+  h->partials_cap = 0;
   h->scratch_dt = static_cast<double*>(mem);
   h->scratch_keys = reinterpret_cast<unsigned long long*>(static_cast<char*>(mem) + 64);
   h->scratch_counter = reinterpret_cast<unsigned int*>(static_cast<char*>(mem) + 192);
@@ -328,6 +331,7 @@ int fvs_destroy(fvs_handle* h) {
   if (!h) return FVS_OK;
   (void)hipStreamSynchronize(h->stream);
   (void)hipFree(h->scratch_dt);
+  if (h->partials) (void)hipFree(h->partials);
   delete h;
   return FVS_OK;
 }
@@ -493,6 +497,19 @@ int fvs_transmissive_fill(fvs_handle* h, const fvs_view* states, const int64_t d
 
 // ---- fused single-patch operators ------------------------------------------
 
+static int EnsurePartials(fvs_handle* h, int64_t n) {
+  if (n <= h->partials_cap) return FVS_OK;
+  if (h->partials) {
+    FVS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    FVS_HIP_CHECK(hipFree(h->partials));
+    h->partials = nullptr;
+    h->partials_cap = 0;
+  }
+  FVS_HIP_CHECK(hipMalloc(&h->partials, n * sizeof(unsigned long long)));
+  h->partials_cap = n;
+  return FVS_OK;
+}
+
 static int FillSweepArgs(fvs_handle* h, const double* const* in, double* const* out,
                          const int64_t n[3], double dx, SweepArgs* a) {
   const int D = h->desc.dim;
@@ -540,9 +557,13 @@ int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* 
   }
   a.dt_val = dt;
   a.check = stable_dt_device != nullptr;
-  a.check_key = h->scratch_keys;
+  const int64_t nb = SweepBlocks(h->desc.dim, dir, a.n, a.seg);
+  if (a.check) {
+    if (int rc = EnsurePartials(h, nb)) return rc;
+    a.partials = h->partials;
+  }
   if (int rc = LaunchSweep(h->desc, h->stream, dir, a)) return rc;
-  if (stable_dt_device) return LaunchKeysToDt(h->stream, h->scratch_keys, 1, stable_dt_device);
+  if (stable_dt_device) return LaunchFinalizeCheck(h->stream, h->partials, nb, stable_dt_device, nullptr);
   return FVS_OK;
 }
 
@@ -560,10 +581,15 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t 
     }
   }
   int64_t nn[3] = {n[0], n[1], D == 3 ? n[2] : 1};
-  if (int rc = LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, h->scratch_keys,
-                              h->scratch_counter, nullptr, nullptr, 0, 0))
-    return rc;
-  return LaunchKeysToDt(h->stream, h->scratch_keys, D, dt_dir_device);
+  for (int d = 0; d < D; ++d) {
+    if (nn[d] < 1) {
+      SetError("patch extent %d must be positive", d);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  if (int rc = EnsurePartials(h, D * CflBlocks(D, nn))) return rc;
+  return LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, h->partials, dt_dir_device, nullptr,
+                        nullptr, 0, 0);
 }
 
 } // extern "C"
@@ -578,6 +604,7 @@ struct fvs_level {
   double* block;          // 3 state buffers
   double* buf[3][5];
   Ctl* ctl;               // device control block
+  unsigned long long* partials;  // per-workgroup minima
   int cur;                // buffer holding the current state
 };
 
@@ -626,14 +653,21 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
     SetError("hipMalloc of the control block failed");
     return FVS_HIP_ERROR;
   }
+  int64_t np = D * CflBlocks(D, n);
+  for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, SweepBlocks(D, d, n, 32));
+  if (hipMalloc(&L->partials, np * sizeof(unsigned long long)) != hipSuccess) {
+    (void)hipFree(L->block);
+    (void)hipFree(L->ctl);
+    delete L;
+    SetError("hipMalloc of the reduction partials failed");
+    return FVS_HIP_ERROR;
+  }
   for (int b = 0; b < 3; ++b)
     for (int c = 0; c < D + 2; ++c) L->buf[b][c] = L->block + (static_cast<int64_t>(b) * (D + 2) + c) * L->ncell;
   L->cur = 0;
   (void)hipMemsetAsync(L->block, 0, bytes, h->stream);
   Ctl init{};
   init.cfl = patch->cfl;
-  init.cfl_key[0] = init.cfl_key[1] = init.cfl_key[2] = 0x7ff0000000000000ull;
-  init.check_key = 0x7ff0000000000000ull;
   (void)hipMemcpyAsync(L->ctl, &init, sizeof init, hipMemcpyHostToDevice, h->stream);
   if (hipStreamSynchronize(h->stream) != hipSuccess) {
     SetError("level initialisation failed");
@@ -648,6 +682,7 @@ int fvs_level_destroy(fvs_level* L) {
   (void)hipStreamSynchronize(L->h->stream);
   (void)hipFree(L->block);
   (void)hipFree(L->ctl);
+  (void)hipFree(L->partials);
   delete L;
   return FVS_OK;
 }
@@ -674,8 +709,6 @@ int fvs_level_set_state(fvs_level* L, const double* const* comps, double t0) {
   Ctl init{};
   init.t = t0;
   init.cfl = L->patch.cfl;
-  init.cfl_key[0] = init.cfl_key[1] = init.cfl_key[2] = 0x7ff0000000000000ull;
-  init.check_key = 0x7ff0000000000000ull;
   FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &init, sizeof init, hipMemcpyHostToDevice, L->h->stream));
   FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
   return FVS_OK;
@@ -753,8 +786,8 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
         for (int i = D - 1; i >= 0; --i) seq[ns++] = dirs[i];
       const double* cin[5];
       for (int q = 0; q < D + 2; ++q) cin[q] = L->buf[cur][q];
-      rc = LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->ctl->cfl_key,
-                          &L->ctl->cfl_counter, L->ctl, &L->ctl->halt, seq[0], strang);
+      rc = LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
+                          &L->ctl->halt, seq[0], strang);
       if (rc) break;
       const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
       int src = b0, dst = b1;
@@ -769,10 +802,14 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
         a.dt_ptr = &L->ctl->split_dt;
         a.halt = &L->ctl->halt;
         a.check = (w > 0);
-        a.check_key = &L->ctl->check_key;
-        a.ctl = L->ctl;
+        a.partials = L->partials;
         rc = LaunchSweep(h->desc, s, seq[w], a);
         if (rc) break;
+        if (a.check) {
+          rc = LaunchFinalizeCheck(s, L->partials, SweepBlocks(D, seq[w], L->n, a.seg), nullptr,
+                                   L->ctl);
+          if (rc) break;
+        }
         src = dst;
         dst = (dst == b1) ? b2 : b1;
       }

@@ -17,6 +17,8 @@ struct fvs_handle {
   double* scratch_dt; // 4 device doubles for reductions of entry points
   unsigned long long* scratch_keys; // 8 device keys
   unsigned int* scratch_counter;    // 4 device counters
+  unsigned long long* partials;     // per-workgroup minima of reductions
+  int64_t partials_cap;             // capacity of partials (keys)
 };
 
 namespace fvs {
@@ -58,12 +60,8 @@ struct SweepArgs {
   int quirk3d;
   int check;              // compute the CFL check of the input in this sweep
   int seg;                // pencil segment length (Y/Z sweeps)
-  unsigned long long* check_key; // atomic-min accumulator of the check
-  unsigned int* done_counter;    // last-block detection (nullable)
+  unsigned long long* partials;  // per-workgroup minimum of the check
   int* halt;              // nonzero: skip all work (nullable)
-  // Finalisation of the check by the last block (level driver only).
-  struct Ctl* ctl;
-  int sweep_slot;
 };
 
 // Device control block of the single-patch level driver.
@@ -77,21 +75,25 @@ struct Ctl {
   int have_failure;   // use failure_dt for the next step
   int halt;           // 0 running, 1 TimeStepTooLarge, 2 final time reached
   int64_t cycle;
-  unsigned long long cfl_key[3];
-  unsigned long long check_key;
-  unsigned int cfl_counter;
-  unsigned int check_counter;
   double dt_dir[3];   // step-start stable dt per direction (decoded)
   double last_check;
 };
 
+// Number of workgroups (= partial minima) of a sweep / of the CFL pass.
+int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg);
+int64_t CflBlocks(int dim, const int64_t n[3]);
 int LaunchSweep(const fvs_method_desc& desc, hipStream_t stream, int dir,
                 const SweepArgs& a);
+// Reduces a sweep's partial minima: writes the value to out (nullable) and,
+// with a control block, applies the split step's CFL check.
+int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials,
+                        int64_t nblocks, double* out, Ctl* ctl);
+// Step-start CFL pass: per-direction partial minima, then a finalize kernel
+// writing dt_dir (out, nullable) or, with a control block, the step's dt.
 int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t stream,
                    const double* const* cons, const int64_t n[3],
-                   const double dx[3], unsigned long long* keys,
-                   unsigned int* counter, Ctl* ctl, int* halt, int first_dir,
-                   int strang);
+                   const double dx[3], unsigned long long* partials,
+                   double* out, Ctl* ctl, int* halt, int first_dir, int strang);
 int LaunchStepEnd(hipStream_t s, Ctl* ctl, double* dt_log, int64_t dt_log_base,
                   int dt_log_len);
 int LaunchKeysToDt(hipStream_t s, unsigned long long* keys, int n, double* out);

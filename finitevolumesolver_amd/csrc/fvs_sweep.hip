@@ -11,11 +11,16 @@
 // Boundaries are transmissive (TransmissiveBoundary.cpp:31-93): a ghost cell
 // holds its row's boundary value, i.e. indices are clamped to the patch.
 //
-// X sweeps (unit stride): one workgroup per row tile of 256 cells; the
-// conservative states, half-step states and fluxes are exchanged through
-// LDS.  Y/Z sweeps: one thread per (x, transverse) pencil, marching along
-// the sweep direction with a register window of three cells, so every load
-// is coalesced along x and no LDS or barrier is needed.
+// X sweeps (unit stride): one cell per lane over the flattened sequence of
+// row slots (cells plus the two ghost cells of each row), neighbour states
+// moving between lanes with wave shuffles; waves are independent.  Y/Z: one
+// thread per (x, transverse) pencil, marching along the sweep direction with
+// a register window of three cells and a one-cell load prefetch, so every
+// load is coalesced along x and no LDS or barrier is needed.
+//
+// Reductions (CFL minima) write one partial per workgroup and a one-block
+// finalize kernel reduces them: no contended device-scope atomics, and the
+// result does not depend on workgroup scheduling.
 #include "fvs_internal.hpp"
 #include "fvs_physics.hpp"
 
@@ -23,54 +28,107 @@ namespace fvs {
 
 namespace {
 
-constexpr int kTX = 256;      // X-sweep tile (cells per workgroup)
-constexpr int kBlock = 256;   // threads per workgroup
+constexpr int kTX = 256;                // CFL-pass x tile (cells per workgroup)
+constexpr int kBlock = 256;             // workgroup size of every sweep kernel
+constexpr unsigned long long kInfKey = 0x7ff0000000000000ull;
 
 __device__ __forceinline__ int64_t Clamp(int64_t i, int64_t n) {
   return i < 0 ? 0 : (i >= n ? n - 1 : i);
 }
 
+__device__ __forceinline__ unsigned long long KMin(unsigned long long a,
+                                                   unsigned long long b) {
+  return b < a ? b : a;
+}
+
 __device__ __forceinline__ unsigned long long WaveMinKey(unsigned long long k) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(k, off, 64);
-    k = o < k ? o : k;
-  }
+  for (int off = 32; off > 0; off >>= 1) k = KMin(k, __shfl_xor(k, off, 64));
   return k;
 }
 
-// Block-wide min of keys; thread 0 performs one atomic min.  Then the last
-// block to finish (if a control block is given) finalises the CFL check.
-__device__ void BlockCheckMin(unsigned long long key, const SweepArgs& a) {
-  __shared__ unsigned long long s_keys[kBlock / 64];
-  key = WaveMinKey(key);
+// Block-wide minimum of NK keys per thread; thread 0 writes them to
+// out[d * stride] (d < NK).  Every thread of the block must call it.
+template <int NK, int NWAVES>
+__device__ __forceinline__ void BlockMinToPartials(const unsigned long long* key,
+                                                   unsigned long long* out,
+                                                   int64_t stride) {
+  __shared__ unsigned long long s_keys[NK][NWAVES];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  if (lane == 0) s_keys[wave] = key;
+#pragma unroll
+  for (int d = 0; d < NK; ++d) {
+    const unsigned long long k = WaveMinKey(key[d]);
+    if (lane == 0) s_keys[d][wave] = k;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long k = s_keys[0];
-    for (int w = 1; w < kBlock / 64; ++w) k = s_keys[w] < k ? s_keys[w] : k;
-    atomicMin(a.check_key, k);
-    if (a.ctl) {
-      __threadfence();
-      const unsigned int total = gridDim.x * gridDim.y * gridDim.z;
-      const unsigned int prev = atomicAdd(&a.ctl->check_counter, 1u);
-      if (prev == total - 1) {
-        __threadfence();
-        const unsigned long long fk = atomicAdd(a.check_key, 0ull);
-        const double v = __longlong_as_double(static_cast<long long>(fk));
-        a.ctl->last_check = v;
-        if (v < a.ctl->split_dt) {
-          // TimeStepTooLarge{level_dt} (DimensionalSplitLevelIntegrator.hpp:243-245)
-          a.ctl->halt = 1;
-          a.ctl->failure_dt = v;
-        }
-        atomicExch(a.check_key, 0x7ff0000000000000ull);
-        atomicExch(&a.ctl->check_counter, 0u);
-      }
+#pragma unroll
+    for (int d = 0; d < NK; ++d) {
+      unsigned long long k = s_keys[d][0];
+      for (int w = 1; w < NWAVES; ++w) k = KMin(k, s_keys[d][w]);
+      out[d * stride] = k;
     }
   }
+}
+
+// Per-cell ingredients of Hllem::ComputeStableDt
+// (src/equations/perfect_gas/HllemMethod.hpp:1152-1216): sqrt(rho), the
+// velocity, the enthalpy (E + p) / rho and the speed of sound.  A face's dt
+// combines two of them with exactly the reference's operations.
+template <int D> struct DtCell {
+  double sq;
+  double u[D];
+  double h;
+  double a;
+};
+
+template <int D>
+__device__ __forceinline__ DtCell<D> MakeDtCell(const double* u, const Gas<double>& g) {
+  double q[D + 4];
+#pragma unroll
+  for (int c = 0; c < D + 2; ++c) q[c] = u[c];
+  CompleteFromCons<D>(q, g);
+  DtCell<D> r;
+  r.sq = rsqrt_ieee(q[0]);
+#pragma unroll
+  for (int i = 0; i < D; ++i) r.u[i] = q[1 + i] / q[0];
+  r.h = (q[D + 1] + q[D + 2]) / q[0];
+  r.a = q[D + 3];
+  return r;
+}
+
+template <int D, int DIR>
+__device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R, double dx,
+                                         double gm1) {
+  const double sqRhoSum = L.sq + R.sq;
+  const double wL = L.sq / sqRhoSum;
+  const double wR = R.sq / sqRhoSum;
+  double roeU[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) roeU[i] = wL * L.u[i] + wR * R.u[i];
+  const double roeH = wL * L.h + wR * R.h;
+  double sq = roeU[0] * roeU[0];
+#pragma unroll
+  for (int i = 1; i < D; ++i) sq += roeU[i] * roeU[i];
+  const double roeA2 = gm1 * (roeH - 0.5 * sq);
+  const double roeA = rsqrt_ieee(roeA2);
+  const double maxA = rmax(L.a, R.a);
+  const double sL = rmin(L.u[DIR] - maxA, roeU[DIR] - roeA);
+  const double sR = rmax(roeU[DIR] + roeA, R.u[DIR] + maxA);
+  const double maxS = rmax(rabs(sL), rabs(sR));
+  return dx / maxS;
+}
+
+template <int D>
+__device__ __forceinline__ DtCell<D> ShflUp(const DtCell<D>& v) {
+  DtCell<D> r;
+  r.sq = __shfl_up(v.sq, 1, 64);
+#pragma unroll
+  for (int i = 0; i < D; ++i) r.u[i] = __shfl_up(v.u[i], 1, 64);
+  r.h = __shfl_up(v.h, 1, 64);
+  r.a = __shfl_up(v.a, 1, 64);
+  return r;
 }
 
 template <int D, int DIR, class T>
@@ -122,108 +180,114 @@ template <class T> __device__ __forceinline__ Gas<T> MakeGas(const SweepArgs& a)
 }
 
 // ---------------------------------------------------------------------------
-// X sweep: workgroup per row tile.
+// X sweep.  Every row of nx cells contributes nx + 2 "slots": the ghost cell
+// -1, the cells 0 .. nx-1 and the ghost cell nx.  The slots of all rows form
+// one sequence; each wave takes 64 consecutive slots, owns the 62 in the
+// middle and overlaps one slot with each neighbouring wave.  A lane
+// reconstructs its cell, computes the flux of the face on the cell's left
+// (from its left neighbour's right half-step state, one shuffle away) and
+// updates its cell with its right neighbour's flux.  Waves are independent:
+// no LDS and no barrier, and ~96% of the lanes do useful work for 256-cell
+// rows (62 of 64 lanes, 256 of 258 slots).
+constexpr int kXOwned = 62;
+
 template <int D, bool LARR, class T>
 __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   constexpr int NC = D + 2;
   constexpr int NQ = D + 4;
   if (a.halt && *a.halt) return;
   const int64_t nx = a.n[0];
-  const int64_t tiles = (nx + kTX - 1) / kTX;
-  const int64_t row = blockIdx.x / tiles;
-  const int64_t x0 = (blockIdx.x % tiles) * kTX;
-  const int cnt = static_cast<int>(nx - x0 < kTX ? nx - x0 : kTX);
-  const int t = threadIdx.x;
-  const int64_t rbase = row * nx;
-
-  __shared__ double su[NC][kTX + 4];   // cells x0-2 .. x0+cnt+1 (clamped)
-  __shared__ T srecL[NQ][kTX + 2];     // recL of cells x0-1 .. x0+cnt
-  __shared__ T srecR[NQ][kTX + 2];     // recR, later overwritten by fluxes
-  __shared__ double spa[2][kTX + 4];   // p, a of the input cells (check)
+  const uint32_t S = static_cast<uint32_t>(nx + 2);
+  const uint32_t Q = static_cast<uint32_t>(a.n[1] * a.n[2]) * S;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  const int64_t q64 = wave * kXOwned + lane - 1;
+  const bool valid = q64 >= 0 && q64 < static_cast<int64_t>(Q);
+  const uint32_t q = valid ? static_cast<uint32_t>(q64) : 0u;
+  const uint32_t row = q / S;
+  const uint32_t sl = q - row * S;           // slot within the row
+  const int64_t cell = static_cast<int64_t>(sl) - 1;
+  const bool real = valid && cell >= 0 && cell < nx;
+  const int64_t rbase = static_cast<int64_t>(row) * nx;
 
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
   const T lh = static_cast<T>(0.5 * dt / a.dx);
   const double dt_dx = dt / a.dx;
   const Gas<T> g = MakeGas<T>(a);
+  const Gas<double> gd = MakeGas<double>(a);
 
-  for (int i = t; i < cnt + 4; i += kBlock) {
-    const int64_t x = Clamp(x0 - 2 + i, nx);
+  double u[NC], um[NC], up[NC];
+  {
+    const int64_t off = rbase + Clamp(cell, nx);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) su[c][i] = a.in[c][rbase + x];
+    for (int k = 0; k < NC; ++k) u[k] = a.in[k][off];
   }
-  __syncthreads();
-
-  for (int i = t; i < cnt + 2; i += kBlock) {
-    const int64_t cell = x0 - 1 + i;
-    T um[NC], u0[NC], up[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      um[c] = static_cast<T>(su[c][i]);
-      u0[c] = static_cast<T>(su[c][i + 1]);
-      up[c] = static_cast<T>(su[c][i + 2]);
+  for (int k = 0; k < NC; ++k) {
+    um[k] = __shfl_up(u[k], 1, 64);
+    up[k] = __shfl_down(u[k], 1, 64);
+  }
+  // The wave's edge lanes read their outer neighbour themselves.
+  if (lane == 0 && real) {
+    const int64_t off = rbase + Clamp(cell - 1, nx);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) um[k] = a.in[k][off];
+  }
+  if (lane == 63 && real) {
+    const int64_t off = rbase + Clamp(cell + 1, nx);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) up[k] = a.in[k][off];
+  }
+
+  T recL[NQ], recR[NQ];
+  {
+    T tm[NC], t0[NC], tp[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      tm[k] = static_cast<T>(um[k]);
+      t0[k] = static_cast<T>(u[k]);
+      tp[k] = static_cast<T>(up[k]);
     }
-    T recL[NQ], recR[NQ];
-    if (cell < 0 || cell >= nx) {
-      GhostRec<D>(u0, g, recL);
+    if (real) {
+      ReconAny<D, 0>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR);
+    } else {
+      GhostRec<D>(t0, g, recL);
 #pragma unroll
       for (int k = 0; k < NQ; ++k) recR[k] = recL[k];
-    } else {
-      ReconAny<D, 0>(um, u0, up, lh, g, a.limiter, a.muscl, recL, recR);
-    }
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      srecL[k][i] = recL[k];
-      srecR[k][i] = recR[k];
-    }
-    if (a.check) {
-      double q[NQ];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) q[c] = su[c][i + 1];
-      CompleteFromCons<D>(q, MakeGas<double>(a));
-      spa[0][i + 1] = q[D + 2];
-      spa[1][i + 1] = q[D + 3];
     }
   }
-  __syncthreads();
+  T recRprev[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) recRprev[k] = __shfl_up(recR[k], 1, 64);
 
-  unsigned long long key = 0x7ff0000000000000ull;
-  for (int i = t; i < cnt + 1; i += kBlock) {
-    T L[NQ], R[NQ], f[NC];
+  // Face sl - 1 (left face of the slot's cell) for slots 1 .. nx + 1.
+  const bool face_lane = valid && lane >= 1 && sl >= 1;
+  T F[NC];
+  if (face_lane) {
+    FaceFlux<D, 0, LARR>(F, recRprev, recL, g, a.quirk3d);
+  } else {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      L[k] = srecR[k][i];
-      R[k] = srecL[k][i + 1];
-    }
-    FaceFlux<D, 0, LARR>(f, L, R, g, a.quirk3d);
-#pragma unroll
-    for (int k = 0; k < NC; ++k) srecR[k][i] = f[k];
-    if (a.check) {
-      double qL[NQ], qR[NQ];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        qL[c] = su[c][i + 1];
-        qR[c] = su[c][i + 2];
-      }
-      qL[D + 2] = spa[0][i + 1];
-      qL[D + 3] = spa[1][i + 1];
-      qR[D + 2] = spa[0][i + 2];
-      qR[D + 3] = spa[1][i + 2];
-      const double dtf = HllemStableDt<D, 0>(qL, qR, a.dx, MakeGas<double>(a));
-      const unsigned long long k2 = DtKey(dtf);
-      key = k2 < key ? k2 : key;
-    }
+    for (int k = 0; k < NC; ++k) F[k] = T(0);
   }
-  __syncthreads();
+  T Fnext[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) Fnext[k] = __shfl_down(F[k], 1, 64);
 
-  for (int i = t; i < cnt; i += kBlock) {
+  const bool owned = lane >= 1 && lane <= kXOwned;
+  if (owned && real) {
+    const int64_t off = rbase + cell;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const double fl = static_cast<double>(srecR[c][i]);
-      const double fr = static_cast<double>(srecR[c][i + 1]);
-      a.out[c][rbase + x0 + i] = su[c][i + 2] + dt_dx * (fl - fr);
+    for (int k = 0; k < NC; ++k) {
+      a.out[k][off] = u[k] + dt_dx * (static_cast<double>(F[k]) - static_cast<double>(Fnext[k]));
     }
   }
-  if (a.check) BlockCheckMin(key, a);
+  if (a.check) {
+    unsigned long long key = kInfKey;
+    const DtCell<D> dc = MakeDtCell<D>(u, gd);
+    const DtCell<D> dp = ShflUp<D>(dc);
+    if (owned && face_lane) key = DtKey(FaceDt<D, 0>(dp, dc, a.dx, gd.gm1));
+    BlockMinToPartials<1, kBlock / 64>(&key, a.partials + blockIdx.x, 0);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -239,12 +303,7 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
   const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   const bool active = p < npencil;
   const int64_t stride = (DIR == 1) ? nx : nx * ny;
-  int64_t base = 0;
-  if (DIR == 1) {
-    base = (p % nx) + (p / nx) * nx * ny;
-  } else {
-    base = p;
-  }
+  const int64_t base = (DIR == 1) ? (p % nx) + (p / nx) * nx * ny : p;
   const int64_t j0 = static_cast<int64_t>(blockIdx.y) * a.seg;
   const int64_t j1 = (j0 + a.seg < nd) ? j0 + a.seg : nd;
 
@@ -254,9 +313,9 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
 
-  unsigned long long key = 0x7ff0000000000000ull;
+  unsigned long long key = kInfKey;
   if (active) {
-    double Um[NC], U0[NC], Up[NC];
+    double Um[NC], U0[NC], Up[NC], Unext[NC];
     auto load = [&](double* u, int64_t j) {
       const int64_t off = base + Clamp(j, nd) * stride;
 #pragma unroll
@@ -264,10 +323,10 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
     };
     load(Um, j0 - 1);
     load(U0, j0);
+    load(Unext, j0 + 1);
     T recRprev[NQ];
     {
-      T tm[NC], t0[NC], tp[NC];
-      T dummy[NQ];
+      T t0[NC];
       if (j0 == 0) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) t0[c] = static_cast<T>(U0[c]);
@@ -275,6 +334,7 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
       } else {
         double Umm[NC];
         load(Umm, j0 - 2);
+        T tm[NC], tp[NC], dummy[NQ];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           tm[c] = static_cast<T>(Umm[c]);
@@ -284,17 +344,15 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
         ReconAny<D, DIR>(tm, t0, tp, lh, g, a.limiter, a.muscl, dummy, recRprev);
       }
     }
-    double cprev[NQ];
-    if (a.check) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) cprev[c] = Um[c];
-      CompleteFromCons<D>(cprev, gd);
-    }
+    DtCell<D> dprev;
+    if (a.check) dprev = MakeDtCell<D>(Um, gd);
     T Fprev[NC];
     for (int64_t j = j0; j <= j1; ++j) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) Up[c] = Unext[c];
+      if (j + 1 < nd) load(Unext, j + 2);
       T recL[NQ], recR[NQ];
       if (j < nd) {
-        load(Up, j + 1);
         T tm[NC], t0[NC], tp[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
@@ -306,10 +364,7 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
       } else {
         T t0[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          t0[c] = static_cast<T>(U0[c]);
-          Up[c] = U0[c];
-        }
+        for (int c = 0; c < NC; ++c) t0[c] = static_cast<T>(U0[c]);
         GhostRec<D>(t0, g, recL);
 #pragma unroll
         for (int k = 0; k < NQ; ++k) recR[k] = recL[k];
@@ -325,15 +380,9 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
         }
       }
       if (a.check && (j < j1 || j1 == nd)) {
-        double c0[NQ];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) c0[c] = U0[c];
-        CompleteFromCons<D>(c0, gd);
-        const double dtf = HllemStableDt<D, DIR>(cprev, c0, a.dx, gd);
-        const unsigned long long k2 = DtKey(dtf);
-        key = k2 < key ? k2 : key;
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) cprev[k] = c0[k];
+        const DtCell<D> d0 = MakeDtCell<D>(U0, gd);
+        key = KMin(key, DtKey(FaceDt<D, DIR>(dprev, d0, a.dx, gd.gm1)));
+        dprev = d0;
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
@@ -345,124 +394,172 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
       for (int k = 0; k < NQ; ++k) recRprev[k] = recR[k];
     }
   }
-  if (a.check) BlockCheckMin(key, a);
+  if (a.check) {
+    BlockMinToPartials<1, kBlock / 64>(&key,
+                                       a.partials + blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x,
+                                       0);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Step-start CFL: stable dt of every direction in one pass.  Each thread
-// owns a cell c and evaluates the faces (c - e_d, c) for every direction d,
-// plus the boundary face after the last cell of each row.
+// Step-start CFL: stable dt of every direction in one pass.  A workgroup is a
+// 256-cell x tile of one row and marches along the last dimension (z in 3D,
+// y in 2D) over a segment; x neighbours come through LDS, the march
+// neighbour is kept in registers and (3D) the y neighbour is recomputed.
+// Boundary faces pair a cell with its transmissive ghost copy.
+constexpr int kCflSeg = 32;
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void PatchCflKernel(
     const double* __restrict__ u0, const double* __restrict__ u1,
     const double* __restrict__ u2, const double* __restrict__ u3,
-    const double* __restrict__ u4, int64_t nx, int64_t ny, int64_t nz,
-    double dx0, double dx1, double dx2, double gamma, double gm1inv,
-    unsigned long long* keys, unsigned int* counter, Ctl* ctl, int* halt,
-    int first_dir, int strang) {
+    const double* __restrict__ u4, int64_t nx, int64_t ny, int64_t nz, double dx0,
+    double dx1, double dx2, double gamma, double gm1inv, unsigned long long* partials,
+    int* halt) {
   constexpr int NC = D + 2;
-  constexpr int NQ = D + 4;
+  constexpr int M = D - 1;  // march dimension
   if (halt && *halt) return;
   const double* in[5] = {u0, u1, u2, u3, u4};
   Gas<double> g;
   g.gamma = gamma;
   g.gm1inv = gm1inv;
   g.gm1 = gamma - 1.0;
-  const int64_t ncell = nx * ny * nz;
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  unsigned long long key[3] = {0x7ff0000000000000ull, 0x7ff0000000000000ull,
-                               0x7ff0000000000000ull};
-  if (c < ncell) {
-    const int64_t x = c % nx;
-    const int64_t y = (c / nx) % ny;
-    const int64_t z = c / (nx * ny);
-    double q[NQ];
+  const int64_t tiles = (nx + kTX - 1) / kTX;
+  const int64_t x0 = (blockIdx.x % tiles) * kTX;
+  const int64_t y = (D == 3) ? static_cast<int64_t>(blockIdx.x / tiles) : 0;
+  const int tx = threadIdx.x;
+  const int64_t x = x0 + tx;
+  const bool active = x < nx;
+  const int64_t xs = active ? x : nx - 1;
+  const int64_t nm = (D == 3) ? nz : ny;
+  const int64_t ms = (D == 3) ? nx * ny : nx;
+  const int64_t m0 = static_cast<int64_t>(blockIdx.y) * kCflSeg;
+  const int64_t m1 = (m0 + kCflSeg < nm) ? m0 + kCflSeg : nm;
+  const int64_t rowoff = (D == 3) ? y * nx : 0;
+  const double dxs[3] = {dx0, dx1, dx2};
+
+  __shared__ DtCell<D> s_cells[kTX];
+  unsigned long long key[3] = {kInfKey, kInfKey, kInfKey};
+
+  auto load_cell = [&](int64_t off) {
+    double u[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) q[k] = in[k][c];
-    CompleteFromCons<D>(q, g);
-    const int64_t pos[3] = {x, y, z};
-    const int64_t len[3] = {nx, ny, nz};
-    const int64_t str[3] = {1, nx, nx * ny};
-    const double dxs[3] = {dx0, dx1, dx2};
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      double dtf;
-      if (pos[d] == 0) {
-        // Face 0: the ghost copy and the first cell.
-        if (d == 0) dtf = HllemStableDtUniform<D, 0>(q, dxs[0], g);
-        else if (d == 1) dtf = HllemStableDtUniform<D, 1>(q, dxs[1], g);
-        else dtf = HllemStableDtUniform<D, (D > 2 ? 2 : 0)>(q, dxs[2], g);
+    for (int c = 0; c < NC; ++c) u[c] = in[c][off];
+    return MakeDtCell<D>(u, g);
+  };
+
+  DtCell<D> dprev = load_cell(xs + rowoff + Clamp(m0 - 1, nm) * ms);
+  for (int64_t m = m0; m < m1; ++m) {
+    const int64_t off = xs + rowoff + m * ms;
+    const DtCell<D> dc = load_cell(off);
+    // March direction (z in 3D, y in 2D).
+    {
+      const DtCell<D>& l = (m == 0) ? dc : dprev;
+      key[M] = KMin(key[M], DtKey(FaceDt<D, M>(l, dc, dxs[M], g.gm1)));
+      if (m == nm - 1) key[M] = KMin(key[M], DtKey(FaceDt<D, M>(dc, dc, dxs[M], g.gm1)));
+    }
+    // y direction in 3D: recompute the y - 1 neighbour.
+    if constexpr (D == 3) {
+      const DtCell<D> dy = (y == 0) ? dc : load_cell(off - nx);
+      key[1] = KMin(key[1], DtKey(FaceDt<D, 1>(dy, dc, dxs[1], g.gm1)));
+      if (y == ny - 1) key[1] = KMin(key[1], DtKey(FaceDt<D, 1>(dc, dc, dxs[1], g.gm1)));
+    }
+    // x direction through LDS.
+    s_cells[tx] = dc;
+    __syncthreads();
+    if (active) {
+      DtCell<D> dl;
+      if (x == 0) {
+        dl = dc;
+      } else if (tx > 0) {
+        dl = s_cells[tx - 1];
       } else {
-        double qm[NQ];
-#pragma unroll
-        for (int k = 0; k < NC; ++k) qm[k] = in[k][c - str[d]];
-        CompleteFromCons<D>(qm, g);
-        if (d == 0) dtf = HllemStableDt<D, 0>(qm, q, dxs[0], g);
-        else if (d == 1) dtf = HllemStableDt<D, 1>(qm, q, dxs[1], g);
-        else dtf = HllemStableDt<D, (D > 2 ? 2 : 0)>(qm, q, dxs[2], g);
+        dl = load_cell(off - 1);
       }
-      unsigned long long k2 = DtKey(dtf);
-      key[d] = k2 < key[d] ? k2 : key[d];
-      if (pos[d] == len[d] - 1) {
-        // Face n: the last cell and its ghost copy.
-        if (d == 0) dtf = HllemStableDtUniform<D, 0>(q, dxs[0], g);
-        else if (d == 1) dtf = HllemStableDtUniform<D, 1>(q, dxs[1], g);
-        else dtf = HllemStableDtUniform<D, (D > 2 ? 2 : 0)>(q, dxs[2], g);
-        k2 = DtKey(dtf);
-        key[d] = k2 < key[d] ? k2 : key[d];
-      }
+      key[0] = KMin(key[0], DtKey(FaceDt<D, 0>(dl, dc, dxs[0], g.gm1)));
+      if (x == nx - 1) key[0] = KMin(key[0], DtKey(FaceDt<D, 0>(dc, dc, dxs[0], g.gm1)));
     }
+    __syncthreads();
+    dprev = dc;
   }
-  __shared__ unsigned long long s_keys[3][kBlock / 64];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    const unsigned long long k = WaveMinKey(key[d]);
-    if (lane == 0) s_keys[d][wave] = k;
-  }
+  if (!active) key[0] = key[1] = key[2] = kInfKey;
+  const int64_t nblocks = static_cast<int64_t>(gridDim.x) * gridDim.y;
+  const int64_t bid = blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x;
+  BlockMinToPartials<D, kBlock / 64>(key, partials + bid, nblocks);
+}
+
+// One-workgroup reduction of partial minima.
+constexpr int kRedBlock = 1024;
+
+__device__ __forceinline__ double KeyToDt(unsigned long long k) {
+  return __longlong_as_double(static_cast<long long>(k));
+}
+
+__device__ unsigned long long ReduceKeys(const unsigned long long* p, int64_t n) {
+  __shared__ unsigned long long s[kRedBlock / 64];
+  unsigned long long k = kInfKey;
+  for (int64_t i = threadIdx.x; i < n; i += kRedBlock) k = KMin(k, p[i]);
+  k = WaveMinKey(k);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = k;
   __syncthreads();
+  unsigned long long r = s[0];
+  for (int w = 1; w < kRedBlock / 64; ++w) r = KMin(r, s[w]);
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kRedBlock) void FinalizeCheckKernel(
+    const unsigned long long* partials, int64_t n, double* out, Ctl* ctl) {
+  if (ctl && ctl->halt) return;
+  const double v = KeyToDt(ReduceKeys(partials, n));
   if (threadIdx.x == 0) {
-    for (int d = 0; d < D; ++d) {
-      unsigned long long k = s_keys[d][0];
-      for (int w = 1; w < kBlock / 64; ++w) k = s_keys[d][w] < k ? s_keys[d][w] : k;
-      atomicMin(&keys[d], k);
-    }
+    if (out) *out = v;
     if (ctl) {
-      __threadfence();
-      const unsigned int prev = atomicAdd(counter, 1u);
-      if (prev == gridDim.x - 1) {
-        __threadfence();
-        // Last block: RunSimulation's limited_dt (RunSimulation.hpp:121-131).
-        double stable = 1.7976931348623157e308; // DimensionalSplitLevelIntegrator min_dt start
-        for (int d = 0; d < D; ++d) {
-          const unsigned long long k = atomicAdd(&keys[d], 0ull);
-          const double v = __longlong_as_double(static_cast<long long>(k));
-          ctl->dt_dir[d] = v;
-          stable = (v < stable) ? v : stable;
-          atomicExch(&keys[d], 0x7ff0000000000000ull);
-        }
-        atomicExch(counter, 0u);
-        if (!(ctl->t < ctl->final_time)) {
-          ctl->halt = 2;
-        } else {
-          const double sdt = ctl->have_failure ? ctl->failure_dt : stable;
-          const double rem = ctl->final_time - ctl->t;
-          const double cdt = ctl->cfl * sdt;
-          const double dt = (cdt < rem) ? cdt : rem;
-          ctl->dt = dt;
-          ctl->split_dt = strang ? 0.5 * dt : dt;
-          // The first sweep's CFL check is the step-start stable dt of its
-          // direction: the same faces of the same state.
-          const double first = ctl->dt_dir[first_dir];
-          ctl->last_check = first;
-          if (first < ctl->split_dt) {
-            ctl->halt = 1;
-            ctl->failure_dt = first;
-          }
-        }
+      ctl->last_check = v;
+      if (v < ctl->split_dt) {
+        // TimeStepTooLarge{level_dt} (DimensionalSplitLevelIntegrator.hpp:243-245)
+        ctl->halt = 1;
+        ctl->failure_dt = v;
       }
     }
+  }
+}
+
+__global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
+    const unsigned long long* partials, int64_t n, int dim, double* out, Ctl* ctl,
+    int first_dir, int strang) {
+  if (ctl && ctl->halt) return;
+  double v[3];
+  for (int d = 0; d < dim; ++d) v[d] = KeyToDt(ReduceKeys(partials + d * n, n));
+  if (threadIdx.x != 0) return;
+  if (out) {
+    for (int d = 0; d < dim; ++d) out[d] = v[d];
+  }
+  if (!ctl) return;
+  // DimensionalSplitLevelIntegrator::ComputeStableDt (:204-219) starts from
+  // DBL_MAX; RunSimulation's limited_dt (RunSimulation.hpp:121-131).
+  double stable = 1.7976931348623157e308;
+  for (int d = 0; d < dim; ++d) {
+    ctl->dt_dir[d] = v[d];
+    stable = (v[d] < stable) ? v[d] : stable;
+  }
+  if (!(ctl->t < ctl->final_time)) {
+    ctl->halt = 2;
+    return;
+  }
+  const double sdt = ctl->have_failure ? ctl->failure_dt : stable;
+  const double rem = ctl->final_time - ctl->t;
+  const double cdt = ctl->cfl * sdt;
+  const double dt = (cdt < rem) ? cdt : rem;
+  ctl->dt = dt;
+  ctl->split_dt = strang ? 0.5 * dt : dt;
+  // The first sweep's CFL check is the step-start stable dt of its
+  // direction: the same faces of the same state.
+  const double first = v[first_dir];
+  ctl->last_check = first;
+  if (first < ctl->split_dt) {
+    ctl->halt = 1;
+    ctl->failure_dt = first;
   }
 }
 
@@ -479,32 +576,36 @@ __global__ void StepEndKernel(Ctl* ctl, double* dt_log, int64_t dt_log_base,
 __global__ void KeysToDtKernel(unsigned long long* keys, int n, double* out) {
   const int i = threadIdx.x;
   if (i < n) {
-    out[i] = __longlong_as_double(static_cast<long long>(keys[i]));
-    keys[i] = 0x7ff0000000000000ull;
+    out[i] = KeyToDt(keys[i]);
+    keys[i] = kInfKey;
   }
 }
 
 __global__ void ResetKeysKernel(unsigned long long* keys, int n, unsigned int* counters,
                                 int nc) {
   const int i = threadIdx.x;
-  if (i < n) keys[i] = 0x7ff0000000000000ull;
+  if (i < n) keys[i] = kInfKey;
   if (i < nc) counters[i] = 0u;
 }
 
 template <int D, bool LARR, class T>
 int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
+  const int64_t blocks = SweepBlocks(D, dir, a.n, a.seg);
+  if (blocks <= 0) return FVS_BAD_ARGUMENT;
   if (dir == 0) {
-    const int64_t tiles = (a.n[0] + kTX - 1) / kTX;
-    const int64_t rows = a.n[1] * (D == 3 ? a.n[2] : 1);
-    const int64_t blocks = tiles * rows;
-    if (blocks <= 0 || blocks > 0x7fffffff) return FVS_BAD_ARGUMENT;
+    if (blocks > 0x7fffffff) return FVS_BAD_ARGUMENT;
+    const int64_t slots = (a.n[0] + 2) * a.n[1] * a.n[2];
+    if (slots >= (int64_t(1) << 32)) {
+      SetError("X sweep: %lld slots exceed the 32-bit slot index", static_cast<long long>(slots));
+      return FVS_BAD_ARGUMENT;
+    }
     hipLaunchKernelGGL((SweepXKernel<D, LARR, T>), dim3(static_cast<unsigned>(blocks)),
                        dim3(kBlock), 0, s, a);
   } else {
     const int64_t npencil = (dir == 1) ? a.n[0] * (D == 3 ? a.n[2] : 1) : a.n[0] * a.n[1];
     const int64_t bx = (npencil + kBlock - 1) / kBlock;
     const int64_t nseg = (a.n[dir] + a.seg - 1) / a.seg;
-    if (bx <= 0 || bx > 0x7fffffff || nseg <= 0 || nseg > 65535) return FVS_BAD_ARGUMENT;
+    if (bx > 0x7fffffff || nseg > 65535) return FVS_BAD_ARGUMENT;
     if (dir == 1) {
       hipLaunchKernelGGL((SweepPencilKernel<D, 1, LARR, T>),
                          dim3(static_cast<unsigned>(bx), static_cast<unsigned>(nseg)),
@@ -525,6 +626,26 @@ int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
 
 } // namespace
 
+int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
+  const int64_t nz = dim == 3 ? n[2] : 1;
+  if (dir == 0) {
+    const int64_t slots = (n[0] + 2) * n[1] * nz;
+    const int64_t waves = (slots + kXOwned - 1) / kXOwned;
+    return (waves + kBlock / 64 - 1) / (kBlock / 64);
+  }
+  const int64_t npencil = (dir == 1) ? n[0] * nz : n[0] * n[1];
+  const int64_t bx = (npencil + kBlock - 1) / kBlock;
+  const int64_t nseg = (n[dir] + seg - 1) / seg;
+  return bx * nseg;
+}
+
+int64_t CflBlocks(int dim, const int64_t n[3]) {
+  const int64_t tiles = (n[0] + kTX - 1) / kTX;
+  const int64_t rows = dim == 3 ? n[1] : 1;
+  const int64_t nm = dim == 3 ? n[2] : n[1];
+  return tiles * rows * ((nm + kCflSeg - 1) / kCflSeg);
+}
+
 int LaunchSweep(const fvs_method_desc& desc, hipStream_t s, int dir, const SweepArgs& a) {
   if (desc.base != FVS_BASE_HLLEM_LARROUTUROU && desc.base != FVS_BASE_HLLEM) {
     SetError("fused sweep: base method %d has no fused kernel", desc.base);
@@ -540,25 +661,38 @@ int LaunchSweep(const fvs_method_desc& desc, hipStream_t s, int dir, const Sweep
   return fp32 ? LaunchTyped<3, false, float>(s, dir, a) : LaunchTyped<3, false, double>(s, dir, a);
 }
 
+int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
+                        double* out, Ctl* ctl) {
+  hipLaunchKernelGGL(FinalizeCheckKernel, dim3(1), dim3(kRedBlock), 0, s, partials, nblocks,
+                     out, ctl);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
 int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
-                   const int64_t n[3], const double dx[3], unsigned long long* keys,
-                   unsigned int* counter, Ctl* ctl, int* halt, int first_dir,
-                   int strang) {
-  const int64_t ncell = n[0] * n[1] * (desc.dim == 3 ? n[2] : 1);
-  const int64_t blocks = (ncell + kBlock - 1) / kBlock;
-  if (blocks <= 0 || blocks > 0x7fffffff) return FVS_BAD_ARGUMENT;
+                   const int64_t n[3], const double dx[3], unsigned long long* partials,
+                   double* out, Ctl* ctl, int* halt, int first_dir, int strang) {
+  const int D = desc.dim;
+  const int64_t tiles = (n[0] + kTX - 1) / kTX;
+  const int64_t rows = D == 3 ? n[1] : 1;
+  const int64_t nm = D == 3 ? n[2] : n[1];
+  const int64_t gx = tiles * rows;
+  const int64_t gy = (nm + kCflSeg - 1) / kCflSeg;
+  if (gx <= 0 || gx > 0x7fffffff || gy <= 0 || gy > 65535) return FVS_BAD_ARGUMENT;
   const double gm1inv = 1.0 / (desc.gamma - 1.0);
-  if (desc.dim == 2) {
-    hipLaunchKernelGGL(PatchCflKernel<2>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
-                       0, s, cons[0], cons[1], cons[2], cons[3], nullptr, n[0], n[1],
-                       int64_t(1), dx[0], dx[1], dx[0], desc.gamma, gm1inv, keys, counter,
-                       ctl, halt, first_dir, strang);
+  if (D == 2) {
+    hipLaunchKernelGGL(PatchCflKernel<2>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
+                       dim3(kBlock), 0, s, cons[0], cons[1], cons[2], cons[3], nullptr, n[0],
+                       n[1], int64_t(1), dx[0], dx[1], dx[0], desc.gamma, gm1inv, partials,
+                       halt);
   } else {
-    hipLaunchKernelGGL(PatchCflKernel<3>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
-                       0, s, cons[0], cons[1], cons[2], cons[3], cons[4], n[0], n[1], n[2],
-                       dx[0], dx[1], dx[2], desc.gamma, gm1inv, keys, counter, ctl, halt,
-                       first_dir, strang);
+    hipLaunchKernelGGL(PatchCflKernel<3>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
+                       dim3(kBlock), 0, s, cons[0], cons[1], cons[2], cons[3], cons[4], n[0],
+                       n[1], n[2], dx[0], dx[1], dx[2], desc.gamma, gm1inv, partials, halt);
   }
+  FVS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(FinalizeCflKernel, dim3(1), dim3(kRedBlock), 0, s, partials, gx * gy, D,
+                     out, ctl, first_dir, strang);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
