@@ -95,6 +95,22 @@ SIGNATURES = {
     "fvs_level_time": (_I, [_P, _PD, _PI64]),
     "fvs_split_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P]),
     "fvs_patch_stable_dt": (_I, [_P, C.POINTER(_P), _PI64, _PD, _P]),
+    "fvs_godunov_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P, _P]),
+    "fvs_newton_stats": (_I, [_P, C.POINTER(C.c_uint64), _I]),
+    "fvs_exact_middle_state": (_I, [_P, _I, _P, _P, C.c_int64, _P, _P, _P, _P, _P]),
+    "fvs_mlp_create": (_I, [_P, C.c_uint64, C.POINTER(_P)]),
+    "fvs_mlp_destroy": (_I, [_P]),
+    "fvs_mlp_num_params": (_I, []),
+    "fvs_mlp_get_params": (_I, [_P, _P]),
+    "fvs_mlp_set_params": (_I, [_P, _P]),
+    "fvs_mlp_generate": (_I, [_P, C.c_int64, C.c_uint64, _D, _PI64]),
+    "fvs_mlp_dataset": (_I, [_P, _PI64, _P, _P, _P, _P, _P]),
+    "fvs_mlp_forward": (_I, [_P, _P, C.c_int64, _P]),
+    "fvs_mlp_gradients": (_I, [_P, _P, _P, _P, C.c_int64, C.c_float, _P, _P]),
+    "fvs_mlp_train": (_I, [_P, _I, C.c_int64, C.c_float, C.c_float, C.c_uint64, _P]),
+    "fvs_mlp_infer_pairs": (_I, [_P, _I, _I, _P, _P, C.c_int64, _P, _P]),
+    "fvs_mlp_face_guess": (_I, [_P, _I, _D, C.POINTER(_P), _PI64, _I, _P]),
+    "fvs_set_mlp": (_I, [_P, _P]),
 }
 
 _lib = None

@@ -280,6 +280,19 @@ int64_t Count(const DevView& v) { return v.ext[0] * v.ext[1] * v.ext[2]; }
 
 using namespace fvs;
 
+static int EnsurePartials(fvs_handle* h, int64_t n) {
+  if (n <= h->partials_cap) return FVS_OK;
+  if (h->partials) {
+    FVS_HIP_CHECK(hipStreamSynchronize(h->stream));
+    FVS_HIP_CHECK(hipFree(h->partials));
+    h->partials = nullptr;
+    h->partials_cap = 0;
+  }
+  FVS_HIP_CHECK(hipMalloc(&h->partials, n * sizeof(unsigned long long)));
+  h->partials_cap = n;
+  return FVS_OK;
+}
+
 extern "C" {
 
 const char* fvs_last_error(void) { return g_last_error.c_str(); }
@@ -308,16 +321,24 @@ int fvs_create(const fvs_method_desc* desc, void* hip_stream, fvs_handle** out) 
     return FVS_HIP_ERROR;
   }
   void* mem = nullptr;
-  if (hipMalloc(&mem, 256) != hipSuccess) {
+  if (hipMalloc(&mem, 512) != hipSuccess) {
     delete h;
     SetError("hipMalloc of the handle scratch failed");
     return FVS_HIP_ERROR;
   }
   h->partials = nullptr;
   h->partials_cap = 0;
+  h->mlp = nullptr;
   h->scratch_dt = static_cast<double*>(mem);
   h->scratch_keys = reinterpret_cast<unsigned long long*>(static_cast<char*>(mem) + 64);
   h->scratch_counter = reinterpret_cast<unsigned int*>(static_cast<char*>(mem) + 192);
+  h->newton_stats = reinterpret_cast<unsigned long long*>(static_cast<char*>(mem) + 256);
+  if (hipMemsetAsync(h->newton_stats, 0, 5 * sizeof(unsigned long long), h->stream) != hipSuccess) {
+    (void)hipFree(mem);
+    delete h;
+    SetError("hipMemsetAsync of the Newton counters failed");
+    return FVS_HIP_ERROR;
+  }
   if (int rc = LaunchResetKeys(h->stream, h->scratch_keys, 8, h->scratch_counter, 4)) {
     (void)hipFree(mem);
     delete h;
@@ -343,11 +364,22 @@ int fvs_stable_dt(fvs_handle* h, const fvs_view* states, double dx, int dir,
     return FVS_BAD_ARGUMENT;
   }
   if (int rc = CheckDir(h, dir)) return rc;
-  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
-    SetError("exact Godunov stable dt is provided by fvs_godunov_stable_dt");
-    return FVS_BAD_ARGUMENT;
-  }
   const int D = h->desc.dim;
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    if (!states || !states->data || states->ncomp < D + 4) {
+      SetError("states view needs %d components", D + 4);
+      return FVS_BAD_ARGUMENT;
+    }
+    const int64_t nb = GodunovViewStableDtBlocks(h->desc, states, dir);
+    if (nb <= 0) {
+      SetError("states box too small for the stencil");
+      return FVS_BAD_ARGUMENT;
+    }
+    if (int rc = EnsurePartials(h, nb)) return rc;
+    if (int rc = LaunchGodunovViewStableDt(h->desc, h->stream, states, dx, dir, h->partials))
+      return rc;
+    return LaunchFinalizeCheck(h->stream, h->partials, nb, dt_device, nullptr);
+  }
   DevView st;
   if (int rc = MakeDevView(states, D + 4, &st)) return rc;
   const int S = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK ? 4 : 2;
@@ -381,10 +413,6 @@ int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* st
     return FVS_BAD_ARGUMENT;
   }
   if (int rc = CheckDir(h, dir)) return rc;
-  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
-    SetError("exact Godunov fluxes are provided by fvs_godunov_fluxes");
-    return FVS_BAD_ARGUMENT;
-  }
   const int D = h->desc.dim;
   DevView fl, st;
   if (int rc = MakeDevView(fluxes, D + 2, &fl)) return rc;
@@ -401,6 +429,10 @@ int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* st
   if (!ContainsBox(states, need_lo, need_hi)) {
     SetError("states box does not cover the flux stencils");
     return FVS_BAD_ARGUMENT;
+  }
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    return LaunchGodunovViewFlux(h->desc, h->stream, fluxes, states, dt, dx, dir,
+                                 h->newton_stats + 2);
   }
   const Gas<double> g = GasOf(h->desc);
   const bool larr = h->desc.base == FVS_BASE_HLLEM_LARROUTUROU;
@@ -497,18 +529,6 @@ int fvs_transmissive_fill(fvs_handle* h, const fvs_view* states, const int64_t d
 
 // ---- fused single-patch operators ------------------------------------------
 
-static int EnsurePartials(fvs_handle* h, int64_t n) {
-  if (n <= h->partials_cap) return FVS_OK;
-  if (h->partials) {
-    FVS_HIP_CHECK(hipStreamSynchronize(h->stream));
-    FVS_HIP_CHECK(hipFree(h->partials));
-    h->partials = nullptr;
-    h->partials_cap = 0;
-  }
-  FVS_HIP_CHECK(hipMalloc(&h->partials, n * sizeof(unsigned long long)));
-  h->partials_cap = n;
-  return FVS_OK;
-}
 
 static int FillSweepArgs(fvs_handle* h, const double* const* in, double* const* out,
                          const int64_t n[3], double dx, SweepArgs* a) {
@@ -555,6 +575,9 @@ int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* 
       return FVS_BAD_ARGUMENT;
     }
   }
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    return fvs_godunov_sweep(h, cons_in, cons_out, n, dt, dx, dir, nullptr, stable_dt_device);
+  }
   a.dt_val = dt;
   a.check = stable_dt_device != nullptr;
   const int64_t nb = SweepBlocks(h->desc.dim, dir, a.n, a.seg);
@@ -587,9 +610,95 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t 
       return FVS_BAD_ARGUMENT;
     }
   }
+  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    const int64_t nb = GodunovCflBlocks(D, nn);
+    if (int rc = EnsurePartials(h, 5 * D * nb)) return rc;
+    if (int rc = LaunchGodunovCfl(h->desc, h->stream, cons, nn, dx, nullptr, h->partials, nullptr))
+      return rc;
+    for (int d = 0; d < D; ++d) {
+      if (int rc = LaunchGodunovFinalize(h->stream, h->partials + 5 * d * nb, nb,
+                                         dt_dir_device + d, h->newton_stats))
+        return rc;
+    }
+    return FVS_OK;
+  }
   if (int rc = EnsurePartials(h, D * CflBlocks(D, nn))) return rc;
   return LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, h->partials, dt_dir_device, nullptr,
                         nullptr, 0, 0);
+}
+
+int fvs_godunov_sweep(fvs_handle* h, const double* const* cons_in, double* const* cons_out,
+                      const int64_t n[3], double dt, double dx, int dir,
+                      const double* guess_device, double* stable_dt_device) {
+  if (!h || !n || !cons_in) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  if (h->desc.base != FVS_BASE_GODUNOV_EXACT || h->desc.reconstruction != FVS_RECON_NONE) {
+    SetError("fvs_godunov_sweep needs a first-order exact-Godunov handle");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int D = h->desc.dim;
+  int64_t nn[3] = {n[0], n[1], D == 3 ? n[2] : 1};
+  for (int c = 0; c < D + 2; ++c) {
+    if (!cons_in[c] || (cons_out && !cons_out[c]) || (cons_out && cons_out[c] == cons_in[c])) {
+      SetError("bad component pointer %d (null, or output aliasing input)", c);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  for (int d = 0; d < D; ++d) {
+    if (nn[d] < 1) {
+      SetError("patch extent %d must be positive", d);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  const int64_t nb = GodunovSweepBlocks(D, dir, nn, 32);
+  if (int rc = EnsurePartials(h, 5 * nb)) return rc;
+  if (int rc = LaunchGodunovSweep(h->desc, h->stream, dir, cons_in, cons_out, nn, dx, nullptr, dt,
+                                  guess_device, h->partials, nullptr, cons_out != nullptr))
+    return rc;
+  return LaunchGodunovFinalize(h->stream, h->partials, nb, stable_dt_device, h->newton_stats);
+}
+
+int fvs_exact_middle_state(fvs_handle* h, int dir, const double* left, const double* right,
+                           int64_t n, const double* guess, double* pM, double* uM,
+                           int* iterations, int* converged) {
+  if (!h || !left || !right || n <= 0) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CheckDir(h, dir)) return rc;
+  return LaunchMiddleState(h->desc.dim, h->stream, left, right, n, dir, h->desc.gamma,
+                           h->desc.newton_tolerance, h->desc.max_newton_iterations, guess, pM, uM,
+                           iterations, converged);
+}
+
+int fvs_set_mlp(fvs_handle* h, fvs_mlp* mlp) {
+  if (!h) {
+    SetError("null handle");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (mlp && (h->desc.base != FVS_BASE_GODUNOV_EXACT || h->desc.reconstruction != FVS_RECON_NONE)) {
+    SetError("a learned guess needs a first-order exact-Godunov handle");
+    return FVS_BAD_ARGUMENT;
+  }
+  h->mlp = mlp;
+  h->desc.newton_guess = mlp ? FVS_GUESS_LEARNED : FVS_GUESS_MEAN_PRESSURE;
+  return FVS_OK;
+}
+
+int fvs_newton_stats(fvs_handle* h, uint64_t stats[5], int reset) {
+  if (!h || !stats) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  unsigned long long tmp[5];
+  FVS_HIP_CHECK(hipMemcpyAsync(tmp, h->newton_stats, sizeof tmp, hipMemcpyDeviceToHost, h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < 5; ++i) stats[i] = tmp[i];
+  if (reset) FVS_HIP_CHECK(hipMemsetAsync(h->newton_stats, 0, sizeof tmp, h->stream));
+  return FVS_OK;
 }
 
 } // extern "C"
@@ -605,6 +714,7 @@ struct fvs_level {
   double* buf[3][5];
   Ctl* ctl;               // device control block
   unsigned long long* partials;  // per-workgroup minima
+  double* guess[3];       // learned Newton guesses per direction (lazy)
   int cur;                // buffer holding the current state
 };
 
@@ -617,8 +727,9 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
   }
   *out = nullptr;
   const int D = h->desc.dim;
-  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
-    SetError("the fused level driver runs HLLEM methods");
+  const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
+  if (godunov && h->desc.reconstruction != FVS_RECON_NONE) {
+    SetError("the fused level driver runs exact Godunov at first order");
     return FVS_BAD_ARGUMENT;
   }
   int64_t n[3] = {patch->n[0], patch->n[1], D == 3 ? patch->n[2] : 1};
@@ -655,6 +766,10 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
   }
   int64_t np = D * CflBlocks(D, n);
   for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, SweepBlocks(D, d, n, 32));
+  if (godunov) {
+    np = std::max<int64_t>(np, 5 * D * GodunovCflBlocks(D, n));
+    for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, 5 * GodunovSweepBlocks(D, d, n, 32));
+  }
   if (hipMalloc(&L->partials, np * sizeof(unsigned long long)) != hipSuccess) {
     (void)hipFree(L->block);
     (void)hipFree(L->ctl);
@@ -683,6 +798,8 @@ int fvs_level_destroy(fvs_level* L) {
   (void)hipFree(L->block);
   (void)hipFree(L->ctl);
   (void)hipFree(L->partials);
+  for (int d = 0; d < 3; ++d)
+    if (L->guess[d]) (void)hipFree(L->guess[d]);
   delete L;
   return FVS_OK;
 }
@@ -754,6 +871,15 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
   hipStream_t s = h->stream;
   const int D = h->desc.dim;
   const bool strang = L->patch.splitting == FVS_SPLIT_STRANG;
+  const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
+  const bool learned = godunov && h->mlp != nullptr;
+  if (learned) {
+    for (int d = 0; d < D; ++d) {
+      if (L->guess[d]) continue;
+      const int64_t nf = (L->n[0] + (d == 0)) * (L->n[1] + (d == 1)) * (L->n[2] + (d == 2));
+      FVS_HIP_CHECK(hipMalloc(&L->guess[d], nf * sizeof(double)));
+    }
+  }
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
   FVS_HIP_CHECK(hipStreamSynchronize(s));
@@ -786,8 +912,25 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
         for (int i = D - 1; i >= 0; --i) seq[ns++] = dirs[i];
       const double* cin[5];
       for (int q = 0; q < D + 2; ++q) cin[q] = L->buf[cur][q];
-      rc = LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
-                          &L->ctl->halt, seq[0], strang);
+      if (godunov) {
+        const int64_t nb = GodunovCflBlocks(D, L->n);
+        if (learned) {
+          for (int d = 0; d < D && rc == FVS_OK; ++d)
+            rc = MlpFaceGuess(h->mlp, s, D, h->desc.gamma, cin, L->n, d, L->guess[d]);
+          if (rc) break;
+        }
+        const double* gptr[3] = {L->guess[0], L->guess[1], L->guess[2]};
+        rc = LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, learned ? gptr : nullptr,
+                              L->partials, &L->ctl->halt);
+        for (int d = 0; d < D && rc == FVS_OK; ++d) {
+          rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
+                                     h->newton_stats, L->ctl);
+        }
+        if (rc == FVS_OK) rc = LaunchStepDt(s, L->ctl, D, seq[0], strang);
+      } else {
+        rc = LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
+                            &L->ctl->halt, seq[0], strang);
+      }
       if (rc) break;
       const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
       int src = b0, dst = b1;
@@ -797,6 +940,25 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
         for (int q = 0; q < D + 2; ++q) outp[q] = L->buf[dst][q];
         const double* inp[5];
         for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
+        if (godunov) {
+          const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, 32);
+          // The first sweep sees the step-start state: its guesses are the
+          // CFL pass's; later sweeps get fresh ones for their input.
+          if (learned && w > 0)
+            rc = MlpFaceGuess(h->mlp, s, D, h->desc.gamma, inp, L->n, seq[w], L->guess[seq[w]]);
+          if (rc) break;
+          rc = LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
+                                  &L->ctl->split_dt, 0.0, learned ? L->guess[seq[w]] : nullptr,
+                                  L->partials, &L->ctl->halt, 1);
+          if (rc == FVS_OK)
+            rc = LaunchGodunovFinalize(s, L->partials, nb, &L->ctl->last_check, h->newton_stats,
+                                       L->ctl);
+          if (rc == FVS_OK && w > 0) rc = LaunchCheckValue(s, L->ctl, &L->ctl->last_check);
+          if (rc) break;
+          src = dst;
+          dst = (dst == b1) ? b2 : b1;
+          continue;
+        }
         rc = FillSweepArgs(h, inp, outp, L->n, L->patch.dx[seq[w]], &a);
         if (rc) break;
         a.dt_ptr = &L->ctl->split_dt;

@@ -19,6 +19,9 @@ struct fvs_handle {
   unsigned int* scratch_counter;    // 4 device counters
   unsigned long long* partials;     // per-workgroup minima of reductions
   int64_t partials_cap;             // capacity of partials (keys)
+  unsigned long long* newton_stats; // 5 device counters: iterations, max,
+                                    // non-converged faces, faces, fallbacks
+  struct fvs_mlp* mlp;              // learned initial guess (nullable)
 };
 
 namespace fvs {
@@ -94,8 +97,36 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t stream,
                    const double* const* cons, const int64_t n[3],
                    const double dx[3], unsigned long long* partials,
                    double* out, Ctl* ctl, int* halt, int first_dir, int strang);
+// Exact-Godunov flux method (fvs_godunov.hip).  Partials hold 5 keys per
+// workgroup: min dt, iteration sum, iteration max, non-converged, faces.
+int64_t GodunovSweepBlocks(int dim, int dir, const int64_t n[3], int seg);
+int64_t GodunovCflBlocks(int dim, const int64_t n[3]);
+int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
+                       const double* const* in, double* const* out, const int64_t n[3],
+                       double dx, const double* dt_ptr, double dt_val, const double* guess,
+                       unsigned long long* partials, int* halt, int update);
+int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
+                          double* out, unsigned long long* stats, Ctl* ctl = nullptr);
+int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
+                     const int64_t n[3], const double dx[3], const double* const* guess,
+                     unsigned long long* partials, int* halt);
+int LaunchGodunovViewFlux(const fvs_method_desc& desc, hipStream_t s, const fvs_view* fluxes,
+                          const fvs_view* states, double dt, double dx, int dir,
+                          unsigned long long* nfail);
+int64_t GodunovViewStableDtBlocks(const fvs_method_desc& desc, const fvs_view* states, int dir);
+int LaunchGodunovViewStableDt(const fvs_method_desc& desc, hipStream_t s, const fvs_view* states,
+                              double dx, int dir, unsigned long long* partials);
+// Learned Newton guesses for the faces of direction dir of a ghost-free
+// patch, launched on stream s (fvs_mlp.hip).
+int MlpFaceGuess(fvs_mlp* m, hipStream_t s, int dim, double gamma, const double* const* cons,
+                 const int64_t n[3], int dir, double* guess);
+int LaunchMiddleState(int dim, hipStream_t s, const double* L, const double* R, int64_t n,
+                      int dir, double gamma, double tol, int max_it, const double* guess,
+                      double* pM, double* uM, int* iters, int* conv);
 int LaunchStepEnd(hipStream_t s, Ctl* ctl, double* dt_log, int64_t dt_log_base,
                   int dt_log_len);
+int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang);
+int LaunchCheckValue(hipStream_t s, Ctl* ctl, const double* value);
 int LaunchKeysToDt(hipStream_t s, unsigned long long* keys, int n, double* out);
 int LaunchResetKeys(hipStream_t s, unsigned long long* keys, int n,
                     unsigned int* counters, int nc);

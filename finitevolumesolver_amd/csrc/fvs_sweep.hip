@@ -563,6 +563,41 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
   }
 }
 
+// Step dt from per-direction stable dts already in ctl->dt_dir (used by
+// methods whose CFL pass reduces into ctl->dt_dir itself).
+__global__ void StepDtKernel(Ctl* ctl, int dim, int first_dir, int strang) {
+  if (ctl->halt) return;
+  double stable = 1.7976931348623157e308;
+  for (int d = 0; d < dim; ++d) stable = (ctl->dt_dir[d] < stable) ? ctl->dt_dir[d] : stable;
+  if (!(ctl->t < ctl->final_time)) {
+    ctl->halt = 2;
+    return;
+  }
+  const double sdt = ctl->have_failure ? ctl->failure_dt : stable;
+  const double rem = ctl->final_time - ctl->t;
+  const double cdt = ctl->cfl * sdt;
+  const double dt = (cdt < rem) ? cdt : rem;
+  ctl->dt = dt;
+  ctl->split_dt = strang ? 0.5 * dt : dt;
+  const double first = ctl->dt_dir[first_dir];
+  ctl->last_check = first;
+  if (first < ctl->split_dt) {
+    ctl->halt = 1;
+    ctl->failure_dt = first;
+  }
+}
+
+// CFL check of a split step whose stable dt was reduced into ctl->last_check.
+__global__ void CheckValueKernel(Ctl* ctl, const double* value) {
+  if (ctl->halt) return;
+  const double v = *value;
+  ctl->last_check = v;
+  if (v < ctl->split_dt) {
+    ctl->halt = 1;
+    ctl->failure_dt = v;
+  }
+}
+
 __global__ void StepEndKernel(Ctl* ctl, double* dt_log, int64_t dt_log_base,
                               int dt_log_len) {
   if (ctl->halt) return;
@@ -701,6 +736,18 @@ int LaunchStepEnd(hipStream_t s, Ctl* ctl, double* dt_log, int64_t dt_log_base,
                   int dt_log_len) {
   hipLaunchKernelGGL(StepEndKernel, dim3(1), dim3(1), 0, s, ctl, dt_log, dt_log_base,
                      dt_log_len);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang) {
+  hipLaunchKernelGGL(StepDtKernel, dim3(1), dim3(1), 0, s, ctl, dim, first_dir, strang);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int LaunchCheckValue(hipStream_t s, Ctl* ctl, const double* value) {
+  hipLaunchKernelGGL(CheckValueKernel, dim3(1), dim3(1), 0, s, ctl, value);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }

@@ -165,6 +165,13 @@ class FluxMethod:
         L.check(L.load().fvs_numeric_fluxes(self.handle, C.byref(fv), C.byref(sv),
                                             float(dt), float(dx), _dir(dir)))
 
+    def newton_stats(self, reset: bool = False) -> dict:
+        """Newton counters of the exact Riemann solver on this handle."""
+        arr = (C.c_uint64 * 5)()
+        L.check(L.load().fvs_newton_stats(self.handle, arr, 1 if reset else 0))
+        return {"iterations": arr[0], "max_iterations": arr[1],
+                "not_converged": arr[2], "faces": arr[3], "fallbacks": arr[4]}
+
     def ComputeStableDt(self, states: PatchView, dx: float, dir,
                         out: torch.Tensor | None = None) -> torch.Tensor:
         """Minimum face stable dt; returns a 1-element device tensor."""
@@ -223,6 +230,28 @@ def split_sweep(flux_method: FluxMethod, cons_in: torch.Tensor, cons_out: torch.
     L.check(L.load().fvs_split_sweep(flux_method.handle, pin, pout, L.i64x3(n), float(dt),
                                      float(dx), _dir(dir),
                                      stable_dt_out.data_ptr() if stable_dt_out is not None else None))
+
+
+def godunov_sweep(flux_method: FluxMethod, cons_in: torch.Tensor,
+                  cons_out: torch.Tensor | None, dt: float, dx: float, dir,
+                  guess: torch.Tensor | None = None,
+                  stable_dt_out: torch.Tensor | None = None) -> None:
+    """First-order exact-Godunov fused sweep; `guess` (float64, direction
+    dir's face box, x fastest) seeds the Newton star-pressure iteration."""
+    D = flux_method.equation.dim
+    _check_cons(cons_in, D)
+    n = (cons_in.shape[3], cons_in.shape[2], cons_in.shape[1])
+    pin = L.ptr_array([cons_in[c].data_ptr() for c in range(D + 2)])
+    pout = None
+    if cons_out is not None:
+        _check_cons(cons_out, D)
+        pout = L.ptr_array([cons_out[c].data_ptr() for c in range(D + 2)])
+    if guess is not None and (guess.dtype != torch.float64 or not guess.is_contiguous()):
+        raise ValueError("guess must be a contiguous float64 tensor")
+    L.check(L.load().fvs_godunov_sweep(
+        flux_method.handle, pin, pout, L.i64x3(n), float(dt), float(dx), _dir(dir),
+        guess.data_ptr() if guess is not None else None,
+        stable_dt_out.data_ptr() if stable_dt_out is not None else None))
 
 
 def patch_stable_dt(flux_method: FluxMethod, cons: torch.Tensor, dx) -> torch.Tensor:

@@ -222,6 +222,87 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons,
                         const int64_t n[3], const double dx[3],
                         double* dt_dir_device);
 
+/* ---- exact Godunov -------------------------------------------------------- */
+
+/* First-order exact-Godunov split sweep (FluxMethod<Godunov<PerfectGas<D>,
+ * ExactRiemannSolver>>, include/fub/flux_method/GodunovMethod.hpp:130-170,
+ * src/equations/perfect_gas/ExactRiemannSolver.hpp:34-353) on a ghost-free
+ * transmissive patch, fused with its CFL check.  One FP64 Newton iteration
+ * per face serves both the flux and the face's signal speeds.
+ * guess_device (nullable): per-face initial star pressure in direction dir's
+ * face box (n with n[dir] + 1, x fastest); NULL uses 0.5 (pL + pR) as the
+ * reference (ExactRiemannSolver.hpp:91).  cons_out may be NULL (CFL only).
+ * Newton iterations are added to the handle's counters (fvs_newton_stats). */
+int fvs_godunov_sweep(fvs_handle* h, const double* const* cons_in,
+                      double* const* cons_out, const int64_t n[3], double dt,
+                      double dx, int dir, const double* guess_device,
+                      double* stable_dt_device);
+
+/* ExactRiemannSolver::ComputeMiddleState (ExactRiemannSolver.hpp:34-97) on n
+ * complete state pairs (device SoA (D+4) x n) along dir, Newton seeded with
+ * guess[i] (nullable: 0.5 (pL + pR)).  Outputs (each nullable): star pressure
+ * and velocity, Newton iterations, converged flag. */
+int fvs_exact_middle_state(fvs_handle* h, int dir, const double* left,
+                           const double* right, int64_t n, const double* guess,
+                           double* pM, double* uM, int* iterations, int* converged);
+
+/* Copies the handle's Newton counters to the host: stats[0] total
+ * iterations, [1] largest per-face count, [2] faces that hit the iteration
+ * cap (the reference throws there, NewtonIteration.hpp:44-50), [3] faces,
+ * [4] faces whose learned guess did not converge and were redone from the
+ * reference's guess.  reset != 0 zeroes them afterwards.  Synchronises. */
+int fvs_newton_stats(fvs_handle* h, uint64_t stats[5], int reset);
+
+/* ---- learned Newton initial guess (6 -> 64 -> 64 -> 1 MLP) ----------------
+ * No counterpart in the reference: the network predicts
+ * r = ln p* - (ln pL + ln pR) / 2 from (ln rhoL, uL/5, ln pL, ln rhoR, uR/5,
+ * ln pR); the exact solver's Newton iteration starts from
+ * sqrt(pL pR) exp(r) and is otherwise unchanged.  Inference runs on the
+ * matrix cores (BF16 operands, FP32 accumulation); training is FP32. */
+typedef struct fvs_mlp fvs_mlp;
+int fvs_mlp_create(void* hip_stream, uint64_t seed, fvs_mlp** out);
+int fvs_mlp_destroy(fvs_mlp* mlp);
+int fvs_mlp_num_params(void);
+int fvs_mlp_get_params(fvs_mlp* mlp, float* host_params);
+int fvs_mlp_set_params(fvs_mlp* mlp, const float* host_params);
+/* Generates n labelled pairs on the device: rho, p log-uniform in
+ * [1e-3, 1e3], u uniform in [-5, 5]; label = FP64 Newton star pressure.
+ * counts[0] usable, [1] vacuum (rejected), [2] not converged (rejected). */
+int fvs_mlp_generate(fvs_mlp* mlp, int64_t n, uint64_t seed, double gamma,
+                     int64_t counts[3]);
+/* Copies the generated set into caller device buffers (each nullable):
+ * prim (6 x n: rhoL uL pL rhoR uR pR), features (6 x n), target r (n),
+ * flag (n), Newton iterations from the reference guess (n); *n gets the
+ * sample count. */
+int fvs_mlp_dataset(fvs_mlp* mlp, int64_t* n, double* prim, float* features,
+                    float* target, int* flag, int* iterations);
+/* FP32 forward of n feature vectors (device, 6 x n) into y (device, n). */
+int fvs_mlp_forward(fvs_mlp* mlp, const float* features, int64_t n, float* y);
+/* Loss over flag == 0 samples (mean of e^2, or with huber > 0 of the Huber
+ * form 2 huber |e| - huber^2 beyond |e| > huber) and its gradient
+ * (fvs_mlp_num_params floats, host) on one device batch. */
+int fvs_mlp_gradients(fvs_mlp* mlp, const float* features, const float* target,
+                      const int* flag, int64_t n, float huber, float* grad_host,
+                      float* loss_host);
+/* Adam on the generated set, `epochs` passes of minibatches of `batch`,
+ * cosine-decayed learning rate from lr, Huber threshold huber (0: MSE). */
+int fvs_mlp_train(fvs_mlp* mlp, int epochs, int64_t batch, float lr, float huber,
+                  uint64_t seed, float* loss_history);
+/* Matrix-core inference on complete state pairs (device SoA (D+4) x n):
+ * network output r (nullable) and initial guess (nullable).  Weak pairs
+ * (|pL - pR| and |uR - uL| (rhoL + rhoR)(aL + aR) / 4 both below
+ * 0.1 (pL + pR)) keep the mean-pressure guess. */
+int fvs_mlp_infer_pairs(fvs_mlp* mlp, int dim, int dir, const double* left,
+                        const double* right, int64_t n, float* r_out, double* guess);
+/* Initial guesses for every face of direction dir of a ghost-free
+ * transmissive patch, in the face-box layout fvs_godunov_sweep reads. */
+int fvs_mlp_face_guess(fvs_mlp* mlp, int dim, double gamma, const double* const* cons,
+                       const int64_t n[3], int dir, double* guess);
+/* Attaches a network to a first-order exact-Godunov handle: the level
+ * driver then seeds every Newton iteration (fluxes and CFL) with it.
+ * NULL detaches.  The handle does not own the network. */
+int fvs_set_mlp(fvs_handle* h, fvs_mlp* mlp);
+
 #ifdef __cplusplus
 }
 #endif

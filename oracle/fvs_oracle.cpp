@@ -191,8 +191,11 @@ int StableDt(const Method& m, const fvs_view* states, double dx, int dir, double
   int64_t lo[3], hi[3];
   BoxOf(states, lo, hi);
   // Faces: the pairs of every stencil whose cells lie in the box
-  // (Shrink(states, dir, {i, S-1-i}), FluxMethod.hpp:346-352).
-  hi[dir] -= (S - 1);
+  // (Shrink(states, dir, {i, S-1-i}), FluxMethod.hpp:346-352).  The
+  // sequential driver shrinks by the full stencil size instead
+  // (Shrink(Box<0>(states), dir, {0, StencilSize}), FluxMethod.hpp:299) and
+  // so skips the box's last face.
+  hi[dir] -= m.seq ? S : (S - 1);
   double min_dt = std::numeric_limits<double>::infinity();
 #pragma omp parallel for collapse(2) schedule(static) reduction(min : min_dt)
   for (int64_t k = lo[2]; k <= hi[2]; ++k) {
@@ -298,7 +301,10 @@ template <int D> struct Level {
   int64_t ncell_g;
 
   Level(const fvs_method_desc* d, const fvs_patch_desc* p) : m(d) {
-    gw = m.width();
+    // Two stencil widths of ghosts: enough that every face of the patch and
+    // its boundary (a cell and its transmissive copy) lies inside the CFL
+    // box for both the simd and the sequential face ranges.
+    gw = 2 * m.width();
     for (int i = 0; i < 3; ++i) {
       n[i] = p->n[i];
       dx[i] = p->dx[i];

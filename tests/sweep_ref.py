@@ -32,7 +32,9 @@ def padded_complete(cons: np.ndarray, D: int, gw: int, gamma: float = 1.4):
 
 def oracle_sweep(desc: L.MethodDesc, cons: np.ndarray, dt: float, dx: float, direction: int):
     D = desc.dim
-    gw = 2 if desc.reconstruction == L.RECON_MUSCL_HANCOCK else 1
+    # Two stencil widths of ghosts, so that the sequential driver's CFL face
+    # range (which skips the box's last face) still covers every face.
+    gw = 2 * (2 if desc.reconstruction == L.RECON_MUSCL_HANCOCK else 1)
     comp, g = padded_complete(cons, D, gw, desc.gamma)
     nz, ny, nx = cons.shape[1:]
     n = [nx, ny, nz]
