@@ -33,6 +33,8 @@ GUESS_MEAN_PRESSURE = 0
 GUESS_LEARNED = 1
 SPLIT_GODUNOV = 0
 SPLIT_STRANG = 1
+BOUNDARY_EXACT = 0
+BOUNDARY_HLLEM = 1
 
 
 class MethodDesc(C.Structure):
@@ -47,6 +49,7 @@ class MethodDesc(C.Structure):
         ("newton_guess", C.c_int),
         ("max_newton_iterations", C.c_int),
         ("newton_tolerance", C.c_double),
+        ("boundary_solver", C.c_int),
     ]
 
 
@@ -58,6 +61,20 @@ class View(C.Structure):
         ("hi", C.c_int64 * 3),
         ("stride", C.c_int64 * 3),
         ("comp_stride", C.c_int64),
+    ]
+
+
+class CutCellData(C.Structure):
+    """fvs_cutcell_data: CutCellData<Rank> as views (include/fvs.h)."""
+    _fields_ = [
+        ("volume_fractions", View),
+        ("face_fractions", View * 3),
+        ("boundary_normals", View),
+        ("boundary_centroids", View),
+        ("unshielded", View * 3),
+        ("shielded_left", View * 3),
+        ("shielded_right", View * 3),
+        ("doubly_shielded", View * 3),
     ]
 
 
@@ -111,6 +128,21 @@ SIGNATURES = {
     "fvs_mlp_infer_pairs": (_I, [_P, _I, _I, _P, _P, C.c_int64, _P, _P]),
     "fvs_mlp_face_guess": (_I, [_P, _I, _D, C.POINTER(_P), _PI64, _I, _P]),
     "fvs_set_mlp": (_I, [_P, _P]),
+    "fvs_kbn_fill_cutcell_data": (_I, [_P, _P, _I, _PI64, _PI64]),
+    "fvs_kbn_is_cut_cell": (_I, [_P, _P, _PI64, _PI64, _P]),
+    "fvs_kbn_reference_states": (_I, [_P, C.POINTER(View), C.POINTER(View), _P]),
+    "fvs_kbn_boundary_fluxes": (_I, [_P, C.POINTER(View), C.POINTER(View), C.POINTER(View), _P,
+                                     _D, _D, _I]),
+    "fvs_kbn_regular_fluxes": (_I, [_P, C.POINTER(View), C.POINTER(View), _P, _D, _D, _I]),
+    "fvs_kbn_stable_dt": (_I, [_P, C.POINTER(View), _P, _D, _I, _P]),
+    "fvs_kbn_cutcell_fluxes": (_I, [_P, C.POINTER(View), C.POINTER(View), C.POINTER(View),
+                                    C.POINTER(View), C.POINTER(View), _P, _D, _D, _I]),
+    "fvs_kbn_update_conservatively": (_I, [_P, C.POINTER(View), C.POINTER(View), C.POINTER(View),
+                                           C.POINTER(View), C.POINTER(View), C.POINTER(View), _P,
+                                           _D, _D, _I]),
+    "fvs_kbn_update_conservatively2": (_I, [_P, C.POINTER(View), C.POINTER(View), C.POINTER(View),
+                                            C.POINTER(View), _P, _D, _D, _I]),
+    "fvs_kbn_complete_from_cons": (_I, [_P, C.POINTER(View), C.POINTER(View), _P]),
 }
 
 _lib = None

@@ -3,6 +3,7 @@
 // sweeps.
 #include "fvs_internal.hpp"
 #include "fvs_physics.hpp"
+#include "fvs_views.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -25,28 +26,6 @@ void SetError(const char* fmt, ...) {
   va_end(ap);
   g_last_error = buf;
 }
-
-namespace {
-
-constexpr int kBlock = 256;
-
-struct DevView {
-  double* data;
-  int64_t lo[3];
-  int64_t ext[3];
-  int64_t stride[3];
-  int64_t comp_stride;
-  __device__ __forceinline__ double* at(int64_t i, int64_t j, int64_t k, int c) const {
-    return data + (i - lo[0]) * stride[0] + (j - lo[1]) * stride[1] +
-           (k - lo[2]) * stride[2] + c * comp_stride;
-  }
-  __device__ __forceinline__ int64_t count() const { return ext[0] * ext[1] * ext[2]; }
-  __device__ __forceinline__ void index(int64_t q, int64_t* idx) const {
-    idx[0] = lo[0] + q % ext[0];
-    idx[1] = lo[1] + (q / ext[0]) % ext[1];
-    idx[2] = lo[2] + q / (ext[0] * ext[1]);
-  }
-};
 
 int MakeDevView(const fvs_view* v, int min_comp, DevView* out) {
   if (!v || !v->data) {
@@ -77,6 +56,10 @@ bool ContainsBox(const fvs_view* v, const int64_t lo[3], const int64_t hi[3]) {
   return true;
 }
 
+namespace {
+
+constexpr int kBlock = 256;
+
 int CheckDesc(const fvs_method_desc* d) {
   if (!d) {
     SetError("null method description");
@@ -93,7 +76,8 @@ int CheckDesc(const fvs_method_desc* d) {
   if (d->base < 0 || d->base > FVS_BASE_GODUNOV_EXACT || d->reconstruction < 0 ||
       d->reconstruction > FVS_RECON_MUSCL_HANCOCK || d->limiter < 0 ||
       d->limiter > FVS_LIMITER_NO_GRADIENT || d->precision < 0 ||
-      d->precision > FVS_FP32_FLUX) {
+      d->precision > FVS_FP32_FLUX || d->boundary_solver < FVS_BOUNDARY_EXACT ||
+      d->boundary_solver > FVS_BOUNDARY_HLLEM) {
     SetError("method enum out of range");
     return FVS_BAD_ARGUMENT;
   }

@@ -11,6 +11,7 @@
 // (iterations, non-converged faces) are accumulated per workgroup.
 #include "fvs_internal.hpp"
 #include "fvs_riemann.hpp"
+#include "fvs_views.hpp"
 
 #include <cstring>
 
@@ -311,21 +312,6 @@ __global__ __launch_bounds__(kRedBlock) void GodunovFinalize(const unsigned long
 
 // Per-face Godunov flux on strided views (FluxMethod seq driver): first
 // order, or MUSCL-Hancock with the sequential (scalar) CompleteFromCons.
-struct DevView {
-  double* data;
-  int64_t lo[3], ext[3], stride[3], comp_stride;
-  __device__ __forceinline__ double* at(const int64_t* c, int k) const {
-    return data + (c[0] - lo[0]) * stride[0] + (c[1] - lo[1]) * stride[1] +
-           (c[2] - lo[2]) * stride[2] + k * comp_stride;
-  }
-  __host__ __device__ __forceinline__ int64_t count() const { return ext[0] * ext[1] * ext[2]; }
-  __device__ __forceinline__ void index(int64_t q, int64_t* idx) const {
-    idx[0] = lo[0] + q % ext[0];
-    idx[1] = lo[1] + (q / ext[0]) % ext[1];
-    idx[2] = lo[2] + q / (ext[0] * ext[1]);
-  }
-};
-
 template <int D, int DIR, int LIM>
 __device__ __forceinline__ void ReconScalarKE(const double* um, const double* u0,
                                               const double* up, double lh,
@@ -724,17 +710,7 @@ int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* c
   return FVS_OK;
 }
 
-static int MakeGView(const fvs_view* v, DevView* out) {
-  out->data = v->data;
-  for (int d = 0; d < 3; ++d) {
-    out->lo[d] = v->lo[d];
-    out->ext[d] = v->hi[d] - v->lo[d] + 1;
-    out->stride[d] = v->stride[d];
-    if (out->ext[d] <= 0) return FVS_BAD_ARGUMENT;
-  }
-  out->comp_stride = v->comp_stride;
-  return FVS_OK;
-}
+static int MakeGView(const fvs_view* v, DevView* out) { return MakeDevView(v, 1, out); }
 
 int LaunchGodunovViewFlux(const fvs_method_desc& desc, hipStream_t s, const fvs_view* fluxes,
                           const fvs_view* states, double dt, double dx, int dir,

@@ -123,7 +123,7 @@ _LIMITERS = {"MinMod": L.LIMITER_MINMOD, "VanLeer": L.LIMITER_VANLEER,
 def make_desc(equation: PerfectGas, base="HLLEM", reconstruction="MusclHancock",
               limiter="MinMod", precision="fp64", quirks=True,
               max_newton_iterations=5000, newton_tolerance=1e-7,
-              newton_guess="mean") -> L.MethodDesc:
+              newton_guess="mean", boundary_solver="exact") -> L.MethodDesc:
     d = L.MethodDesc()
     d.dim = equation.dim
     d.gamma = equation.gamma
@@ -136,6 +136,8 @@ def make_desc(equation: PerfectGas, base="HLLEM", reconstruction="MusclHancock",
     d.newton_guess = L.GUESS_LEARNED if newton_guess in ("learned", 1) else L.GUESS_MEAN_PRESSURE
     d.max_newton_iterations = max_newton_iterations
     d.newton_tolerance = newton_tolerance
+    d.boundary_solver = (L.BOUNDARY_HLLEM if boundary_solver in ("hllem", "HLLEM", 1)
+                         else L.BOUNDARY_EXACT)
     return d
 
 

@@ -90,7 +90,17 @@ typedef struct fvs_method_desc {
   int newton_guess;   /* enum fvs_newton_guess */
   int max_newton_iterations; /* 5000 in the reference */
   double newton_tolerance;   /* 1e-7 in the reference (absolute |f|) */
+  int boundary_solver;       /* enum fvs_boundary_solver (cut-cell methods) */
 } fvs_method_desc;
+
+/* Riemann solver of the cut-cell boundary problems, the RiemannSolver
+ * template argument of KbnCutCellMethod
+ * (include/fub/cutcell_method/KbnStabilisation.hpp:53-55).
+ * FVS_BOUNDARY_EXACT: ExactRiemannSolver (the template default).
+ * FVS_BOUNDARY_HLLEM: Hllem::SolveRiemannProblem with the handle's
+ *   Larrouturou setting (src/equations/perfect_gas/HllemMethod.hpp:25-369),
+ *   as examples/AMReX/EB/2D/Ramp.cpp:92-95 constructs it. */
+enum fvs_boundary_solver { FVS_BOUNDARY_EXACT = 0, FVS_BOUNDARY_HLLEM = 1 };
 
 /* Strided view over a box of cells or faces, the analogue of
  * PatchDataView<double, R+1, layout_stride>.  Element (i, j, k, c) lives at
@@ -105,6 +115,30 @@ typedef struct fvs_view {
   int64_t stride[3];
   int64_t comp_stride;
 } fvs_view;
+
+/* Embedded-boundary geometry of one patch, CutCellData<Rank>
+ * (include/fub/CutCellData.hpp:48-75), as views over cells or faces:
+ *   volume_fractions     cells, 1 component (alpha)
+ *   face_fractions[d]    faces normal to d, 1 component (beta)
+ *   boundary_normals     cells, D components, AMReX orientation (pointing
+ *                        into the body; GetBoundaryNormal negates it,
+ *                        src/CutCellData.cpp:90-105)
+ *   boundary_centroids   cells, D components, cell-relative in [-1/2, 1/2]
+ *   unshielded[d], shielded_left[d], shielded_right[d], doubly_shielded[d]
+ *                        faces normal to d, 2 components: 0 the absolute
+ *                        fraction, 1 the fraction relative to beta, the
+ *                        layout of src/AMReX/cutcell/PatchHierarchy.cpp:395-431
+ * Views of unused directions may be empty (data NULL). */
+typedef struct fvs_cutcell_data {
+  fvs_view volume_fractions;
+  fvs_view face_fractions[3];
+  fvs_view boundary_normals;
+  fvs_view boundary_centroids;
+  fvs_view unshielded[3];
+  fvs_view shielded_left[3];
+  fvs_view shielded_right[3];
+  fvs_view doubly_shielded[3];
+} fvs_cutcell_data;
 
 typedef struct fvs_handle fvs_handle;
 
@@ -302,6 +336,81 @@ int fvs_mlp_face_guess(fvs_mlp* mlp, int dim, double gamma, const double* const*
  * driver then seeds every Newton iteration (fluxes and CFL) with it.
  * NULL detaches.  The handle does not own the network. */
 int fvs_set_mlp(fvs_handle* h, fvs_mlp* mlp);
+
+/* ---- KBN h-box cut-cell method (KbnCutCellMethod) ---------------------- */
+
+/* FillCutCellData (src/AMReX/cutcell/FillCutCellData.cpp:28-118): for every
+ * face of the box [face_lo, face_hi] normal to dir, the unshielded,
+ * shielded-from-left, shielded-from-right and doubly shielded fractions
+ * (absolute and relative to beta) from the face fractions of the face and
+ * its two neighbours along dir.  Writes geom->unshielded[dir] and friends. */
+int fvs_kbn_fill_cutcell_data(fvs_handle* h, const fvs_cutcell_data* geom, int dir,
+                              const int64_t face_lo[3], const int64_t face_hi[3]);
+
+/* IsCutCell (src/CutCellData.cpp:225-256) for every cell of [lo, hi]:
+ * flags (device int32, x fastest over the box) get 1 for cut cells. */
+int fvs_kbn_is_cut_cell(fvs_handle* h, const fvs_cutcell_data* geom, const int64_t lo[3],
+                        const int64_t hi[3], int32_t* flags);
+
+/* KbnCutCellMethod::PreAdvanceHierarchy (KbnStabilisation.hpp:188-210):
+ * for every cut cell of states' box with positive density, the reference
+ * state from the boundary Riemann problem (state reflected at the wall
+ * against itself, in the frame rotated onto the boundary normal). */
+int fvs_kbn_reference_states(fvs_handle* h, const fvs_view* references,
+                             const fvs_view* states, const fvs_cutcell_data* geom);
+
+/* KbnCutCellMethod::ComputeBoundaryFluxes (KbnStabilisation.hpp:212-273),
+ * single-block path (reference masks 0): for every cut cell of references'
+ * box, F_B = F_dir(reference) + (p_boundary - p_reference) e_dir.  Other
+ * cells of boundary_fluxes are left untouched. */
+int fvs_kbn_boundary_fluxes(fvs_handle* h, const fvs_view* boundary_fluxes,
+                            const fvs_view* states, const fvs_view* references,
+                            const fvs_cutcell_data* geom, double dt, double dx, int dir);
+
+/* KbnCutCellMethod::ComputeRegularFluxes (KbnStabilisation.hpp:347-429):
+ * the masked flux method over the faces of fluxes' box; faces with beta = 0
+ * get a zero flux, slopes next to covered cells are zero. */
+int fvs_kbn_regular_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* states,
+                           const fvs_cutcell_data* geom, double dt, double dx, int dir);
+
+/* KbnCutCellMethod::ComputeStableDt (KbnStabilisation.hpp:277-345): minimum
+ * of the masked stable dt over the faces of states' box shrunk by the
+ * stencil; written to *dt_device. */
+int fvs_kbn_stable_dt(fvs_handle* h, const fvs_view* states, const fvs_cutcell_data* geom,
+                      double dx, int dir, double* dt_device);
+
+/* KbnCutCellMethod::ComputeCutCellFluxes -> ComputeStableFluxComponents
+ * (src/cutcell_method/KbnStabilisation.cpp:55-184): stabilised and singly
+ * shielded fluxes over the faces of regular_fluxes' box. */
+int fvs_kbn_cutcell_fluxes(fvs_handle* h, const fvs_view* stabilised,
+                           const fvs_view* shielded_left, const fvs_view* shielded_right,
+                           const fvs_view* boundary_fluxes, const fvs_view* regular_fluxes,
+                           const fvs_cutcell_data* geom, double dt, double dx, int dir);
+
+/* cutcell::TimeIntegrator::UpdateConservatively
+ * (src/AMReX/cutcell/TimeIntegrator.cpp:65-411) in place on the cells of
+ * scratch's box (fluxes on the faces enclosing them), the h-box update with
+ * absolute shielded fractions; covered cells become 0. */
+int fvs_kbn_update_conservatively(fvs_handle* h, const fvs_view* scratch,
+                                  const fvs_view* unshielded, const fvs_view* stabilised,
+                                  const fvs_view* shielded_left, const fvs_view* shielded_right,
+                                  const fvs_view* boundary_fluxes,
+                                  const fvs_cutcell_data* geom, double dt, double dx, int dir);
+
+/* cutcell::TimeIntegrator2::UpdateConservatively
+ * (src/AMReX/cutcell/TimeIntegrator.cpp:413-508): dest = src + dt/(dx alpha)
+ * (betaL F_L - betaR F_R - (betaL - betaR) F_B) on cells with alpha > 0 and
+ * betaL != betaR, the regular update where betaL == betaR; covered cells are
+ * left untouched. */
+int fvs_kbn_update_conservatively2(fvs_handle* h, const fvs_view* dest, const fvs_view* src,
+                                   const fvs_view* stabilised, const fvs_view* boundary_fluxes,
+                                   const fvs_cutcell_data* geom, double dt, double dx, int dir);
+
+/* cutcell::Reconstruction::CompleteFromCons, SIMD kernel with volume mask
+ * (include/fub/AMReX/cutcell/Reconstruction.hpp:123-157): complete states
+ * over complete's box, the zero state where alpha = 0 or rho <= 0. */
+int fvs_kbn_complete_from_cons(fvs_handle* h, const fvs_view* complete, const fvs_view* cons,
+                               const fvs_cutcell_data* geom);
 
 #ifdef __cplusplus
 }

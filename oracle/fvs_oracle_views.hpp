@@ -1,0 +1,132 @@
+// TEST INFRASTRUCTURE ONLY: view access and per-face helpers shared by the
+// oracle's translation units (fvs_oracle.cpp, fvs_kbn_oracle.cpp).
+#pragma once
+
+#include "fvs_oracle.hpp"
+
+#include "../include/fvs.h"
+
+namespace fvs_oracle {
+struct ViewAcc {
+  const fvs_view* v;
+  double* at(int64_t i, int64_t j, int64_t k, int c) const {
+    return v->data + (i - v->lo[0]) * v->stride[0] + (j - v->lo[1]) * v->stride[1] +
+           (k - v->lo[2]) * v->stride[2] + c * v->comp_stride;
+  }
+};
+
+inline void Shift(int64_t idx[3], int dir, int64_t by) { idx[dir] += by; }
+
+template <int D>
+void LoadState(State<D>& q, const ViewAcc& a, const int64_t idx[3]) {
+  for (int c = 0; c < D + 4; ++c) C(q, c) = *a.at(idx[0], idx[1], idx[2], c);
+}
+
+struct Method {
+  Gas gas;
+  int base;
+  int recon;
+  int limiter;
+  bool quirk3d;
+  double tol;
+  int max_it;
+  bool seq; // sequential (scalar) semantics: required for exact Godunov
+  explicit Method(const fvs_method_desc* d)
+      : gas(d->gamma), base(d->base), recon(d->reconstruction),
+        limiter(d->limiter), quirk3d(d->quirks != 0),
+        tol(d->newton_tolerance > 0 ? d->newton_tolerance : 1e-7),
+        max_it(d->max_newton_iterations > 0 ? d->max_newton_iterations : 5000),
+        seq(d->base == FVS_BASE_GODUNOV_EXACT) {}
+  int width() const { return recon == FVS_RECON_MUSCL_HANCOCK ? 2 : 1; }
+};
+
+// Base two-state flux (Hllem array path or Godunov scalar path).
+template <int D>
+void BaseFlux(const Method& m, Cons<D>& f, const State<D>& L, const State<D>& R,
+              int dir, int* newton_fail) {
+  if (m.base == FVS_BASE_GODUNOV_EXACT) {
+    State<D> sol{};
+    MiddleState ms = SolveRiemannProblem<D>(m.gas, sol, L, R, dir, -1.0, m.tol, m.max_it);
+    if (!ms.converged && newton_fail) *newton_fail = 1;
+    Flux<D>(f, sol, dir);
+  } else {
+    HllemFlux<D>(m.gas, f, L, R, dir, m.base == FVS_BASE_HLLEM_LARROUTUROU, m.quirk3d);
+  }
+}
+
+template <int D>
+double BaseStableDt(const Method& m, const State<D>& L, const State<D>& R, double dx,
+                    int dir) {
+  if (m.base == FVS_BASE_GODUNOV_EXACT) {
+    return GodunovStableDt<D>(m.gas, L, R, dx, dir, m.tol, m.max_it);
+  }
+  return HllemStableDt<D>(m.gas, L, R, dx, dir);
+}
+
+// MusclHancock::ComputeNumericFlux (MusclHancockMethod.hpp:294-356 scalar,
+// :419-485 array).  The only difference between the two on this path is the
+// kinetic-energy reduction order inside CompleteFromCons (3D).
+template <int D>
+void MusclHancockFlux(const Method& m, Cons<D>& flux, const State<D> st[4],
+                      double dt, double dx, int dir, int* newton_fail) {
+  const double lambda_half = 0.5 * dt / dx;
+  State<D> rec[2];
+  for (int side = 0; side < 2; ++side) {
+    const State<D>* s3 = st + side; // first<3>() or last<3>()
+    State<D> qL{}, qR{};
+    for (int k = 0; k < D + 2; ++k) {
+      const double slope = LimitedSlope(m.limiter, C(s3[0], k), C(s3[1], k), C(s3[2], k));
+      C(qL, k) = C(s3[1], k) - 0.5 * slope;
+      C(qR, k) = C(s3[1], k) + 0.5 * slope;
+    }
+    if (m.seq) {
+      CompleteFromConsScalar<D>(m.gas, qL);
+      CompleteFromConsScalar<D>(m.gas, qR);
+    } else {
+      CompleteFromConsArray<D>(m.gas, qL);
+      CompleteFromConsArray<D>(m.gas, qR);
+    }
+    Cons<D> fL, fR;
+    Flux<D>(fL, qL, dir);
+    Flux<D>(fR, qR, dir);
+    State<D>& r = rec[side];
+    const State<D>& base = side == 0 ? qR : qL;
+    for (int k = 0; k < D + 2; ++k) {
+      C(r, k) = C(base, k) + lambda_half * (C(fL, k) - C(fR, k));
+    }
+    if (m.seq) {
+      CompleteFromConsScalar<D>(m.gas, r);
+    } else {
+      CompleteFromConsArray<D>(m.gas, r);
+    }
+  }
+  BaseFlux<D>(m, flux, rec[0], rec[1], dir, newton_fail);
+}
+
+template <int D>
+void FaceFlux(const Method& m, Cons<D>& flux, const State<D>* st, double dt,
+              double dx, int dir, int* newton_fail) {
+  if (m.recon == FVS_RECON_MUSCL_HANCOCK) {
+    MusclHancockFlux<D>(m, flux, st, dt, dx, dir, newton_fail);
+  } else {
+    BaseFlux<D>(m, flux, st[0], st[1], dir, newton_fail);
+  }
+}
+
+template <int D>
+double FaceStableDt(const Method& m, const State<D>* st, double dx, int dir) {
+  // MusclHancock::ComputeStableDt uses the middle two cells (MusclHancockMethod.hpp:214-218).
+  if (m.recon == FVS_RECON_MUSCL_HANCOCK) {
+    return BaseStableDt<D>(m, st[1], st[2], dx, dir);
+  }
+  return BaseStableDt<D>(m, st[0], st[1], dx, dir);
+}
+
+inline void BoxOf(const fvs_view* v, int64_t lo[3], int64_t hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = v->lo[d];
+    hi[d] = v->hi[d];
+  }
+}
+
+} // namespace fvs_oracle
