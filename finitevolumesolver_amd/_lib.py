@@ -143,6 +143,13 @@ SIGNATURES = {
     "fvs_kbn_update_conservatively2": (_I, [_P, C.POINTER(View), C.POINTER(View), C.POINTER(View),
                                             C.POINTER(View), _P, _D, _D, _I]),
     "fvs_kbn_complete_from_cons": (_I, [_P, C.POINTER(View), C.POINTER(View), _P]),
+    "fvs_kbn_level_create": (_I, [_P, C.POINTER(PatchDesc), _P, _I, C.POINTER(_P)]),
+    "fvs_kbn_level_destroy": (_I, [_P]),
+    "fvs_kbn_level_ncut": (_I, [_P]),
+    "fvs_kbn_level_set_state": (_I, [_P, C.POINTER(_P), _D]),
+    "fvs_kbn_level_get_state": (_I, [_P, C.POINTER(_P)]),
+    "fvs_kbn_level_time": (_I, [_P, _PD, _PI64]),
+    "fvs_kbn_level_advance": (_I, [_P, _I, _D, _I, _PD, C.POINTER(_I), C.POINTER(_I)]),
 }
 
 _lib = None

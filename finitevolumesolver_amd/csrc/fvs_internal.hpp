@@ -123,6 +123,11 @@ int MlpFaceGuess(fvs_mlp* m, hipStream_t s, int dim, double gamma, const double*
 int LaunchMiddleState(int dim, hipStream_t s, const double* L, const double* R, int64_t n,
                       int dir, double gamma, double tol, int max_it, const double* guess,
                       double* pM, double* uM, int* iters, int* conv);
+// Reduces `dim` consecutive groups of nblocks partial minima into the
+// per-direction stable dts (out, nullable; ctl->dt_dir) and the step dt
+// with the first split step's CFL check.
+int LaunchFinalizeCfl(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
+                      int dim, double* out, Ctl* ctl, int first_dir, int strang);
 int LaunchStepEnd(hipStream_t s, Ctl* ctl, double* dt_log, int64_t dt_log_base,
                   int dt_log_len);
 int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang);

@@ -21,7 +21,11 @@
 #include "fvs_riemann.hpp"
 #include "fvs_views.hpp"
 
+#include <algorithm>
 #include <cfloat>
+#include <cstring>
+#include <new>
+#include <vector>
 
 namespace fvs {
 
@@ -1138,6 +1142,740 @@ int fvs_kbn_complete_from_cons(fvs_handle* h, const fvs_view* complete, const fv
     hipLaunchKernelGGL(KbnCompleteKernel<3>, dim3(Blocks(cv.count())), dim3(kBlock), 0, h->stream, cv, uv, alpha, g);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Single-patch 2D cut-cell level: the sequence of the reference's
+// DimensionalSplitLevelIntegrator with the cut-cell IntegratorContext
+// (reference states per level advance, then per split step the boundary
+// fill, KBN CFL check, boundary/regular/stabilised fluxes, cut-cell update
+// and masked reconstruction).  The state is stored as conservative
+// variables with two ghost layers; complete states are recomputed from them
+// with the masked reconstruction wherever a kernel needs them (the values
+// the reference stores).  Kernels per split step:
+//   ghost fill (transmissive, along the sweep direction)
+//   boundary fluxes over the list of cut cells (a dense zero-initialised
+//     array; only cut cells are ever written)
+//   regular fluxes + masked CFL of every face, with per-workgroup minima
+//   h-box update of every interior cell, the stabilised fluxes of its two
+//     faces recomputed from the regular fluxes and the face fractions
+namespace fvs {
+namespace {
+
+constexpr int kG = 2;  // ghost layers
+
+struct LevelGeom {
+  const double* alpha;      // [ngy][ngx]
+  const double* beta[2];    // x: [ngy][ngx + 1], y: [ngy + 1][ngx]
+  const double* normals;    // [2][ngy][ngx] (AMReX orientation)
+  const double* centroids;  // [2][ngy][ngx]
+  int64_t ngx, ngy;
+  __device__ __forceinline__ int64_t cell(int64_t i, int64_t j) const {
+    return (j + kG) * ngx + (i + kG);
+  }
+  // face f of direction d at transverse index t (interior coordinates)
+  __device__ __forceinline__ double beta_at(int d, int64_t i, int64_t j) const {
+    return d == 0 ? beta[0][(j + kG) * (ngx + 1) + (i + kG)]
+                  : beta[1][(j + kG) * ngx + (i + kG)];
+  }
+};
+
+struct LevelState {
+  double* u[4];
+};
+
+__device__ __forceinline__ void LoadCons(const LevelState& s, int64_t c, double* q) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = s.u[k][c];
+}
+
+__global__ void GhostFillKernel(LevelState st, int64_t nx, int64_t ny, int64_t ngx, int dir) {
+  // one thread per (ghost layer pair, transverse grown index)
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t nt = dir == 0 ? ny + 2 * kG : nx + 2 * kG;
+  if (q >= nt * 2 * kG) return;
+  const int64_t t = q % nt - kG;
+  const int g = static_cast<int>(q / nt);  // 0..2G-1
+  const int64_t n = dir == 0 ? nx : ny;
+  const int64_t dst = g < kG ? -(g + 1) : n + (g - kG);
+  const int64_t src = g < kG ? 0 : n - 1;
+  int64_t di, dj, si, sj;
+  if (dir == 0) {
+    di = dst, si = src, dj = sj = t;
+  } else {
+    dj = dst, sj = src, di = si = t;
+  }
+  const int64_t cd = (dj + kG) * ngx + (di + kG);
+  const int64_t cs = (sj + kG) * ngx + (si + kG);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) st.u[k][cd] = st.u[k][cs];
+}
+
+__device__ __forceinline__ unsigned long long BlockMinKey(unsigned long long k) {
+  __shared__ unsigned long long s[kBlock / 64];
+  k = WaveMinKey(k);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = k;
+  __syncthreads();
+  unsigned long long m = s[0];
+  for (int w = 1; w < kBlock / 64; ++w) m = s[w] < m ? s[w] : m;
+  return m;
+}
+
+// Complete state of interior/ghost cell c with the volume mask.
+__device__ __forceinline__ void CellComplete(const LevelState& st, const LevelGeom& g, int64_t c,
+                                             const Gas<double>& gas, double* q) {
+  LoadCons(st, c, q);
+  CompleteMasked<2>(q, g.alpha[c] > 0.0, gas);
+}
+
+// Masked HLLEM CFL of the faces 0..n of direction DIR (interior rows); the
+// grid is padded to `nblocks` workgroups so that every direction writes the
+// same number of partial minima.
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void LevelDtKernel(LevelState st, LevelGeom g, int64_t nx,
+                                                        int64_t ny, double dx, Gas<double> gas,
+                                                        unsigned long long* partials,
+                                                        const int* halt) {
+  if (*halt) return;
+  const int64_t fx = nx + (DIR == 0), fy = ny + (DIR == 1);
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  unsigned long long key = 0x7ff0000000000000ull;
+  if (q < fx * fy) {
+    const int64_t i = q % fx, j = q / fx;
+    double dt = DBL_MAX;
+    if (g.beta_at(DIR, i, j) > 0.0) {
+      double L[6], R[6];
+      CellComplete(st, g, g.cell(i - (DIR == 0), j - (DIR == 1)), gas, L);
+      CellComplete(st, g, g.cell(i, j), gas, R);
+      dt = HllemStableDt<2, DIR>(L, R, dx, gas);
+    }
+    key = DtKey(dt);
+  }
+  const unsigned long long m = BlockMinKey(key);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// Regular (masked MUSCL-Hancock + HLLEM) flux of every face 0..n of DIR and
+// its masked CFL value (input state of the sweep).
+template <int DIR, int LIM, bool LARR>
+__global__ __launch_bounds__(kBlock) void LevelFluxKernel(LevelState st, LevelGeom g, int64_t nx,
+                                                          int64_t ny, double dx,
+                                                          const double* dt_ptr, Gas<double> gas,
+                                                          double* F, unsigned long long* partials,
+                                                          const int* halt) {
+  if (*halt) return;
+  const int64_t fx = nx + (DIR == 0), fy = ny + (DIR == 1);
+  const int64_t nf = fx * fy;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  unsigned long long key = 0x7ff0000000000000ull;
+  if (q < nf) {
+    const int64_t i = q % fx, j = q / fx;
+    const double beta = g.beta_at(DIR, i, j);
+    double out[4] = {0.0, 0.0, 0.0, 0.0};
+    double dt = DBL_MAX;
+    if (beta > 0.0) {
+      double u[4][6];
+      double a[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int64_t c = g.cell(i + (DIR == 0) * (s - 2), j + (DIR == 1) * (s - 2));
+        CellComplete(st, g, c, gas, u[s]);
+        a[s] = g.alpha[c];
+      }
+      dt = HllemStableDt<2, DIR>(u[1], u[2], dx, gas);
+      const bool mL = a[0] > 0.0 && a[1] > 0.0 && a[2] > 0.0;
+      const bool mR = a[1] > 0.0 && a[2] > 0.0 && a[3] > 0.0;
+      const double lh = 0.5 * (*dt_ptr) / dx;
+      double L[6], R[6], dummy[6];
+      MaskedRecon<2, DIR, LIM>(u[0], u[1], u[2], mL, lh, gas, dummy, L);
+      MaskedRecon<2, DIR, LIM>(u[1], u[2], u[3], mR, lh, gas, R, dummy);
+      HllemFlux<2, DIR, LARR, false>(out, L, R, gas);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k * nf + q] = out[k];
+    key = DtKey(dt);
+  }
+  const unsigned long long m = BlockMinKey(key);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// Reference states of the cut cells (list) from the step-start state.
+__global__ __launch_bounds__(kBlock) void LevelRefsKernel(LevelState st, LevelGeom g,
+                                                          const int32_t* cut, int ncut,
+                                                          BoundaryParams bp, double* refs,
+                                                          const int* halt) {
+  if (*halt) return;
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= ncut) return;
+  const int64_t c = cut[q];
+  double state[6];
+  CellComplete(st, g, c, bp.g, state);
+  if (!(state[0] > 0.0)) return;  // references keep their previous value
+  const int64_t plane = g.ngx * g.ngy;
+  double normal[2] = {-g.normals[c], -g.normals[plane + c]};
+  double sol[6];
+  WallProblem<2>(sol, state, normal, bp);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) refs[k * ncut + q] = sol[k];
+}
+
+// Boundary fluxes of the cut cells (list) from the sweep's input state.
+__global__ __launch_bounds__(kBlock) void LevelBoundaryFluxKernel(LevelState st, LevelGeom g,
+                                                                  const int32_t* cut, int ncut,
+                                                                  BoundaryParams bp,
+                                                                  const double* refs, double* fb,
+                                                                  int dir, const int* halt) {
+  if (*halt) return;
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= ncut) return;
+  const int64_t c = cut[q];
+  double state[6], ref[6];
+  CellComplete(st, g, c, bp.g, state);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) ref[k] = refs[k * ncut + q];
+  const int64_t plane = g.ngx * g.ngy;
+  double normal[2] = {-g.normals[c], -g.normals[plane + c]};
+  double flux[4];
+  if (dir == 0) PhysFlux<2, 0>(flux, ref);
+  else PhysFlux<2, 1>(flux, ref);
+  double sol[6];
+  WallProblem<2>(sol, state, normal, bp);
+  flux[1 + dir] += sol[4] - ref[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) fb[k * plane + c] = flux[k];
+}
+
+// Face fractions of face i (absolute and relative) from the face fractions
+// of faces i - 1, i, i + 1 (FillCutCellData__ arithmetic).
+struct FaceFrac {
+  double us, sl, sr, us_rel, sl_rel, sr_rel;
+};
+__device__ __forceinline__ FaceFrac Fractions(double bl, double bm, double br) {
+  FaceFrac f;
+  const double dBetaL = rmax(0.0, bm - bl);
+  const double dBetaR = rmax(0.0, bm - br);
+  f.us = rmax(0.0, bm - rmax(dBetaL, dBetaR));
+  const double doubly = rmin(dBetaL, dBetaR);
+  f.sl = dBetaL - doubly;
+  f.sr = dBetaR - doubly;
+  const double b = bm > 0 ? bm : 1.0;
+  f.us_rel = f.us / b;
+  f.sl_rel = f.sl / b;
+  f.sr_rel = f.sr / b;
+  return f;
+}
+
+// h-box update (TimeIntegrator, or TimeIntegrator2 when `second`) of every
+// interior cell; output buffer `out` (interior only).
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, LevelState out,
+                                                            LevelGeom g, int64_t nx, int64_t ny,
+                                                            const double* F, const double* fb,
+                                                            const double* dt_ptr, double dx,
+                                                            int second, const int* halt) {
+  if (*halt) return;
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= nx * ny) return;
+  const int64_t i = q % nx, j = q / nx;
+  const int64_t c = g.cell(i, j);
+  const double alpha = g.alpha[c];
+  if (alpha == 0.0) {
+    // covered cells: zero after the masked reconstruction (both integrators)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out.u[k][c] = 0.0;
+    return;
+  }
+  const double dt_over_dx = (*dt_ptr) / dx;
+  const int64_t fx = nx + (DIR == 0), fy = ny + (DIR == 1), nf = fx * fy;
+  const int64_t fL = j * fx + i;                 // face i (left) of cell (i, j)
+  const int64_t fR = fL + (DIR == 0 ? 1 : fx);   // face i + 1 (right)
+  const int di = DIR == 0, dj = DIR == 1;
+  const double bLL = g.beta_at(DIR, i - di, j - dj);
+  const double bL = g.beta_at(DIR, i, j);
+  const double bR = g.beta_at(DIR, i + di, j + dj);
+  const double bRR = g.beta_at(DIR, i + 2 * di, j + 2 * dj);
+  const FaceFrac fl = Fractions(bLL, bL, bR);
+  const FaceFrac fr = Fractions(bL, bR, bRR);
+  const int64_t plane = g.ngx * g.ngy;
+  const int64_t cl = g.cell(i - di, j - dj), cr = g.cell(i + di, j + dj);
+  const double* xc = g.centroids + DIR * plane;
+  const bool affL = fl.sl_rel != 0.0 || fl.sr_rel != 0.0;
+  const bool affR = fr.sl_rel != 0.0 || fr.sr_rel != 0.0;
+  // boundary-side blend weights of the two faces (ComputeStableFluxes_Row)
+  const double dL_l = affL ? Clamp01(0.5 - xc[cl]) : 0.5;
+  const double dR_l = affL ? Clamp01(0.5 + xc[c]) : 0.5;
+  const double dL_r = affR ? Clamp01(0.5 - xc[c]) : 0.5;
+  const double dR_r = affR ? Clamp01(0.5 + xc[cr]) : 0.5;
+  const double center = xc[c];
+  double res[4];
+  for (int k = 0; k < 4; ++k) {
+    const double Fl = F[k * nf + fL];
+    const double Fr = F[k * nf + fR];
+    const double fbl = affL ? fb[k * plane + cl] : 0.0;
+    const double fbc = (affL || affR || bL != bR) ? fb[k * plane + c] : 0.0;
+    const double fbr = affR ? fb[k * plane + cr] : 0.0;
+    // stabilised fluxes and the shielded ones of both faces
+    const double sL_l = dL_l * Fl + (1.0 - dL_l) * fbl;
+    const double sR_l = dR_l * Fl + (1.0 - dR_l) * fbc;
+    const double Fs_l = fl.us_rel * Fl + fl.sl_rel * sL_l + fl.sr_rel * sR_l;
+    const double sL_r = dL_r * Fr + (1.0 - dL_r) * fbc;
+    const double sR_r = dR_r * Fr + (1.0 - dR_r) * fbr;
+    const double Fs_r = fr.us_rel * Fr + fr.sl_rel * sL_r + fr.sr_rel * sR_r;
+    const double prev = in.u[k][c];
+    double next;
+    if (second) {
+      next = (bL == bR) ? prev + dt_over_dx * (Fs_l - Fs_r)
+                        : prev + dt_over_dx / alpha * (bL * Fs_l - bR * Fs_r - (bL - bR) * fbc);
+    } else if (bL == bR) {
+      next = prev + dt_over_dx * (Fs_l - Fs_r);
+    } else if (bL > bR) {
+      const double dLg = Clamp01((fl.sr * (0.5 + center)) / alpha);
+      next = prev + dt_over_dx * (fl.us / alpha) * (Fl - Fs_r) +
+             dt_over_dx * (fl.sl / alpha) * (sL_l - Fs_r) + dt_over_dx * dLg * (Fl - fbc);
+    } else {
+      const double dRg = Clamp01((fr.sl * (0.5 - center)) / alpha);
+      next = prev + dt_over_dx * (fr.us / alpha) * (Fs_l - Fr) +
+             dt_over_dx * (fr.sr / alpha) * (Fs_l - sR_r) + dt_over_dx * dRg * (fbc - Fr);
+    }
+    res[k] = next;
+  }
+  // the masked reconstruction turns a cell without positive density into
+  // the zero state
+  const bool keep = res[0] > 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out.u[k][c] = keep ? res[k] : 0.0;
+}
+
+// Masked reconstruction of the interior at set_state: covered cells and
+// cells without positive density become the zero state.
+__global__ void LevelMaskKernel(LevelState st, LevelGeom g, int64_t nx, int64_t ny) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= nx * ny) return;
+  const int64_t c = g.cell(q % nx, q / nx);
+  if (g.alpha[c] > 0.0 && st.u[0][c] > 0.0) return;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) st.u[k][c] = 0.0;
+}
+
+__global__ void CopyGeomKernel(DevView v, int comp, int64_t lo0, int64_t lo1, int64_t nx,
+                               int64_t ny, double* dst) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (q >= nx * ny) return;
+  const int64_t i = q % nx, j = q / nx;
+  dst[q] = *v.at(lo0 + i, lo1 + j, 0, comp);
+}
+
+} // namespace
+} // namespace fvs
+
+struct fvs_kbn_level {
+  fvs_handle* h;
+  fvs_patch_desc patch;
+  int integrator;
+  int64_t nx, ny, ngx, ngy;
+  double* buf[3][4];
+  double* geom;  // alpha | beta_x | beta_y | normals(2) | centroids(2)
+  LevelGeom lg;
+  int32_t* cut;
+  int ncut;
+  double* refs;
+  double* fb;
+  double* F;
+  unsigned long long* partials;
+  int64_t nblocks;
+  Ctl* ctl;
+  int cur;
+};
+
+namespace {
+
+LevelState StateOf(fvs_kbn_level* L, int b) {
+  LevelState s;
+  for (int k = 0; k < 4; ++k) s.u[k] = L->buf[b][k];
+  return s;
+}
+
+} // namespace
+
+extern "C" {
+
+int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
+                         const fvs_cutcell_data* geom, int integrator, fvs_kbn_level** out) {
+  if (!h || !patch || !geom || !out || (integrator != 0 && integrator != 1)) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  *out = nullptr;
+  if (h->desc.dim != 2 || h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+    SetError("the cut-cell level is 2D with an HLLEM base method");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (h->desc.reconstruction != FVS_RECON_MUSCL_HANCOCK) {
+    SetError("the cut-cell level uses MUSCL-Hancock reconstruction");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int64_t nx = patch->n[0], ny = patch->n[1];
+  if (nx < 4 || ny < 4 || nx * ny > (int64_t(1) << 31)) {
+    SetError("patch size out of range");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int64_t ngx = nx + 2 * kG, ngy = ny + 2 * kG;
+  const int64_t lo[3] = {-kG, -kG, 0}, hi[3] = {nx - 1 + kG, ny - 1 + kG, 0};
+  const int64_t bxhi[3] = {nx + kG, ny - 1 + kG, 0}, byhi[3] = {nx - 1 + kG, ny + kG, 0};
+  if (!ContainsBox(&geom->volume_fractions, lo, hi) ||
+      !ContainsBox(&geom->face_fractions[0], lo, bxhi) ||
+      !ContainsBox(&geom->face_fractions[1], lo, byhi) ||
+      !ContainsBox(&geom->boundary_normals, lo, hi) || geom->boundary_normals.ncomp < 2 ||
+      !ContainsBox(&geom->boundary_centroids, lo, hi) || geom->boundary_centroids.ncomp < 2) {
+    SetError("geometry must cover the patch grown by two ghost layers");
+    return FVS_BAD_ARGUMENT;
+  }
+  fvs_kbn_level* L = new (std::nothrow) fvs_kbn_level();
+  if (!L) {
+    SetError("out of host memory");
+    return FVS_BAD_ARGUMENT;
+  }
+  std::memset(L, 0, sizeof *L);
+  L->h = h;
+  L->patch = *patch;
+  L->integrator = integrator;
+  L->nx = nx;
+  L->ny = ny;
+  L->ngx = ngx;
+  L->ngy = ngy;
+  hipStream_t s = h->stream;
+  const int64_t ncg = ngx * ngy;
+  auto fail = [&](int rc) {
+    fvs_kbn_level_destroy(L);
+    return rc;
+  };
+  for (int b = 0; b < 3; ++b)
+    for (int k = 0; k < 4; ++k)
+      if (hipMalloc(&L->buf[b][k], ncg * sizeof(double)) != hipSuccess)
+        return fail(FVS_HIP_ERROR);
+  const int64_t nbx = ngy * (ngx + 1), nby = (ngy + 1) * ngx;
+  const int64_t gsize = ncg + nbx + nby + 4 * ncg;
+  if (hipMalloc(&L->geom, gsize * sizeof(double)) != hipSuccess) return fail(FVS_HIP_ERROR);
+  double* ga = L->geom;
+  double* gbx = ga + ncg;
+  double* gby = gbx + nbx;
+  double* gn = gby + nby;
+  double* gc = gn + 2 * ncg;
+  // dense copies of the caller's geometry views
+  auto copy = [&](const fvs_view* v, int comp, int64_t cx, int64_t cy, double* dst) {
+    DevView dv;
+    if (MakeDevView(v, comp + 1, &dv)) return FVS_BAD_ARGUMENT;
+    hipLaunchKernelGGL(CopyGeomKernel, dim3(Blocks(cx * cy)), dim3(kBlock), 0, s, dv, comp,
+                       -kG, -kG, cx, cy, dst);
+    return hipGetLastError() == hipSuccess ? FVS_OK : FVS_HIP_ERROR;
+  };
+  int rc = copy(&geom->volume_fractions, 0, ngx, ngy, ga);
+  if (!rc) rc = copy(&geom->face_fractions[0], 0, ngx + 1, ngy, gbx);
+  if (!rc) rc = copy(&geom->face_fractions[1], 0, ngx, ngy + 1, gby);
+  if (!rc) rc = copy(&geom->boundary_normals, 0, ngx, ngy, gn);
+  if (!rc) rc = copy(&geom->boundary_normals, 1, ngx, ngy, gn + ncg);
+  if (!rc) rc = copy(&geom->boundary_centroids, 0, ngx, ngy, gc);
+  if (!rc) rc = copy(&geom->boundary_centroids, 1, ngx, ngy, gc + ncg);
+  if (rc) return fail(rc);
+  L->lg.alpha = ga;
+  L->lg.beta[0] = gbx;
+  L->lg.beta[1] = gby;
+  L->lg.normals = gn;
+  L->lg.centroids = gc;
+  L->lg.ngx = ngx;
+  L->lg.ngy = ngy;
+  // list of the interior cut cells
+  int32_t* flags = nullptr;
+  if (hipMalloc(&flags, nx * ny * sizeof(int32_t)) != hipSuccess) return fail(FVS_HIP_ERROR);
+  {
+    GeomViews gv;
+    if (GeomOf(geom, 2, &gv)) {
+      (void)hipFree(flags);
+      return fail(FVS_BAD_ARGUMENT);
+    }
+    const int64_t ilo[3] = {0, 0, 0}, ihi[3] = {nx - 1, ny - 1, 0};
+    const DevView box = BoxView(ilo, ihi);
+    hipLaunchKernelGGL(IsCutCellKernel<2>, dim3(Blocks(box.count())), dim3(kBlock), 0, s, gv, box,
+                       flags);
+  }
+  std::vector<int32_t> hflags(nx * ny);
+  if (hipMemcpyAsync(hflags.data(), flags, nx * ny * sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFree(flags);
+    return fail(FVS_HIP_ERROR);
+  }
+  (void)hipFree(flags);
+  std::vector<int32_t> cut;
+  for (int64_t j = 0; j < ny; ++j)
+    for (int64_t i = 0; i < nx; ++i)
+      if (hflags[j * nx + i]) cut.push_back(static_cast<int32_t>((j + kG) * ngx + (i + kG)));
+  L->ncut = static_cast<int>(cut.size());
+  const int nc = L->ncut > 0 ? L->ncut : 1;
+  if (hipMalloc(&L->cut, nc * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&L->refs, 6 * nc * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L->fb, 4 * ncg * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L->F, 4 * (nx + 1) * (ny + 1) * sizeof(double)) != hipSuccess ||
+      hipMalloc(&L->ctl, sizeof(Ctl)) != hipSuccess)
+    return fail(FVS_HIP_ERROR);
+  if (L->ncut > 0 && hipMemcpyAsync(L->cut, cut.data(), L->ncut * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(FVS_HIP_ERROR);
+  if (hipMemsetAsync(L->fb, 0, 4 * ncg * sizeof(double), s) != hipSuccess ||
+      hipMemsetAsync(L->refs, 0, 6 * nc * sizeof(double), s) != hipSuccess)
+    return fail(FVS_HIP_ERROR);
+  for (int b = 0; b < 3; ++b)
+    for (int k = 0; k < 4; ++k)
+      if (hipMemsetAsync(L->buf[b][k], 0, ncg * sizeof(double), s) != hipSuccess)
+        return fail(FVS_HIP_ERROR);
+  L->nblocks = (std::max((nx + 1) * ny, nx * (ny + 1)) + kBlock - 1) / kBlock;
+  if (hipMalloc(&L->partials, 2 * L->nblocks * sizeof(unsigned long long)) != hipSuccess)
+    return fail(FVS_HIP_ERROR);
+  Ctl c{};
+  c.cfl = patch->cfl;
+  if (hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(FVS_HIP_ERROR);
+  *out = L;
+  return FVS_OK;
+}
+
+int fvs_kbn_level_destroy(fvs_kbn_level* L) {
+  if (!L) return FVS_OK;
+  (void)hipStreamSynchronize(L->h->stream);
+  for (int b = 0; b < 3; ++b)
+    for (int k = 0; k < 4; ++k)
+      if (L->buf[b][k]) (void)hipFree(L->buf[b][k]);
+  void* ps[] = {L->geom, L->cut, L->refs, L->fb, L->F, L->partials, L->ctl};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  delete L;
+  return FVS_OK;
+}
+
+int fvs_kbn_level_ncut(fvs_kbn_level* L) { return L ? L->ncut : -1; }
+
+int fvs_kbn_level_set_state(fvs_kbn_level* L, const double* const* comps, double t0) {
+  if (!L || !comps) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  hipStream_t s = L->h->stream;
+  L->cur = 0;
+  for (int k = 0; k < 4; ++k) {
+    if (!comps[k]) {
+      SetError("null component");
+      return FVS_BAD_ARGUMENT;
+    }
+    FVS_HIP_CHECK(hipMemcpy2DAsync(L->buf[0][k] + kG * L->ngx + kG, L->ngx * sizeof(double),
+                                   comps[k], L->nx * sizeof(double), L->nx * sizeof(double), L->ny,
+                                   hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(LevelMaskKernel, dim3(Blocks(L->nx * L->ny)), dim3(kBlock), 0, s,
+                     StateOf(L, 0), L->lg, L->nx, L->ny);
+  FVS_LAUNCH_CHECK();
+  Ctl c{};
+  c.t = t0;
+  c.cfl = L->patch.cfl;
+  FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+  FVS_HIP_CHECK(hipStreamSynchronize(s));
+  return FVS_OK;
+}
+
+int fvs_kbn_level_get_state(fvs_kbn_level* L, double* const* comps) {
+  if (!L || !comps) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  hipStream_t s = L->h->stream;
+  for (int k = 0; k < 4; ++k)
+    FVS_HIP_CHECK(hipMemcpy2DAsync(comps[k], L->nx * sizeof(double), L->buf[L->cur][k] + kG * L->ngx + kG,
+                                   L->ngx * sizeof(double), L->nx * sizeof(double), L->ny,
+                                   hipMemcpyDeviceToDevice, s));
+  FVS_HIP_CHECK(hipStreamSynchronize(s));
+  return FVS_OK;
+}
+
+int fvs_kbn_level_time(fvs_kbn_level* L, double* t, int64_t* cycle) {
+  if (!L) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  Ctl c;
+  FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, L->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
+  if (t) *t = c.t;
+  if (cycle) *cycle = c.cycle;
+  return FVS_OK;
+}
+
+// Enqueues one split sweep (input buffer src, output dst) of direction dir.
+static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
+  fvs_handle* h = L->h;
+  hipStream_t s = h->stream;
+  const LevelState in = StateOf(L, src), out = StateOf(L, dst);
+  const int* halt = &L->ctl->halt;
+  Gas<double> gas;
+  gas.gamma = h->desc.gamma;
+  gas.gm1inv = 1.0 / (h->desc.gamma - 1.0);
+  gas.gm1 = h->desc.gamma - 1.0;
+  const int64_t nt = dir == 0 ? L->ny + 2 * kG : L->nx + 2 * kG;
+  hipLaunchKernelGGL(GhostFillKernel, dim3(Blocks(nt * 2 * kG)), dim3(kBlock), 0, s, in, L->nx,
+                     L->ny, L->ngx, dir);
+  FVS_LAUNCH_CHECK();
+  const BoundaryParams bp = BoundaryOf(h->desc);
+  if (L->ncut > 0) {
+    hipLaunchKernelGGL(LevelBoundaryFluxKernel, dim3(Blocks(L->ncut)), dim3(kBlock), 0, s, in,
+                       L->lg, L->cut, L->ncut, bp, L->refs, L->fb, dir, halt);
+    FVS_LAUNCH_CHECK();
+  }
+  const double dx = L->patch.dx[dir];
+  const int64_t nf = (L->nx + (dir == 0)) * (L->ny + (dir == 1));
+  const unsigned nb = Blocks(nf);
+  const bool larr = h->desc.base == FVS_BASE_HLLEM_LARROUTUROU;
+  const double* dtp = &L->ctl->split_dt;
+#define FVS_LF(DR, LM, LA)                                                                    \
+  hipLaunchKernelGGL((LevelFluxKernel<DR, LM, LA>), dim3(nb), dim3(kBlock), 0, s, in, L->lg,  \
+                     L->nx, L->ny, dx, dtp, gas, L->F, L->partials, halt)
+#define FVS_LF_LIM(DR, LA)                                                     \
+  switch (h->desc.limiter) {                                                   \
+  case kMinMod: FVS_LF(DR, kMinMod, LA); break;                                \
+  case kVanLeer: FVS_LF(DR, kVanLeer, LA); break;                              \
+  case kNoLimiter: FVS_LF(DR, kNoLimiter, LA); break;                          \
+  default: FVS_LF(DR, kNoGradient, LA); break;                                 \
+  }
+  if (dir == 0) {
+    if (larr) { FVS_LF_LIM(0, true) } else { FVS_LF_LIM(0, false) }
+  } else {
+    if (larr) { FVS_LF_LIM(1, true) } else { FVS_LF_LIM(1, false) }
+  }
+#undef FVS_LF_LIM
+#undef FVS_LF
+  FVS_LAUNCH_CHECK();
+  if (check) {
+    if (int rc = LaunchFinalizeCheck(s, L->partials, nb, nullptr, L->ctl)) return rc;
+  }
+  const unsigned ub = Blocks(L->nx * L->ny);
+  if (dir == 0)
+    hipLaunchKernelGGL(LevelUpdateKernel<0>, dim3(ub), dim3(kBlock), 0, s, in, out, L->lg, L->nx,
+                       L->ny, L->F, L->fb, dtp, dx, L->integrator, halt);
+  else
+    hipLaunchKernelGGL(LevelUpdateKernel<1>, dim3(ub), dim3(kBlock), 0, s, in, out, L->lg, L->nx,
+                       L->ny, L->F, L->fb, dtp, dx, L->integrator, halt);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, int batch,
+                          double* dt_log, int* steps_done, int* failures) {
+  if (!L || max_steps < 0 || batch < 1) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  fvs_handle* h = L->h;
+  hipStream_t s = h->stream;
+  const bool strang = L->patch.splitting == FVS_SPLIT_STRANG;
+  Gas<double> gas;
+  gas.gamma = h->desc.gamma;
+  gas.gm1inv = 1.0 / (h->desc.gamma - 1.0);
+  gas.gm1 = h->desc.gamma - 1.0;
+  const BoundaryParams bp = BoundaryOf(h->desc);
+  Ctl c;
+  FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
+  FVS_HIP_CHECK(hipStreamSynchronize(s));
+  c.final_time = final_time;
+  c.cfl = L->patch.cfl;
+  c.halt = 0;
+  FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+  const int64_t cycle0 = c.cycle;
+  int64_t cycle = c.cycle;
+  int nfail = 0;
+  double* dlog = nullptr;
+  if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
+  int rc = FVS_OK;
+  std::vector<int> start_buf;
+  const int* halt = &L->ctl->halt;
+  while (cycle - cycle0 < max_steps) {
+    const int k = static_cast<int>(std::min<int64_t>(batch, max_steps - (cycle - cycle0)));
+    start_buf.assign(k + 1, 0);
+    int cur = L->cur;
+    for (int sidx = 0; sidx < k && rc == FVS_OK; ++sidx) {
+      start_buf[sidx] = cur;
+      const int64_t cyc = cycle + sidx;
+      const int is_odd = static_cast<int>(cyc % 2);
+      const int dirs[2] = {is_odd ? 1 : 0, is_odd ? 0 : 1};
+      int seq[4];
+      int ns = 0;
+      seq[ns++] = dirs[0];
+      seq[ns++] = dirs[1];
+      if (strang) {
+        seq[ns++] = dirs[1];
+        seq[ns++] = dirs[0];
+      }
+      // step-start CFL of both directions (ghosts of each direction filled)
+      const LevelState st0 = StateOf(L, cur);
+      for (int d = 0; d < 2; ++d) {
+        const int64_t nt = d == 0 ? L->ny + 2 * kG : L->nx + 2 * kG;
+        hipLaunchKernelGGL(GhostFillKernel, dim3(Blocks(nt * 2 * kG)), dim3(kBlock), 0, s, st0,
+                           L->nx, L->ny, L->ngx, d);
+        if (d == 0)
+          hipLaunchKernelGGL(LevelDtKernel<0>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
+                             L->lg, L->nx, L->ny, L->patch.dx[0], gas, L->partials, halt);
+        else
+          hipLaunchKernelGGL(LevelDtKernel<1>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
+                             L->lg, L->nx, L->ny, L->patch.dx[1], gas, L->partials + L->nblocks,
+                             halt);
+      }
+      rc = LaunchFinalizeCfl(s, L->partials, L->nblocks, 2, nullptr, L->ctl, seq[0], strang);
+      if (rc) break;
+      if (L->ncut > 0) {
+        hipLaunchKernelGGL(LevelRefsKernel, dim3(Blocks(L->ncut)), dim3(kBlock), 0, s, st0, L->lg,
+                           L->cut, L->ncut, bp, L->refs, halt);
+      }
+      const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
+      int src = b0, dst = b1;
+      for (int w = 0; w < ns && rc == FVS_OK; ++w) {
+        rc = KbnSweep(L, seq[w], src, dst, w > 0);
+        src = dst;
+        dst = (dst == b1) ? b2 : b1;
+      }
+      if (rc) break;
+      cur = src;
+      rc = LaunchStepEnd(s, L->ctl, dlog, cycle0, max_steps);
+    }
+    if (rc) break;
+    FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
+    FVS_HIP_CHECK(hipStreamSynchronize(s));
+    const int64_t completed = c.cycle - cycle;
+    if (c.halt == 0) {
+      L->cur = cur;
+      cycle = c.cycle;
+      continue;
+    }
+    L->cur = start_buf[completed];
+    cycle = c.cycle;
+    if (c.halt == 2) break;
+    ++nfail;
+    c.halt = 0;
+    c.have_failure = 1;
+    FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+    if (nfail > 64) {
+      SetError("more than 64 consecutive CFL failures");
+      rc = FVS_TIME_STEP_TOO_LARGE;
+      break;
+    }
+  }
+  if (dlog) {
+    if (rc == FVS_OK && cycle > cycle0)
+      (void)hipMemcpyAsync(dt_log, dlog, (cycle - cycle0) * sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(dlog);
+  }
+  if (steps_done) *steps_done = static_cast<int>(cycle - cycle0);
+  if (failures) *failures = nfail;
+  return rc;
 }
 
 } // extern "C"

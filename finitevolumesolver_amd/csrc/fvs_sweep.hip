@@ -704,6 +704,14 @@ int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials, int64
   return FVS_OK;
 }
 
+int LaunchFinalizeCfl(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
+                      int dim, double* out, Ctl* ctl, int first_dir, int strang) {
+  hipLaunchKernelGGL(FinalizeCflKernel, dim3(1), dim3(kRedBlock), 0, s, partials, nblocks, dim,
+                     out, ctl, first_dir, strang);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
 int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                    const int64_t n[3], const double dx[3], unsigned long long* partials,
                    double* out, Ctl* ctl, int* halt, int first_dir, int strang) {

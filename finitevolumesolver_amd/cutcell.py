@@ -203,3 +203,69 @@ class Reconstruction:
         cv, uv, s = complete.c_view(), cons.c_view(), geom.c_struct()
         L.check(L.load().fvs_kbn_complete_from_cons(flux_method.handle, C.byref(cv), C.byref(uv),
                                                     C.byref(s)))
+
+
+class CutCellLevel:
+    """Single-patch 2D cut-cell level on the GPU (fvs_kbn_level_*): the
+    reference's cut-cell split sequence with transmissive boundaries.
+
+    geom: CutCellData over the patch grown by two ghost layers."""
+
+    INTEGRATORS = {"TimeIntegrator": 0, "TimeIntegrator2": 1}
+
+    def __init__(self, flux_method: FluxMethod, n, dx, geom: CutCellData,
+                 integrator="TimeIntegrator", splitting="Godunov", cfl=0.8):
+        self._lib = L.load()
+        self.flux_method = flux_method
+        self.n = (int(n[0]), int(n[1]))
+        pd = L.PatchDesc()
+        pd.n = L.i64x3((self.n[0], self.n[1], 1))
+        pd.dx = (C.c_double * 3)(float(dx[0]), float(dx[1]), 1.0)
+        pd.splitting = L.SPLIT_STRANG if splitting == "Strang" else L.SPLIT_GODUNOV
+        pd.cfl = cfl
+        self._geom = geom
+        s = geom.c_struct()
+        lv = C.c_void_p()
+        L.check(self._lib.fvs_kbn_level_create(flux_method.handle, C.byref(pd), C.byref(s),
+                                               self.INTEGRATORS[integrator], C.byref(lv)))
+        self.level = lv
+        self.device = geom.alpha.data.device
+
+    def __del__(self):
+        lv = getattr(self, "level", None)
+        if lv is not None and lv.value:
+            self._lib.fvs_kbn_level_destroy(lv)
+            self.level = None
+
+    @property
+    def ncut(self) -> int:
+        return self._lib.fvs_kbn_level_ncut(self.level)
+
+    def set_state(self, cons: torch.Tensor, t0: float = 0.0) -> None:
+        cons = cons.reshape(4, self.n[1], self.n[0]).contiguous()
+        self._cons_in = cons
+        L.check(self._lib.fvs_kbn_level_set_state(
+            self.level, L.ptr_array([cons[c].data_ptr() for c in range(4)]), t0))
+
+    def get_state(self) -> torch.Tensor:
+        out = torch.empty((4, self.n[1], self.n[0]), dtype=torch.float64, device=self.device)
+        L.check(self._lib.fvs_kbn_level_get_state(
+            self.level, L.ptr_array([out[c].data_ptr() for c in range(4)])))
+        return out
+
+    def advance(self, max_steps: int, final_time: float = float("inf"), batch: int = 16,
+                return_dts: bool = False):
+        dts = np.zeros(max(max_steps, 1))
+        done, fails = C.c_int(), C.c_int()
+        L.check(self._lib.fvs_kbn_level_advance(
+            self.level, max_steps, final_time, batch,
+            dts.ctypes.data_as(C.POINTER(C.c_double)), C.byref(done), C.byref(fails)))
+        if return_dts:
+            return done.value, fails.value, dts[:done.value].tolist()
+        return done.value, fails.value
+
+    def time(self):
+        t = C.c_double()
+        cyc = C.c_int64()
+        L.check(self._lib.fvs_kbn_level_time(self.level, C.byref(t), C.byref(cyc)))
+        return t.value, cyc.value

@@ -9,6 +9,8 @@ src/equations/PerfectGas.cpp:146-158).
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 MASK64 = (1 << 64) - 1
@@ -159,3 +161,26 @@ def random_states(n: int, dim: int, seed: int, gamma: float = 1.4,
         sq = (mom[0] * mom[0] + mom[1] * mom[1]) + mom[2] * mom[2]
     E = 0.5 * sq / rho + p * gm1inv
     return np.stack([rho, *mom, E])
+
+
+def shock_channel(nx: int, ny: int, mach: float = 2.0, x_shock: float = -1.0,
+                  gamma: float = 1.4):
+    """Channel [-1.5, 2.5] x [-1, 1]: a Mach `mach` shock at x_shock moving
+    into gas at rest with rho = 1.4, p = 1 (a = 1); post-shock state from
+    the Rankine-Hugoniot relations (the reference's ShockMachnumber,
+    include/fub/initial_data/ShockMachnumber.hpp:41-65).  Returns cons
+    (4, ny, nx) and dx."""
+    dx = 4.0 / nx
+    rho1, p1, u1 = 1.4, 1.0, 0.0
+    a1 = math.sqrt(gamma * p1 / rho1)
+    m2 = mach * mach
+    p2 = p1 * (1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0))
+    rho2 = rho1 * (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    u2 = mach * a1 * (1.0 - rho1 / rho2)
+    x = -1.5 + (np.arange(nx) + 0.5) * dx
+    post = np.broadcast_to(x[None, :] < x_shock, (ny, nx))
+    rho = np.where(post, rho2, rho1)
+    u = np.where(post, u2, u1)
+    p = np.where(post, p2, p1)
+    cons = cons_from_prim(rho, [u, np.zeros_like(u)], p, gamma)
+    return cons, dx

@@ -412,6 +412,32 @@ int fvs_kbn_update_conservatively2(fvs_handle* h, const fvs_view* dest, const fv
 int fvs_kbn_complete_from_cons(fvs_handle* h, const fvs_view* complete, const fvs_view* cons,
                                const fvs_cutcell_data* geom);
 
+/* ---- single-patch cut-cell level (2D) --------------------------------------
+ * RunSimulation over one 2D patch of n[0] x n[1] cells with two ghost layers
+ * and transmissive boundaries, advancing with the reference's cut-cell
+ * split sequence: reference states at every step
+ * (src/AMReX/cutcell/IntegratorContext.cpp:800-806), then per split step the
+ * boundary fill, the KBN CFL check, boundary / regular / stabilised fluxes
+ * (include/fub/AMReX/cutcell/FluxMethod.hpp:153-257), the cut-cell update
+ * (integrator 0: TimeIntegrator, 1: TimeIntegrator2) and the masked
+ * reconstruction.  The handle must be 2D, MUSCL-Hancock with an HLLEM base.
+ * geom covers the patch grown by two ghost layers (cells -2 .. n+1, faces
+ * -2 .. n+2); the level keeps a dense device copy. */
+typedef struct fvs_kbn_level fvs_kbn_level;
+int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
+                         const fvs_cutcell_data* geom, int integrator,
+                         fvs_kbn_level** out);
+int fvs_kbn_level_destroy(fvs_kbn_level* level);
+/* Number of interior cut cells. */
+int fvs_kbn_level_ncut(fvs_kbn_level* level);
+/* Copies 4 interior component arrays (device, n[0]*n[1] each, x fastest). */
+int fvs_kbn_level_set_state(fvs_kbn_level* level, const double* const* comps, double t0);
+int fvs_kbn_level_get_state(fvs_kbn_level* level, double* const* comps);
+int fvs_kbn_level_time(fvs_kbn_level* level, double* t, int64_t* cycle);
+/* Same contract as fvs_level_advance. */
+int fvs_kbn_level_advance(fvs_kbn_level* level, int max_steps, double final_time,
+                          int batch, double* dt_log, int* steps_done, int* failures);
+
 #ifdef __cplusplus
 }
 #endif

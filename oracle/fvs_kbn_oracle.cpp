@@ -27,8 +27,10 @@
 
 #include "../include/fvs.h"
 
+#include <algorithm>
 #include <cstring>
 #include <limits>
+#include <vector>
 
 using namespace fvs_oracle;
 
@@ -617,9 +619,249 @@ int FillCutCellData(const fvs_cutcell_data* geom, int dir, const int64_t lo[3],
   return FVS_OK;
 }
 
+// ---- single-patch cut-cell level (2D) -----------------------------------------------
+//
+// The sequence of DimensionalSplitLevelIntegrator::AdvanceLevelNonRecursively
+// (include/fub/solver/DimensionalSplitLevelIntegrator.hpp:221-276) with the
+// cut-cell IntegratorContext: reference states at every level advance
+// (src/AMReX/cutcell/IntegratorContext.cpp:800-806), then per split step the
+// boundary fill of the direction, the KBN CFL check, boundary, regular and
+// stabilised fluxes (include/fub/AMReX/cutcell/FluxMethod.hpp:153-257), the
+// cut-cell update and the masked reconstruction.  Two ghost layers,
+// transmissive boundaries; the geometry covers the grown box.
+struct KbnLevel {
+  static constexpr int D = 2;
+  static constexpr int G = 2;
+  const fvs_method_desc* desc;
+  fvs_cutcell_data geom;
+  int integrator;
+  int64_t n[2];
+  int64_t ng[2];
+  double dx[2];
+  std::vector<double> scratch, refs, fb, flux[4];
+
+  fvs_view Grid(std::vector<double>& buf, int ncomp, const int64_t lo[3], const int64_t hi[3]) {
+    fvs_view v{};
+    v.ncomp = ncomp;
+    v.stride[0] = 1;
+    v.stride[1] = ng[0];
+    v.stride[2] = ng[0] * ng[1];
+    v.comp_stride = ng[0] * ng[1];
+    int64_t off = 0;
+    for (int d = 0; d < 3; ++d) {
+      v.lo[d] = lo[d];
+      v.hi[d] = hi[d];
+      off += (lo[d] + (d < D ? G : 0)) * v.stride[d];
+    }
+    v.data = buf.data() + off;
+    return v;
+  }
+  void Interior(int64_t lo[3], int64_t hi[3]) const {
+    lo[0] = lo[1] = lo[2] = 0;
+    hi[0] = n[0] - 1;
+    hi[1] = n[1] - 1;
+    hi[2] = 0;
+  }
+  fvs_view FaceView(std::vector<double>& buf, int dir) {
+    fvs_view v{};
+    int64_t lo[3], hi[3];
+    Interior(lo, hi);
+    hi[dir] += 1;
+    v.ncomp = D + 2;
+    v.stride[0] = 1;
+    v.stride[1] = hi[0] - lo[0] + 1;
+    v.stride[2] = v.stride[1] * (hi[1] - lo[1] + 1);
+    v.comp_stride = v.stride[2];
+    for (int d = 0; d < 3; ++d) {
+      v.lo[d] = lo[d];
+      v.hi[d] = hi[d];
+    }
+    v.data = buf.data();
+    return v;
+  }
+
+  KbnLevel(const fvs_method_desc* d, const fvs_patch_desc* p, const fvs_cutcell_data* g, int integ)
+      : desc(d), geom(*g), integrator(integ) {
+    for (int i = 0; i < 2; ++i) {
+      n[i] = p->n[i];
+      ng[i] = n[i] + 2 * G;
+      dx[i] = p->dx[i];
+    }
+    scratch.assign(ng[0] * ng[1] * (D + 4), 0.0);
+    refs.assign(ng[0] * ng[1] * (D + 4), 0.0);
+    fb.assign(ng[0] * ng[1] * (D + 2), 0.0);
+    for (auto& f : flux) f.assign((n[0] + 1) * (n[1] + 1) * (D + 2), 0.0);
+    // shielded fractions of every face with both neighbours in the grown box
+    for (int dir = 0; dir < D; ++dir) {
+      int64_t lo[3] = {-G, -G, 0}, hi[3] = {n[0] - 1 + G, n[1] - 1 + G, 0};
+      hi[dir] += 1;
+      lo[dir] += 1;
+      hi[dir] -= 1;
+      FillCutCellData(&geom, dir, lo, hi);
+    }
+  }
+
+  void FillGhosts(int dir) {
+    int64_t glo[3] = {-G, -G, 0}, ghi[3] = {n[0] - 1 + G, n[1] - 1 + G, 0};
+    int64_t ilo[3], ihi[3];
+    Interior(ilo, ihi);
+    fvs_view v = Grid(scratch, D + 4, glo, ghi);
+    TransmissiveFill(&v, ilo, ihi, dir);
+  }
+
+  double DirStableDt(int dir) {
+    FillGhosts(dir);
+    int64_t lo[3], hi[3];
+    Interior(lo, hi);
+    const int w = desc->reconstruction == FVS_RECON_MUSCL_HANCOCK ? 2 : 1;
+    lo[dir] -= w;
+    hi[dir] += w;
+    fvs_view s = Grid(scratch, D + 4, lo, hi);
+    double dt;
+    KbnStableDt<D>(desc, &s, &geom, dx[dir], dir, &dt);
+    return dt;
+  }
+
+  void ReferenceStates() {
+    int64_t lo[3], hi[3];
+    Interior(lo, hi);
+    fvs_view rv = Grid(refs, D + 4, lo, hi);
+    fvs_view sv = Grid(scratch, D + 4, lo, hi);
+    ::ReferenceStates<D>(desc, &rv, &sv, &geom);
+  }
+
+  int SplitStep(int dir, double split_dt, double* level_dt) {
+    const double local_dt = DirStableDt(dir);
+    *level_dt = local_dt;
+    if (local_dt < split_dt) return FVS_TIME_STEP_TOO_LARGE;
+    int64_t glo[3] = {-G, -G, 0}, ghi[3] = {n[0] - 1 + G, n[1] - 1 + G, 0};
+    int64_t ilo[3], ihi[3];
+    Interior(ilo, ihi);
+    std::fill(fb.begin(), fb.end(), 0.0);
+    fvs_view fbv = Grid(fb, D + 2, glo, ghi);
+    fvs_view sg = Grid(scratch, D + 4, glo, ghi);
+    fvs_view ri = Grid(refs, D + 4, ilo, ihi);
+    BoundaryFluxes<D>(desc, &fbv, &sg, &ri, &geom, dir);
+    fvs_view F = FaceView(flux[0], dir), Fs = FaceView(flux[1], dir);
+    fvs_view FsL = FaceView(flux[2], dir), FsR = FaceView(flux[3], dir);
+    RegularFluxes<D>(desc, &F, &sg, &geom, split_dt, dx[dir], dir);
+    CutCellFluxes(D, &Fs, &FsL, &FsR, &fbv, &F, &geom, dir);
+    fvs_view ui = Grid(scratch, D + 2, ilo, ihi);
+    if (integrator == 0) {
+      Update(&ui, &F, &Fs, &FsL, &FsR, &fbv, &geom, split_dt, dx[dir], dir);
+    } else {
+      Update2(&ui, &ui, &Fs, &fbv, &geom, split_dt, dx[dir], dir);
+    }
+    fvs_view ci = Grid(scratch, D + 4, ilo, ihi);
+    CompleteMasked<D>(desc->gamma, &ci, &ui, &geom);
+    return FVS_OK;
+  }
+};
+
+int KbnLevelRun(const fvs_method_desc* d, const fvs_patch_desc* p, const fvs_cutcell_data* g,
+                int integrator, double* cons, double t0, int max_steps, double final_time,
+                double* dt_log, int* steps_done, int* failures) {
+  KbnLevel L(d, p, g, integrator);
+  constexpr int D = 2;
+  const int64_t ncell = L.n[0] * L.n[1];
+  {
+    int64_t ilo[3], ihi[3];
+    L.Interior(ilo, ihi);
+    fvs_view ci = L.Grid(L.scratch, D + 4, ilo, ihi);
+    fvs_view uv{};
+    uv.data = cons;
+    uv.ncomp = D + 2;
+    uv.stride[0] = 1;
+    uv.stride[1] = L.n[0];
+    uv.stride[2] = ncell;
+    uv.comp_stride = ncell;
+    for (int i = 0; i < 3; ++i) {
+      uv.lo[i] = ilo[i];
+      uv.hi[i] = ihi[i];
+    }
+    CompleteMasked<D>(d->gamma, &ci, &uv, &L.geom);
+  }
+  double t = t0;
+  int64_t cycle = 0;
+  int nfail = 0, consecutive = 0;
+  bool have_failure = false;
+  double failure_dt = 0.0;
+  std::vector<double> backup;
+  int rc = FVS_OK;
+  while (t < final_time && cycle < max_steps) {
+    backup = L.scratch;
+    double stable = std::numeric_limits<double>::max();
+    if (have_failure) {
+      stable = failure_dt;
+    } else {
+      for (int dir = 0; dir < D; ++dir) stable = std::min(stable, L.DirStableDt(dir));
+    }
+    const double dt = std::min(final_time - t, p->cfl * stable);
+    L.ReferenceStates();
+    const int is_odd = static_cast<int>(cycle % 2);
+    int dirs[2] = {is_odd ? 1 : 0, is_odd ? 0 : 1};
+    std::vector<std::pair<int, double>> seq;
+    if (p->splitting == FVS_SPLIT_STRANG) {
+      seq = {{dirs[0], 0.5 * dt}, {dirs[1], 0.5 * dt}, {dirs[1], 0.5 * dt}, {dirs[0], 0.5 * dt}};
+    } else {
+      seq = {{dirs[0], dt}, {dirs[1], dt}};
+    }
+    bool failed = false;
+    for (auto& st : seq) {
+      double level_dt;
+      const int r = L.SplitStep(st.first, st.second, &level_dt);
+      if (r == FVS_TIME_STEP_TOO_LARGE) {
+        failed = true;
+        failure_dt = level_dt;
+        break;
+      }
+      if (r != FVS_OK) {
+        rc = r;
+        break;
+      }
+    }
+    if (rc != FVS_OK) break;
+    if (failed) {
+      L.scratch = backup;
+      have_failure = true;
+      ++nfail;
+      if (++consecutive > 64) {
+        rc = FVS_TIME_STEP_TOO_LARGE;
+        break;
+      }
+      continue;
+    }
+    consecutive = 0;
+    have_failure = false;
+    if (dt_log) dt_log[cycle] = dt;
+    t += dt;
+    ++cycle;
+  }
+  {
+    int64_t ilo[3], ihi[3];
+    L.Interior(ilo, ihi);
+    fvs_view ci = L.Grid(L.scratch, D + 4, ilo, ihi);
+    for (int c = 0; c < D + 2; ++c)
+      for (int64_t j = 0; j < L.n[1]; ++j)
+        for (int64_t i = 0; i < L.n[0]; ++i) cons[c * ncell + j * L.n[0] + i] = At(ci, i, j, 0, c);
+  }
+  if (steps_done) *steps_done = static_cast<int>(cycle);
+  if (failures) *failures = nfail;
+  return rc;
+}
+
 } // namespace
 
 extern "C" {
+
+int oracle_kbn_level_run(const fvs_method_desc* desc, const fvs_patch_desc* patch,
+                         const fvs_cutcell_data* geom, int integrator, double* cons, double t0,
+                         int max_steps, double final_time, double* dt_log, int* steps_done,
+                         int* failures) {
+  if (!desc || desc->dim != 2) return FVS_BAD_ARGUMENT;
+  return KbnLevelRun(desc, patch, geom, integrator, cons, t0, max_steps, final_time, dt_log,
+                     steps_done, failures);
+}
 
 int oracle_kbn_fill_cutcell_data(const fvs_cutcell_data* geom, int dir, const int64_t lo[3],
                                  const int64_t hi[3]) {

@@ -138,29 +138,6 @@ int CompleteView(const Gas& g, const fvs_view* complete, const fvs_view* cons) {
   return FVS_OK;
 }
 
-int TransmissiveFill(const fvs_view* v, const int64_t dlo[3], const int64_t dhi[3],
-                     int dir) {
-  ViewAcc a{v};
-  int64_t lo[3], hi[3];
-  BoxOf(v, lo, hi);
-  for (int64_t k = lo[2]; k <= hi[2]; ++k)
-    for (int64_t j = lo[1]; j <= hi[1]; ++j)
-      for (int64_t i = lo[0]; i <= hi[0]; ++i) {
-        int64_t idx[3] = {i, j, k};
-        int64_t src[3] = {i, j, k};
-        if (idx[dir] < dlo[dir]) {
-          src[dir] = dlo[dir];
-        } else if (idx[dir] > dhi[dir]) {
-          src[dir] = dhi[dir];
-        } else {
-          continue;
-        }
-        for (int c = 0; c < v->ncomp; ++c)
-          *a.at(i, j, k, c) = *a.at(src[0], src[1], src[2], c);
-      }
-  return FVS_OK;
-}
-
 int CheckDesc(const fvs_method_desc* d) {
   if (!d || (d->dim != 2 && d->dim != 3)) return FVS_BAD_ARGUMENT;
   return FVS_OK;

@@ -66,6 +66,8 @@ def load():
             "oracle_kbn_complete_from_cons": (I, [I, D, C.POINTER(L.View), C.POINTER(L.View), P]),
             "oracle_kbn_rotate": (I, [I, P, P, P, P]),
             "oracle_hllem_riemann_state": (I, [I, D, I, I, I64, P, P, P]),
+            "oracle_kbn_level_run": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.PatchDesc), P, I, P,
+                                         D, I, D, _PD, _PI, _PI]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -250,3 +252,25 @@ def kbn_rotate(frm, to, mom):
                              ptr(np.ascontiguousarray(to, dtype=np.float64)),
                              ptr(np.ascontiguousarray(mom, dtype=np.float64)), ptr(out))
     return out
+
+
+def kbn_level_run(desc: L.MethodDesc, geom, cons: np.ndarray, dx, n_steps, integrator=0,
+                  splitting=L.SPLIT_GODUNOV, cfl=0.8, final_time=1e300):
+    """The oracle's single-patch cut-cell level; geom: Geometry2D over the
+    patch grown by two ghost layers.  Returns (cons, dts, failures)."""
+    h = HostCutCellData(geom)
+    s = h.c_struct()
+    cons = np.ascontiguousarray(cons.copy())
+    ny, nx = cons.shape[-2:]
+    pd = L.PatchDesc()
+    pd.n = L.i64x3((nx, ny, 1))
+    pd.dx = (C.c_double * 3)(dx[0], dx[1], 1.0)
+    pd.splitting = splitting
+    pd.cfl = cfl
+    dts = np.zeros(max(n_steps, 1))
+    done, fails = C.c_int(), C.c_int()
+    rc = load().oracle_kbn_level_run(C.byref(desc), C.byref(pd), C.byref(s), integrator, ptr(cons),
+                                     0.0, n_steps, final_time, dts.ctypes.data_as(_PD),
+                                     C.byref(done), C.byref(fails))
+    assert rc == 0, rc
+    return cons, dts[:done.value], fails.value

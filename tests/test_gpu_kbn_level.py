@@ -1,0 +1,68 @@
+"""GPU single-patch cut-cell level against the oracle's level driver.
+
+A Mach 2 shock in the channel with the cylinder and the 30-degree wedge
+(the cut-cell benchmark geometry at reduced resolution, with small cells):
+with the HLLEM boundary solver the run is bit-exact (values); with the
+exact boundary solver (pow) it meets the 1e-12 bar."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from tolerances import tolerance_ratio
+
+import finitevolumesolver_amd as fvs
+from finitevolumesolver_amd import cutcell as K
+from finitevolumesolver_amd.geometry import channel_geometry
+from finitevolumesolver_amd.initial_data import shock_channel
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("solver,integrator,splitting", [
+    ("hllem", "TimeIntegrator", "Godunov"),
+    ("hllem", "TimeIntegrator2", "Godunov"),
+    ("exact", "TimeIntegrator", "Godunov"),
+    ("hllem", "TimeIntegrator", "Strang"),
+])
+def test_cutcell_level_matches_oracle(cuda, solver, integrator, splitting):
+    import torch
+    nx, ny, steps = 160, 80, 24
+    geom = channel_geometry(nx, ny, perturb=True)
+    cons, dx = shock_channel(nx, ny, x_shock=-0.4)
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
+                        limiter="MinMod", boundary_solver=solver)
+    g = K.CutCellData.from_geometry2d(geom, cuda)
+    lvl = K.CutCellLevel(fm, (nx, ny), (dx, dx), g, integrator=integrator, splitting=splitting,
+                         cfl=0.8)
+    _note(f"cut-cell level {solver}/{integrator}/{splitting}: {lvl.ncut} cut cells, "
+          f"{geom.meta.get('perturbed')} shrunk")
+    lvl.set_state(torch.from_numpy(np.ascontiguousarray(cons)).to(cuda))
+    done, fails, dts = lvl.advance(steps, batch=8, return_dts=True)
+    got = lvl.get_state().cpu().numpy()
+    ref, ref_dts, ref_fails = O.kbn_level_run(
+        fm._h.desc, geom, cons, (dx, dx), steps,
+        integrator=K.CutCellLevel.INTEGRATORS[integrator],
+        splitting=fvs._lib.SPLIT_STRANG if splitting == "Strang" else fvs._lib.SPLIT_GODUNOV)
+    ref = ref.reshape(got.shape)
+    assert done == steps and fails == ref_fails
+    if integrator == "TimeIntegrator":
+        # the h-box update keeps every uncovered cell physical; the plain
+        # conservative form (TimeIntegrator2) divides by sliver volume
+        # fractions and may lose cells, identically on both sides
+        live = geom.alpha[2:-2, 2:-2] > 0
+        assert np.all(np.isfinite(got)) and np.all(got[0][live] > 0)
+    if solver == "hllem":
+        np.testing.assert_array_equal(np.array(dts), ref_dts)
+        np.testing.assert_array_equal(got, ref)
+        _note("  state and dt sequence bit-exact")
+    else:
+        assert np.max(np.abs(np.array(dts) - ref_dts) / ref_dts) <= 1e-12
+        r = tolerance_ratio(got, ref)
+        _note(f"  tolerance ratio {r:.3f}")
+        assert r <= 1.0
