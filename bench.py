@@ -11,6 +11,10 @@ fused split sweeps (each with its own CFL check).  `value` is interior
 cell-updates per second summed over the sweeps (Gcell-updates/s), with the
 state resident in HBM before the timed region starts.
 
+The JSON line also carries `variants`: the same workload with FP32
+reconstruction/flux, the 384^3 exact-Godunov workload with and without the
+learned Newton guess, and the 2048x1024 cut-cell channel.
+
 Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--precision fp64]
 """
 from __future__ import annotations
@@ -99,10 +103,9 @@ def kernels_only(args, torch, fvs) -> int:
     CFL pass) --kernel-reps times on the 256^3 state, then one device copy of
     a 1 GiB buffer: the copy's FETCH_SIZE/WRITE_SIZE calibrate the counters
     (it moves exactly 1 GiB each way)."""
-    from finitevolumesolver_amd.initial_data import fourier_modes_3d
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
     n = args.size
-    cons, dx = fourier_modes_3d(n)
-    state = torch.from_numpy(cons).cuda()
+    state, dx = fourier_modes_3d_device(n, seed0=1)
     out = torch.empty_like(state)
     sdt = torch.empty(1, dtype=torch.float64, device="cuda")
     fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=args.precision)
@@ -128,6 +131,118 @@ def kernels_only(args, torch, fvs) -> int:
     return 0
 
 
+def bench_fp32(state, dx, n, steps, torch, fvs) -> dict:
+    """FP32 reconstruction/flux with FP64 accumulation on the same workload."""
+    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision="fp32")
+    lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
+    lvl.set_state(state)
+    lvl.advance(1, batch=1)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    done, fails = lvl.advance(steps, batch=steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    out = {"value": round(n ** 3 * 3 * done / (ms * 1e-3) / 1e9, 4), "unit": "Gcell-updates/s",
+           "steps": done, "ms_per_step": round(ms / max(done, 1), 4),
+           "dtype": "f32 reconstruction and flux, f64 update"}
+    try:
+        out["accuracy"] = l1_vs_oracle("fp32", torch)
+    except Exception as e:
+        out["accuracy"] = {"error": str(e)}
+    return out
+
+
+def bench_godunov(n: int, steps: int, torch, fvs) -> dict:
+    """3D first-order exact-Godunov on the Fourier-mode field (seeds 11..15):
+    plain FP64 (mean-pressure Newton guess) against the learned guess (a
+    network trained here on the GPU), both FP64 Newton; L1 density error of
+    the learned form against the plain FP64 GPU result (itself within 1e-12
+    of the CPU oracle)."""
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
+    from finitevolumesolver_amd.newton_guess import LearnedNewtonGuess
+    log(f"[godunov] {n}^3 initial state")
+    state, dx = fourier_modes_3d_device(n, seed0=11)
+    res = {"workload": f"3D {n}^3 exact Godunov (first order), Godunov split, CFL 0.8, "
+                       f"FP64 Newton, Fourier-mode field seeds 11-15"}
+    finals = {}
+    t0 = time.perf_counter()
+    net = LearnedNewtonGuess(seed=1)
+    counts = net.generate(1 << 20, seed=42)
+    hist = net.train(epochs=24, batch=1024, lr=5e-3, huber=0.25, seed=3)
+    torch.cuda.synchronize()
+    res["training"] = {"pairs": 1 << 20, "usable": counts["usable"], "vacuum": counts["vacuum"],
+                       "epochs": 24, "seconds": round(time.perf_counter() - t0, 2),
+                       "final_loss": float(hist[-1])}
+    for form in ("plain", "learned"):
+        fm = fvs.FluxMethod(fvs.PerfectGas(3), base="Godunov", reconstruction="None")
+        if form == "learned":
+            net.attach(fm)
+        lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
+        lvl.set_state(state)
+        fm.newton_stats(reset=True)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record()
+        done, fails = lvl.advance(steps, batch=steps)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        st = fm.newton_stats()
+        res[form] = {"value": round(n ** 3 * 3 * done / (ms * 1e-3) / 1e9, 4),
+                     "unit": "Gcell-updates/s", "steps": done,
+                     "ms_per_step": round(ms / max(done, 1), 3),
+                     "newton_iterations_per_face": round(st["iterations"] / max(st["faces"], 1), 4),
+                     "newton_max_iterations": st["max_iterations"],
+                     "fallbacks": st["fallbacks"], "not_converged": st["not_converged"]}
+        finals[form] = lvl.get_state()[0].clone()
+        log(f"[godunov] {form}: {res[form]}")
+        del lvl
+    d = (finals["learned"] - finals["plain"]).abs()
+    res["learned_vs_plain"] = {"l1_density": float(d.sum() / finals["plain"].abs().sum()),
+                               "linf_density": float((d / finals["plain"].abs()).max())}
+    res["speedup_learned_over_plain"] = round(res["learned"]["value"] / res["plain"]["value"], 4)
+    return res
+
+
+def bench_cutcell(nx: int, ny: int, steps: int, torch, fvs) -> dict:
+    """2D cut-cell channel: Mach 2 shock (x = -0.4) towards the cylinder and
+    the 30-degree wedge, small cells shrunk to volume fractions in
+    [1e-4, 1e-2]; MUSCL-Hancock(MinMod) + HLLEM regular fluxes, KBN h-box
+    stabilisation with the exact boundary solver, Godunov split, CFL 0.8."""
+    from finitevolumesolver_amd import cutcell as K
+    from finitevolumesolver_amd.geometry import channel_geometry
+    from finitevolumesolver_amd.initial_data import shock_channel
+    t0 = time.perf_counter()
+    geom = channel_geometry(nx, ny)
+    cons, dx = shock_channel(nx, ny, x_shock=-0.4)
+    log(f"[cutcell] geometry {nx}x{ny} in {time.perf_counter() - t0:.1f} s")
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock")
+    g = K.CutCellData.from_geometry2d(geom, torch.device("cuda:0"))
+    lvl = K.CutCellLevel(fm, (nx, ny), (dx, dx), g)
+    lvl.set_state(torch.from_numpy(cons).cuda())
+    lvl.advance(1, batch=1)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    done, fails = lvl.advance(steps, batch=steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    value = nx * ny * 2 * done / (ms * 1e-3) / 1e9
+    alg = 80 * nx * ny * 2 * done / (ms * 1e-3) / 1e9
+    return {"workload": f"2D {nx}x{ny} cut-cell channel, cylinder + 30-degree wedge, "
+                        f"Mach 2 shock, MUSCL-Hancock+HLLEM with KBN h-box",
+            "value": round(value, 4), "unit": "Gcell-updates/s", "steps": done,
+            "cut_cells": lvl.ncut, "shrunk_cells": geom.meta.get("perturbed"),
+            "ms_per_step": round(ms / max(done, 1), 4),
+            "hbm_frac_at_80B": round(alg / HBM_PEAK_GBS, 4)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +255,11 @@ def main() -> int:
     ap.add_argument("--kernels-only", action="store_true",
                     help="launch each fused kernel --kernel-reps times (and a plain "
                          "copy of known size) for hardware-counter runs; no timing line")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the FP32-flux, exact-Godunov and cut-cell variants")
+    ap.add_argument("--godunov-size", type=int, default=384)
+    ap.add_argument("--godunov-steps", type=int, default=3)
+    ap.add_argument("--cutcell-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -154,11 +274,10 @@ def main() -> int:
     n = args.size
     if args.kernels_only:
         return kernels_only(args, torch, fvs)
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
     log(f"[bench] generating {n}^3 initial state")
-    cons, dx = fourier_modes_3d(n)
     dev = torch.device("cuda:0")
-    t_cons = torch.from_numpy(cons).to(dev)
-    del cons
+    t_cons, dx = fourier_modes_3d_device(n, seed0=1)
     fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=args.precision)
     lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
     lvl.set_state(t_cons)
@@ -228,6 +347,20 @@ def main() -> int:
         result["accuracy"] = l1_vs_oracle(args.precision, torch)
     except Exception as e:  # accuracy is reported, never required for timing
         result["accuracy"] = {"error": str(e)}
+    if not args.no_variants:
+        variants = {}
+        del lvl, t_out, state
+        torch.cuda.empty_cache()
+        log("[bench] variant: FP32 flux")
+        variants["fp32_flux_256"] = bench_fp32(t_cons, dx, n, args.steps, torch, fvs)
+        del t_cons
+        torch.cuda.empty_cache()
+        log("[bench] variant: exact Godunov, plain and learned guess")
+        variants["godunov_exact"] = bench_godunov(args.godunov_size, args.godunov_steps, torch, fvs)
+        torch.cuda.empty_cache()
+        log("[bench] variant: cut-cell channel")
+        variants["cutcell_2048x1024"] = bench_cutcell(2048, 1024, args.cutcell_steps, torch, fvs)
+        result["variants"] = variants
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")
         result["cpu_baseline"] = cpu_baseline(192, 16, args.precision)

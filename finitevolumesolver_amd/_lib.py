@@ -150,6 +150,23 @@ SIGNATURES = {
     "fvs_kbn_level_get_state": (_I, [_P, C.POINTER(_P)]),
     "fvs_kbn_level_time": (_I, [_P, _PD, _PI64]),
     "fvs_kbn_level_advance": (_I, [_P, _I, _D, _I, _PD, C.POINTER(_I), C.POINTER(_I)]),
+    "fvs_batch_create": (_I, [_P, C.c_int64, _P, _P, C.POINTER(_P)]),
+    "fvs_batch_destroy": (_I, [_P]),
+    "fvs_batch_pool": (_P, [_P, _I]),
+    "fvs_batch_pool_size": (C.c_int64, [_P]),
+    "fvs_batch_offset": (C.c_int64, [_P, C.c_int64]),
+    "fvs_batch_upload": (_I, [_P, _I, _P]),
+    "fvs_batch_download": (_I, [_P, _I, _P]),
+    "fvs_batch_set_dt": (_I, [_P, _D]),
+    "fvs_batch_fill_ghosts": (_I, [_P, _I]),
+    "fvs_batch_sweep": (_I, [_P, _I, _I, _P]),
+    "fvs_batch_tune": (_I, [_P, _I, _I, C.POINTER(_I), C.POINTER(C.c_float)]),
+    "fvs_batch_tile_shape": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "fvs_batch_capture": (_I, [_P]),
+    "fvs_batch_replay": (_I, [_P, _I]),
+    "fvs_batch_cfl": (_P, [_P]),
+    "fvs_batch_read_cfl": (_I, [_P, _P]),
+    "fvs_batch_set_shape": (_I, [_P, _I, _I]),
 }
 
 _lib = None

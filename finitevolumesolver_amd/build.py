@@ -21,7 +21,7 @@ LIB = LIBDIR / "libfvs_hip.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["fvs_api.hip", "fvs_sweep.hip", "fvs_godunov.hip", "fvs_mlp.hip", "fvs_kbn.hip"]
+SOURCES = ["fvs_api.hip", "fvs_sweep.hip", "fvs_godunov.hip", "fvs_mlp.hip", "fvs_kbn.hip", "fvs_batch.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

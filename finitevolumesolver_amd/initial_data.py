@@ -184,3 +184,72 @@ def shock_channel(nx: int, ny: int, mach: float = 2.0, x_shock: float = -1.0,
     p = np.where(post, p2, p1)
     cons = cons_from_prim(rho, [u, np.zeros_like(u)], p, gamma)
     return cons, dx
+
+
+def _splitmix_uniform_device(seed: int, n: int, torch, device):
+    """The SplitMix64 stream of `seed` (as SplitMix64(seed).uniform(n)),
+    generated on the device with wrapping int64 arithmetic."""
+    def srl(z, s):  # logical right shift of int64 bit patterns
+        return (z >> s) & ((1 << (64 - s)) - 1)
+
+    def wrap(v):  # a uint64 constant as its int64 bit pattern
+        return v - (1 << 64) if v >= (1 << 63) else v
+
+    golden = wrap(0x9E3779B97F4A7C15)
+    idx = torch.arange(1, n + 1, dtype=torch.int64, device=device)
+    z = wrap(seed & MASK64) + idx * golden
+    z = (z ^ srl(z, 30)) * wrap(0xBF58476D1CE4E5B9)
+    z = (z ^ srl(z, 27)) * wrap(0x94D049BB133111EB)
+    z = z ^ srl(z, 31)
+    return srl(z, 11).to(torch.float64) * (1.0 / (1 << 53))
+
+
+def fourier_modes_3d_device(n: int, seed0: int = 1, gamma: float = 1.4, device="cuda"):
+    """fourier_modes_3d generated on the device (torch FP64) for large
+    benchmark grids: the same mode draws and SplitMix64 streams; values
+    agree with the host version to rounding.  Returns (cons (5, n, n, n)
+    device tensor, dx)."""
+    import torch
+    x = torch.from_numpy(cell_centers(n, 0.0, 1.0)).to(device)
+    X, Y, Z = x.view(1, 1, n), x.view(1, n, 1), x.view(n, 1, 1)
+
+    def field(seed):
+        rng = SplitMix64(seed)
+        ks, amps, phis = [], [], []
+        while len(ks) < 8:
+            k = (rng.uniform(3) * 9).astype(np.int64) - 4
+            if np.max(np.abs(k)) < 1:
+                continue
+            ks.append(k)
+            amps.append(rng.uniform(1)[0] * 2.0 - 1.0)
+            phis.append(rng.uniform(1)[0] * 2.0 * np.pi)
+        amps = np.asarray(amps)
+        amps = amps / np.sum(np.abs(amps))
+        f = torch.zeros((n, n, n), dtype=torch.float64, device=device)
+        for k, a, ph in zip(ks, amps, phis):
+            arg = ph + 2.0 * np.pi * (float(k[0]) * X + float(k[1]) * Y + float(k[2]) * Z)
+            f += float(a) * torch.cos(arg)
+        return f
+
+    rho = 1.0 + 0.2 * field(seed0)
+    p = 1.0 + 0.1 * field(seed0 + 1)
+    vel = []
+    m = (n ** 3 + 1) // 2
+    for s in range(3):
+        u = _splitmix_uniform_device(seed0 + 2 + s, 2 * m, torch, device)
+        u1, u2 = u[:m], u[m:]
+        r = torch.sqrt(-2.0 * torch.log(1.0 - u1))
+        v = torch.cat([r * torch.cos(2 * np.pi * u2), r * torch.sin(2 * np.pi * u2)])[:n ** 3]
+        v = v.view(n, n, n) * 0.1
+        for _ in range(3):
+            for ax in range(3):
+                v = (torch.roll(v, -2, ax) + torch.roll(v, -1, ax) + v + torch.roll(v, 1, ax)
+                     + torch.roll(v, 2, ax)) / 5.0
+        vel.append(v)
+    gm1inv = 1.0 / (gamma - 1.0)
+    mom = [rho * v for v in vel]
+    sq = (mom[0] * mom[0] + mom[1] * mom[1]) + mom[2] * mom[2]
+    E = 0.5 * sq / rho + p * gm1inv
+    cons = torch.stack([rho, *mom, E]).contiguous()
+    dx = 1.0 / n
+    return cons, (dx, dx, dx)

@@ -438,6 +438,49 @@ int fvs_kbn_level_time(fvs_kbn_level* level, double* t, int64_t* cycle);
 int fvs_kbn_level_advance(fvs_kbn_level* level, int max_steps, double final_time,
                           int batch, double* dt_log, int* steps_done, int* failures);
 
+/* ---- batched sweeps over the patches of an AMR level ----------------------
+ * All patches of a level (2D, uneven sizes, two ghost layers each) in one
+ * pool per state buffer; a sweep is one persistent launch over a tile list
+ * of every patch (the reference sweeps patch by patch, ForEachFab,
+ * include/fub/AMReX/ForEachFab.hpp:48-57, with FluxMethod::
+ * ComputeNumericFluxes and HyperbolicPatchIntegrator::UpdateConservatively
+ * per patch).  Handle: 2D MUSCL-Hancock + HLLEM (Larrouturou); precision
+ * FVS_FP32_FLUX runs reconstruction and flux in FP32, the update in FP64.
+ * Patch p occupies 4 * (nx+4) * (ny+4) doubles from fvs_batch_offset(p):
+ * component-major, x fastest, cell (-2, -2) first. */
+typedef struct fvs_batch fvs_batch;
+/* n: host, 2 per patch (nx, ny); dx: host, one per patch (square cells). */
+int fvs_batch_create(fvs_handle* h, int64_t count, const int32_t* n, const double* dx,
+                     fvs_batch** out);
+int fvs_batch_destroy(fvs_batch* batch);
+double* fvs_batch_pool(fvs_batch* batch, int which);
+int64_t fvs_batch_pool_size(fvs_batch* batch);
+int64_t fvs_batch_offset(fvs_batch* batch, int64_t patch);
+int fvs_batch_upload(fvs_batch* batch, int which, const double* src_device);
+int fvs_batch_download(fvs_batch* batch, int which, double* dst_device);
+/* Time step the sweeps read from device memory. */
+int fvs_batch_set_dt(fvs_batch* batch, double dt);
+/* Transmissive ghost layers of every patch of pool `which`. */
+int fvs_batch_fill_ghosts(fvs_batch* batch, int which);
+/* One sweep of direction dir from pool src into pool 1 - src (interior
+ * cells); the minimum HLLEM stable dt of all patch faces of the sweep
+ * (nullable) is written to *dt_device. */
+int fvs_batch_sweep(fvs_batch* batch, int dir, int src, double* dt_device);
+/* Autotuner: times every candidate tile shape of direction dir over `reps`
+ * sweeps and keeps the fastest (shape index, per-shape ms; nullable). */
+int fvs_batch_tune(fvs_batch* batch, int dir, int reps, int* shape, float* ms);
+int fvs_batch_tile_shape(fvs_batch* batch, int dir, int* tx, int* ty, int* ntiles);
+/* Selects candidate tile shape `shape` (0..3) of direction dir. */
+int fvs_batch_set_shape(fvs_batch* batch, int dir, int shape);
+/* Captures one split step (fill 0, X 0 -> 1, fill 1, Y 1 -> 0) into a
+ * hipGraph; fvs_batch_replay launches it `steps` times, stream-ordered with
+ * the handle's stream.  The sweeps' CFL minima land in fvs_batch_cfl. */
+int fvs_batch_capture(fvs_batch* batch);
+int fvs_batch_replay(fvs_batch* batch, int steps);
+double* fvs_batch_cfl(fvs_batch* batch);
+/* Host copy of the two CFL minima (synchronises the handle's stream). */
+int fvs_batch_read_cfl(fvs_batch* batch, double* host2);
+
 #ifdef __cplusplus
 }
 #endif

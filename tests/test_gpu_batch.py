@@ -1,0 +1,160 @@
+"""Batched AMR-level sweeps against the per-patch FP64 oracle.
+
+Patches of uneven sizes ({16, 24, 32, 48, 64} per axis, seeded) with
+ghost cells filled by the harness; each batched sweep (one launch over all
+patches) must equal the oracle's per-patch FluxMethod + UpdateConservatively
+bit for bit, for every candidate tile shape; the hipGraph-captured split
+step must equal the eager sequence on every replay."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fields import rough_field
+
+import finitevolumesolver_amd as fvs
+from finitevolumesolver_amd.batch import GHOSTS, PatchBatch
+from finitevolumesolver_amd.initial_data import SplitMix64
+
+pytestmark = pytest.mark.gpu
+SIDES = (16, 24, 32, 48, 64)
+
+
+def _note(msg):
+    print(msg, flush=True)
+
+
+def _sizes(count, seed):
+    rng = SplitMix64(seed)
+    u = rng.uniform(2 * count)
+    return [(SIDES[int(u[2 * p] * 5)], SIDES[int(u[2 * p + 1] * 5)]) for p in range(count)]
+
+
+def _fill_pool(batch, pool, seed):
+    import torch
+    for p, (nx, ny) in enumerate(batch.sizes):
+        cons = rough_field((1, ny + 2 * GHOSTS, nx + 2 * GHOSTS), seed=seed + p)
+        batch.patch_view(pool, p).copy_(torch.from_numpy(cons.reshape(4, ny + 4, nx + 4)))
+
+
+def _oracle_sweep(desc, cons, dt, dx, d):
+    """Per-patch reference: fluxes over faces 0..n of the interior rows and
+    the update of the interior; cons (4, ny+4, nx+4) with ghosts."""
+    ny, nx = cons.shape[1] - 4, cons.shape[2] - 4
+    comp = O.complete_from_cons(cons.reshape(4, -1), 2).reshape(6, 1, ny + 4, nx + 4)
+    sv = O.host_view(np.ascontiguousarray(comp), (-2, -2, 0))
+    fshape = (4, 1, ny + (d == 1), nx + (d == 0))
+    f = np.zeros(fshape)
+    O.load().oracle_numeric_fluxes(C.byref(desc), C.byref(O.host_view(f)), C.byref(sv), dt, dx, d)
+    prev = np.ascontiguousarray(cons[:, 2:-2, 2:-2].reshape(4, 1, ny, nx))
+    nxt = prev.copy()
+    O.load().oracle_update_conservatively(C.byref(O.host_view(nxt)), C.byref(O.host_view(prev)),
+                                          C.byref(O.host_view(f)), dt, dx, d)
+    # CFL of every face 0..n of the interior rows
+    c = comp[:, 0, 2:-2, :] if d == 0 else comp[:, 0, :, 2:-2]
+    if d == 0:
+        L, R = c[:, :, 1:nx + 2], c[:, :, 2:nx + 3]
+    else:
+        L, R = c[:, 1:ny + 2, :], c[:, 2:ny + 3, :]
+    dts = O.hllem_stable_dt(np.ascontiguousarray(L.reshape(6, -1)),
+                            np.ascontiguousarray(R.reshape(6, -1)), 2, d, dx)
+    return nxt[:, 0], float(dts.min())
+
+
+@pytest.fixture(scope="module")
+def level(cuda):
+    sizes = _sizes(40, seed=10)
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock")
+    dx = [1.0 / 64 if p % 2 == 0 else 1.0 / 128 for p in range(len(sizes))]
+    batch = PatchBatch(fm, sizes, dx)
+    pool = batch.new_pool()
+    _fill_pool(batch, pool, seed=100)
+    _note(f"batched level: {len(sizes)} patches, {sum(a * b for a, b in sizes)} cells")
+    return fm, batch, pool, dx
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_batched_sweep_matches_per_patch_oracle(level, cuda, shape):
+    import torch
+    fm, batch, pool, dx = level
+    desc = fm._h.desc
+    dt = 0.002
+    host = pool.cpu().numpy()
+    for d in range(2):
+        batch.set_shape(d, shape)
+        batch.upload(pool, 0)
+        batch.upload(pool, 1)
+        batch.set_dt(dt)
+        dt_dev = torch.empty(1, dtype=torch.float64, device=cuda)
+        batch.sweep(d, 0, dt_dev)
+        out = batch.download(1).cpu().numpy()
+        cfl = np.inf
+        for p, (nx, ny) in enumerate(batch.sizes):
+            o = batch.offsets[p]
+            cons = host[o:o + 4 * (nx + 4) * (ny + 4)].reshape(4, ny + 4, nx + 4)
+            ref, pdt = _oracle_sweep(desc, cons, dt, dx[p], d)
+            got = out[o:o + 4 * (nx + 4) * (ny + 4)].reshape(4, ny + 4, nx + 4)[:, 2:-2, 2:-2]
+            np.testing.assert_array_equal(got, ref, err_msg=f"patch {p} dir {d} shape {shape}")
+            cfl = min(cfl, pdt)
+        assert dt_dev.item() == cfl
+        tx, ty, nt = batch.tile_shape(d)
+        _note(f"  dir {d} shape {tx}x{ty}: {nt} tiles, bit-exact, cfl {cfl!r}")
+
+
+def test_graph_replay_matches_eager(level, cuda):
+    import torch
+    fm, batch, pool, dx = level
+    batch.set_dt(0.001)
+    # eager reference: fill, X 0 -> 1, fill, Y 1 -> 0, three times
+    batch.upload(pool, 0)
+    eager = []
+    for _ in range(3):
+        batch.fill_ghosts(0)
+        batch.sweep(0, 0)
+        batch.fill_ghosts(1)
+        batch.sweep(1, 1)
+        eager.append(batch.download(0).cpu().numpy())
+    batch.upload(pool, 0)
+    batch.capture()
+    for k in range(3):
+        batch.replay(1)
+        junk = torch.randn(1 << 16, device=cuda).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        got = batch.download(0).cpu().numpy()
+        np.testing.assert_array_equal(got, eager[k], err_msg=f"replay {k}")
+        assert np.isfinite(junk.item())
+    cfl = batch.cfl()
+    assert np.all(cfl > 0) and np.all(np.isfinite(cfl))
+    _note(f"  graph replay bit-exact over 3 steps; cfl {cfl}")
+
+
+def test_autotuner_and_fp32_flux(level, cuda):
+    import torch
+    fm, batch, pool, dx = level
+    for d in range(2):
+        best, ms = batch.tune(d, reps=3)
+        _note(f"  tuner dir {d}: shape {best} of per-shape ms {[round(m, 4) for m in ms]}")
+        assert ms[best] == min(ms)
+    fm32 = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
+                          precision="fp32")
+    b32 = PatchBatch(fm32, batch.sizes, dx)
+    for b in (batch, b32):
+        b.upload(pool, 0)
+        b.set_dt(0.002)
+        b.sweep(0, 0)
+        b.sweep(1, 1)
+    r64 = batch.download(0).cpu().numpy()
+    r32 = b32.download(0).cpu().numpy()
+    rho64, rho32 = [], []
+    for p, (nx, ny) in enumerate(batch.sizes):
+        o = batch.offsets[p]
+        sl = slice(o, o + (nx + 4) * (ny + 4))
+        rho64.append(r64[sl].reshape(ny + 4, nx + 4)[2:-2, 2:-2].ravel())
+        rho32.append(r32[sl].reshape(ny + 4, nx + 4)[2:-2, 2:-2].ravel())
+    a, b = np.concatenate(rho64), np.concatenate(rho32)
+    l1 = np.sum(np.abs(a - b)) / np.sum(np.abs(a))
+    _note(f"  FP32 flux vs FP64: L1 density {l1:.2e}, Linf {np.max(np.abs(a - b) / a):.2e}")
+    assert l1 < 1e-6
