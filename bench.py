@@ -13,7 +13,8 @@ state resident in HBM before the timed region starts.
 
 The JSON line also carries `variants`: the same workload with FP32
 reconstruction/flux, the 384^3 exact-Godunov workload with and without the
-learned Newton guess, and the 2048x1024 cut-cell channel.
+learned Newton guess, the 2048x1024 cut-cell channel and the batched AMR
+levels of the shock-bubble hierarchy.
 
 Usage: python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--precision fp64]
 """
@@ -120,6 +121,39 @@ def kernels_only(args, torch, fvs) -> int:
     for _ in range(args.kernel_reps):
         fvs.patch_stable_dt(fm, state, dx)
     torch.cuda.synchronize()
+    if not args.no_variants:
+        # first-order exact-Godunov sweeps (plain guess) on the same state
+        fg = fvs.FluxMethod(fvs.PerfectGas(3), base="Godunov", reconstruction="None")
+        for d, name in enumerate("XYZ"):
+            log(f"[kernels] godunov sweep {name}")
+            for _ in range(args.kernel_reps):
+                fvs.godunov_sweep(fg, state, out, dt, dx[d], d)
+            torch.cuda.synchronize()
+        del out
+        torch.cuda.empty_cache()
+        log("[kernels] learned-guess inference (matrix cores)")
+        from finitevolumesolver_amd.newton_guess import LearnedNewtonGuess
+        net = LearnedNewtonGuess(seed=1)
+        npairs = 1 << 22
+        net.generate(npairs, seed=42)
+        prim = net.dataset()["prim"]
+        def complete(rho, u, p):
+            q = torch.zeros((6, npairs), dtype=torch.float64, device=prim.device)
+            q[0], q[1], q[4] = rho, rho * u, p
+            q[3] = 0.5 * rho * u * u + p / 0.4
+            q[5] = torch.sqrt(1.4 * p / rho)
+            return q
+        left = complete(prim[0], prim[1], prim[2])
+        right = complete(prim[3], prim[4], prim[5])
+        for _ in range(args.kernel_reps):
+            net.infer_pairs(2, 0, left, right)
+        torch.cuda.synchronize()
+        del net, prim, left, right
+        torch.cuda.empty_cache()
+        log("[kernels] cut-cell level steps")
+        bench_cutcell(2048, 1024, args.kernel_reps, torch, fvs)
+        log("[kernels] batched AMR level steps")
+        bench_amr(args.kernel_reps, torch, fvs)
     log("[kernels] copy 1 GiB")
     a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
@@ -243,6 +277,71 @@ def bench_cutcell(nx: int, ny: int, steps: int, torch, fvs) -> dict:
             "hbm_frac_at_80B": round(alg / HBM_PEAK_GBS, 4)}
 
 
+def bench_amr(steps: int, torch, fvs) -> dict:
+    """Batched AMR levels of the 2D shock-bubble hierarchy: level 0 one 512^2
+    patch, levels 1 and 2 with 300 patches each (sides from {16..64},
+    seeded, non-overlapping); each level's split step (ghost fill, X sweep,
+    ghost fill, Y sweep) is one hipGraph replay of two persistent launches
+    with autotuned tiles.  FP32 reconstruction/flux with FP64 update (and
+    FP64 for comparison).  A coarse step advances level 1 twice and level 2
+    four times (refinement ratio 2)."""
+    import numpy as np
+    from finitevolumesolver_amd.batch import GHOSTS, PatchBatch
+    from finitevolumesolver_amd.initial_data import (amr_patch_layout, cons_from_prim,
+                                                     shock_bubble_state)
+    layouts = [([(0, 0, 512, 512)], 2.0 / 512)]
+    for lev in (1, 2):
+        layouts.append(amr_patch_layout(lev))
+    res = {"workload": "2D shock-bubble AMR levels: 512^2 base + 2 x 300 patches "
+                       "(ratio 2), per-patch split sweeps, transmissive patch ghosts"}
+    for prec in ("fp32", "fp64"):
+        fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
+                            precision=prec)
+        per_level = []
+        total_updates = 0.0
+        total_ms = 0.0
+        for lev, (lay, dx) in enumerate(layouts):
+            sizes = [(a[2], a[3]) for a in lay]
+            batch = PatchBatch(fm, sizes, dx)
+            pool = batch.new_pool()
+            for p, (i0, j0, nx, ny) in enumerate(lay):
+                xs = -1.0 + (np.arange(i0 - GHOSTS, i0 + nx + GHOSTS) + 0.5) * dx
+                ys = -1.0 + (np.arange(j0 - GHOSTS, j0 + ny + GHOSTS) + 0.5) * dx
+                X, Y = np.meshgrid(xs, ys)
+                rho, u, v, pr = shock_bubble_state(X, Y)
+                cons = cons_from_prim(rho, [u, v], pr).reshape(4, ny + 4, nx + 4)
+                batch.patch_view(pool, p).copy_(torch.from_numpy(cons))
+            batch.upload(pool, 0)
+            batch.set_dt(0.3 * dx / 3.0)
+            shapes = [batch.tune(d, reps=3)[0] for d in (0, 1)]
+            batch.upload(pool, 0)
+            batch.capture()
+            batch.replay(1)
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            ev0.record()
+            batch.replay(steps)
+            ev1.record()
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / steps
+            cells = sum(nx * ny for nx, ny in sizes)
+            sub = 2 ** lev
+            total_updates += 2.0 * cells * sub
+            total_ms += ms * sub
+            per_level.append({"level": lev, "patches": len(sizes), "cells": cells,
+                              "ms_per_step": round(ms, 4),
+                              "gcell_updates_per_s": round(2 * cells / (ms * 1e-3) / 1e9, 3),
+                              "tiles": [batch.tile_shape(d)[:2] for d in (0, 1)],
+                              "tuned_shapes": shapes})
+            del batch, pool
+        res[prec] = {"levels": per_level,
+                     "coarse_step_ms": round(total_ms, 4),
+                     "value": round(total_updates / (total_ms * 1e-3) / 1e9, 4),
+                     "unit": "Gcell-updates/s"}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +359,7 @@ def main() -> int:
     ap.add_argument("--godunov-size", type=int, default=384)
     ap.add_argument("--godunov-steps", type=int, default=3)
     ap.add_argument("--cutcell-steps", type=int, default=20)
+    ap.add_argument("--amr-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -360,6 +460,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("[bench] variant: cut-cell channel")
         variants["cutcell_2048x1024"] = bench_cutcell(2048, 1024, args.cutcell_steps, torch, fvs)
+        torch.cuda.empty_cache()
+        log("[bench] variant: batched AMR levels")
+        variants["batched_amr"] = bench_amr(args.amr_steps, torch, fvs)
         result["variants"] = variants
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

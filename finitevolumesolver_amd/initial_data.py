@@ -253,3 +253,56 @@ def fourier_modes_3d_device(n: int, seed0: int = 1, gamma: float = 1.4, device="
     cons = torch.stack([rho, *mom, E]).contiguous()
     dx = 1.0 / n
     return cons, (dx, dx, dx)
+
+
+def amr_patch_layout(level: int, count: int = 300, base: int = 512, ratio: int = 2,
+                     sides=(16, 24, 32, 48, 64), band=(-0.75, 0.35)):
+    """Patches of one refined level of the 2D shock-bubble hierarchy: `count`
+    patches with sides drawn independently per axis from `sides` (SplitMix64
+    seed 10 + level), placed without overlap on an 8-aligned lattice by
+    shelf packing inside the x band `band` of [-1, 1]^2 (continuing to the
+    right of it when the band is full).  Returns a list of (i0, j0, nx, ny)
+    in the level's cell indices and the level's cell width."""
+    n = base * ratio ** level
+    dx = 2.0 / n
+    rng = SplitMix64(10 + level)
+    u = rng.uniform(2 * count)
+    sizes = [(sides[int(u[2 * p] * len(sides))], sides[int(u[2 * p + 1] * len(sides))])
+             for p in range(count)]
+    x0 = int((band[0] + 1.0) / dx) // 8 * 8
+    x1 = int((band[1] + 1.0) / dx) // 8 * 8
+    out = []
+    cx, cy, shelf = x0, 0, 0
+    for nx, ny in sizes:
+        if cy + ny > n:
+            raise ValueError("level too small for the patches")
+        if cx + nx > x1 and cx > x0:
+            cx = x0
+            cy += shelf
+            shelf = 0
+        if cy + ny > n:  # band full: continue in the next band to the right
+            x0, x1 = x1, min(n, x1 + (x1 - x0))
+            cx, cy, shelf = x0, 0, 0
+        out.append((cx, cy, nx, ny))
+        cx += (nx + 7) // 8 * 8
+        shelf = max(shelf, (ny + 7) // 8 * 8)
+    return out, dx
+
+
+def shock_bubble_state(x, y, mach: float = 1.5, x_shock: float = -0.5, r_bubble: float = 0.2,
+                       rho_bubble: float = 0.138, gamma: float = 1.4):
+    """Primitive state (rho, u, v, p) of the 2D shock-bubble problem at points
+    x, y: a Mach `mach` shock at x_shock moving right into rho = 1, p = 1
+    (Rankine-Hugoniot post-shock state behind it), a light bubble of radius
+    r_bubble at the origin."""
+    m2 = mach * mach
+    a1 = math.sqrt(gamma)
+    p2 = 1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0)
+    rho2 = (gamma + 1.0) * m2 / ((gamma - 1.0) * m2 + 2.0)
+    u2 = mach * a1 * (1.0 - 1.0 / rho2)
+    post = x < x_shock
+    bubble = (x * x + y * y) < r_bubble * r_bubble
+    rho = np.where(post, rho2, np.where(bubble, rho_bubble, 1.0))
+    u = np.where(post, u2, 0.0)
+    p = np.where(post, p2, 1.0)
+    return rho, u, np.zeros_like(u), p
