@@ -78,13 +78,16 @@ def cpu_baseline(n: int, steps: int, precision: str) -> dict:
 
 
 def l1_vs_oracle(precision: str, torch) -> dict:
-    """L1 / Linf error of the GPU path against the FP64 oracle on a 48^3
-    sample of the same workload (4 steps)."""
+    """Error of the GPU path against the FP64 oracle on a 48^3 sample of the
+    same workload (4 steps): L1 / Linf of density, relative to the oracle's
+    density, and the largest per-cell relative error over all components
+    (with the 1e-13 x component-maximum floor of the parity tests)."""
     sys.path.insert(0, str(ROOT / "tests"))
     import numpy as np
     import oracle_lib as O
     import finitevolumesolver_amd as fvs
     from finitevolumesolver_amd.initial_data import fourier_modes_3d
+    from tolerances import tolerance_ratio
     n = 48
     cons, dx = fourier_modes_3d(n)
     fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=precision)
@@ -95,13 +98,16 @@ def l1_vs_oracle(precision: str, torch) -> dict:
     fm64 = fvs.FluxMethod(fvs.PerfectGas(3))
     ref, _, _ = O.level_run(fm64._h.desc, cons, dx, 4)
     d = np.abs(got[0] - ref[0])
-    return {"l1_density": float(d.mean()), "linf_density": float(d.max()),
+    return {"l1_density": float(d.sum() / np.abs(ref[0]).sum()),
+            "linf_density": float((d / np.abs(ref[0])).max()),
+            "max_rel_per_cell": float(tolerance_ratio(got, ref) * 1e-12),
             "bitexact": bool(np.array_equal(got, ref)), "sample": "48^3, 4 steps"}
 
 
 def kernels_only(args, torch, fvs) -> int:
     """Each fused kernel (X, Y, Z sweep without and with the CFL check, the
-    CFL pass) --kernel-reps times on the 256^3 state, then one device copy of
+    CFL pass; IEEE FP64, refined-reciprocal FP64 and FP32 flux) --kernel-reps
+    times on the 256^3 state, then one device copy of
     a 1 GiB buffer: the copy's FETCH_SIZE/WRITE_SIZE calibrate the counters
     (it moves exactly 1 GiB each way)."""
     from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
@@ -109,18 +115,19 @@ def kernels_only(args, torch, fvs) -> int:
     state, dx = fourier_modes_3d_device(n, seed0=1)
     out = torch.empty_like(state)
     sdt = torch.empty(1, dtype=torch.float64, device="cuda")
-    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=args.precision)
     dt = 0.5 * 0.8 / n
-    for d, name in enumerate("XYZ"):
-        for check in (False, True):
-            log(f"[kernels] sweep {name} check={check}")
-            for _ in range(args.kernel_reps):
-                fvs.split_sweep(fm, state, out, dt, dx[d], d, sdt if check else None)
-            torch.cuda.synchronize()
-    log("[kernels] CFL")
-    for _ in range(args.kernel_reps):
-        fvs.patch_stable_dt(fm, state, dx)
-    torch.cuda.synchronize()
+    for prec in ("fp64", "fp64_rcp", "fp32"):
+        fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=prec)
+        for d, name in enumerate("XYZ"):
+            for check in (False, True):
+                log(f"[kernels] {prec} sweep {name} check={check}")
+                for _ in range(args.kernel_reps):
+                    fvs.split_sweep(fm, state, out, dt, dx[d], d, sdt if check else None)
+                torch.cuda.synchronize()
+        log(f"[kernels] {prec} CFL")
+        for _ in range(args.kernel_reps):
+            fvs.patch_stable_dt(fm, state, dx)
+        torch.cuda.synchronize()
     if not args.no_variants:
         # first-order exact-Godunov sweeps (plain guess) on the same state
         fg = fvs.FluxMethod(fvs.PerfectGas(3), base="Godunov", reconstruction="None")
@@ -165,9 +172,14 @@ def kernels_only(args, torch, fvs) -> int:
     return 0
 
 
-def bench_fp32(state, dx, n, steps, torch, fvs) -> dict:
-    """FP32 reconstruction/flux with FP64 accumulation on the same workload."""
-    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision="fp32")
+_DTYPES = {"fp64": "f64 (IEEE division and square root)",
+           "fp64_rcp": "f64 (refined hardware reciprocals)",
+           "fp32": "f32 reconstruction and flux, f64 update"}
+
+
+def bench_level(state, dx, n, steps, precision, torch, fvs) -> dict:
+    """The same 3D workload in another arithmetic mode."""
+    fm = fvs.FluxMethod(fvs.PerfectGas(3), precision=precision)
     lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
     lvl.set_state(state)
     lvl.advance(1, batch=1)
@@ -181,9 +193,9 @@ def bench_fp32(state, dx, n, steps, torch, fvs) -> dict:
     ms = ev0.elapsed_time(ev1)
     out = {"value": round(n ** 3 * 3 * done / (ms * 1e-3) / 1e9, 4), "unit": "Gcell-updates/s",
            "steps": done, "ms_per_step": round(ms / max(done, 1), 4),
-           "dtype": "f32 reconstruction and flux, f64 update"}
+           "dtype": _DTYPES[precision]}
     try:
-        out["accuracy"] = l1_vs_oracle("fp32", torch)
+        out["accuracy"] = l1_vs_oracle(precision, torch)
     except Exception as e:
         out["accuracy"] = {"error": str(e)}
     return out
@@ -348,7 +360,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp64_rcp", choices=["fp64", "fp64_rcp", "fp32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--kernels-only", action="store_true",
@@ -426,12 +438,13 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-flux/f64-update",
+        "dtype": "f64" if args.precision != "fp32" else "f32-flux/f64-update",
         "data": "synthetic",
         "config": {
             "workload": f"3D uniform grid {n}^3 (one patch), MUSCL-Hancock(MinMod)+HLLEM "
                         f"(Larrouturou), X-Y-Z Godunov split, CFL 0.8, transmissive, "
                         f"{args.precision}",
+            "arithmetic": _DTYPES[args.precision],
             "cells": n ** 3, "sweeps_per_step": 3, "steps_completed": done,
             "cfl_restarts": fails, "parallelism": "single GPU",
         },
@@ -451,8 +464,11 @@ def main() -> int:
         variants = {}
         del lvl, t_out, state
         torch.cuda.empty_cache()
-        log("[bench] variant: FP32 flux")
-        variants["fp32_flux_256"] = bench_fp32(t_cons, dx, n, args.steps, torch, fvs)
+        for prec in ("fp64", "fp64_rcp", "fp32"):
+            if prec == args.precision:
+                continue
+            log(f"[bench] variant: {prec}")
+            variants[f"{prec}_{n}"] = bench_level(t_cons, dx, n, args.steps, prec, torch, fvs)
         del t_cons
         torch.cuda.empty_cache()
         log("[bench] variant: exact Godunov, plain and learned guess")

@@ -29,6 +29,7 @@ LIMITER_NONE = 2
 LIMITER_NO_GRADIENT = 3
 FP64 = 0
 FP32_FLUX = 1
+FP64_RCP = 2
 GUESS_MEAN_PRESSURE = 0
 GUESS_LEARNED = 1
 SPLIT_GODUNOV = 0

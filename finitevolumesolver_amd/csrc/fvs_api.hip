@@ -76,7 +76,7 @@ int CheckDesc(const fvs_method_desc* d) {
   if (d->base < 0 || d->base > FVS_BASE_GODUNOV_EXACT || d->reconstruction < 0 ||
       d->reconstruction > FVS_RECON_MUSCL_HANCOCK || d->limiter < 0 ||
       d->limiter > FVS_LIMITER_NO_GRADIENT || d->precision < 0 ||
-      d->precision > FVS_FP32_FLUX || d->boundary_solver < FVS_BOUNDARY_EXACT ||
+      d->precision > FVS_FP64_RCP || d->boundary_solver < FVS_BOUNDARY_EXACT ||
       d->boundary_solver > FVS_BOUNDARY_HLLEM) {
     SetError("method enum out of range");
     return FVS_BAD_ARGUMENT;

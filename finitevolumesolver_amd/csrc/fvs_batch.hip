@@ -68,8 +68,10 @@ __device__ __forceinline__ unsigned long long WaveMinK(unsigned long long k) {
 
 // Tile = TX x TY threads; along the sweep direction the T_d threads hold
 // cells start-1 .. start+T_d-2, of which start .. start+T_d-3 are owned.
-template <int DIR, int LIM, class T, int TX, int TY>
+template <int DIR, int LIM, class T, int TX, int TY, bool FAST>
 __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
+  using TA = typename Policies<T, FAST>::TA;
+  using DA = typename Policies<T, FAST>::DA;
   constexpr int NQ = 6, NC = 4;
   constexpr int TD = DIR == 0 ? TX : TY;
   __shared__ T recL[NQ][TY][TX];
@@ -116,9 +118,9 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
           L[k] = um[k];
           R[k] = u0[k];
         }
-        CompleteFromCons<2>(L, a.g);
-        CompleteFromCons<2>(R, a.g);
-        const double sdt = HllemStableDt<2, DIR>(L, R, P.dx, a.g);
+        CompleteFromCons<2, double, DA>(L, a.g);
+        CompleteFromCons<2, double, DA>(R, a.g);
+        const double sdt = HllemStableDt<2, DIR, double, DA>(L, R, P.dx, a.g);
         const unsigned long long key = DtKey(sdt);
         kmin = key < kmin ? key : kmin;
       }
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
         p[k] = static_cast<T>(up[k]);
       }
       const T lh = static_cast<T>(0.5 * dt / P.dx);
-      ReconstructCell<2, DIR, LIM, T>(m, z, p, lh, gT, rl, rr);
+      ReconstructCell<2, DIR, LIM, T, TA>(m, z, p, lh, gT, rl, rr);
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
         recL[k][ty][tx] = rl[k];
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
         L[k] = recR[k][pty][ptx];
         R[k] = recL[k][ty][tx];
       }
-      HllemFlux<2, DIR, true, false, T>(f, L, R, gT);
+      HllemFlux<2, DIR, true, false, T, TA>(f, L, R, gT);
 #pragma unroll
       for (int k = 0; k < NC; ++k) flx[k][ty][tx] = f[k];
     }
@@ -213,40 +215,40 @@ constexpr Shape kShapesX[] = {{64, 4}, {32, 8}, {128, 2}, {64, 2}};
 constexpr Shape kShapesY[] = {{16, 16}, {32, 8}, {64, 4}, {8, 32}};
 constexpr int kNumShapes = 4;
 
-template <int DIR, class T, int TX, int TY>
+template <int DIR, class T, int TX, int TY, bool FAST>
 void LaunchShape(hipStream_t s, const BatchArgs& a, int limiter, unsigned grid) {
   switch (limiter) {
   case kMinMod:
-    hipLaunchKernelGGL((BatchSweepKernel<DIR, kMinMod, T, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, a);
+    hipLaunchKernelGGL((BatchSweepKernel<DIR, kMinMod, T, TX, TY, FAST>), dim3(grid), dim3(TX * TY), 0, s, a);
     break;
   case kVanLeer:
-    hipLaunchKernelGGL((BatchSweepKernel<DIR, kVanLeer, T, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, a);
+    hipLaunchKernelGGL((BatchSweepKernel<DIR, kVanLeer, T, TX, TY, FAST>), dim3(grid), dim3(TX * TY), 0, s, a);
     break;
   case kNoLimiter:
-    hipLaunchKernelGGL((BatchSweepKernel<DIR, kNoLimiter, T, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, a);
+    hipLaunchKernelGGL((BatchSweepKernel<DIR, kNoLimiter, T, TX, TY, FAST>), dim3(grid), dim3(TX * TY), 0, s, a);
     break;
   default:
-    hipLaunchKernelGGL((BatchSweepKernel<DIR, kNoGradient, T, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, a);
+    hipLaunchKernelGGL((BatchSweepKernel<DIR, kNoGradient, T, TX, TY, FAST>), dim3(grid), dim3(TX * TY), 0, s, a);
     break;
   }
 }
 
-template <class T>
+template <class T, bool FAST>
 void LaunchSweepShape(hipStream_t s, int dir, int shape, const BatchArgs& a, int limiter,
                       unsigned grid) {
   if (dir == 0) {
     switch (shape) {
-    case 0: LaunchShape<0, T, 64, 4>(s, a, limiter, grid); break;
-    case 1: LaunchShape<0, T, 32, 8>(s, a, limiter, grid); break;
-    case 2: LaunchShape<0, T, 128, 2>(s, a, limiter, grid); break;
-    default: LaunchShape<0, T, 64, 2>(s, a, limiter, grid); break;
+    case 0: LaunchShape<0, T, 64, 4, FAST>(s, a, limiter, grid); break;
+    case 1: LaunchShape<0, T, 32, 8, FAST>(s, a, limiter, grid); break;
+    case 2: LaunchShape<0, T, 128, 2, FAST>(s, a, limiter, grid); break;
+    default: LaunchShape<0, T, 64, 2, FAST>(s, a, limiter, grid); break;
     }
   } else {
     switch (shape) {
-    case 0: LaunchShape<1, T, 16, 16>(s, a, limiter, grid); break;
-    case 1: LaunchShape<1, T, 32, 8>(s, a, limiter, grid); break;
-    case 2: LaunchShape<1, T, 64, 4>(s, a, limiter, grid); break;
-    default: LaunchShape<1, T, 8, 32>(s, a, limiter, grid); break;
+    case 0: LaunchShape<1, T, 16, 16, FAST>(s, a, limiter, grid); break;
+    case 1: LaunchShape<1, T, 32, 8, FAST>(s, a, limiter, grid); break;
+    case 2: LaunchShape<1, T, 64, 4, FAST>(s, a, limiter, grid); break;
+    default: LaunchShape<1, T, 8, 32, FAST>(s, a, limiter, grid); break;
     }
   }
 }
@@ -325,9 +327,11 @@ int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int pr
   a.limiter = b->h->desc.limiter;
   a.partials = b->partials;
   if (precision == FVS_FP32_FLUX)
-    LaunchSweepShape<float>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
+    LaunchSweepShape<float, true>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
+  else if (precision == FVS_FP64_RCP)
+    LaunchSweepShape<double, true>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
   else
-    LaunchSweepShape<double>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
+    LaunchSweepShape<double, false>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
   FVS_LAUNCH_CHECK();
   return LaunchFinalizeCheck(s, b->partials, b->grid[dir], dt_out, nullptr);
 }

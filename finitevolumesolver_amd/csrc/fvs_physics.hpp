@@ -16,10 +16,25 @@
 //
 // min/max follow std::min/std::max argument semantics ((b < a) ? b : a),
 // which is what Eigen 3.3's scalar and packet min/max reproduce.
+//
+// Every division and square root goes through an arithmetic policy A:
+//   IeeeArith<T>  correctly rounded a / b and sqrt (the reference's
+//                 arithmetic; the FP64 kernels are bit-exact with it);
+//   FastArith<T>  a shared reciprocal per denominator from the hardware
+//                 estimate (v_rcp_f64 refined by two Newton steps), quotients
+//                 as products, and sqrt from v_rsq_f64 refined by one
+//                 Goldschmidt step and one residual correction.  Each result
+//                 is within a few units in the last place of the correctly
+//                 rounded one; a zero denominator yields NaN instead of inf.
+//                 The float form uses the single-precision estimates as is.
+// Sharing a reciprocal changes nothing under IeeeArith (Inv keeps the
+// denominator and every quotient is still a / b).
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 namespace fvs {
 
@@ -34,6 +49,77 @@ template <class T> struct Gas {
   T gamma;  // 1.4
   T gm1inv; // 1 / (gamma - 1), evaluated in double on the host
   T gm1;    // gamma - 1 (stable-dt formula)
+};
+
+template <class T> struct IeeeArith {
+  static constexpr bool kFast = false;
+  struct Inv {
+    T b;
+  };
+  static __device__ __forceinline__ Inv inv(T b) { return Inv{b}; }
+  static __device__ __forceinline__ T div(T a, const Inv& i) { return a / i.b; }
+  static __device__ __forceinline__ T quot(T a, T b) { return a / b; }
+  static __device__ __forceinline__ T sqrt(T x) { return rsqrt_ieee(x); }
+  // e_int / (1 / (gamma - 1))
+  static __device__ __forceinline__ T pressure(T e_int, const Gas<T>& g) {
+    return e_int / g.gm1inv;
+  }
+};
+
+template <class T> struct FastArith;
+
+template <> struct FastArith<double> {
+  static constexpr bool kFast = true;
+  struct Inv {
+    double y;
+  };
+  static __device__ __forceinline__ Inv inv(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return Inv{y};
+  }
+  static __device__ __forceinline__ double div(double a, const Inv& i) { return a * i.y; }
+  static __device__ __forceinline__ double quot(double a, double b) { return a * inv(b).y; }
+  static __device__ __forceinline__ double sqrt(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double s = x * r;
+    double h = 0.5 * r;
+    const double e = __builtin_fma(-h, s, 0.5);
+    s = __builtin_fma(s, e, s);
+    h = __builtin_fma(h, e, h);
+    const double d = __builtin_fma(-s, s, x);
+    s = __builtin_fma(d, h, s);
+    return (x == 0.0) ? x : s;
+  }
+  static __device__ __forceinline__ double pressure(double e_int, const Gas<double>& g) {
+    return e_int * g.gm1;
+  }
+};
+
+template <> struct FastArith<float> {
+  static constexpr bool kFast = true;
+  struct Inv {
+    float y;
+  };
+  static __device__ __forceinline__ Inv inv(float b) { return Inv{__builtin_amdgcn_rcpf(b)}; }
+  static __device__ __forceinline__ float div(float a, const Inv& i) { return a * i.y; }
+  static __device__ __forceinline__ float quot(float a, float b) {
+    return a * __builtin_amdgcn_rcpf(b);
+  }
+  static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float pressure(float e_int, const Gas<float>& g) {
+    return e_int * g.gm1;
+  }
+};
+
+// Arithmetic policies of a kernel: the flux arithmetic in T and the CFL
+// arithmetic (always double).
+template <class T, bool FAST> struct Policies {
+  using TA = typename std::conditional<FAST, FastArith<T>, IeeeArith<T>>::type;
+  using DA = typename std::conditional<FAST, FastArith<double>, IeeeArith<double>>::type;
 };
 
 // Component layout of a Complete state in registers: rho, mom[0..D-1], E, p, a.
@@ -57,21 +143,25 @@ template <int D, class T> __device__ __forceinline__ T SqNormArray(const T* m) {
   }
 }
 
-// PerfectGas::CompleteFromCons (array form): fills p and a of q.
-template <int D, class T>
-__device__ __forceinline__ void CompleteFromCons(T* q, const Gas<T>& g) {
+// PerfectGas::CompleteFromCons (array form): fills p and a of q and returns
+// the density's reciprocal (for further quotients by the same density).
+template <int D, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ typename A::Inv CompleteFromCons(T* q, const Gas<T>& g) {
   using I = Idx<D>;
-  const T e_kin = T(0.5) * SqNormArray<D>(q + I::kMom) / q[I::kRho];
+  const typename A::Inv ir = A::inv(q[I::kRho]);
+  const T e_kin = A::div(T(0.5) * SqNormArray<D>(q + I::kMom), ir);
   const T e_int = q[I::kE] - e_kin;
-  q[I::kP] = e_int / g.gm1inv;
-  q[I::kA] = rsqrt_ieee(g.gamma * q[I::kP] / q[I::kRho]);
+  q[I::kP] = A::pressure(e_int, g);
+  q[I::kA] = A::sqrt(A::div(g.gamma * q[I::kP], ir));
+  return ir;
 }
 
-// PerfectGas::Flux in direction DIR of a complete state.
-template <int D, int DIR, class T>
-__device__ __forceinline__ void PhysFlux(T* f, const T* q) {
+// PerfectGas::Flux in direction DIR of a complete state; ir is the
+// reciprocal of its density.
+template <int D, int DIR, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void PhysFlux(T* f, const T* q, const typename A::Inv& ir) {
   using I = Idx<D>;
-  const T velocity = q[I::kMom + DIR] / q[I::kRho];
+  const T velocity = A::div(q[I::kMom + DIR], ir);
   f[I::kRho] = q[I::kMom + DIR];
 #pragma unroll
   for (int d = 0; d < D; ++d) f[I::kMom + d] = velocity * q[I::kMom + d];
@@ -79,9 +169,14 @@ __device__ __forceinline__ void PhysFlux(T* f, const T* q) {
   f[I::kE] = velocity * (q[I::kE] + q[I::kP]);
 }
 
+template <int D, int DIR, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void PhysFlux(T* f, const T* q) {
+  PhysFlux<D, DIR, T, A>(f, q, A::inv(q[Idx<D>::kRho]));
+}
+
 enum { kMinMod = 0, kVanLeer = 1, kNoLimiter = 2, kNoGradient = 3 };
 
-template <int LIM, class T>
+template <int LIM, class T, class A = IeeeArith<T>>
 __device__ __forceinline__ T LimitedSlope(T qL, T qM, T qR) {
   const T sL = qM - qL;
   const T sR = qR - qM;
@@ -89,7 +184,7 @@ __device__ __forceinline__ T LimitedSlope(T qL, T qM, T qR) {
     return (sR > T(0)) ? rmax(T(0), rmin(sL, sR)) : rmin(T(0), rmax(sL, sR));
   } else if constexpr (LIM == kVanLeer) {
     const T r = T(2) * sL * sR;
-    return (r > T(0)) ? r / (sL + sR) : T(0);
+    return (r > T(0)) ? A::quot(r, sL + sR) : T(0);
   } else if constexpr (LIM == kNoLimiter) {
     return T(0.5) * (sL + sR);
   } else {
@@ -101,48 +196,59 @@ __device__ __forceinline__ T LimitedSlope(T qL, T qM, T qR) {
 // um (left), u0 (centre), up (right).  Produces the complete half-step
 // states at the cell's left face (recL = rec_[1] of that face) and right
 // face (recR = rec_[0] of that face).  lh = 0.5 * dt / dx.
-template <int D, int DIR, int LIM, class T>
+// irL / irR receive the reciprocals of recL's and recR's densities.
+template <int D, int DIR, int LIM, class T, class A = IeeeArith<T>>
 __device__ __forceinline__ void ReconstructCell(const T* um, const T* u0, const T* up,
-                                                T lh, const Gas<T>& g, T* recL,
-                                                T* recR) {
+                                                T lh, const Gas<T>& g, T* recL, T* recR,
+                                                typename A::Inv& irL, typename A::Inv& irR) {
   using I = Idx<D>;
   T qL[I::kComp], qR[I::kComp];
 #pragma unroll
   for (int k = 0; k < I::kCons; ++k) {
-    const T s = LimitedSlope<LIM>(um[k], u0[k], up[k]);
+    const T s = LimitedSlope<LIM, T, A>(um[k], u0[k], up[k]);
     qL[k] = u0[k] - T(0.5) * s;
     qR[k] = u0[k] + T(0.5) * s;
   }
-  CompleteFromCons<D>(qL, g);
-  CompleteFromCons<D>(qR, g);
+  const typename A::Inv iqL = CompleteFromCons<D, T, A>(qL, g);
+  const typename A::Inv iqR = CompleteFromCons<D, T, A>(qR, g);
   T fL[I::kCons], fR[I::kCons];
-  PhysFlux<D, DIR>(fL, qL);
-  PhysFlux<D, DIR>(fR, qR);
+  PhysFlux<D, DIR, T, A>(fL, qL, iqL);
+  PhysFlux<D, DIR, T, A>(fR, qR, iqR);
 #pragma unroll
   for (int k = 0; k < I::kCons; ++k) {
     const T dq = lh * (fL[k] - fR[k]);
     recR[k] = qR[k] + dq;
     recL[k] = qL[k] + dq;
   }
-  CompleteFromCons<D>(recR, g);
-  CompleteFromCons<D>(recL, g);
+  irR = CompleteFromCons<D, T, A>(recR, g);
+  irL = CompleteFromCons<D, T, A>(recL, g);
+}
+
+template <int D, int DIR, int LIM, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void ReconstructCell(const T* um, const T* u0, const T* up,
+                                                T lh, const Gas<T>& g, T* recL,
+                                                T* recR) {
+  typename A::Inv irL, irR;
+  ReconstructCell<D, DIR, LIM, T, A>(um, u0, up, lh, g, recL, recR, irL, irR);
 }
 
 // Reconstruction of a ghost cell whose three stencil cells are copies of the
 // same boundary state u (transmissive boundary): every limiter returns a zero
 // slope, qL = qR = u, the half-step correction vanishes and the result is the
 // complete state of u (equal in value to the general formula).
-template <int D, class T>
-__device__ __forceinline__ void GhostRec(const T* u, const Gas<T>& g, T* rec) {
+template <int D, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ typename A::Inv GhostRec(const T* u, const Gas<T>& g, T* rec) {
 #pragma unroll
   for (int k = 0; k < D + 2; ++k) rec[k] = u[k];
-  CompleteFromCons<D>(rec, g);
+  return CompleteFromCons<D, T, A>(rec, g);
 }
 
 // Hllem<PerfectGas<D>, Larrouturou> numerical flux in direction DIR between
 // complete states L and R; writes the D+2 flux components.
-template <int D, int DIR, bool LARR, bool QUIRK, class T>
-__device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
+// irL / irR: reciprocals of the two densities.
+template <int D, int DIR, bool LARR, bool QUIRK, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void HllemFlux(T* flux, const T* L, const typename A::Inv& irL,
+                                          const T* R, const typename A::Inv& irR,
                                           const Gas<T>& g) {
   using I = Idx<D>;
   constexpr int M = I::kMom;
@@ -152,12 +258,13 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
   const T aR = R[I::kA];
   const T pL = L[I::kP];
   const T pR = R[I::kP];
-  const T sqRhoL = rsqrt_ieee(rhoL);
-  const T sqRhoR = rsqrt_ieee(rhoR);
+  const T sqRhoL = A::sqrt(rhoL);
+  const T sqRhoR = A::sqrt(rhoR);
   const T sqRhoSum = sqRhoL + sqRhoR;
-  const T sqRhoL_over_Sum = sqRhoL / sqRhoSum;
-  const T sqRhoR_over_Sum = sqRhoR / sqRhoSum;
-  const T roeGamma = (sqRhoL * g.gamma + sqRhoR * g.gamma) / sqRhoSum;
+  const typename A::Inv isum = A::inv(sqRhoSum);
+  const T sqRhoL_over_Sum = A::div(sqRhoL, isum);
+  const T sqRhoR_over_Sum = A::div(sqRhoR, isum);
+  const T roeGamma = A::div(sqRhoL * g.gamma + sqRhoR * g.gamma, isum);
   const T gm1 = roeGamma - T(1);
   if constexpr (LARR) {
     // Transverse velocities and kinetic energies (HllemMethod.hpp:729-745);
@@ -165,8 +272,8 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
     T vL[D], vR[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      vL[i] = L[M + i] / rhoL;
-      vR[i] = R[M + i] / rhoR;
+      vL[i] = A::div(L[M + i], irL);
+      vR[i] = A::div(R[M + i], irR);
     }
     T eKinRL = T(0), eKinRR = T(0);
 #pragma unroll
@@ -191,14 +298,14 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
     const T fR_m = uRd * mR + pR;
     const T fL_E = uLd * (EL + pL);
     const T fR_E = uRd * (ER + pR);
-    const T hL = (EL + pL) / rhoL;
-    const T hR = (ER + pR) / rhoR;
+    const T hL = A::div(EL + pL, irL);
+    const T hR = A::div(ER + pR, irR);
     const T roeU = sqRhoL_over_Sum * uLd + sqRhoR_over_Sum * uRd;
     const T roeH = sqRhoL_over_Sum * hL + sqRhoR_over_Sum * hR;
     const T squaredNormRoeU = roeU * roeU;
     const T squaredNormRoeU_half = T(0.5) * squaredNormRoeU;
     const T roeA2 = gm1 * (roeH - squaredNormRoeU_half);
-    const T roeA = rsqrt_ieee(roeA2);
+    const T roeA = A::sqrt(roeA2);
     const T maxA = rmax(aL, aR);
     const T sL1 = uLd - maxA;
     const T sL2 = roeU - roeA;
@@ -211,16 +318,17 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
     const T bLbR = bL * bR;
     const T db = bR - bL;
     const T db_positive = (db > T(0)) ? db : T(1);
-    const T hlle_rho = (bR * fL_rho - bL * fR_rho + bLbR * (rhoR - rhoL)) / db_positive;
-    const T hlle_m = (bR * fL_m - bL * fR_m + bLbR * (mR - mL)) / db_positive;
-    const T hlle_E = (bR * fL_E - bL * fR_E + bLbR * (ER - EL)) / db_positive;
+    const typename A::Inv idb = A::inv(db_positive);
+    const T hlle_rho = A::div(bR * fL_rho - bL * fR_rho + bLbR * (rhoR - rhoL), idb);
+    const T hlle_m = A::div(bR * fL_m - bL * fR_m + bLbR * (mR - mL), idb);
+    const T hlle_E = A::div(bR * fL_E - bL * fR_E + bLbR * (ER - EL), idb);
     const T u_bar = T(0.5) * (sR + sL);
-    const T delta = roeA / (roeA + rabs(u_bar));
-    const T b = bLbR / db_positive;
+    const T delta = A::quot(roeA, roeA + rabs(u_bar));
+    const T b = A::div(bLbR, idb);
     const T deltaRho = rhoR - rhoL;
     const T deltaRhoU = mR - mL;
     const T deltaRhoE = ER - EL;
-    const T gm1_over_roeA2 = gm1 / roeA2;
+    const T gm1_over_roeA2 = A::quot(gm1, roeA2);
     const T l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
     const T lUd = gm1_over_roeA2 * roeU;
     const T lE = -gm1_over_roeA2;
@@ -252,15 +360,15 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
   } else {
     // Plain HLLEM (Larrouturou = false): every component.
     T fL[I::kCons], fR[I::kCons];
-    PhysFlux<D, DIR>(fL, L);
-    PhysFlux<D, DIR>(fR, R);
-    const T hL = (L[I::kE] + pL) / rhoL;
-    const T hR = (R[I::kE] + pR) / rhoR;
+    PhysFlux<D, DIR, T, A>(fL, L, irL);
+    PhysFlux<D, DIR, T, A>(fR, R, irR);
+    const T hL = A::div(L[I::kE] + pL, irL);
+    const T hR = A::div(R[I::kE] + pR, irR);
     T uL[D], uR[D], roeU[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      uL[i] = L[M + i] / rhoL;
-      uR[i] = R[M + i] / rhoR;
+      uL[i] = A::div(L[M + i], irL);
+      uR[i] = A::div(R[M + i], irR);
       roeU[i] = sqRhoL_over_Sum * uL[i] + sqRhoR_over_Sum * uR[i];
     }
     const T roeH = sqRhoL_over_Sum * hL + sqRhoR_over_Sum * hR;
@@ -269,7 +377,7 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
     for (int i = 1; i < D; ++i) squaredNormRoeU += roeU[i] * roeU[i];
     const T squaredNormRoeU_half = T(0.5) * squaredNormRoeU;
     const T roeA2 = gm1 * (roeH - squaredNormRoeU_half);
-    const T roeA = rsqrt_ieee(roeA2);
+    const T roeA = A::sqrt(roeA2);
     const T maxA = rmax(aL, aR);
     const T sL = rmin(uL[DIR] - maxA, roeU[DIR] - roeA);
     const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
@@ -278,19 +386,20 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
     const T bLbR = bL * bR;
     const T db = bR - bL;
     const T db_positive = (db > T(0)) ? db : T(1);
+    const typename A::Inv idb = A::inv(db_positive);
     T hlle[I::kCons];
 #pragma unroll
     for (int k = 0; k < I::kCons; ++k)
-      hlle[k] = (bR * fL[k] - bL * fR[k] + bLbR * (R[k] - L[k])) / db_positive;
+      hlle[k] = A::div(bR * fL[k] - bL * fR[k] + bLbR * (R[k] - L[k]), idb);
     const T u_bar = T(0.5) * (sR + sL);
-    const T delta = roeA / (roeA + rabs(u_bar));
-    const T b = bLbR / db_positive;
+    const T delta = A::quot(roeA, roeA + rabs(u_bar));
+    const T b = A::div(bLbR, idb);
     const T deltaRho = rhoR - rhoL;
     T deltaRhoU[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) deltaRhoU[i] = R[M + i] - L[M + i];
     const T deltaRhoE = R[I::kE] - L[I::kE];
-    const T gm1_over_roeA2 = gm1 / roeA2;
+    const T gm1_over_roeA2 = A::quot(gm1, roeA2);
     const T l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
     T alpha_2 = l21 * deltaRho;
 #pragma unroll
@@ -322,26 +431,35 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R,
   }
 }
 
+template <int D, int DIR, bool LARR, bool QUIRK, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void HllemFlux(T* flux, const T* L, const T* R, const Gas<T>& g) {
+  HllemFlux<D, DIR, LARR, QUIRK, T, A>(flux, L, A::inv(L[Idx<D>::kRho]), R,
+                                       A::inv(R[Idx<D>::kRho]), g);
+}
+
 // Hllem::ComputeStableDt (array form) between complete states L and R.
-template <int D, int DIR, class T>
+template <int D, int DIR, class T, class A = IeeeArith<T>>
 __device__ __forceinline__ T HllemStableDt(const T* L, const T* R, T dx,
                                            const Gas<T>& g) {
   using I = Idx<D>;
   constexpr int M = I::kMom;
   const T rhoL = L[I::kRho];
   const T rhoR = R[I::kRho];
-  const T hL = (L[I::kE] + L[I::kP]) / rhoL;
-  const T hR = (R[I::kE] + R[I::kP]) / rhoR;
-  const T sqRhoL = rsqrt_ieee(rhoL);
-  const T sqRhoR = rsqrt_ieee(rhoR);
+  const typename A::Inv irL = A::inv(rhoL);
+  const typename A::Inv irR = A::inv(rhoR);
+  const T hL = A::div(L[I::kE] + L[I::kP], irL);
+  const T hR = A::div(R[I::kE] + R[I::kP], irR);
+  const T sqRhoL = A::sqrt(rhoL);
+  const T sqRhoR = A::sqrt(rhoR);
   const T sqRhoSum = sqRhoL + sqRhoR;
-  const T wL = sqRhoL / sqRhoSum;
-  const T wR = sqRhoR / sqRhoSum;
+  const typename A::Inv isum = A::inv(sqRhoSum);
+  const T wL = A::div(sqRhoL, isum);
+  const T wR = A::div(sqRhoR, isum);
   T uL[D], uR[D], roeU[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    uL[i] = L[M + i] / rhoL;
-    uR[i] = R[M + i] / rhoR;
+    uL[i] = A::div(L[M + i], irL);
+    uR[i] = A::div(R[M + i], irR);
     roeU[i] = wL * uL[i] + wR * uR[i];
   }
   const T roeH = wL * hL + wR * hR;
@@ -349,37 +467,38 @@ __device__ __forceinline__ T HllemStableDt(const T* L, const T* R, T dx,
 #pragma unroll
   for (int i = 1; i < D; ++i) sq += roeU[i] * roeU[i];
   const T roeA2 = g.gm1 * (roeH - T(0.5) * sq);
-  const T roeA = rsqrt_ieee(roeA2);
+  const T roeA = A::sqrt(roeA2);
   const T maxA = rmax(L[I::kA], R[I::kA]);
   const T sL = rmin(uL[DIR] - maxA, roeU[DIR] - roeA);
   const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
   const T maxS = rmax(rabs(sL), rabs(sR));
-  return dx / maxS;
+  return A::quot(dx, maxS);
 }
 
 // Stable dt of a face whose two states are the same state Q (boundary face
 // between a cell and its transmissive ghost copy).  With L == R the Roe
 // weights are exactly 1/2 and every Roe average equals the state's own value,
 // so this equals HllemStableDt(Q, Q) bit for bit.
-template <int D, int DIR, class T>
+template <int D, int DIR, class T, class A = IeeeArith<T>>
 __device__ __forceinline__ T HllemStableDtUniform(const T* Q, T dx, const Gas<T>& g) {
   using I = Idx<D>;
   constexpr int M = I::kMom;
   const T rho = Q[I::kRho];
-  const T h = (Q[I::kE] + Q[I::kP]) / rho;
+  const typename A::Inv ir = A::inv(rho);
+  const T h = A::div(Q[I::kE] + Q[I::kP], ir);
   T u[D];
 #pragma unroll
-  for (int i = 0; i < D; ++i) u[i] = Q[M + i] / rho;
+  for (int i = 0; i < D; ++i) u[i] = A::div(Q[M + i], ir);
   T sq = u[0] * u[0];
 #pragma unroll
   for (int i = 1; i < D; ++i) sq += u[i] * u[i];
   const T roeA2 = g.gm1 * (h - T(0.5) * sq);
-  const T roeA = rsqrt_ieee(roeA2);
+  const T roeA = A::sqrt(roeA2);
   const T a = Q[I::kA];
   const T sL = rmin(u[DIR] - a, u[DIR] - roeA);
   const T sR = rmax(u[DIR] + roeA, u[DIR] + a);
   const T maxS = rmax(rabs(sL), rabs(sR));
-  return dx / maxS;
+  return A::quot(dx, maxS);
 }
 
 // Order-preserving integer key of a non-negative double (for atomic min).

@@ -83,27 +83,28 @@ template <int D> struct DtCell {
   double a;
 };
 
-template <int D>
+template <int D, class A>
 __device__ __forceinline__ DtCell<D> MakeDtCell(const double* u, const Gas<double>& g) {
   double q[D + 4];
 #pragma unroll
   for (int c = 0; c < D + 2; ++c) q[c] = u[c];
-  CompleteFromCons<D>(q, g);
+  const typename A::Inv ir = CompleteFromCons<D, double, A>(q, g);
   DtCell<D> r;
-  r.sq = rsqrt_ieee(q[0]);
+  r.sq = A::sqrt(q[0]);
 #pragma unroll
-  for (int i = 0; i < D; ++i) r.u[i] = q[1 + i] / q[0];
-  r.h = (q[D + 1] + q[D + 2]) / q[0];
+  for (int i = 0; i < D; ++i) r.u[i] = A::div(q[1 + i], ir);
+  r.h = A::div(q[D + 1] + q[D + 2], ir);
   r.a = q[D + 3];
   return r;
 }
 
-template <int D, int DIR>
+template <int D, int DIR, class A>
 __device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R, double dx,
                                          double gm1) {
   const double sqRhoSum = L.sq + R.sq;
-  const double wL = L.sq / sqRhoSum;
-  const double wR = R.sq / sqRhoSum;
+  const typename A::Inv isum = A::inv(sqRhoSum);
+  const double wL = A::div(L.sq, isum);
+  const double wR = A::div(R.sq, isum);
   double roeU[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) roeU[i] = wL * L.u[i] + wR * R.u[i];
@@ -112,12 +113,12 @@ __device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R,
 #pragma unroll
   for (int i = 1; i < D; ++i) sq += roeU[i] * roeU[i];
   const double roeA2 = gm1 * (roeH - 0.5 * sq);
-  const double roeA = rsqrt_ieee(roeA2);
+  const double roeA = A::sqrt(roeA2);
   const double maxA = rmax(L.a, R.a);
   const double sL = rmin(L.u[DIR] - maxA, roeU[DIR] - roeA);
   const double sR = rmax(roeU[DIR] + roeA, R.u[DIR] + maxA);
   const double maxS = rmax(rabs(sL), rabs(sR));
-  return dx / maxS;
+  return A::quot(dx, maxS);
 }
 
 template <int D>
@@ -131,45 +132,49 @@ __device__ __forceinline__ DtCell<D> ShflUp(const DtCell<D>& v) {
   return r;
 }
 
-template <int D, int DIR, class T>
+template <int D, int DIR, class T, class A>
 __device__ __forceinline__ void ReconAny(const T* um, const T* u0, const T* up, T lh,
                                          const Gas<T>& g, int limiter, int muscl,
-                                         T* recL, T* recR) {
+                                         T* recL, T* recR, typename A::Inv& irL,
+                                         typename A::Inv& irR) {
   if (!muscl) {
-    GhostRec<D>(u0, g, recL);
+    irL = GhostRec<D, T, A>(u0, g, recL);
+    irR = irL;
 #pragma unroll
     for (int k = 0; k < D + 4; ++k) recR[k] = recL[k];
     return;
   }
   switch (limiter) {
   case kMinMod:
-    ReconstructCell<D, DIR, kMinMod>(um, u0, up, lh, g, recL, recR);
+    ReconstructCell<D, DIR, kMinMod, T, A>(um, u0, up, lh, g, recL, recR, irL, irR);
     break;
   case kVanLeer:
-    ReconstructCell<D, DIR, kVanLeer>(um, u0, up, lh, g, recL, recR);
+    ReconstructCell<D, DIR, kVanLeer, T, A>(um, u0, up, lh, g, recL, recR, irL, irR);
     break;
   case kNoLimiter:
-    ReconstructCell<D, DIR, kNoLimiter>(um, u0, up, lh, g, recL, recR);
+    ReconstructCell<D, DIR, kNoLimiter, T, A>(um, u0, up, lh, g, recL, recR, irL, irR);
     break;
   default:
-    ReconstructCell<D, DIR, kNoGradient>(um, u0, up, lh, g, recL, recR);
+    ReconstructCell<D, DIR, kNoGradient, T, A>(um, u0, up, lh, g, recL, recR, irL, irR);
     break;
   }
 }
 
-template <int D, int DIR, bool LARR, class T>
-__device__ __forceinline__ void FaceFlux(T* f, const T* L, const T* R, const Gas<T>& g,
-                                         int quirk) {
+template <int D, int DIR, bool LARR, class T, class A>
+__device__ __forceinline__ void FaceFlux(T* f, const T* L, const typename A::Inv& irL,
+                                         const T* R, const typename A::Inv& irR,
+                                         const Gas<T>& g, int quirk) {
   if constexpr (D == 3) {
     if (quirk) {
-      HllemFlux<D, DIR, LARR, true>(f, L, R, g);
+      HllemFlux<D, DIR, LARR, true, T, A>(f, L, irL, R, irR, g);
     } else {
-      HllemFlux<D, DIR, LARR, false>(f, L, R, g);
+      HllemFlux<D, DIR, LARR, false, T, A>(f, L, irL, R, irR, g);
     }
   } else {
-    HllemFlux<D, DIR, LARR, false>(f, L, R, g);
+    HllemFlux<D, DIR, LARR, false, T, A>(f, L, irL, R, irR, g);
   }
 }
+
 
 template <class T> __device__ __forceinline__ Gas<T> MakeGas(const SweepArgs& a) {
   Gas<T> g;
@@ -191,8 +196,10 @@ template <class T> __device__ __forceinline__ Gas<T> MakeGas(const SweepArgs& a)
 // rows (62 of 64 lanes, 256 of 258 slots).
 constexpr int kXOwned = 62;
 
-template <int D, bool LARR, class T>
+template <int D, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
+  using TA = typename Policies<T, FAST>::TA;
+  using DA = typename Policies<T, FAST>::DA;
   constexpr int NC = D + 2;
   constexpr int NQ = D + 4;
   if (a.halt && *a.halt) return;
@@ -227,19 +234,25 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
     um[k] = __shfl_up(u[k], 1, 64);
     up[k] = __shfl_down(u[k], 1, 64);
   }
-  // The wave's edge lanes read their outer neighbour themselves.
-  if (lane == 0 && real) {
+  // The wave's edge lanes read their outer neighbour themselves, and so do
+  // the ghost slots, whose shuffled neighbour belongs to the adjacent row.
+  // With clamped indices a ghost slot's stencil is three copies of the
+  // boundary cell: its zero-slope reconstruction equals GhostRec (the ghost's
+  // complete state, up to the sign of zero components), so every lane runs
+  // the same reconstruction and no wave takes two paths.
+  if (lane == 0 || cell < 0) {
     const int64_t off = rbase + Clamp(cell - 1, nx);
 #pragma unroll
     for (int k = 0; k < NC; ++k) um[k] = a.in[k][off];
   }
-  if (lane == 63 && real) {
+  if (lane == 63 || cell >= nx) {
     const int64_t off = rbase + Clamp(cell + 1, nx);
 #pragma unroll
     for (int k = 0; k < NC; ++k) up[k] = a.in[k][off];
   }
 
   T recL[NQ], recR[NQ];
+  typename TA::Inv irL, irR;
   {
     T tm[NC], t0[NC], tp[NC];
 #pragma unroll
@@ -248,23 +261,23 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
       t0[k] = static_cast<T>(u[k]);
       tp[k] = static_cast<T>(up[k]);
     }
-    if (real) {
-      ReconAny<D, 0>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR);
-    } else {
-      GhostRec<D>(t0, g, recL);
-#pragma unroll
-      for (int k = 0; k < NQ; ++k) recR[k] = recL[k];
-    }
+    ReconAny<D, 0, T, TA>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR, irL, irR);
   }
   T recRprev[NQ];
 #pragma unroll
   for (int k = 0; k < NQ; ++k) recRprev[k] = __shfl_up(recR[k], 1, 64);
+  typename TA::Inv irRprev;
+  if constexpr (TA::kFast) {
+    irRprev.y = __shfl_up(irR.y, 1, 64);
+  } else {
+    irRprev = TA::inv(recRprev[0]);
+  }
 
   // Face sl - 1 (left face of the slot's cell) for slots 1 .. nx + 1.
   const bool face_lane = valid && lane >= 1 && sl >= 1;
   T F[NC];
   if (face_lane) {
-    FaceFlux<D, 0, LARR>(F, recRprev, recL, g, a.quirk3d);
+    FaceFlux<D, 0, LARR, T, TA>(F, recRprev, irRprev, recL, irL, g, a.quirk3d);
   } else {
 #pragma unroll
     for (int k = 0; k < NC; ++k) F[k] = T(0);
@@ -283,17 +296,19 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   }
   if (a.check) {
     unsigned long long key = kInfKey;
-    const DtCell<D> dc = MakeDtCell<D>(u, gd);
+    const DtCell<D> dc = MakeDtCell<D, DA>(u, gd);
     const DtCell<D> dp = ShflUp<D>(dc);
-    if (owned && face_lane) key = DtKey(FaceDt<D, 0>(dp, dc, a.dx, gd.gm1));
+    if (owned && face_lane) key = DtKey(FaceDt<D, 0, DA>(dp, dc, a.dx, gd.gm1));
     BlockMinToPartials<1, kBlock / 64>(&key, a.partials + blockIdx.x, 0);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Y / Z sweep: pencil march.
-template <int D, int DIR, bool LARR, class T>
+template <int D, int DIR, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
+  using TA = typename Policies<T, FAST>::TA;
+  using DA = typename Policies<T, FAST>::DA;
   constexpr int NC = D + 2;
   constexpr int NQ = D + 4;
   if (a.halt && *a.halt) return;
@@ -325,33 +340,37 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
     load(U0, j0);
     load(Unext, j0 + 1);
     T recRprev[NQ];
+    typename TA::Inv irRprev;
     {
       T t0[NC];
       if (j0 == 0) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) t0[c] = static_cast<T>(U0[c]);
-        GhostRec<D>(t0, g, recRprev);
+        irRprev = GhostRec<D, T, TA>(t0, g, recRprev);
       } else {
         double Umm[NC];
         load(Umm, j0 - 2);
         T tm[NC], tp[NC], dummy[NQ];
+        typename TA::Inv idummy;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
           tm[c] = static_cast<T>(Umm[c]);
           t0[c] = static_cast<T>(Um[c]);
           tp[c] = static_cast<T>(U0[c]);
         }
-        ReconAny<D, DIR>(tm, t0, tp, lh, g, a.limiter, a.muscl, dummy, recRprev);
+        ReconAny<D, DIR, T, TA>(tm, t0, tp, lh, g, a.limiter, a.muscl, dummy, recRprev, idummy,
+                                irRprev);
       }
     }
     DtCell<D> dprev;
-    if (a.check) dprev = MakeDtCell<D>(Um, gd);
+    if (a.check) dprev = MakeDtCell<D, DA>(Um, gd);
     T Fprev[NC];
     for (int64_t j = j0; j <= j1; ++j) {
 #pragma unroll
       for (int c = 0; c < NC; ++c) Up[c] = Unext[c];
       if (j + 1 < nd) load(Unext, j + 2);
       T recL[NQ], recR[NQ];
+      typename TA::Inv irL, irR;
       if (j < nd) {
         T tm[NC], t0[NC], tp[NC];
 #pragma unroll
@@ -360,17 +379,18 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
           t0[c] = static_cast<T>(U0[c]);
           tp[c] = static_cast<T>(Up[c]);
         }
-        ReconAny<D, DIR>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR);
+        ReconAny<D, DIR, T, TA>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR, irL, irR);
       } else {
         T t0[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) t0[c] = static_cast<T>(U0[c]);
-        GhostRec<D>(t0, g, recL);
+        irL = GhostRec<D, T, TA>(t0, g, recL);
+        irR = irL;
 #pragma unroll
         for (int k = 0; k < NQ; ++k) recR[k] = recL[k];
       }
       T F[NC];
-      FaceFlux<D, DIR, LARR>(F, recRprev, recL, g, a.quirk3d);
+      FaceFlux<D, DIR, LARR, T, TA>(F, recRprev, irRprev, recL, irL, g, a.quirk3d);
       if (j > j0) {
         const int64_t off = base + (j - 1) * stride;
 #pragma unroll
@@ -380,8 +400,8 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
         }
       }
       if (a.check && (j < j1 || j1 == nd)) {
-        const DtCell<D> d0 = MakeDtCell<D>(U0, gd);
-        key = KMin(key, DtKey(FaceDt<D, DIR>(dprev, d0, a.dx, gd.gm1)));
+        const DtCell<D> d0 = MakeDtCell<D, DA>(U0, gd);
+        key = KMin(key, DtKey(FaceDt<D, DIR, DA>(dprev, d0, a.dx, gd.gm1)));
         dprev = d0;
       }
 #pragma unroll
@@ -392,6 +412,7 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
       }
 #pragma unroll
       for (int k = 0; k < NQ; ++k) recRprev[k] = recR[k];
+      irRprev = irR;
     }
   }
   if (a.check) {
@@ -402,41 +423,39 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Step-start CFL: stable dt of every direction in one pass.  A workgroup is a
-// 256-cell x tile of one row and marches along the last dimension (z in 3D,
-// y in 2D) over a segment; x neighbours come through LDS, the march
-// neighbour is kept in registers and (3D) the y neighbour is recomputed.
-// Boundary faces pair a cell with its transmissive ghost copy.
+// 2D step-start CFL: stable dt of both directions in one pass.  A workgroup
+// is a 256-cell x tile marching along y over a segment; x neighbours come
+// through LDS and the y neighbour is kept in registers.  Boundary faces pair
+// a cell with its transmissive ghost copy.
 constexpr int kCflSeg = 32;
 
-template <int D>
-__global__ __launch_bounds__(kBlock) void PatchCflKernel(
+template <bool FAST>
+__global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
     const double* __restrict__ u0, const double* __restrict__ u1,
-    const double* __restrict__ u2, const double* __restrict__ u3,
-    const double* __restrict__ u4, int64_t nx, int64_t ny, int64_t nz, double dx0,
-    double dx1, double dx2, double gamma, double gm1inv, unsigned long long* partials,
+    const double* __restrict__ u2, const double* __restrict__ u3, int64_t nx, int64_t ny,
+    double dx0, double dx1, double gamma, double gm1inv, unsigned long long* partials,
     int* halt) {
+  using DA = typename Policies<double, FAST>::DA;
+  constexpr int D = 2;
   constexpr int NC = D + 2;
-  constexpr int M = D - 1;  // march dimension
+  constexpr int M = 1;  // march dimension
   if (halt && *halt) return;
-  const double* in[5] = {u0, u1, u2, u3, u4};
+  const double* in[4] = {u0, u1, u2, u3};
   Gas<double> g;
   g.gamma = gamma;
   g.gm1inv = gm1inv;
   g.gm1 = gamma - 1.0;
-  const int64_t tiles = (nx + kTX - 1) / kTX;
-  const int64_t x0 = (blockIdx.x % tiles) * kTX;
-  const int64_t y = (D == 3) ? static_cast<int64_t>(blockIdx.x / tiles) : 0;
+  const int64_t x0 = static_cast<int64_t>(blockIdx.x) * kTX;
   const int tx = threadIdx.x;
   const int64_t x = x0 + tx;
   const bool active = x < nx;
   const int64_t xs = active ? x : nx - 1;
-  const int64_t nm = (D == 3) ? nz : ny;
-  const int64_t ms = (D == 3) ? nx * ny : nx;
+  const int64_t nm = ny;
+  const int64_t ms = nx;
   const int64_t m0 = static_cast<int64_t>(blockIdx.y) * kCflSeg;
   const int64_t m1 = (m0 + kCflSeg < nm) ? m0 + kCflSeg : nm;
-  const int64_t rowoff = (D == 3) ? y * nx : 0;
-  const double dxs[3] = {dx0, dx1, dx2};
+  const int64_t rowoff = 0;
+  const double dxs[2] = {dx0, dx1};
 
   __shared__ DtCell<D> s_cells[kTX];
   unsigned long long key[3] = {kInfKey, kInfKey, kInfKey};
@@ -445,24 +464,18 @@ __global__ __launch_bounds__(kBlock) void PatchCflKernel(
     double u[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) u[c] = in[c][off];
-    return MakeDtCell<D>(u, g);
+    return MakeDtCell<D, DA>(u, g);
   };
 
   DtCell<D> dprev = load_cell(xs + rowoff + Clamp(m0 - 1, nm) * ms);
   for (int64_t m = m0; m < m1; ++m) {
     const int64_t off = xs + rowoff + m * ms;
     const DtCell<D> dc = load_cell(off);
-    // March direction (z in 3D, y in 2D).
+    // March direction y.
     {
       const DtCell<D>& l = (m == 0) ? dc : dprev;
-      key[M] = KMin(key[M], DtKey(FaceDt<D, M>(l, dc, dxs[M], g.gm1)));
-      if (m == nm - 1) key[M] = KMin(key[M], DtKey(FaceDt<D, M>(dc, dc, dxs[M], g.gm1)));
-    }
-    // y direction in 3D: recompute the y - 1 neighbour.
-    if constexpr (D == 3) {
-      const DtCell<D> dy = (y == 0) ? dc : load_cell(off - nx);
-      key[1] = KMin(key[1], DtKey(FaceDt<D, 1>(dy, dc, dxs[1], g.gm1)));
-      if (y == ny - 1) key[1] = KMin(key[1], DtKey(FaceDt<D, 1>(dc, dc, dxs[1], g.gm1)));
+      key[M] = KMin(key[M], DtKey(FaceDt<D, M, DA>(l, dc, dxs[M], g.gm1)));
+      if (m == nm - 1) key[M] = KMin(key[M], DtKey(FaceDt<D, M, DA>(dc, dc, dxs[M], g.gm1)));
     }
     // x direction through LDS.
     s_cells[tx] = dc;
@@ -476,16 +489,105 @@ __global__ __launch_bounds__(kBlock) void PatchCflKernel(
       } else {
         dl = load_cell(off - 1);
       }
-      key[0] = KMin(key[0], DtKey(FaceDt<D, 0>(dl, dc, dxs[0], g.gm1)));
-      if (x == nx - 1) key[0] = KMin(key[0], DtKey(FaceDt<D, 0>(dc, dc, dxs[0], g.gm1)));
+      key[0] = KMin(key[0], DtKey(FaceDt<D, 0, DA>(dl, dc, dxs[0], g.gm1)));
+      if (x == nx - 1) key[0] = KMin(key[0], DtKey(FaceDt<D, 0, DA>(dc, dc, dxs[0], g.gm1)));
     }
     __syncthreads();
     dprev = dc;
   }
-  if (!active) key[0] = key[1] = key[2] = kInfKey;
+  if (!active) key[0] = key[1] = kInfKey;
   const int64_t nblocks = static_cast<int64_t>(gridDim.x) * gridDim.y;
   const int64_t bid = blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x;
   BlockMinToPartials<D, kBlock / 64>(key, partials + bid, nblocks);
+}
+
+// 3D step-start CFL.  A workgroup is a tile of 9 x rows of 64 cells (one
+// wave each) marching along z over a segment.  Lanes 1..62 of a wave own
+// their cells and lanes 0 and 63 hold the x neighbours, so x faces pair
+// neighbouring lanes through a shuffle; row 0 is a halo row (the y - 1
+// neighbours of row 1) and y faces read the row below from LDS; z faces pair
+// the plane with the previous one kept in registers.  Each cell's
+// DtCell is computed 64 * 9 / (62 * 8) = 1.16 times instead of twice.
+// Tiles are grouped per XCD (workgroups are dispatched round-robin over the 8
+// XCDs): XCD k gets a contiguous range of tiles, so vertically neighbouring
+// tiles, whose halo rows overlap, share one L2.
+constexpr int kC3Rows = 9;
+constexpr int kC3OwnX = 62;
+constexpr int kC3OwnY = kC3Rows - 1;
+constexpr int kXcds = 8;
+
+template <bool FAST>
+__global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
+    const double* __restrict__ u0, const double* __restrict__ u1,
+    const double* __restrict__ u2, const double* __restrict__ u3,
+    const double* __restrict__ u4, int64_t nx, int64_t ny, int64_t nz, double dx0,
+    double dx1, double dx2, double gamma, double gm1inv, int64_t ntiles,
+    unsigned long long* partials, int* halt) {
+  using DA = typename Policies<double, FAST>::DA;
+  constexpr int NC = 5;
+  if (halt && *halt) return;
+  const double* in[NC] = {u0, u1, u2, u3, u4};
+  Gas<double> g;
+  g.gamma = gamma;
+  g.gm1inv = gm1inv;
+  g.gm1 = gamma - 1.0;
+  const int lane = threadIdx.x & 63;
+  const int row = threadIdx.x >> 6;
+  const int64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
+  const int64_t tile = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  const int64_t tiles_x = (nx + kC3OwnX - 1) / kC3OwnX;
+  const bool live = tile < ntiles;
+  const int64_t bx = live ? tile % tiles_x : 0;
+  const int64_t by = live ? tile / tiles_x : 0;
+  const int64_t x = bx * kC3OwnX + lane - 1;
+  const int64_t y = by * kC3OwnY + row - 1;
+  const int64_t xs = Clamp(x, nx);
+  const int64_t ys = Clamp(y, ny);
+  const bool own = live && lane >= 1 && lane <= kC3OwnX && x < nx && row >= 1 && y < ny;
+  const int64_t m0 = static_cast<int64_t>(blockIdx.y) * kCflSeg;
+  const int64_t m1 = (m0 + kCflSeg < nz) ? m0 + kCflSeg : nz;
+  const int64_t plane = nx * ny;
+  const int64_t col = xs + ys * nx;
+
+  __shared__ DtCell<3> s_cells[2][kC3Rows][64];
+  unsigned long long key[3] = {kInfKey, kInfKey, kInfKey};
+  auto load_cell = [&](int64_t m) {
+    double u[NC];
+    const int64_t off = col + Clamp(m, nz) * plane;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) u[c] = in[c][off];
+    return MakeDtCell<3, DA>(u, g);
+  };
+  DtCell<3> dprev = load_cell(m0 - 1);
+  int buf = 0;
+  for (int64_t m = m0; m < m1; ++m) {
+    const DtCell<3> dc = load_cell(m);
+    s_cells[buf][row][lane] = dc;
+    const DtCell<3> dl = ShflUp<3>(dc);
+    __syncthreads();
+    if (own) {
+      key[0] = KMin(key[0], DtKey(FaceDt<3, 0, DA>(x == 0 ? dc : dl, dc, dx0, g.gm1)));
+      if (x == nx - 1) key[0] = KMin(key[0], DtKey(FaceDt<3, 0, DA>(dc, dc, dx0, g.gm1)));
+      const DtCell<3> db = s_cells[buf][row - 1][lane];
+      key[1] = KMin(key[1], DtKey(FaceDt<3, 1, DA>(y == 0 ? dc : db, dc, dx1, g.gm1)));
+      if (y == ny - 1) key[1] = KMin(key[1], DtKey(FaceDt<3, 1, DA>(dc, dc, dx1, g.gm1)));
+      key[2] = KMin(key[2], DtKey(FaceDt<3, 2, DA>(m == 0 ? dc : dprev, dc, dx2, g.gm1)));
+      if (m == nz - 1) key[2] = KMin(key[2], DtKey(FaceDt<3, 2, DA>(dc, dc, dx2, g.gm1)));
+    }
+    dprev = dc;
+    buf ^= 1;
+  }
+  const int64_t nblocks = static_cast<int64_t>(gridDim.x) * gridDim.y;
+  const int64_t bid = blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x;
+  BlockMinToPartials<3, kC3Rows>(key, partials + bid, nblocks);
+}
+
+int64_t Cfl3Tiles(int64_t nx, int64_t ny) {
+  return ((nx + kC3OwnX - 1) / kC3OwnX) * ((ny + kC3OwnY - 1) / kC3OwnY);
+}
+
+int64_t Cfl3GridX(int64_t nx, int64_t ny) {
+  return ((Cfl3Tiles(nx, ny) + kXcds - 1) / kXcds) * kXcds;
 }
 
 // One-workgroup reduction of partial minima.
@@ -623,7 +725,7 @@ __global__ void ResetKeysKernel(unsigned long long* keys, int n, unsigned int* c
   if (i < nc) counters[i] = 0u;
 }
 
-template <int D, bool LARR, class T>
+template <int D, bool LARR, class T, bool FAST>
 int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
   const int64_t blocks = SweepBlocks(D, dir, a.n, a.seg);
   if (blocks <= 0) return FVS_BAD_ARGUMENT;
@@ -634,7 +736,7 @@ int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
       SetError("X sweep: %lld slots exceed the 32-bit slot index", static_cast<long long>(slots));
       return FVS_BAD_ARGUMENT;
     }
-    hipLaunchKernelGGL((SweepXKernel<D, LARR, T>), dim3(static_cast<unsigned>(blocks)),
+    hipLaunchKernelGGL((SweepXKernel<D, LARR, T, FAST>), dim3(static_cast<unsigned>(blocks)),
                        dim3(kBlock), 0, s, a);
   } else {
     const int64_t npencil = (dir == 1) ? a.n[0] * (D == 3 ? a.n[2] : 1) : a.n[0] * a.n[1];
@@ -642,12 +744,12 @@ int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
     const int64_t nseg = (a.n[dir] + a.seg - 1) / a.seg;
     if (bx > 0x7fffffff || nseg > 65535) return FVS_BAD_ARGUMENT;
     if (dir == 1) {
-      hipLaunchKernelGGL((SweepPencilKernel<D, 1, LARR, T>),
+      hipLaunchKernelGGL((SweepPencilKernel<D, 1, LARR, T, FAST>),
                          dim3(static_cast<unsigned>(bx), static_cast<unsigned>(nseg)),
                          dim3(kBlock), 0, s, a);
     } else {
       if constexpr (D == 3) {
-        hipLaunchKernelGGL((SweepPencilKernel<D, 2, LARR, T>),
+        hipLaunchKernelGGL((SweepPencilKernel<D, 2, LARR, T, FAST>),
                            dim3(static_cast<unsigned>(bx), static_cast<unsigned>(nseg)),
                            dim3(kBlock), 0, s, a);
       } else {
@@ -657,6 +759,23 @@ int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
   }
   FVS_LAUNCH_CHECK();
   return FVS_OK;
+}
+
+// FP64: IEEE arithmetic (bit-exact with the reference); FP64 reciprocal:
+// double with refined hardware reciprocals; FP32 flux: float flux arithmetic
+// with the single-precision estimates.
+template <int D>
+int LaunchPrecision(int precision, bool larr, hipStream_t s, int dir, const SweepArgs& a) {
+  if (precision == FVS_FP32_FLUX) {
+    return larr ? LaunchTyped<D, true, float, true>(s, dir, a)
+                : LaunchTyped<D, false, float, true>(s, dir, a);
+  }
+  if (precision == FVS_FP64_RCP) {
+    return larr ? LaunchTyped<D, true, double, true>(s, dir, a)
+                : LaunchTyped<D, false, double, true>(s, dir, a);
+  }
+  return larr ? LaunchTyped<D, true, double, false>(s, dir, a)
+              : LaunchTyped<D, false, double, false>(s, dir, a);
 }
 
 } // namespace
@@ -675,10 +794,9 @@ int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
 }
 
 int64_t CflBlocks(int dim, const int64_t n[3]) {
+  if (dim == 3) return Cfl3GridX(n[0], n[1]) * ((n[2] + kCflSeg - 1) / kCflSeg);
   const int64_t tiles = (n[0] + kTX - 1) / kTX;
-  const int64_t rows = dim == 3 ? n[1] : 1;
-  const int64_t nm = dim == 3 ? n[2] : n[1];
-  return tiles * rows * ((nm + kCflSeg - 1) / kCflSeg);
+  return tiles * ((n[1] + kCflSeg - 1) / kCflSeg);
 }
 
 int LaunchSweep(const fvs_method_desc& desc, hipStream_t s, int dir, const SweepArgs& a) {
@@ -687,13 +805,8 @@ int LaunchSweep(const fvs_method_desc& desc, hipStream_t s, int dir, const Sweep
     return FVS_BAD_ARGUMENT;
   }
   const bool larr = desc.base == FVS_BASE_HLLEM_LARROUTUROU;
-  const bool fp32 = desc.precision == FVS_FP32_FLUX;
-  if (desc.dim == 2) {
-    if (larr) return fp32 ? LaunchTyped<2, true, float>(s, dir, a) : LaunchTyped<2, true, double>(s, dir, a);
-    return fp32 ? LaunchTyped<2, false, float>(s, dir, a) : LaunchTyped<2, false, double>(s, dir, a);
-  }
-  if (larr) return fp32 ? LaunchTyped<3, true, float>(s, dir, a) : LaunchTyped<3, true, double>(s, dir, a);
-  return fp32 ? LaunchTyped<3, false, float>(s, dir, a) : LaunchTyped<3, false, double>(s, dir, a);
+  if (desc.dim == 2) return LaunchPrecision<2>(desc.precision, larr, s, dir, a);
+  return LaunchPrecision<3>(desc.precision, larr, s, dir, a);
 }
 
 int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
@@ -716,22 +829,22 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
                    const int64_t n[3], const double dx[3], unsigned long long* partials,
                    double* out, Ctl* ctl, int* halt, int first_dir, int strang) {
   const int D = desc.dim;
-  const int64_t tiles = (n[0] + kTX - 1) / kTX;
-  const int64_t rows = D == 3 ? n[1] : 1;
+  const int64_t gx = D == 3 ? Cfl3GridX(n[0], n[1]) : (n[0] + kTX - 1) / kTX;
   const int64_t nm = D == 3 ? n[2] : n[1];
-  const int64_t gx = tiles * rows;
   const int64_t gy = (nm + kCflSeg - 1) / kCflSeg;
   if (gx <= 0 || gx > 0x7fffffff || gy <= 0 || gy > 65535) return FVS_BAD_ARGUMENT;
   const double gm1inv = 1.0 / (desc.gamma - 1.0);
+  const bool fast = desc.precision != FVS_FP64;
+  const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
   if (D == 2) {
-    hipLaunchKernelGGL(PatchCflKernel<2>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
-                       dim3(kBlock), 0, s, cons[0], cons[1], cons[2], cons[3], nullptr, n[0],
-                       n[1], int64_t(1), dx[0], dx[1], dx[0], desc.gamma, gm1inv, partials,
-                       halt);
+    auto* k = fast ? PatchCfl2Kernel<true> : PatchCfl2Kernel<false>;
+    hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, s, cons[0], cons[1], cons[2], cons[3], n[0],
+                       n[1], dx[0], dx[1], desc.gamma, gm1inv, partials, halt);
   } else {
-    hipLaunchKernelGGL(PatchCflKernel<3>, dim3(static_cast<unsigned>(gx), static_cast<unsigned>(gy)),
-                       dim3(kBlock), 0, s, cons[0], cons[1], cons[2], cons[3], cons[4], n[0],
-                       n[1], n[2], dx[0], dx[1], dx[2], desc.gamma, gm1inv, partials, halt);
+    auto* k = fast ? PatchCfl3Kernel<true> : PatchCfl3Kernel<false>;
+    hipLaunchKernelGGL(k, grid, dim3(64 * kC3Rows), 0, s, cons[0], cons[1], cons[2], cons[3],
+                       cons[4], n[0], n[1], n[2], dx[0], dx[1], dx[2], desc.gamma, gm1inv,
+                       Cfl3Tiles(n[0], n[1]), partials, halt);
   }
   FVS_LAUNCH_CHECK();
   hipLaunchKernelGGL(FinalizeCflKernel, dim3(1), dim3(kRedBlock), 0, s, partials, gx * gy, D,

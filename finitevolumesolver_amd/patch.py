@@ -131,7 +131,14 @@ def make_desc(equation: PerfectGas, base="HLLEM", reconstruction="MusclHancock",
     d.reconstruction = (L.RECON_MUSCL_HANCOCK if reconstruction in ("MusclHancock", 1)
                         else L.RECON_NONE)
     d.limiter = _LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
-    d.precision = L.FP32_FLUX if precision in ("fp32", "fp32_flux", 1) else L.FP64
+    if precision in ("fp32", "fp32_flux", 1):
+        d.precision = L.FP32_FLUX
+    elif precision in ("fp64_rcp", 2):
+        d.precision = L.FP64_RCP
+    elif precision in ("fp64", 0):
+        d.precision = L.FP64
+    else:
+        raise ValueError(f"unknown precision {precision!r} (fp64, fp64_rcp, fp32_flux)")
     d.quirks = 1 if quirks else 0
     d.newton_guess = L.GUESS_LEARNED if newton_guess in ("learned", 1) else L.GUESS_MEAN_PRESSURE
     d.max_newton_iterations = max_newton_iterations

@@ -68,8 +68,20 @@ enum fvs_limiter {
 };
 
 /* Arithmetic of reconstruction and flux.  The conservative update, Newton
- * iterations and cut-cell divisions are FP64 in every mode. */
-enum fvs_precision { FVS_FP64 = 0, FVS_FP32_FLUX = 1 };
+ * iterations and cut-cell divisions are FP64 in every mode.
+ * FVS_FP64:      IEEE double, correctly rounded division and square root (the
+ *                reference's arithmetic; results are bit-identical to it).
+ * FVS_FP32_FLUX: float reconstruction and flux with the hardware reciprocal
+ *                and square-root estimates; FP64 CFL and accumulation.
+ * FVS_FP64_RCP:  double throughout, but each division is a product with a
+ *                reciprocal refined from the hardware estimate (shared by all
+ *                quotients of one denominator) and each square root comes from
+ *                a refined reciprocal square root: every result is within a
+ *                few units in the last place, so a sweep agrees with FVS_FP64
+ *                to about 1e-15 relative.  Fused sweeps, level drivers and
+ *                batched sweeps honour it; the operator entry points compute
+ *                as FVS_FP64. */
+enum fvs_precision { FVS_FP64 = 0, FVS_FP32_FLUX = 1, FVS_FP64_RCP = 2 };
 
 /* Newton initial guess of the exact Riemann solver. */
 enum fvs_newton_guess {

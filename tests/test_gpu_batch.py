@@ -145,6 +145,7 @@ def test_autotuner_and_fp32_flux(level, cuda):
         b.upload(pool, 0)
         b.set_dt(0.002)
         b.sweep(0, 0)
+        b.fill_ghosts(1)
         b.sweep(1, 1)
     r64 = batch.download(0).cpu().numpy()
     r32 = b32.download(0).cpu().numpy()
