@@ -38,20 +38,19 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_kernel_launches(fn, reps: int, torch) -> float:
-    """Average duration (ms) of fn() measured with HIP events on the stream the
-    library launches on (torch's current stream, handed to the library)."""
-    stream = torch.cuda.current_stream()
-    fn()
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(reps):
-        fn()
-    end.record(stream)
-    end.synchronize()
-    return start.elapsed_time(end) / reps
+def measured_traffic(precision: str, kernel: str):
+    """HBM bytes per launch of the dominant sweep kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by what it
+    reads for a 1 GiB device copy), or None when the summary is absent."""
+    path = ROOT / "profiles" / "sweep3d_256_hbm_traffic.txt"
+    if not path.exists():
+        return None
+    names = {"X": "SweepXKernel", "Y": "SweepPencilKernel<3, 1", "Z": "SweepPencilKernel<3, 2"}
+    mode = {"fp64": "double, false", "fp64_rcp": "double, true", "fp32": "float, true"}[precision]
+    for line in path.read_text().splitlines():
+        if line.startswith(names[kernel]) and mode in line and "bytes/launch" in line:
+            return int(line.split("bytes/launch")[1].split()[0])
+    return None
 
 
 def cpu_baseline(n: int, steps: int, precision: str) -> dict:
@@ -411,21 +410,20 @@ def main() -> int:
     value = updates / (ms * 1e-3) / 1e9
     log(f"[bench] {done} steps ({fails} CFL restarts) in {ms:.2f} ms -> {value:.3f} Gcell-updates/s")
 
-    # Per-kernel timing of each fused sweep direction and of the CFL pass.
-    t_out = torch.empty_like(t_cons)
-    sdt = torch.empty(1, dtype=torch.float64, device=dev)
-    state = lvl.get_state()
-    dt = 0.5 * 0.8 / n
-    per = {}
-    for d, name in enumerate("XYZ"):
-        per[name] = time_kernel_launches(
-            lambda d=d: fvs.split_sweep(fm, state, t_out, dt, dx[d], d, sdt), args.kernel_reps, torch)
-    per["CFL"] = time_kernel_launches(lambda: fvs.patch_stable_dt(fm, state, dx),
-                                      args.kernel_reps, torch)
+    # Per-kernel durations, measured live: the level repeats the timed steps
+    # with HIP events recorded on its stream around every launch.
+    lvl.set_timing(True)
+    lvl.advance(args.steps, batch=args.steps)
+    times = lvl.kernel_times()
+    lvl.set_timing(False)
+    per = {"X": times["sweep_x"], "Y": times["sweep_y"], "Z": times["sweep_z"],
+           "CFL": times["cfl"]}
+    per = {k: ms / max(cnt, 1) for k, (ms, cnt) in per.items()}
     dominant = max(("X", "Y", "Z"), key=lambda k: per[k])
     alg_bytes = BYTES_PER_CELL_UPDATE_3D * n ** 3
     achieved = alg_bytes / (per[dominant] * 1e-3) / 1e9
     log("[bench] per-launch ms: " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
+    traffic = measured_traffic(args.precision, dominant)
 
     result = {
         "metric": "Gcell-updates/s per split sweep, % HBM roofline, L1 err vs FP64 CPU oracle",
@@ -451,7 +449,7 @@ def main() -> int:
         "roofline": {
             "bound": "hbm", "kernel": f"fused sweep {dominant}",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_3D,
             "per_launch_ms": {k: round(v, 4) for k, v in per.items()},
         },
@@ -462,7 +460,7 @@ def main() -> int:
         result["accuracy"] = {"error": str(e)}
     if not args.no_variants:
         variants = {}
-        del lvl, t_out, state
+        del lvl
         torch.cuda.empty_cache()
         for prec in ("fp64", "fp64_rcp", "fp32"):
             if prec == args.precision:

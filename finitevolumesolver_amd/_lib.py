@@ -30,6 +30,8 @@ LIMITER_NO_GRADIENT = 3
 FP64 = 0
 FP32_FLUX = 1
 FP64_RCP = 2
+LEVEL_TIMERS = 4
+TIMER_NAMES = ("sweep_x", "sweep_y", "sweep_z", "cfl")
 GUESS_MEAN_PRESSURE = 0
 GUESS_LEARNED = 1
 SPLIT_GODUNOV = 0
@@ -111,6 +113,8 @@ SIGNATURES = {
     "fvs_level_get_state": (_I, [_P, C.POINTER(_P)]),
     "fvs_level_advance": (_I, [_P, _I, _D, _I, _PD, C.POINTER(_I), C.POINTER(_I)]),
     "fvs_level_time": (_I, [_P, _PD, _PI64]),
+    "fvs_level_set_timing": (_I, [_P, _I]),
+    "fvs_level_kernel_times": (_I, [_P, _PD, _PI64]),
     "fvs_split_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P]),
     "fvs_patch_stable_dt": (_I, [_P, C.POINTER(_P), _PI64, _PD, _P]),
     "fvs_godunov_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P, _P]),

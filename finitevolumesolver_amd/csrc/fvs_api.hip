@@ -700,7 +700,63 @@ struct fvs_level {
   unsigned long long* partials;  // per-workgroup minima
   double* guess[3];       // learned Newton guesses per direction (lazy)
   int cur;                // buffer holding the current state
+  // Kernel timing (off by default): HIP events recorded on the level's stream
+  // around every launch, reduced per kernel class when read.
+  int timing = 0;
+  struct Mark {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> spare;
+  double t_ms[FVS_LEVEL_TIMERS] = {};
+  int64_t t_n[FVS_LEVEL_TIMERS] = {};
 };
+
+namespace {
+
+hipEvent_t TakeEvent(fvs_level* L) {
+  hipEvent_t e = nullptr;
+  if (!L->spare.empty()) {
+    e = L->spare.back();
+    L->spare.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
+// Runs launch() and, when timing is on, brackets it with two events.
+template <class F> int Timed(fvs_level* L, hipStream_t s, int cls, F&& launch) {
+  if (!L->timing) return launch();
+  hipEvent_t a = TakeEvent(L), b = TakeEvent(L);
+  if (!a || !b) {
+    SetError("hipEventCreate failed");
+    return FVS_HIP_ERROR;
+  }
+  FVS_HIP_CHECK(hipEventRecord(a, s));
+  const int rc = launch();
+  FVS_HIP_CHECK(hipEventRecord(b, s));
+  L->marks.push_back({cls, a, b});
+  return rc;
+}
+
+int CollectMarks(fvs_level* L) {
+  if (L->marks.empty()) return FVS_OK;
+  FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
+  for (const auto& m : L->marks) {
+    float ms = 0.0f;
+    FVS_HIP_CHECK(hipEventElapsedTime(&ms, m.a, m.b));
+    L->t_ms[m.cls] += ms;
+    L->t_n[m.cls] += 1;
+    L->spare.push_back(m.a);
+    L->spare.push_back(m.b);
+  }
+  L->marks.clear();
+  return FVS_OK;
+}
+
+} // namespace
 
 extern "C" {
 
@@ -784,7 +840,36 @@ int fvs_level_destroy(fvs_level* L) {
   (void)hipFree(L->partials);
   for (int d = 0; d < 3; ++d)
     if (L->guess[d]) (void)hipFree(L->guess[d]);
+  for (const auto& m : L->marks) {
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
+  for (hipEvent_t e : L->spare) (void)hipEventDestroy(e);
   delete L;
+  return FVS_OK;
+}
+
+int fvs_level_set_timing(fvs_level* L, int on) {
+  if (!L) {
+    SetError("null level");
+    return FVS_BAD_ARGUMENT;
+  }
+  L->timing = on ? 1 : 0;
+  return FVS_OK;
+}
+
+int fvs_level_kernel_times(fvs_level* L, double* ms, int64_t* count) {
+  if (!L || !ms || !count) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (int rc = CollectMarks(L)) return rc;
+  for (int k = 0; k < FVS_LEVEL_TIMERS; ++k) {
+    ms[k] = L->t_ms[k];
+    count[k] = L->t_n[k];
+    L->t_ms[k] = 0.0;
+    L->t_n[k] = 0;
+  }
   return FVS_OK;
 }
 
@@ -904,16 +989,20 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
           if (rc) break;
         }
         const double* gptr[3] = {L->guess[0], L->guess[1], L->guess[2]};
-        rc = LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, learned ? gptr : nullptr,
-                              L->partials, &L->ctl->halt);
+        rc = Timed(L, s, FVS_TIMER_CFL, [&] {
+          return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, learned ? gptr : nullptr,
+                                  L->partials, &L->ctl->halt);
+        });
         for (int d = 0; d < D && rc == FVS_OK; ++d) {
           rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
                                      h->newton_stats, L->ctl);
         }
         if (rc == FVS_OK) rc = LaunchStepDt(s, L->ctl, D, seq[0], strang);
       } else {
-        rc = LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
-                            &L->ctl->halt, seq[0], strang);
+        rc = Timed(L, s, FVS_TIMER_CFL, [&] {
+          return LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
+                                &L->ctl->halt, seq[0], strang);
+        });
       }
       if (rc) break;
       const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
@@ -931,9 +1020,12 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
           if (learned && w > 0)
             rc = MlpFaceGuess(h->mlp, s, D, h->desc.gamma, inp, L->n, seq[w], L->guess[seq[w]]);
           if (rc) break;
-          rc = LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
-                                  &L->ctl->split_dt, 0.0, learned ? L->guess[seq[w]] : nullptr,
-                                  L->partials, &L->ctl->halt, 1);
+          rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] {
+            return LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
+                                      &L->ctl->split_dt, 0.0,
+                                      learned ? L->guess[seq[w]] : nullptr, L->partials,
+                                      &L->ctl->halt, 1);
+          });
           if (rc == FVS_OK)
             rc = LaunchGodunovFinalize(s, L->partials, nb, &L->ctl->last_check, h->newton_stats,
                                        L->ctl);
@@ -949,7 +1041,7 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
         a.halt = &L->ctl->halt;
         a.check = (w > 0);
         a.partials = L->partials;
-        rc = LaunchSweep(h->desc, s, seq[w], a);
+        rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] { return LaunchSweep(h->desc, s, seq[w], a); });
         if (rc) break;
         if (a.check) {
           rc = LaunchFinalizeCheck(s, L->partials, SweepBlocks(D, seq[w], L->n, a.seg), nullptr,

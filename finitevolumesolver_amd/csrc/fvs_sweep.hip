@@ -47,6 +47,42 @@ __device__ __forceinline__ unsigned long long WaveMinKey(unsigned long long k) {
   return k;
 }
 
+// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane i gets
+// lane i - 1 (ShiftUp) or lane i + 1 (ShiftDown); the end lane keeps its own
+// value, as __shfl_up / __shfl_down(x, 1) do.  A DPP move is one VALU
+// instruction with no LDS round trip.
+__device__ __forceinline__ int DppUp(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int DppDown(int x) {
+  return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false);
+}
+template <bool UP> __device__ __forceinline__ double Shift(double x) {
+  const int64_t v = __double_as_longlong(x);
+  const int lo = static_cast<int>(v);
+  const int hi = static_cast<int>(v >> 32);
+  const int rlo = UP ? DppUp(lo) : DppDown(lo);
+  const int rhi = UP ? DppUp(hi) : DppDown(hi);
+  return __longlong_as_double((static_cast<int64_t>(rhi) << 32) | static_cast<uint32_t>(rlo));
+}
+template <bool UP> __device__ __forceinline__ float Shift(float x) {
+  const int v = __float_as_int(x);
+  return __int_as_float(UP ? DppUp(v) : DppDown(v));
+}
+template <class T> __device__ __forceinline__ T ShiftUp(T x) { return Shift<true>(x); }
+template <class T> __device__ __forceinline__ T ShiftDown(T x) { return Shift<false>(x); }
+
+// Loads and stores as a wave-uniform base pointer plus a 32-bit byte offset:
+// the base lives in scalar registers, the offset in one vector register shared
+// by every component, and the global accesses take the scalar-base form with
+// no 64-bit address arithmetic per component.
+__device__ __forceinline__ double LdOff(const double* base, uint32_t byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void StOff(double* base, uint32_t byte_off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
 // Block-wide minimum of NK keys per thread; thread 0 writes them to
 // out[d * stride] (d < NK).  Every thread of the block must call it.
 template <int NK, int NWAVES>
@@ -124,11 +160,11 @@ __device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R,
 template <int D>
 __device__ __forceinline__ DtCell<D> ShflUp(const DtCell<D>& v) {
   DtCell<D> r;
-  r.sq = __shfl_up(v.sq, 1, 64);
+  r.sq = ShiftUp(v.sq);
 #pragma unroll
-  for (int i = 0; i < D; ++i) r.u[i] = __shfl_up(v.u[i], 1, 64);
-  r.h = __shfl_up(v.h, 1, 64);
-  r.a = __shfl_up(v.a, 1, 64);
+  for (int i = 0; i < D; ++i) r.u[i] = ShiftUp(v.u[i]);
+  r.h = ShiftUp(v.h);
+  r.a = ShiftUp(v.a);
   return r;
 }
 
@@ -210,12 +246,19 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
   const int64_t q64 = wave * kXOwned + lane - 1;
   const bool valid = q64 >= 0 && q64 < static_cast<int64_t>(Q);
-  const uint32_t q = valid ? static_cast<uint32_t>(q64) : 0u;
+  // Lanes outside the slot sequence take its nearest end, so that slots grow
+  // with the lane index and every lane's row is at least lane 0's (the row
+  // offsets below are unsigned and relative to lane 0's row).
+  const uint32_t q = valid ? static_cast<uint32_t>(q64) : (q64 < 0 ? 0u : Q - 1u);
   const uint32_t row = q / S;
   const uint32_t sl = q - row * S;           // slot within the row
   const int64_t cell = static_cast<int64_t>(sl) - 1;
   const bool real = valid && cell >= 0 && cell < nx;
-  const int64_t rbase = static_cast<int64_t>(row) * nx;
+  // The wave's first row is its uniform base; lanes address their row
+  // relative to it (a wave spans at most 64 / (nx + 2) + 2 rows).
+  const uint32_t row0 = __builtin_amdgcn_readfirstlane(row);
+  const uint32_t rbase = (row - row0) * static_cast<uint32_t>(nx);
+  const int64_t wbase = static_cast<int64_t>(row0) * nx;
 
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
   const T lh = static_cast<T>(0.5 * dt / a.dx);
@@ -225,14 +268,14 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
 
   double u[NC], um[NC], up[NC];
   {
-    const int64_t off = rbase + Clamp(cell, nx);
+    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(Clamp(cell, nx)));
 #pragma unroll
-    for (int k = 0; k < NC; ++k) u[k] = a.in[k][off];
+    for (int k = 0; k < NC; ++k) u[k] = LdOff(a.in[k] + wbase, off);
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    um[k] = __shfl_up(u[k], 1, 64);
-    up[k] = __shfl_down(u[k], 1, 64);
+    um[k] = ShiftUp(u[k]);
+    up[k] = ShiftDown(u[k]);
   }
   // The wave's edge lanes read their outer neighbour themselves, and so do
   // the ghost slots, whose shuffled neighbour belongs to the adjacent row.
@@ -241,14 +284,14 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   // complete state, up to the sign of zero components), so every lane runs
   // the same reconstruction and no wave takes two paths.
   if (lane == 0 || cell < 0) {
-    const int64_t off = rbase + Clamp(cell - 1, nx);
+    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(Clamp(cell - 1, nx)));
 #pragma unroll
-    for (int k = 0; k < NC; ++k) um[k] = a.in[k][off];
+    for (int k = 0; k < NC; ++k) um[k] = LdOff(a.in[k] + wbase, off);
   }
   if (lane == 63 || cell >= nx) {
-    const int64_t off = rbase + Clamp(cell + 1, nx);
+    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(Clamp(cell + 1, nx)));
 #pragma unroll
-    for (int k = 0; k < NC; ++k) up[k] = a.in[k][off];
+    for (int k = 0; k < NC; ++k) up[k] = LdOff(a.in[k] + wbase, off);
   }
 
   T recL[NQ], recR[NQ];
@@ -265,10 +308,10 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   }
   T recRprev[NQ];
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) recRprev[k] = __shfl_up(recR[k], 1, 64);
+  for (int k = 0; k < NQ; ++k) recRprev[k] = ShiftUp(recR[k]);
   typename TA::Inv irRprev;
   if constexpr (TA::kFast) {
-    irRprev.y = __shfl_up(irR.y, 1, 64);
+    irRprev.y = ShiftUp(irR.y);
   } else {
     irRprev = TA::inv(recRprev[0]);
   }
@@ -284,14 +327,15 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   }
   T Fnext[NC];
 #pragma unroll
-  for (int k = 0; k < NC; ++k) Fnext[k] = __shfl_down(F[k], 1, 64);
+  for (int k = 0; k < NC; ++k) Fnext[k] = ShiftDown(F[k]);
 
   const bool owned = lane >= 1 && lane <= kXOwned;
   if (owned && real) {
-    const int64_t off = rbase + cell;
+    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(cell));
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      a.out[k][off] = u[k] + dt_dx * (static_cast<double>(F[k]) - static_cast<double>(Fnext[k]));
+      StOff(a.out[k] + wbase, off,
+            u[k] + dt_dx * (static_cast<double>(F[k]) - static_cast<double>(Fnext[k])));
     }
   }
   if (a.check) {
@@ -319,6 +363,11 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
   const bool active = p < npencil;
   const int64_t stride = (DIR == 1) ? nx : nx * ny;
   const int64_t base = (DIR == 1) ? (p % nx) + (p / nx) * nx * ny : p;
+  // Uniform part of the addresses: the plane of the block's first pencil (Y
+  // sweeps; a block spans at most 256 / nx + 2 planes) and the marching index.
+  const int64_t ubase =
+      (DIR == 1) ? ((static_cast<int64_t>(blockIdx.x) * kBlock) / nx) * nx * ny : 0;
+  const uint32_t toff = 8u * static_cast<uint32_t>(base - ubase);
   const int64_t j0 = static_cast<int64_t>(blockIdx.y) * a.seg;
   const int64_t j1 = (j0 + a.seg < nd) ? j0 + a.seg : nd;
 
@@ -332,9 +381,9 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
   if (active) {
     double Um[NC], U0[NC], Up[NC], Unext[NC];
     auto load = [&](double* u, int64_t j) {
-      const int64_t off = base + Clamp(j, nd) * stride;
+      const int64_t uoff = ubase + Clamp(j, nd) * stride;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) u[c] = a.in[c][off];
+      for (int c = 0; c < NC; ++c) u[c] = LdOff(a.in[c] + uoff, toff);
     };
     load(Um, j0 - 1);
     load(U0, j0);
@@ -392,11 +441,11 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
       T F[NC];
       FaceFlux<D, DIR, LARR, T, TA>(F, recRprev, irRprev, recL, irL, g, a.quirk3d);
       if (j > j0) {
-        const int64_t off = base + (j - 1) * stride;
+        const int64_t uoff = ubase + (j - 1) * stride;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          a.out[c][off] = Um[c] + dt_dx * (static_cast<double>(Fprev[c]) -
-                                           static_cast<double>(F[c]));
+          StOff(a.out[c] + uoff, toff,
+                Um[c] + dt_dx * (static_cast<double>(Fprev[c]) - static_cast<double>(F[c])));
         }
       }
       if (a.check && (j < j1 || j1 == nd)) {
@@ -466,11 +515,23 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
     for (int c = 0; c < NC; ++c) u[c] = in[c][off];
     return MakeDtCell<D, DA>(u, g);
   };
+  auto load = [&](double* u, int64_t m) {
+    const int64_t off = xs + rowoff + Clamp(m, nm) * ms;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) u[c] = in[c][off];
+  };
 
-  DtCell<D> dprev = load_cell(xs + rowoff + Clamp(m0 - 1, nm) * ms);
+  double ucur[NC], unext[NC];
+  load(unext, m0 - 1);
+  DtCell<D> dprev = MakeDtCell<D, DA>(unext, g);
+  load(ucur, m0);
   for (int64_t m = m0; m < m1; ++m) {
     const int64_t off = xs + rowoff + m * ms;
-    const DtCell<D> dc = load_cell(off);
+    // The next row's load is in flight while this row is reduced.
+    if (m + 1 < m1) load(unext, m + 1);
+    const DtCell<D> dc = MakeDtCell<D, DA>(ucur, g);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ucur[c] = unext[c];
     // March direction y.
     {
       const DtCell<D>& l = (m == 0) ? dc : dprev;
@@ -551,17 +612,27 @@ __global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
 
   __shared__ DtCell<3> s_cells[2][kC3Rows][64];
   unsigned long long key[3] = {kInfKey, kInfKey, kInfKey};
-  auto load_cell = [&](int64_t m) {
-    double u[NC];
+  auto load = [&](double* u, int64_t m) {
     const int64_t off = col + Clamp(m, nz) * plane;
 #pragma unroll
     for (int c = 0; c < NC; ++c) u[c] = in[c][off];
-    return MakeDtCell<3, DA>(u, g);
   };
-  DtCell<3> dprev = load_cell(m0 - 1);
+  double ucur[NC], un1[NC], un2[NC];
+  load(un2, m0 - 1);
+  DtCell<3> dprev = MakeDtCell<3, DA>(un2, g);
+  load(ucur, m0);
+  load(un1, m0 + 1);
   int buf = 0;
   for (int64_t m = m0; m < m1; ++m) {
-    const DtCell<3> dc = load_cell(m);
+    // The loads of the next two planes are in flight while this plane is
+    // reduced (indices are clamped, so reading past the segment is harmless).
+    load(un2, m + 2);
+    const DtCell<3> dc = MakeDtCell<3, DA>(ucur, g);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      ucur[c] = un1[c];
+      un1[c] = un2[c];
+    }
     s_cells[buf][row][lane] = dc;
     const DtCell<3> dl = ShflUp<3>(dc);
     __syncthreads();
@@ -729,6 +800,16 @@ template <int D, bool LARR, class T, bool FAST>
 int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
   const int64_t blocks = SweepBlocks(D, dir, a.n, a.seg);
   if (blocks <= 0) return FVS_BAD_ARGUMENT;
+  // Per-thread byte offsets from the uniform base are 32-bit: a Z sweep spans
+  // one x-y plane, a Y sweep 256 / nx + 2 planes.
+  const int64_t span = dir == 0 ? 0
+                       : dir == 1 ? (kBlock / a.n[0] + 2) * a.n[0] * a.n[1]
+                                  : a.n[0] * a.n[1];
+  if (span * 8 >= (int64_t(1) << 32)) {
+    SetError("fused sweep: a %lld x %lld plane exceeds the 32-bit offset range",
+             static_cast<long long>(a.n[0]), static_cast<long long>(a.n[1]));
+    return FVS_BAD_ARGUMENT;
+  }
   if (dir == 0) {
     if (blocks > 0x7fffffff) return FVS_BAD_ARGUMENT;
     const int64_t slots = (a.n[0] + 2) * a.n[1] * a.n[2];

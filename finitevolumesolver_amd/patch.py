@@ -343,3 +343,14 @@ class SinglePatchLevel:
         cyc = C.c_int64()
         L.check(self._lib.fvs_level_time(self.h, C.byref(t), C.byref(cyc)))
         return t.value, cyc.value
+
+    def set_timing(self, on: bool = True) -> None:
+        """Brackets every kernel launch of advance() with HIP events."""
+        L.check(self._lib.fvs_level_set_timing(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        """{class: (summed ms, launches)} since the last call (see set_timing)."""
+        ms = (C.c_double * L.LEVEL_TIMERS)()
+        cnt = (C.c_int64 * L.LEVEL_TIMERS)()
+        L.check(self._lib.fvs_level_kernel_times(self.h, ms, cnt))
+        return {name: (ms[k], cnt[k]) for k, name in enumerate(L.TIMER_NAMES)}

@@ -251,6 +251,23 @@ int fvs_level_advance(fvs_level* level, int max_steps, double final_time,
 /* Time point and cycle count of the current state. */
 int fvs_level_time(fvs_level* level, double* t, int64_t* cycles);
 
+/* Kernel timing of the level driver (no counterpart in the reference; it
+ * replaces reading a profiler).  With timing on, fvs_level_advance brackets
+ * every launch on the level's stream with HIP events; fvs_level_kernel_times
+ * synchronises the stream, writes the summed milliseconds and launch counts
+ * per class since the last read (FVS_LEVEL_TIMERS entries each) and resets
+ * them.  Classes: split sweep in x / y / z (the sweep kernel alone) and the
+ * step-start CFL pass (reduction and finalize). */
+enum fvs_level_timer {
+  FVS_TIMER_SWEEP_X = 0,
+  FVS_TIMER_SWEEP_Y = 1,
+  FVS_TIMER_SWEEP_Z = 2,
+  FVS_TIMER_CFL = 3,
+  FVS_LEVEL_TIMERS = 4
+};
+int fvs_level_set_timing(fvs_level* level, int on);
+int fvs_level_kernel_times(fvs_level* level, double* ms, int64_t* count);
+
 /* One fused split sweep (flux + update + completion + CFL check of the
  * input) from cons_in to cons_out (distinct arrays, D+2 components of
  * n[0]*n[1]*n[2] each, x fastest, transmissive boundaries).  Writes the
