@@ -201,55 +201,78 @@ def bench_level(state, dx, n, steps, precision, torch, fvs) -> dict:
 
 
 def bench_godunov(n: int, steps: int, torch, fvs) -> dict:
-    """3D first-order exact-Godunov on the Fourier-mode field (seeds 11..15):
-    plain FP64 (mean-pressure Newton guess) against the learned guess (a
-    network trained here on the GPU), both FP64 Newton; L1 density error of
-    the learned form against the plain FP64 GPU result (itself within 1e-12
-    of the CPU oracle)."""
-    from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
+    """3D first-order exact Godunov at n^3 on two fields: the smooth
+    Fourier-mode field (seeds 11-15) and a strong-wave field of 4^3-cell
+    blocks (density and pressure log-uniform in [0.1, 10], velocities in
+    [-1, 1]).  Four forms per field: plain FP64 (mean-pressure Newton
+    guess), learned guess (network trained here on the GPU, inferred inline
+    on the matrix cores), mixed precision (FP64 Newton with exp/log powers,
+    float sample and flux, FP64 update) and mixed precision with the learned
+    guess.  L1 density error of each form against the plain FP64 GPU result
+    (itself within 1e-12 of the CPU oracle)."""
+    from finitevolumesolver_amd.initial_data import blocky_field_3d_device, fourier_modes_3d_device
     from finitevolumesolver_amd.newton_guess import LearnedNewtonGuess
-    log(f"[godunov] {n}^3 initial state")
-    state, dx = fourier_modes_3d_device(n, seed0=11)
     res = {"workload": f"3D {n}^3 exact Godunov (first order), Godunov split, CFL 0.8, "
-                       f"FP64 Newton, Fourier-mode field seeds 11-15"}
-    finals = {}
+                       f"FP64 Newton; smooth Fourier-mode field and strong-wave block field"}
     t0 = time.perf_counter()
     net = LearnedNewtonGuess(seed=1)
-    counts = net.generate(1 << 20, seed=42)
-    hist = net.train(epochs=24, batch=1024, lr=5e-3, huber=0.25, seed=3)
+    counts = net.generate(1 << 22, seed=42)
+    hist = net.train(epochs=12, batch=1024, lr=5e-3, huber=0.25, seed=3)
     torch.cuda.synchronize()
-    res["training"] = {"pairs": 1 << 20, "usable": counts["usable"], "vacuum": counts["vacuum"],
-                       "epochs": 24, "seconds": round(time.perf_counter() - t0, 2),
+    res["training"] = {"pairs": 1 << 22, "usable": counts["usable"], "vacuum": counts["vacuum"],
+                       "epochs": 12, "seconds": round(time.perf_counter() - t0, 2),
                        "final_loss": float(hist[-1])}
-    for form in ("plain", "learned"):
-        fm = fvs.FluxMethod(fvs.PerfectGas(3), base="Godunov", reconstruction="None")
-        if form == "learned":
-            net.attach(fm)
-        lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
-        lvl.set_state(state)
-        fm.newton_stats(reset=True)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev1 = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        ev0.record()
-        done, fails = lvl.advance(steps, batch=steps)
-        ev1.record()
-        torch.cuda.synchronize()
-        ms = ev0.elapsed_time(ev1)
-        st = fm.newton_stats()
-        res[form] = {"value": round(n ** 3 * 3 * done / (ms * 1e-3) / 1e9, 4),
-                     "unit": "Gcell-updates/s", "steps": done,
-                     "ms_per_step": round(ms / max(done, 1), 3),
-                     "newton_iterations_per_face": round(st["iterations"] / max(st["faces"], 1), 4),
-                     "newton_max_iterations": st["max_iterations"],
-                     "fallbacks": st["fallbacks"], "not_converged": st["not_converged"]}
-        finals[form] = lvl.get_state()[0].clone()
-        log(f"[godunov] {form}: {res[form]}")
-        del lvl
-    d = (finals["learned"] - finals["plain"]).abs()
-    res["learned_vs_plain"] = {"l1_density": float(d.sum() / finals["plain"].abs().sum()),
-                               "linf_density": float((d / finals["plain"].abs()).max())}
-    res["speedup_learned_over_plain"] = round(res["learned"]["value"] / res["plain"]["value"], 4)
+    forms = (("plain", "fp64", False), ("learned", "fp64", True),
+             ("mixed", "fp32", False), ("mixed_learned", "fp32", True))
+    for field in ("smooth", "strong"):
+        log(f"[godunov] {n}^3 {field} field")
+        if field == "smooth":
+            state, dx = fourier_modes_3d_device(n, seed0=11)
+        else:
+            state, dx = blocky_field_3d_device(n, seed=5)
+        out = {}
+        finals = {}
+        for form, prec, learned in forms:
+            fm = fvs.FluxMethod(fvs.PerfectGas(3), base="Godunov", reconstruction="None",
+                                precision=prec)
+            if learned:
+                net.attach(fm)
+            lvl = fvs.SinglePatchLevel(fm, (n, n, n), dx, splitting="Godunov", cfl=0.8)
+            lvl.set_state(state)
+            fm.newton_stats(reset=True)
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            ev0.record()
+            done, fails = lvl.advance(steps, batch=steps)
+            ev1.record()
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1)
+            st = fm.newton_stats()
+            out[form] = {"value": round(n ** 3 * 3 * done / (ms * 1e-3) / 1e9, 4),
+                         "unit": "Gcell-updates/s", "steps": done,
+                         "ms_per_step": round(ms / max(done, 1), 3),
+                         "newton_iterations_per_face":
+                             round(st["iterations"] / max(st["faces"], 1), 4),
+                         "newton_max_iterations": st["max_iterations"],
+                         "fallback_rate": round(st["fallbacks"] / max(st["faces"], 1), 6),
+                         "not_converged": st["not_converged"]}
+            finals[form] = lvl.get_state()[0].clone()
+            log(f"[godunov] {field} {form}: {out[form]}")
+            del lvl
+        ref = finals["plain"]
+        for form in ("learned", "mixed", "mixed_learned"):
+            d = (finals[form] - ref).abs()
+            out[form]["l1_density_vs_fp64"] = float(d.sum() / ref.abs().sum())
+            out[form]["linf_density_vs_fp64"] = float((d / ref.abs()).max())
+        it0 = out["plain"]["newton_iterations_per_face"]
+        out["iterations_saved_learned"] = round(1.0 - out["learned"]["newton_iterations_per_face"]
+                                                / max(it0, 1e-30), 4)
+        out["speedup_mixed_learned_over_plain"] = round(
+            out["mixed_learned"]["value"] / out["plain"]["value"], 4)
+        res[field] = out
+        del state, finals
+        torch.cuda.empty_cache()
     return res
 
 

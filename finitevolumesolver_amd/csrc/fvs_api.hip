@@ -698,7 +698,6 @@ struct fvs_level {
   double* buf[3][5];
   Ctl* ctl;               // device control block
   unsigned long long* partials;  // per-workgroup minima
-  double* guess[3];       // learned Newton guesses per direction (lazy)
   int cur;                // buffer holding the current state
   // Kernel timing (off by default): HIP events recorded on the level's stream
   // around every launch, reduced per kernel class when read.
@@ -838,8 +837,6 @@ int fvs_level_destroy(fvs_level* L) {
   (void)hipFree(L->block);
   (void)hipFree(L->ctl);
   (void)hipFree(L->partials);
-  for (int d = 0; d < 3; ++d)
-    if (L->guess[d]) (void)hipFree(L->guess[d]);
   for (const auto& m : L->marks) {
     (void)hipEventDestroy(m.a);
     (void)hipEventDestroy(m.b);
@@ -941,14 +938,9 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
   const int D = h->desc.dim;
   const bool strang = L->patch.splitting == FVS_SPLIT_STRANG;
   const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
-  const bool learned = godunov && h->mlp != nullptr;
-  if (learned) {
-    for (int d = 0; d < D; ++d) {
-      if (L->guess[d]) continue;
-      const int64_t nf = (L->n[0] + (d == 0)) * (L->n[1] + (d == 1)) * (L->n[2] + (d == 2));
-      FVS_HIP_CHECK(hipMalloc(&L->guess[d], nf * sizeof(double)));
-    }
-  }
+  // With a network attached, the Godunov kernels infer each face's guess
+  // inline (no guess arrays, no extra pass over the state).
+  const fvs_mlp* net = godunov ? h->mlp : nullptr;
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
   FVS_HIP_CHECK(hipStreamSynchronize(s));
@@ -983,15 +975,9 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
       for (int q = 0; q < D + 2; ++q) cin[q] = L->buf[cur][q];
       if (godunov) {
         const int64_t nb = GodunovCflBlocks(D, L->n);
-        if (learned) {
-          for (int d = 0; d < D && rc == FVS_OK; ++d)
-            rc = MlpFaceGuess(h->mlp, s, D, h->desc.gamma, cin, L->n, d, L->guess[d]);
-          if (rc) break;
-        }
-        const double* gptr[3] = {L->guess[0], L->guess[1], L->guess[2]};
         rc = Timed(L, s, FVS_TIMER_CFL, [&] {
-          return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, learned ? gptr : nullptr,
-                                  L->partials, &L->ctl->halt);
+          return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, nullptr, L->partials,
+                                  &L->ctl->halt, net);
         });
         for (int d = 0; d < D && rc == FVS_OK; ++d) {
           rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
@@ -1015,16 +1001,10 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
         for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
         if (godunov) {
           const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, 32);
-          // The first sweep sees the step-start state: its guesses are the
-          // CFL pass's; later sweeps get fresh ones for their input.
-          if (learned && w > 0)
-            rc = MlpFaceGuess(h->mlp, s, D, h->desc.gamma, inp, L->n, seq[w], L->guess[seq[w]]);
-          if (rc) break;
           rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] {
             return LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
-                                      &L->ctl->split_dt, 0.0,
-                                      learned ? L->guess[seq[w]] : nullptr, L->partials,
-                                      &L->ctl->halt, 1);
+                                      &L->ctl->split_dt, 0.0, nullptr, L->partials,
+                                      &L->ctl->halt, 1, net);
           });
           if (rc == FVS_OK)
             rc = LaunchGodunovFinalize(s, L->partials, nb, &L->ctl->last_check, h->newton_stats,

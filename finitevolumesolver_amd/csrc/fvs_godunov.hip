@@ -10,10 +10,12 @@
 // its tolerance and its safeguards are unchanged.  Newton statistics
 // (iterations, non-converged faces) are accumulated per workgroup.
 #include "fvs_internal.hpp"
+#include "fvs_mlp_infer.hpp"
 #include "fvs_riemann.hpp"
 #include "fvs_views.hpp"
 
 #include <cstring>
+#include <type_traits>
 
 namespace fvs {
 
@@ -95,6 +97,8 @@ struct GodunovArgs {
   double tol;
   int max_it;
   const double* guess;   // per-face initial guess (nullable)
+  const short* tiles;    // learned guess computed inline (nullable)
+  const float* bias;
   unsigned long long* partials;  // 5 per workgroup
   int* halt;
   int seg;
@@ -115,8 +119,34 @@ __device__ __forceinline__ void ShflUpState(const double* q, double* r) {
   for (int k = 0; k < D + 4; ++k) r[k] = __shfl_up(q[k], 1, 64);
 }
 
-// One face: Newton (shared by flux and CFL), flux, dt.
+// Mixed form of the flux: the sampled state and its physical flux in float.
 template <int D, int DIR>
+__device__ __forceinline__ void GodunovFluxMixed(double* flux, const double* L, const double* R,
+                                                 const MiddleState& ms, const Gas<double>& g) {
+  constexpr int NQ = D + 4;
+  float lf[NQ], rf[NQ], sol[NQ], ff[D + 2];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    lf[k] = static_cast<float>(L[k]);
+    rf[k] = static_cast<float>(R[k]);
+    sol[k] = 0.0f;
+  }
+  Gas<float> gf;
+  gf.gamma = static_cast<float>(g.gamma);
+  gf.gm1inv = static_cast<float>(g.gm1inv);
+  gf.gm1 = static_cast<float>(g.gm1);
+  if (0.0 < L[0] && 0.0 < R[0])
+    SampleRiemannT<D, DIR, float>(sol, lf, rf, static_cast<float>(ms.pM),
+                                  static_cast<float>(ms.uM), gf);
+  PhysFlux<D, DIR, float>(ff, sol);
+#pragma unroll
+  for (int k = 0; k < D + 2; ++k) flux[k] = static_cast<double>(ff[k]);
+}
+
+// One face: Newton (shared by flux and CFL), flux, dt.  MIXED: the Newton
+// iteration stays FP64 (with exp/log powers) and the sample and flux are
+// float.
+template <int D, int DIR, bool MIXED>
 __device__ __forceinline__ void GodunovFace(const double* L, const double* R, double guess,
                                             bool has_guess, const GodunovArgs& a,
                                             const Gas<double>& g, double* F, double* dt_face,
@@ -126,22 +156,28 @@ __device__ __forceinline__ void GodunovFace(const double* L, const double* R, do
   np.max_it = a.max_it;
   using I = Idx<D>;
   const double mean = 0.5 * (L[I::kP] + R[I::kP]);
-  MiddleState ms = ComputeMiddleState<D, DIR>(L, R, g, np, has_guess ? guess : mean);
+  MiddleState ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, has_guess ? guess : mean);
   int it = ms.iterations;
   int fallback = 0;
   if (has_guess && !ms.converged) {
     // The reference's initial guess is the fallback for a learned guess.
-    ms = ComputeMiddleState<D, DIR>(L, R, g, np, mean);
+    ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, mean);
     it += ms.iterations;
     fallback = 1;
   }
-  GodunovFluxFromMiddle<D, DIR>(F, L, R, ms, g);
+  if constexpr (MIXED) {
+    GodunovFluxMixed<D, DIR>(F, L, R, ms, g);
+  } else {
+    GodunovFluxFromMiddle<D, DIR>(F, L, R, ms, g);
+  }
   *dt_face = GodunovDtFromMiddle<D, DIR>(L, R, ms.pM, g, a.dx);
   *iters = it;
   *conv = ms.converged | (fallback << 1);
 }
 
-template <int D>
+// LEARNED: the network's guess is inferred inline (a.tiles set); without it
+// the kernel carries none of the matrix-core code or its registers.
+template <int D, bool MIXED, bool LEARNED>
 __global__ __launch_bounds__(kBlock) void GodunovSweepX(GodunovArgs a) {
   constexpr int NC = D + 2, NQ = D + 4;
   if (a.halt && *a.halt) return;
@@ -175,12 +211,18 @@ __global__ __launch_bounds__(kBlock) void GodunovSweepX(GodunovArgs a) {
   const bool owned = lane >= 1 && lane <= kXOwned;
   double F[NC];
   unsigned long long key = kInfKey, it_sum = 0, it_max = 0, nfail = 0, nfaces = 0;
+  // Inline learned guess: every lane takes part in the wave's matrix-core step.
+  double inline_guess = 0.0;
+  if constexpr (LEARNED) inline_guess = mlp::LearnedGuess<D, 0>(L, u, a.tiles, a.bias);
   if (face_lane) {
     const int64_t f = static_cast<int64_t>(sl) - 1;
-    const double guess = a.guess ? a.guess[static_cast<int64_t>(row) * (nx + 1) + f] : 0.0;
+    const double guess = LEARNED   ? inline_guess
+                         : a.guess ? a.guess[static_cast<int64_t>(row) * (nx + 1) + f]
+                                   : 0.0;
     double dtf;
     int it, conv;
-    GodunovFace<D, 0>(L, u, guess, a.guess != nullptr, a, g, F, &dtf, &it, &conv);
+    GodunovFace<D, 0, MIXED>(L, u, guess, LEARNED || a.guess != nullptr, a, g, F, &dtf, &it,
+                             &conv);
     if (owned) {
       key = DtKey(dtf);
       it_sum = it;
@@ -203,15 +245,18 @@ __global__ __launch_bounds__(kBlock) void GodunovSweepX(GodunovArgs a) {
   BlockReduceGodunov(key, it_sum, it_max, nfail, nfaces, a.partials + 5 * blockIdx.x);
 }
 
-template <int D, int DIR>
-__global__ __launch_bounds__(kBlock) void GodunovSweepPencil(GodunovArgs a) {
+template <int D, int DIR, bool MIXED, bool LEARNED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void GodunovSweepPencil(GodunovArgs a) {
   constexpr int NC = D + 2, NQ = D + 4;
   if (a.halt && *a.halt) return;
   const int64_t nx = a.n[0], ny = a.n[1], nz = a.n[2];
   const int64_t nd = a.n[DIR];
   const int64_t npencil = (DIR == 1) ? nx * nz : nx * ny;
-  const int64_t p = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const bool active = p < npencil;
+  const int64_t p0 = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const bool active = p0 < npencil;
+  // Lanes past the last pencil repeat it (all lanes of a wave march together,
+  // which the inline matrix-core guess needs); their results are not kept.
+  const int64_t p = active ? p0 : npencil - 1;
   const int64_t stride = (DIR == 1) ? nx : nx * ny;
   const int64_t base = (DIR == 1) ? (p % nx) + (p / nx) * nx * ny : p;
   // Face (x, j, z) of direction DIR in the face box (n with n[DIR] + 1).
@@ -225,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void GodunovSweepPencil(GodunovArgs a) {
   (void)nz;
 
   unsigned long long key = kInfKey, it_sum = 0, it_max = 0, nfail = 0, nfaces = 0;
-  if (active) {
+  {
     auto load = [&](double* q, int64_t j) {
       const int64_t off = base + Clamp(j, nd) * stride;
 #pragma unroll
@@ -239,16 +284,22 @@ __global__ __launch_bounds__(kBlock) void GodunovSweepPencil(GodunovArgs a) {
       load(q0, j);
       double F[NC], dtf;
       int it, conv;
-      const double guess = a.guess ? a.guess[fbase + j * fstride] : 0.0;
-      GodunovFace<D, DIR>(qm, q0, guess, a.guess != nullptr, a, g, F, &dtf, &it, &conv);
-      if (j < j1 || j1 == nd) {
+      double guess = 0.0;
+      if constexpr (LEARNED) {
+        guess = mlp::LearnedGuess<D, DIR>(qm, q0, a.tiles, a.bias);
+      } else if (a.guess) {
+        guess = a.guess[fbase + j * fstride];
+      }
+      GodunovFace<D, DIR, MIXED>(qm, q0, guess, LEARNED || a.guess != nullptr, a, g, F, &dtf,
+                                 &it, &conv);
+      if (active && (j < j1 || j1 == nd)) {
         key = KMin(key, DtKey(dtf));
         it_sum += it;
         it_max = (unsigned long long)it > it_max ? it : it_max;
         nfail += (conv & 1) ? 0 : 1;
         nfaces += 1 + (static_cast<unsigned long long>(conv >> 1) << 32);
       }
-      if (a.update && j > j0) {
+      if (a.update && active && j > j0) {
         const int64_t off = base + (j - 1) * stride;
 #pragma unroll
         for (int k = 0; k < NC; ++k) a.out[k][off] = qm[k] + dt_dx * (Fprev[k] - F[k]);
@@ -439,22 +490,28 @@ struct GodunovCflArgs {
   double gamma, gm1inv, tol;
   int max_it;
   const double* guess[3];
+  const short* tiles;  // learned guess computed inline (nullable)
+  const float* bias;
   unsigned long long* partials;  // 5 per workgroup and direction
   int* halt;
 };
 
-template <int D, int DIR>
+// Newton seed of a face: the inline learned guess (lg, computed by the whole
+// wave), else the face's guess array entry, else the mean pressure.
+template <int D, int DIR, bool MIXED, bool LEARNED>
 __device__ __forceinline__ void CflFace(const double* L, const double* R, const GodunovCflArgs& a,
-                                        const Gas<double>& g, int64_t fidx,
+                                        const Gas<double>& g, int64_t fidx, double lg,
                                         unsigned long long* acc) {
   NewtonParams np;
   np.tol = a.tol;
   np.max_it = a.max_it;
   const double mean = 0.5 * (L[D + 2] + R[D + 2]);
-  MiddleState ms = ComputeMiddleState<D, DIR>(L, R, g, np, a.guess[DIR] ? a.guess[DIR][fidx] : mean);
+  const bool seeded = LEARNED || a.guess[DIR] != nullptr;
+  const double x0 = LEARNED ? lg : a.guess[DIR] ? a.guess[DIR][fidx] : mean;
+  MiddleState ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, x0);
   unsigned long long it = ms.iterations, fb = 0;
-  if (a.guess[DIR] && !ms.converged) {
-    ms = ComputeMiddleState<D, DIR>(L, R, g, np, mean);
+  if (seeded && !ms.converged) {
+    ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, mean);
     it += ms.iterations;
     fb = 1;
   }
@@ -466,8 +523,8 @@ __device__ __forceinline__ void CflFace(const double* L, const double* R, const 
   acc[4] += 1 + (fb << 32);
 }
 
-template <int D>
-__global__ __launch_bounds__(kBlock) void GodunovCflKernel(GodunovCflArgs a) {
+template <int D, bool MIXED, bool LEARNED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void GodunovCflKernel(GodunovCflArgs a) {
   constexpr int NC = D + 2, NQ = D + 4;
   constexpr int M = D - 1;
   if (a.halt && *a.halt) return;
@@ -504,6 +561,18 @@ __global__ __launch_bounds__(kBlock) void GodunovCflKernel(GodunovCflArgs a) {
     const int64_t fx = nx + (d == 0), fy = ny + (d == 1);
     return cx + fx * (cy + fy * cz);
   };
+  // Waves with an active lane run the inline learned guess with all lanes.
+  const bool wave_live = __ballot(active) != 0;
+  auto learned = [&](auto dir_tag, const double* l, const double* r) -> double {
+    constexpr int DD = decltype(dir_tag)::value;
+    if constexpr (LEARNED) {
+      if (wave_live) return mlp::LearnedGuess<D, DD>(l, r, a.tiles, a.bias);
+    }
+    return 0.0;
+  };
+  using X0 = std::integral_constant<int, 0>;
+  using X1 = std::integral_constant<int, 1>;
+  using XM = std::integral_constant<int, M>;
   double qprev[NQ];
   load(xs + rowoff + Clamp(m0 - 1, nm) * ms, qprev);
   for (int64_t m = m0; m < m1; ++m) {
@@ -511,27 +580,38 @@ __global__ __launch_bounds__(kBlock) void GodunovCflKernel(GodunovCflArgs a) {
     const int64_t cy = (D == 3) ? y : m, cz = (D == 3) ? m : 0;
     double qc[NQ];
     load(off, qc);
-    if (active) {
-      // March direction.
+    // March direction.
+    {
       const double* l = (m == 0) ? qc : qprev;
-      CflFace<D, M>(l, qc, a, g, (M == 1) ? fidx(1, xs, m, 0) : fidx(2, xs, cy, m), acc[M]);
-      if (m == nm - 1)
-        CflFace<D, M>(qc, qc, a, g, (M == 1) ? fidx(1, xs, m + 1, 0) : fidx(2, xs, cy, m + 1), acc[M]);
-      if constexpr (D == 3) {
-        double qy[NQ];
-        if (y == 0) {
-          for (int k = 0; k < NQ; ++k) qy[k] = qc[k];
-        } else {
-          load(off - nx, qy);
-        }
-        CflFace<D, 1>(qy, qc, a, g, fidx(1, xs, y, cz), acc[1]);
-        if (y == ny - 1) CflFace<D, 1>(qc, qc, a, g, fidx(1, xs, y + 1, cz), acc[1]);
+      const double lg = learned(XM{}, l, qc);
+      if (active) {
+        CflFace<D, M, MIXED, LEARNED>(l, qc, a, g, (M == 1) ? fidx(1, xs, m, 0) : fidx(2, xs, cy, m), lg,
+                             acc[M]);
+        if (m == nm - 1)
+          CflFace<D, M, MIXED, LEARNED>(qc, qc, a, g,
+                               (M == 1) ? fidx(1, xs, m + 1, 0) : fidx(2, xs, cy, m + 1),
+                               0.5 * (qc[D + 2] + qc[D + 2]), acc[M]);
+      }
+    }
+    if constexpr (D == 3) {
+      double qy[NQ];
+      if (y == 0) {
+        for (int k = 0; k < NQ; ++k) qy[k] = qc[k];
+      } else {
+        load(off - nx, qy);
+      }
+      const double lg = learned(X1{}, qy, qc);
+      if (active) {
+        CflFace<D, 1, MIXED, LEARNED>(qy, qc, a, g, fidx(1, xs, y, cz), lg, acc[1]);
+        if (y == ny - 1)
+          CflFace<D, 1, MIXED, LEARNED>(qc, qc, a, g, fidx(1, xs, y + 1, cz), 0.5 * (qc[D + 2] + qc[D + 2]),
+                               acc[1]);
       }
     }
 #pragma unroll
     for (int k = 0; k < NQ; ++k) s_q[k][tx] = qc[k];
     __syncthreads();
-    if (active) {
+    {
       double ql[NQ];
       if (x == 0) {
         for (int k = 0; k < NQ; ++k) ql[k] = qc[k];
@@ -540,8 +620,13 @@ __global__ __launch_bounds__(kBlock) void GodunovCflKernel(GodunovCflArgs a) {
       } else {
         load(off - 1, ql);
       }
-      CflFace<D, 0>(ql, qc, a, g, fidx(0, xs, cy, cz), acc[0]);
-      if (x == nx - 1) CflFace<D, 0>(qc, qc, a, g, fidx(0, xs + 1, cy, cz), acc[0]);
+      const double lg = learned(X0{}, ql, qc);
+      if (active) {
+        CflFace<D, 0, MIXED, LEARNED>(ql, qc, a, g, fidx(0, xs, cy, cz), lg, acc[0]);
+        if (x == nx - 1)
+          CflFace<D, 0, MIXED, LEARNED>(qc, qc, a, g, fidx(0, xs + 1, cy, cz),
+                               0.5 * (qc[D + 2] + qc[D + 2]), acc[0]);
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -626,7 +711,8 @@ int64_t GodunovCflBlocks(int dim, const int64_t n[3]) {
 int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
                        const double* const* in, double* const* out, const int64_t n[3],
                        double dx, const double* dt_ptr, double dt_val, const double* guess,
-                       unsigned long long* partials, int* halt, int update) {
+                       unsigned long long* partials, int* halt, int update,
+                       const fvs_mlp* net) {
   GodunovArgs a;
   std::memset(&a, 0, sizeof a);
   const int D = desc.dim;
@@ -643,10 +729,13 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
   a.tol = desc.newton_tolerance;
   a.max_it = desc.max_newton_iterations;
   a.guess = guess;
+  if (net) MlpOperands(net, &a.tiles, &a.bias);
   a.partials = partials;
   a.halt = halt;
   a.seg = 32;
   a.update = update;
+  const bool mixed = desc.precision == FVS_FP32_FLUX;
+  const bool learned = a.tiles != nullptr;
   const int64_t blocks = GodunovSweepBlocks(D, dir, a.n, a.seg);
   if (blocks <= 0) return FVS_BAD_ARGUMENT;
   if (dir == 0) {
@@ -654,21 +743,28 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
       SetError("Godunov X sweep: patch too large for 32-bit slot indices");
       return FVS_BAD_ARGUMENT;
     }
-    if (D == 2) hipLaunchKernelGGL(GodunovSweepX<2>, dim3(unsigned(blocks)), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(GodunovSweepX<3>, dim3(unsigned(blocks)), dim3(kBlock), 0, s, a);
+    using K = void (*)(GodunovArgs);
+    const K ks[2][2][2] = {{{GodunovSweepX<2, false, false>, GodunovSweepX<2, false, true>},
+                            {GodunovSweepX<2, true, false>, GodunovSweepX<2, true, true>}},
+                           {{GodunovSweepX<3, false, false>, GodunovSweepX<3, false, true>},
+                            {GodunovSweepX<3, true, false>, GodunovSweepX<3, true, true>}}};
+    hipLaunchKernelGGL(ks[D == 3][mixed][learned], dim3(unsigned(blocks)), dim3(kBlock), 0, s, a);
   } else {
     const int64_t npencil = (dir == 1) ? a.n[0] * a.n[2] : a.n[0] * a.n[1];
     const int64_t bx = (npencil + kBlock - 1) / kBlock;
     const int64_t nseg = (a.n[dir] + a.seg - 1) / a.seg;
     if (bx > 0x7fffffff || nseg > 65535) return FVS_BAD_ARGUMENT;
     const dim3 grid{static_cast<unsigned>(bx), static_cast<unsigned>(nseg), 1u};
-    if (D == 2) {
-      hipLaunchKernelGGL((GodunovSweepPencil<2, 1>), grid, dim3(kBlock), 0, s, a);
-    } else if (dir == 1) {
-      hipLaunchKernelGGL((GodunovSweepPencil<3, 1>), grid, dim3(kBlock), 0, s, a);
-    } else {
-      hipLaunchKernelGGL((GodunovSweepPencil<3, 2>), grid, dim3(kBlock), 0, s, a);
-    }
+    using K = void (*)(GodunovArgs);
+    const K ks[3][2][2] = {
+        {{GodunovSweepPencil<2, 1, false, false>, GodunovSweepPencil<2, 1, false, true>},
+         {GodunovSweepPencil<2, 1, true, false>, GodunovSweepPencil<2, 1, true, true>}},
+        {{GodunovSweepPencil<3, 1, false, false>, GodunovSweepPencil<3, 1, false, true>},
+         {GodunovSweepPencil<3, 1, true, false>, GodunovSweepPencil<3, 1, true, true>}},
+        {{GodunovSweepPencil<3, 2, false, false>, GodunovSweepPencil<3, 2, false, true>},
+         {GodunovSweepPencil<3, 2, true, false>, GodunovSweepPencil<3, 2, true, true>}}};
+    const int which = D == 2 ? 0 : (dir == 1 ? 1 : 2);
+    hipLaunchKernelGGL(ks[which][mixed][learned], grid, dim3(kBlock), 0, s, a);
   }
   FVS_LAUNCH_CHECK();
   return FVS_OK;
@@ -684,7 +780,7 @@ int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int
 
 int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                      const int64_t n[3], const double dx[3], const double* const* guess,
-                     unsigned long long* partials, int* halt) {
+                     unsigned long long* partials, int* halt, const fvs_mlp* net) {
   GodunovCflArgs a;
   std::memset(&a, 0, sizeof a);
   const int D = desc.dim;
@@ -698,14 +794,22 @@ int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* c
   a.gm1inv = 1.0 / (desc.gamma - 1.0);
   a.tol = desc.newton_tolerance;
   a.max_it = desc.max_newton_iterations;
+  if (net) MlpOperands(net, &a.tiles, &a.bias);
   a.partials = partials;
   a.halt = halt;
+  const bool mixed = desc.precision == FVS_FP32_FLUX;
   const int64_t tiles = (a.n[0] + kTX - 1) / kTX;
   const int64_t gx = tiles * (D == 3 ? a.n[1] : 1);
   const int64_t gy = ((D == 3 ? a.n[2] : a.n[1]) + kCflSeg - 1) / kCflSeg;
   if (gx > 0x7fffffff || gy > 65535) return FVS_BAD_ARGUMENT;
-  if (D == 2) hipLaunchKernelGGL(GodunovCflKernel<2>, dim3(unsigned(gx), unsigned(gy)), dim3(kBlock), 0, s, a);
-  else hipLaunchKernelGGL(GodunovCflKernel<3>, dim3(unsigned(gx), unsigned(gy)), dim3(kBlock), 0, s, a);
+  using K = void (*)(GodunovCflArgs);
+  const K ks[2][2][2] = {{{GodunovCflKernel<2, false, false>, GodunovCflKernel<2, false, true>},
+                          {GodunovCflKernel<2, true, false>, GodunovCflKernel<2, true, true>}},
+                         {{GodunovCflKernel<3, false, false>, GodunovCflKernel<3, false, true>},
+                          {GodunovCflKernel<3, true, false>, GodunovCflKernel<3, true, true>}}};
+  const bool learned = a.tiles != nullptr;
+  hipLaunchKernelGGL(ks[D == 3][mixed][learned], dim3(unsigned(gx), unsigned(gy)), dim3(kBlock),
+                     0, s, a);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }

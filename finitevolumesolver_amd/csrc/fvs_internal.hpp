@@ -104,12 +104,13 @@ int64_t GodunovCflBlocks(int dim, const int64_t n[3]);
 int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
                        const double* const* in, double* const* out, const int64_t n[3],
                        double dx, const double* dt_ptr, double dt_val, const double* guess,
-                       unsigned long long* partials, int* halt, int update);
+                       unsigned long long* partials, int* halt, int update,
+                       const fvs_mlp* net = nullptr);
 int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
                           double* out, unsigned long long* stats, Ctl* ctl = nullptr);
 int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                      const int64_t n[3], const double dx[3], const double* const* guess,
-                     unsigned long long* partials, int* halt);
+                     unsigned long long* partials, int* halt, const fvs_mlp* net = nullptr);
 int LaunchGodunovViewFlux(const fvs_method_desc& desc, hipStream_t s, const fvs_view* fluxes,
                           const fvs_view* states, double dt, double dx, int dir,
                           unsigned long long* nfail);
@@ -120,6 +121,8 @@ int LaunchGodunovViewStableDt(const fvs_method_desc& desc, hipStream_t s, const 
 // patch, launched on stream s (fvs_mlp.hip).
 int MlpFaceGuess(fvs_mlp* m, hipStream_t s, int dim, double gamma, const double* const* cons,
                  const int64_t n[3], int dir, double* guess);
+// The network's packed matrix-core operands, for kernels that infer inline.
+void MlpOperands(const fvs_mlp* m, const short** tiles, const float** bias);
 int LaunchMiddleState(int dim, hipStream_t s, const double* L, const double* R, int64_t n,
                       int dir, double gamma, double tol, int max_it, const double* guess,
                       double* pM, double* uM, int* iters, int* conv);

@@ -13,6 +13,7 @@
 // come from the FP64 Newton iteration itself.  The Newton iteration that
 // refines the guess is unchanged (include/fub/NewtonIteration.hpp:28-52).
 #include "fvs_internal.hpp"
+#include "fvs_mlp_infer.hpp"
 #include "fvs_riemann.hpp"
 
 #include <cmath>
@@ -24,136 +25,9 @@ namespace fvs {
 
 namespace {
 
-// FP32 master parameter layout: W1 [64][8] (inputs padded to 8, columns 6
-// and 7 stay zero), b1 [64], W2 [64][64], b2 [64], W3 [64], b3.
-constexpr int kH = 64;
-constexpr int kIn = 6;
-constexpr int kInPad = 8;
-constexpr int kOffW1 = 0;
-constexpr int kOffB1 = kOffW1 + kH * kInPad;
-constexpr int kOffW2 = kOffB1 + kH;
-constexpr int kOffB2 = kOffW2 + kH * kH;
-constexpr int kOffW3 = kOffB2 + kH;
-constexpr int kOffB3 = kOffW3 + kH;
-constexpr int kNParams = kOffB3 + 1;  // 4801
-
-// Packed inference operands: 24 A tiles of 64 lanes x 4 bf16
-// (layer 1: tiles 0..3 [m]; layer 2: 4 + 4 m + kb; layer 3: 20 + kb).
-constexpr int kNTiles = 24;
-constexpr int kTileShorts = 64 * 4;
+using namespace mlp;
 
 constexpr int kBlock = 256;
-constexpr float kLogScale = 0.14476482730108395f;  // 1 / ln(1000)
-
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
-typedef float floatx4_t __attribute__((ext_vector_type(4)));
-
-__host__ __device__ __forceinline__ short ToBf16(float f) {
-  unsigned int u;
-  std::memcpy(&u, &f, 4);
-  const unsigned int lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;  // round to nearest even (finite values)
-  return static_cast<short>(u >> 16);
-}
-
-__device__ __forceinline__ floatx4_t Mfma16(bf16x4_t a, bf16x4_t b, floatx4_t c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
-template <int D>
-__device__ __forceinline__ void Features(const double* L, const double* R, int dir, float* x) {
-  x[0] = logf(static_cast<float>(L[0])) * kLogScale;
-  x[1] = static_cast<float>(L[1 + dir] / L[0]) * 0.2f;
-  x[2] = logf(static_cast<float>(L[D + 2])) * kLogScale;
-  x[3] = logf(static_cast<float>(R[0])) * kLogScale;
-  x[4] = static_cast<float>(R[1 + dir] / R[0]) * 0.2f;
-  x[5] = logf(static_cast<float>(R[D + 2])) * kLogScale;
-}
-
-__device__ __forceinline__ double GuessFromOutput(double pL, double pR, float r) {
-  return exp(0.5 * (log(pL) + log(pR)) + static_cast<double>(r));
-}
-
-// Matrix-core inference for the 64 faces of one wave (lane = face).
-// Operand layouts of v_mfma_f32_16x16x16_bf16 per 16x16x16 block:
-//   A[i][k] : lane i + 16 (k / 4), element k % 4
-//   B[k][j] : lane j + 16 (k / 4), element k % 4
-//   D[i][j] : lane j + 16 (i / 4), element i % 4
-// The feature transpose into B and the output gather use cross-lane
-// shuffles only (no LDS, no barrier), so a wave may skip the call alone.
-__device__ float MlpWave(const float* xf, const short* __restrict__ tiles,
-                         const float* __restrict__ bias) {
-  const int lane = threadIdx.x & 63;
-  const int kq = lane >> 4;
-  const int r16 = lane & 15;
-  bf16x4_t h1[4][4];  // [face tile][hidden tile]
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    // B[k][j] = feature k of face t * 16 + j (k < 6; rows 6..15 are zero)
-    const int src = t * 16 + r16;
-    bf16x4_t bx;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const float lo = __shfl(xf[e], src, 64);
-      const float hi = (e < 2) ? __shfl(xf[4 + e], src, 64) : 0.0f;
-      const float v = (kq == 0) ? lo : ((kq == 1) ? hi : 0.0f);
-      bx[e] = ToBf16(v);
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const bf16x4_t aw = *reinterpret_cast<const bf16x4_t*>(tiles + (m * 64 + lane) * 4);
-      floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
-      acc = Mfma16(aw, bx, acc);
-      bf16x4_t hb;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) hb[e] = ToBf16(tanhf(acc[e] + bias[m * 16 + kq * 4 + e]));
-      h1[t][m] = hb;
-    }
-  }
-  float outs[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    bf16x4_t h2[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) {
-        const bf16x4_t aw =
-            *reinterpret_cast<const bf16x4_t*>(tiles + ((4 + m * 4 + kb) * 64 + lane) * 4);
-        acc = Mfma16(aw, h1[t][kb], acc);
-      }
-      bf16x4_t hb;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) hb[e] = ToBf16(tanhf(acc[e] + bias[kH + m * 16 + kq * 4 + e]));
-      h2[m] = hb;
-    }
-    floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-      const bf16x4_t aw = *reinterpret_cast<const bf16x4_t*>(tiles + ((20 + kb) * 64 + lane) * 4);
-      acc = Mfma16(aw, h2[kb], acc);
-    }
-    // output of face t * 16 + j sits in element 0 of lane j
-    outs[t] = __shfl(acc[0], r16, 64) + bias[2 * kH];
-  }
-  return (kq == 0) ? outs[0] : (kq == 1) ? outs[1] : (kq == 2) ? outs[2] : outs[3];
-}
-
-// Pairs for which the mean pressure already lies close to the star pressure
-// (acoustic estimates |pL - pR| and |uR - uL| (rhoL + rhoR)(aL + aR) / 4 both
-// below kWeakPair (pL + pR)): Newton from 0.5 (pL + pR) converges in two or
-// three steps there, which a BF16 network cannot beat, so they keep the
-// reference guess.
-constexpr double kWeakPair = 0.1;
-
-__device__ __forceinline__ bool WeakPair(double rhoL, double uL, double pL, double aL,
-                                         double rhoR, double uR, double pR, double aR) {
-  const double sp = pL + pR;
-  const double ep = fabs(pL - pR);
-  const double eu = fabs(uR - uL) * (rhoL + rhoR) * (aL + aR) * 0.25;
-  return ep < kWeakPair * sp && eu < kWeakPair * sp;
-}
 
 // Packs the FP32 master parameters into bf16 A-operand tiles and a bias
 // vector (b1, b2, b3).
@@ -584,6 +458,11 @@ uint64_t HostSplitMix(uint64_t& s) {
 
 // Initial guesses for every face of direction dir of a ghost-free
 // transmissive patch (face box n with n[dir] + 1, x fastest), on stream s.
+void fvs::MlpOperands(const fvs_mlp* m, const short** tiles, const float** bias) {
+  *tiles = m->tiles;
+  *bias = m->bias;
+}
+
 int fvs::MlpFaceGuess(fvs_mlp* m, hipStream_t s, int dim, double gamma,
                       const double* const* cons, const int64_t n[3], int dir, double* guess) {
   if (!m || !cons || !n || !guess || (dim != 2 && dim != 3) || dir < 0 || dir >= dim) {

@@ -11,13 +11,33 @@
 // scalar reduction order (m0^2 + m1^2) + m2^2.  pow is the device libm's;
 // it can differ from the host libm in the last bit, so these results agree
 // with the CPU to rounding, not bit for bit.
+//
+// Mixed form (FAST = true): the Newton iteration stays FP64 but evaluates
+// x^e as exp(e log x) (FP64 exp and log, a few ulps instead of pow's
+// correctly rounded-like accuracy), and the sampled state and its flux are
+// computed in float with x^e = exp2(e log2 x) from the hardware estimates.
 #pragma once
 
 #include "fvs_physics.hpp"
 
 namespace fvs {
 
-template <int D> __device__ __forceinline__ double SqNormScalar(const double* m) {
+// x^e for the Newton pressure function (FP64).
+template <bool FAST> __device__ __forceinline__ double PowN(double x, double e) {
+  if constexpr (FAST) {
+    return exp(e * log(x));
+  } else {
+    return pow(x, e);
+  }
+}
+
+// x^e of the sampled state (float or double).
+__device__ __forceinline__ double PowS(double x, double e) { return pow(x, e); }
+__device__ __forceinline__ float PowS(float x, float e) {
+  return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(x));
+}
+
+template <int D, class T> __device__ __forceinline__ T SqNormScalar(const T* m) {
   if constexpr (D == 2) {
     return m[0] * m[0] + m[1] * m[1];
   } else {
@@ -48,27 +68,29 @@ struct MiddleState {
 };
 
 // Toro's pressure function of one side and its derivative.
-struct SideFn {
+template <bool FAST = false> struct SideFn {
   double rho, p, a, A, B;
   __device__ __forceinline__ double f(double x, double gm1, double exponent) const {
     if (x > p) {
       return (x - p) * rsqrt_ieee(A / (x + B));
     } else {
-      return 2.0 * a / gm1 * (pow(x / p, exponent) - 1.0);
+      return 2.0 * a / gm1 * (PowN<FAST>(x / p, exponent) - 1.0);
     }
   }
   __device__ __forceinline__ double df(double x, double d_exponent) const {
     if (x > p) {
       return rsqrt_ieee(A / (B + x)) * (1.0 - 0.5 * (x - p) / (B + x));
     } else {
-      return pow(x / p, d_exponent) / (rho * a);
+      return PowN<FAST>(x / p, d_exponent) / (rho * a);
     }
   }
 };
 
 // ComputeMiddleState with an explicit initial guess x0 (the reference's is
-// 0.5 * (pL + pR)).
-template <int D, int DIR>
+// 0.5 * (pL + pR)).  The side functions at the final iterate, which the
+// reference evaluates once more for the middle velocity, are the ones the
+// last convergence test computed (same arguments, same values).
+template <int D, int DIR, bool FAST = false>
 __device__ __forceinline__ MiddleState ComputeMiddleState(const double* L, const double* R,
                                                           const Gas<double>& eq,
                                                           const NewtonParams& np,
@@ -77,7 +99,7 @@ __device__ __forceinline__ MiddleState ComputeMiddleState(const double* L, const
   const double g = eq.gamma;
   const double gp1 = g + 1.0;
   const double gm1 = g - 1.0;
-  SideFn left, right;
+  SideFn<FAST> left, right;
   left.rho = L[I::kRho];
   left.p = L[I::kP];
   left.a = L[I::kA];
@@ -95,7 +117,9 @@ __device__ __forceinline__ MiddleState ComputeMiddleState(const double* L, const
   const double du = uR - uL;
   // NewtonIteration (NewtonIteration.hpp:28-52).
   double x = x0;
-  double fx = right.f(x, gm1, exponent) + left.f(x, gm1, exponent) + du;
+  double fR = right.f(x, gm1, exponent);
+  double fL = left.f(x, gm1, exponent);
+  double fx = fR + fL + du;
   int counter = 0;
   double err = rabs(fx);
   while (err > np.tol && counter < np.max_it) {
@@ -103,14 +127,14 @@ __device__ __forceinline__ MiddleState ComputeMiddleState(const double* L, const
     const double offset = fx / dfx;
     const double next = x - offset;
     x = ((next - next) != (next - next) || next < 0.0) ? 0.5 * x : next;
-    fx = right.f(x, gm1, exponent) + left.f(x, gm1, exponent) + du;
+    fR = right.f(x, gm1, exponent);
+    fL = left.f(x, gm1, exponent);
+    fx = fR + fL + du;
     err = rabs(fx);
     counter += 1;
   }
   MiddleState ms;
   ms.pM = x;
-  const double fL = left.f(x, gm1, exponent);
-  const double fR = right.f(x, gm1, exponent);
   ms.uM = 0.5 * (uL + uR) + 0.5 * (fR - fL);
   ms.iterations = counter;
   ms.converged = !(counter == np.max_it && fx > np.tol);
@@ -157,43 +181,41 @@ __device__ __forceinline__ double GodunovDtFromMiddle(const double* L, const dou
 }
 
 // SolveRiemannProblem: the complete state at x/t = 0 given the middle state.
-template <int D, int DIR>
-__device__ __forceinline__ void SampleRiemann(double* state, const double* left,
-                                              const double* right, const MiddleState& ms,
-                                              const Gas<double>& eq) {
+// T = double reproduces the reference; T = float is the mixed form's sample.
+template <int D, int DIR, class T>
+__device__ __forceinline__ void SampleRiemannT(T* state, const T* left, const T* right, T pM,
+                                               T uM, const Gas<T>& eq) {
   using I = Idx<D>;
   constexpr int NQ = I::kComp;
   constexpr int M = I::kMom;
-  auto set_momentum = [&](double momentum, const double* side) {
+  auto set_momentum = [&](T momentum, const T* side) {
 #pragma unroll
     for (int d = 0; d < D; ++d) state[M + d] = state[I::kRho] * (side[M + d] / side[I::kRho]);
     state[M + DIR] = momentum;
   };
   auto from_prim = [&]() {
-    const double rhoE_kin = 0.5 * SqNormScalar<D>(state + M) / state[I::kRho];
-    const double rhoE_internal = state[I::kP] * eq.gm1inv;
+    const T rhoE_kin = T(0.5) * SqNormScalar<D>(state + M) / state[I::kRho];
+    const T rhoE_internal = state[I::kP] * eq.gm1inv;
     state[I::kE] = rhoE_internal + rhoE_kin;
     state[I::kA] = rsqrt_ieee(eq.gamma * state[I::kP] / state[I::kRho]);
   };
-  auto copy = [&](const double* s) {
+  auto copy = [&](const T* s) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) state[k] = s[k];
   };
-  const double pM = ms.pM;
-  const double uM = ms.uM;
-  const double g = eq.gamma;
-  const double gp1 = g + 1.0;
-  const double gm1 = g - 1.0;
-  if (0.0 <= uM) {
-    const double uL = left[M + DIR] / left[I::kRho];
-    const double aL = left[I::kA];
-    const double pL = left[I::kP];
-    const double p_ratio = pM / pL;
-    const double g_ratio = gm1 / gp1;
+  const T g = eq.gamma;
+  const T gp1 = g + T(1.0);
+  const T gm1 = g - T(1.0);
+  if (T(0.0) <= uM) {
+    const T uL = left[M + DIR] / left[I::kRho];
+    const T aL = left[I::kA];
+    const T pL = left[I::kP];
+    const T p_ratio = pM / pL;
+    const T g_ratio = gm1 / gp1;
     if (pM > pL) {
-      const double shock_speed = uL - aL * rsqrt_ieee((gp1 * p_ratio + gm1) / (2.0 * g));
-      if (shock_speed < 0.0) {
-        state[I::kRho] = left[I::kRho] * (p_ratio + g_ratio) / (g_ratio * p_ratio + 1.0);
+      const T shock_speed = uL - aL * rsqrt_ieee((gp1 * p_ratio + gm1) / (T(2.0) * g));
+      if (shock_speed < T(0.0)) {
+        state[I::kRho] = left[I::kRho] * (p_ratio + g_ratio) / (g_ratio * p_ratio + T(1.0));
         set_momentum(state[I::kRho] * uM, left);
         state[I::kP] = pM;
         from_prim();
@@ -201,34 +223,34 @@ __device__ __forceinline__ void SampleRiemann(double* state, const double* left,
         copy(left);
       }
     } else {
-      const double aM = aL * pow(p_ratio, gm1 / (2.0 * g));
-      const double shock_speed_TL = uM - aM;
-      const double shock_speed_HL = uL - aL;
-      if (0.0 < shock_speed_HL) {
+      const T aM = aL * PowS(p_ratio, gm1 / (T(2.0) * g));
+      const T shock_speed_TL = uM - aM;
+      const T shock_speed_HL = uL - aL;
+      if (T(0.0) < shock_speed_HL) {
         copy(left);
-      } else if (shock_speed_TL < 0.0) {
-        state[I::kRho] = left[I::kRho] * pow(p_ratio, 1 / g);
+      } else if (shock_speed_TL < T(0.0)) {
+        state[I::kRho] = left[I::kRho] * PowS(p_ratio, T(1) / g);
         set_momentum(state[I::kRho] * uM, left);
         state[I::kP] = pM;
         from_prim();
       } else {
-        state[I::kRho] = left[I::kRho] * pow(2.0 / gp1 + g_ratio * uL / aL, 2.0 / gm1);
-        const double u = 2.0 / gp1 * (aL + 0.5 * gm1 * uL);
+        state[I::kRho] = left[I::kRho] * PowS(T(2.0) / gp1 + g_ratio * uL / aL, T(2.0) / gm1);
+        const T u = T(2.0) / gp1 * (aL + T(0.5) * gm1 * uL);
         set_momentum(state[I::kRho] * u, left);
-        state[I::kP] = pL * pow(2.0 / gp1 + g_ratio * uL / aL, 2.0 * g / gm1);
+        state[I::kP] = pL * PowS(T(2.0) / gp1 + g_ratio * uL / aL, T(2.0) * g / gm1);
         from_prim();
       }
     }
   } else {
-    const double uR = right[M + DIR] / right[I::kRho];
-    const double aR = right[I::kA];
-    const double pR = right[I::kP];
-    const double p_ratio = pM / pR;
-    const double g_ratio = gm1 / gp1;
+    const T uR = right[M + DIR] / right[I::kRho];
+    const T aR = right[I::kA];
+    const T pR = right[I::kP];
+    const T p_ratio = pM / pR;
+    const T g_ratio = gm1 / gp1;
     if (pM > pR) {
-      const double shock_speed = uR + aR * rsqrt_ieee((gp1 * p_ratio + gm1) / (2.0 * g));
-      if (0.0 < shock_speed) {
-        state[I::kRho] = right[I::kRho] * (p_ratio + g_ratio) / (g_ratio * p_ratio + 1.0);
+      const T shock_speed = uR + aR * rsqrt_ieee((gp1 * p_ratio + gm1) / (T(2.0) * g));
+      if (T(0.0) < shock_speed) {
+        state[I::kRho] = right[I::kRho] * (p_ratio + g_ratio) / (g_ratio * p_ratio + T(1.0));
         set_momentum(state[I::kRho] * uM, right);
         state[I::kP] = pM;
         from_prim();
@@ -236,25 +258,32 @@ __device__ __forceinline__ void SampleRiemann(double* state, const double* left,
         copy(right);
       }
     } else {
-      const double aM = aR * pow(p_ratio, gm1 / (2.0 * g));
-      const double shock_speed_TR = uM + aM;
-      const double shock_speed_HR = uR + aR;
-      if (shock_speed_HR <= 0.0) {
+      const T aM = aR * PowS(p_ratio, gm1 / (T(2.0) * g));
+      const T shock_speed_TR = uM + aM;
+      const T shock_speed_HR = uR + aR;
+      if (shock_speed_HR <= T(0.0)) {
         copy(right);
-      } else if (shock_speed_TR >= 0.0) {
-        state[I::kRho] = right[I::kRho] * pow(p_ratio, 1 / g);
+      } else if (shock_speed_TR >= T(0.0)) {
+        state[I::kRho] = right[I::kRho] * PowS(p_ratio, T(1) / g);
         set_momentum(state[I::kRho] * uM, right);
         state[I::kP] = pM;
         from_prim();
       } else {
-        state[I::kRho] = right[I::kRho] * pow(2.0 / gp1 - g_ratio * uR / aR, 2.0 / gm1);
-        const double u = 2.0 / gp1 * (-aR + 0.5 * gm1 * uR);
+        state[I::kRho] = right[I::kRho] * PowS(T(2.0) / gp1 - g_ratio * uR / aR, T(2.0) / gm1);
+        const T u = T(2.0) / gp1 * (-aR + T(0.5) * gm1 * uR);
         set_momentum(state[I::kRho] * u, right);
-        state[I::kP] = pR * pow(2.0 / gp1 - g_ratio * uR / aR, 2.0 * g / gm1);
+        state[I::kP] = pR * PowS(T(2.0) / gp1 - g_ratio * uR / aR, T(2.0) * g / gm1);
         from_prim();
       }
     }
   }
+}
+
+template <int D, int DIR>
+__device__ __forceinline__ void SampleRiemann(double* state, const double* left,
+                                              const double* right, const MiddleState& ms,
+                                              const Gas<double>& eq) {
+  SampleRiemannT<D, DIR, double>(state, left, right, ms.pM, ms.uM, eq);
 }
 
 // Godunov<PerfectGas<D>, ExactRiemannSolver> numeric flux between complete

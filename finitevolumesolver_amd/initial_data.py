@@ -255,6 +255,34 @@ def fourier_modes_3d_device(n: int, seed0: int = 1, gamma: float = 1.4, device="
     return cons, (dx, dx, dx)
 
 
+def blocky_field_3d_device(n: int, seed: int = 5, block: int = 4, gamma: float = 1.4,
+                           umax: float = 0.5, device="cuda"):
+    """Strong-wave 3D field on the device: piecewise-constant blocks of
+    block^3 cells with density and pressure log-uniform in [0.1, 10] and
+    velocities uniform in [-umax, umax] (SplitMix64 streams per variable),
+    so nearly every block face is a strong Riemann problem.  With umax = 0.5
+    no face generates vacuum (|du| <= 1 < 2 (aL + aR) / (gamma - 1), the
+    sound speeds being at least sqrt(1.4 * 0.1 / 10)).  Returns (cons
+    (5, n, n, n) device tensor, dx)."""
+    import torch
+    nb = (n + block - 1) // block
+
+    def field(s, lo, hi, log):
+        v = _splitmix_uniform_device(seed * 16 + s, nb ** 3, torch, device).view(nb, nb, nb)
+        v = torch.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * v) if log else lo + (hi - lo) * v
+        v = v.repeat_interleave(block, 0).repeat_interleave(block, 1).repeat_interleave(block, 2)
+        return v[:n, :n, :n].contiguous()
+
+    rho, p = field(0, 0.1, 10.0, True), field(1, 0.1, 10.0, True)
+    vel = [field(2 + d, -umax, umax, False) for d in range(3)]
+    cons = torch.empty((5, n, n, n), dtype=torch.float64, device=device)
+    cons[0] = rho
+    for d in range(3):
+        cons[1 + d] = rho * vel[d]
+    cons[4] = p / (gamma - 1.0) + 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+    return cons, (1.0 / n, 1.0 / n, 1.0 / n)
+
+
 def amr_patch_layout(level: int, count: int = 300, base: int = 512, ratio: int = 2,
                      sides=(16, 24, 32, 48, 64), band=(-0.75, 0.35)):
     """Patches of one refined level of the 2D shock-bubble hierarchy: `count`
