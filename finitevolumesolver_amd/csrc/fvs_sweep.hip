@@ -223,14 +223,18 @@ template <class T> __device__ __forceinline__ Gas<T> MakeGas(const SweepArgs& a)
 // ---------------------------------------------------------------------------
 // X sweep.  Every row of nx cells contributes nx + 2 "slots": the ghost cell
 // -1, the cells 0 .. nx-1 and the ghost cell nx.  The slots of all rows form
-// one sequence; each wave takes 64 consecutive slots, owns the 62 in the
-// middle and overlaps one slot with each neighbouring wave.  A lane
-// reconstructs its cell, computes the flux of the face on the cell's left
-// (from its left neighbour's right half-step state, one shuffle away) and
-// updates its cell with its right neighbour's flux.  Waves are independent:
-// no LDS and no barrier, and ~96% of the lanes do useful work for 256-cell
-// rows (62 of 64 lanes, 256 of 258 slots).
-constexpr int kXOwned = 62;
+// one sequence cut into chunks of 64 consecutive slots: lanes 2..61 own
+// their slots, lanes 0, 1 and 62, 63 overlap the neighbouring chunks.  A lane
+// reconstructs its cell from its neighbours one DPP shift away, computes the
+// flux of the face on the cell's left (lanes 2..62) and updates its cell with
+// its right neighbour's flux; the outer lanes' reconstructions are never
+// used, so no lane reads memory beyond its own slot.  Waves are persistent
+// (a few per SIMD) and walk the chunks with a stride, each loading the next
+// chunk while it computes the current one, so every wave keeps a load in
+// flight behind its FP64 work; no LDS, no barrier until the final CFL
+// reduction.
+constexpr int kXOwned = 60;
+constexpr int kXWavesPerCu = 16;
 
 template <int D, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
@@ -242,23 +246,10 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   const int64_t nx = a.n[0];
   const uint32_t S = static_cast<uint32_t>(nx + 2);
   const uint32_t Q = static_cast<uint32_t>(a.n[1] * a.n[2]) * S;
+  const int64_t nchunks = (static_cast<int64_t>(Q) + kXOwned - 1) / kXOwned;
   const int lane = threadIdx.x & 63;
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
-  const int64_t q64 = wave * kXOwned + lane - 1;
-  const bool valid = q64 >= 0 && q64 < static_cast<int64_t>(Q);
-  // Lanes outside the slot sequence take its nearest end, so that slots grow
-  // with the lane index and every lane's row is at least lane 0's (the row
-  // offsets below are unsigned and relative to lane 0's row).
-  const uint32_t q = valid ? static_cast<uint32_t>(q64) : (q64 < 0 ? 0u : Q - 1u);
-  const uint32_t row = q / S;
-  const uint32_t sl = q - row * S;           // slot within the row
-  const int64_t cell = static_cast<int64_t>(sl) - 1;
-  const bool real = valid && cell >= 0 && cell < nx;
-  // The wave's first row is its uniform base; lanes address their row
-  // relative to it (a wave spans at most 64 / (nx + 2) + 2 rows).
-  const uint32_t row0 = __builtin_amdgcn_readfirstlane(row);
-  const uint32_t rbase = (row - row0) * static_cast<uint32_t>(nx);
-  const int64_t wbase = static_cast<int64_t>(row0) * nx;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * (kBlock / 64);
+  const bool owned = lane >= 2 && lane < 2 + kXOwned;
 
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
   const T lh = static_cast<T>(0.5 * dt / a.dx);
@@ -266,85 +257,121 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
 
-  double u[NC], um[NC], up[NC];
-  {
-    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(Clamp(cell, nx)));
+  // Slot of this lane in chunk c.  Lanes outside the slot sequence take its
+  // nearest end, so that slots grow with the lane index and every lane's row
+  // is at least lane 0's (row offsets are unsigned and relative to lane 0's
+  // row, a wave spanning at most 64 / (nx + 2) + 2 rows).
+  struct Slot {
+    bool valid;
+    uint32_t sl;
+    int64_t cell;
+    uint32_t rbase;
+    int64_t wbase;
+  };
+  auto locate = [&](int64_t c) {
+    Slot s;
+    const int64_t q64 = c * kXOwned + lane - 2;
+    s.valid = q64 >= 0 && q64 < static_cast<int64_t>(Q);
+    const uint32_t q = s.valid ? static_cast<uint32_t>(q64) : (q64 < 0 ? 0u : Q - 1u);
+    const uint32_t row = q / S;
+    s.sl = q - row * S;
+    s.cell = static_cast<int64_t>(s.sl) - 1;
+    const uint32_t row0 = __builtin_amdgcn_readfirstlane(row);
+    s.rbase = (row - row0) * static_cast<uint32_t>(nx);
+    s.wbase = static_cast<int64_t>(row0) * nx;
+    return s;
+  };
+  auto load = [&](const Slot& s, double* u) {
+    const uint32_t off = 8u * (s.rbase + static_cast<uint32_t>(Clamp(s.cell, nx)));
 #pragma unroll
-    for (int k = 0; k < NC; ++k) u[k] = LdOff(a.in[k] + wbase, off);
-  }
-#pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    um[k] = ShiftUp(u[k]);
-    up[k] = ShiftDown(u[k]);
-  }
-  // The wave's edge lanes read their outer neighbour themselves, and so do
-  // the ghost slots, whose shuffled neighbour belongs to the adjacent row.
-  // With clamped indices a ghost slot's stencil is three copies of the
-  // boundary cell: its zero-slope reconstruction equals GhostRec (the ghost's
-  // complete state, up to the sign of zero components), so every lane runs
-  // the same reconstruction and no wave takes two paths.
-  if (lane == 0 || cell < 0) {
-    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(Clamp(cell - 1, nx)));
-#pragma unroll
-    for (int k = 0; k < NC; ++k) um[k] = LdOff(a.in[k] + wbase, off);
-  }
-  if (lane == 63 || cell >= nx) {
-    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(Clamp(cell + 1, nx)));
-#pragma unroll
-    for (int k = 0; k < NC; ++k) up[k] = LdOff(a.in[k] + wbase, off);
-  }
+    for (int k = 0; k < NC; ++k) u[k] = LdOff(a.in[k] + s.wbase, off);
+  };
 
-  T recL[NQ], recR[NQ];
-  typename TA::Inv irL, irR;
-  {
-    T tm[NC], t0[NC], tp[NC];
+  unsigned long long key = kInfKey;
+  int64_t c = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  Slot cur = locate(c < nchunks ? c : 0);
+  double u[NC];
+  if (c < nchunks) load(cur, u);
+  for (; c < nchunks; c += nwaves) {
+    // The next chunk's load is in flight while this chunk is computed.
+    const int64_t cn = c + nwaves;
+    const Slot nxt = locate(cn < nchunks ? cn : c);
+    double un[NC];
+    if (cn < nchunks) load(nxt, un);
+
+    const bool real = cur.valid && cur.cell >= 0 && cur.cell < nx;
+    double um[NC], up[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
-      tm[k] = static_cast<T>(um[k]);
-      t0[k] = static_cast<T>(u[k]);
-      tp[k] = static_cast<T>(up[k]);
+      um[k] = ShiftUp(u[k]);
+      up[k] = ShiftDown(u[k]);
     }
-    ReconAny<D, 0, T, TA>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR, irL, irR);
-  }
-  T recRprev[NQ];
+    // A ghost slot's stencil is three copies of the boundary cell (its own
+    // clamped value): its zero-slope reconstruction equals the ghost's
+    // complete state (up to the sign of zero components), so every lane runs
+    // the same reconstruction and no wave takes two paths.
+    if (cur.cell < 0) {
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) recRprev[k] = ShiftUp(recR[k]);
-  typename TA::Inv irRprev;
-  if constexpr (TA::kFast) {
-    irRprev.y = ShiftUp(irR.y);
-  } else {
-    irRprev = TA::inv(recRprev[0]);
-  }
-
-  // Face sl - 1 (left face of the slot's cell) for slots 1 .. nx + 1.
-  const bool face_lane = valid && lane >= 1 && sl >= 1;
-  T F[NC];
-  if (face_lane) {
-    FaceFlux<D, 0, LARR, T, TA>(F, recRprev, irRprev, recL, irL, g, a.quirk3d);
-  } else {
-#pragma unroll
-    for (int k = 0; k < NC; ++k) F[k] = T(0);
-  }
-  T Fnext[NC];
-#pragma unroll
-  for (int k = 0; k < NC; ++k) Fnext[k] = ShiftDown(F[k]);
-
-  const bool owned = lane >= 1 && lane <= kXOwned;
-  if (owned && real) {
-    const uint32_t off = 8u * (rbase + static_cast<uint32_t>(cell));
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      StOff(a.out[k] + wbase, off,
-            u[k] + dt_dx * (static_cast<double>(F[k]) - static_cast<double>(Fnext[k])));
+      for (int k = 0; k < NC; ++k) um[k] = u[k];
     }
+    if (cur.cell >= nx) {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) up[k] = u[k];
+    }
+
+    T recL[NQ], recR[NQ];
+    typename TA::Inv irL, irR;
+    {
+      T tm[NC], t0[NC], tp[NC];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        tm[k] = static_cast<T>(um[k]);
+        t0[k] = static_cast<T>(u[k]);
+        tp[k] = static_cast<T>(up[k]);
+      }
+      ReconAny<D, 0, T, TA>(tm, t0, tp, lh, g, a.limiter, a.muscl, recL, recR, irL, irR);
+    }
+    T recRprev[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) recRprev[k] = ShiftUp(recR[k]);
+    typename TA::Inv irRprev;
+    if constexpr (TA::kFast) {
+      irRprev.y = ShiftUp(irR.y);
+    } else {
+      irRprev = TA::inv(recRprev[0]);
+    }
+
+    // Face sl - 1 (left face of the slot's cell) for slots 1 .. nx + 1.
+    const bool face_lane = cur.valid && lane >= 2 && lane <= 2 + kXOwned && cur.sl >= 1;
+    T F[NC];
+    if (face_lane) {
+      FaceFlux<D, 0, LARR, T, TA>(F, recRprev, irRprev, recL, irL, g, a.quirk3d);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) F[k] = T(0);
+    }
+    T Fnext[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) Fnext[k] = ShiftDown(F[k]);
+
+    if (owned && real) {
+      const uint32_t off = 8u * (cur.rbase + static_cast<uint32_t>(cur.cell));
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        StOff(a.out[k] + cur.wbase, off,
+              u[k] + dt_dx * (static_cast<double>(F[k]) - static_cast<double>(Fnext[k])));
+      }
+    }
+    if (a.check) {
+      const DtCell<D> dc = MakeDtCell<D, DA>(u, gd);
+      const DtCell<D> dp = ShflUp<D>(dc);
+      if (owned && face_lane) key = KMin(key, DtKey(FaceDt<D, 0, DA>(dp, dc, a.dx, gd.gm1)));
+    }
+    cur = nxt;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) u[k] = un[k];
   }
-  if (a.check) {
-    unsigned long long key = kInfKey;
-    const DtCell<D> dc = MakeDtCell<D, DA>(u, gd);
-    const DtCell<D> dp = ShflUp<D>(dc);
-    if (owned && face_lane) key = DtKey(FaceDt<D, 0, DA>(dp, dc, a.dx, gd.gm1));
-    BlockMinToPartials<1, kBlock / 64>(&key, a.partials + blockIdx.x, 0);
-  }
+  if (a.check) BlockMinToPartials<1, kBlock / 64>(&key, a.partials + blockIdx.x, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -859,6 +886,21 @@ int LaunchPrecision(int precision, bool larr, hipStream_t s, int dir, const Swee
               : LaunchTyped<D, false, double, false>(s, dir, a);
 }
 
+// Compute units of the current device (queried once per device).
+int DeviceCus() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cached[dev] = cus;
+  }
+  return cached[dev];
+}
+
 } // namespace
 
 int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
@@ -866,7 +908,9 @@ int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
   if (dir == 0) {
     const int64_t slots = (n[0] + 2) * n[1] * nz;
     const int64_t waves = (slots + kXOwned - 1) / kXOwned;
-    return (waves + kBlock / 64 - 1) / (kBlock / 64);
+    const int64_t blocks = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    const int64_t persistent = static_cast<int64_t>(DeviceCus()) * kXWavesPerCu / (kBlock / 64);
+    return blocks < persistent ? blocks : persistent;
   }
   const int64_t npencil = (dir == 1) ? n[0] * nz : n[0] * n[1];
   const int64_t bx = (npencil + kBlock - 1) / kBlock;
