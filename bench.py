@@ -53,6 +53,56 @@ def measured_traffic(precision: str, kernel: str):
     return None
 
 
+def measured_copy_bandwidth(torch) -> float:
+    """Achievable HBM bandwidth on this GPU: a 2 GiB device-to-device copy
+    (bytes read + written per second, best of five)."""
+    a = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    best = 0.0
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * a.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del a, b
+    torch.cuda.empty_cache()
+    return best
+
+
+def bench_c1(torch, fvs) -> dict:
+    """C1: 2D 256^2 circular Riemann problem on [-1, 1]^2, MUSCL-Hancock
+    (MinMod) + HLLEM, Strang split, CFL 0.8, 100 steps, FP64 (IEEE), with the
+    oracle's final state compared bit for bit (the state is cache resident:
+    launch-bound, reported without a roofline)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import numpy as np
+    import oracle_lib as O
+    from finitevolumesolver_amd import _lib as L
+    from finitevolumesolver_amd.initial_data import circular_riemann_2d
+    cons, dx = circular_riemann_2d(256)
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=2))
+    lvl = fvs.SinglePatchLevel(fm, (256, 256), dx, splitting="Strang", cfl=0.8)
+    lvl.set_state(torch.from_numpy(cons).cuda())
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    done, fails = lvl.advance(100, batch=100)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    got = lvl.get_state().cpu().numpy()
+    ref, _, _ = O.level_run(fm._h.desc, cons, dx, 100, L.SPLIT_STRANG)
+    return {"workload": "2D 256^2 circular Riemann, MUSCL-Hancock+HLLEM, Strang, 100 steps, fp64",
+            "value": round(256 * 256 * 4 * done / (ms * 1e-3) / 1e9, 4),
+            "unit": "Gcell-updates/s", "steps": done, "ms_per_step": round(ms / max(done, 1), 4),
+            "bitexact_vs_oracle": bool(np.array_equal(got, ref))}
+
+
 def cpu_baseline(n: int, steps: int, precision: str) -> dict:
     """Times the FP64 CPU oracle (this project's restatement of the reference
     algorithm; the reference itself cannot be built without its libraries)
@@ -70,10 +120,29 @@ def cpu_baseline(n: int, steps: int, precision: str) -> dict:
     _, dts, _ = O.level_run(desc, cons, dx, steps)
     el = time.perf_counter() - t0
     updates = n ** 3 * 3 * len(dts)
+    # one core on a smaller sample of the same workload
+    O.load().oracle_set_threads(1)
+    n1, s1 = 64, 4
+    c1, dx1 = fourier_modes_3d(n1)
+    t1 = time.perf_counter()
+    _, d1, _ = O.level_run(desc, c1, dx1, s1)
+    el1 = time.perf_counter() - t1
+    O.load().oracle_set_threads(threads)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": updates / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle level driver, {n}^3 patch, {len(dts)} steps x 3 sweeps "
-                      f"({updates / 1e6:.1f} M cell-updates, {el:.1f} s, {threads} threads)"}
+                      f"({updates / 1e6:.1f} M cell-updates, {el:.1f} s, {threads} threads)",
+            "single_core": {"value": n1 ** 3 * 3 * len(d1) / el1 / 1e9,
+                            "sample": f"{n1}^3 patch, {len(d1)} steps, 1 thread, {el1:.1f} s"},
+            "cpu_model": model}
 
 
 def l1_vs_oracle(precision: str, torch) -> dict:
@@ -200,7 +269,7 @@ def bench_level(state, dx, n, steps, precision, torch, fvs) -> dict:
     return out
 
 
-def bench_godunov(n: int, steps: int, torch, fvs) -> dict:
+def bench_godunov(n: int, steps: int, torch, fvs, epochs: int = 12) -> dict:
     """3D first-order exact Godunov at n^3 on two fields: the smooth
     Fourier-mode field (seeds 11-15) and a strong-wave field of 4^3-cell
     blocks (density and pressure log-uniform in [0.1, 10], velocities in
@@ -217,10 +286,10 @@ def bench_godunov(n: int, steps: int, torch, fvs) -> dict:
     t0 = time.perf_counter()
     net = LearnedNewtonGuess(seed=1)
     counts = net.generate(1 << 22, seed=42)
-    hist = net.train(epochs=12, batch=1024, lr=5e-3, huber=0.25, seed=3)
+    hist = net.train(epochs=epochs, batch=1024, lr=5e-3, huber=0.25, seed=3)
     torch.cuda.synchronize()
     res["training"] = {"pairs": 1 << 22, "usable": counts["usable"], "vacuum": counts["vacuum"],
-                       "epochs": 12, "seconds": round(time.perf_counter() - t0, 2),
+                       "epochs": epochs, "seconds": round(time.perf_counter() - t0, 2),
                        "final_loss": float(hist[-1])}
     forms = (("plain", "fp64", False), ("learned", "fp64", True),
              ("mixed", "fp32", False), ("mixed_learned", "fp32", True))
@@ -394,6 +463,9 @@ def main() -> int:
     ap.add_argument("--godunov-steps", type=int, default=3)
     ap.add_argument("--cutcell-steps", type=int, default=20)
     ap.add_argument("--amr-steps", type=int, default=20)
+    ap.add_argument("--godunov-epochs", type=int, default=12)
+    ap.add_argument("--only", choices=["godunov", "c1", "cutcell", "amr"],
+                    help="run one variant alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -408,6 +480,14 @@ def main() -> int:
     n = args.size
     if args.kernels_only:
         return kernels_only(args, torch, fvs)
+    if args.only:
+        run = {"godunov": lambda: bench_godunov(args.godunov_size, args.godunov_steps, torch, fvs,
+                                                epochs=args.godunov_epochs),
+               "c1": lambda: bench_c1(torch, fvs),
+               "cutcell": lambda: bench_cutcell(2048, 1024, args.cutcell_steps, torch, fvs),
+               "amr": lambda: bench_amr(args.amr_steps, torch, fvs)}[args.only]
+        print(json.dumps({args.only: run()}), flush=True)
+        return 0
     from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
     log(f"[bench] generating {n}^3 initial state")
     dev = torch.device("cuda:0")
@@ -447,6 +527,7 @@ def main() -> int:
     achieved = alg_bytes / (per[dominant] * 1e-3) / 1e9
     log("[bench] per-launch ms: " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
     traffic = measured_traffic(args.precision, dominant)
+    hbm_copy = measured_copy_bandwidth(torch)
 
     result = {
         "metric": "Gcell-updates/s per split sweep, % HBM roofline, L1 err vs FP64 CPU oracle",
@@ -475,6 +556,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_3D,
             "per_launch_ms": {k: round(v, 4) for k, v in per.items()},
+            "hbm_copy_GBs": round(hbm_copy, 1),
+            "frac_of_copy": round(achieved / hbm_copy, 4),
         },
     }
     try:
@@ -494,6 +577,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("[bench] variant: exact Godunov, plain and learned guess")
         variants["godunov_exact"] = bench_godunov(args.godunov_size, args.godunov_steps, torch, fvs)
+        torch.cuda.empty_cache()
+        log("[bench] variant: C1 2D circular Riemann")
+        variants["c1_2d_256"] = bench_c1(torch, fvs)
         torch.cuda.empty_cache()
         log("[bench] variant: cut-cell channel")
         variants["cutcell_2048x1024"] = bench_cutcell(2048, 1024, args.cutcell_steps, torch, fvs)
