@@ -21,6 +21,7 @@ FVS_HIP_ERROR = 4
 BASE_HLLEM_LARROUTUROU = 0
 BASE_HLLEM = 1
 BASE_GODUNOV_EXACT = 2
+BASE_HLL_EINFELDT = 3
 RECON_NONE = 0
 RECON_MUSCL_HANCOCK = 1
 LIMITER_MINMOD = 0

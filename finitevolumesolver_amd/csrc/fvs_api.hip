@@ -73,7 +73,7 @@ int CheckDesc(const fvs_method_desc* d) {
     SetError("gamma must exceed 1");
     return FVS_BAD_ARGUMENT;
   }
-  if (d->base < 0 || d->base > FVS_BASE_GODUNOV_EXACT || d->reconstruction < 0 ||
+  if (d->base < 0 || d->base > FVS_BASE_HLL_EINFELDT || d->reconstruction < 0 ||
       d->reconstruction > FVS_RECON_MUSCL_HANCOCK || d->limiter < 0 ||
       d->limiter > FVS_LIMITER_NO_GRADIENT || d->precision < 0 ||
       d->precision > FVS_FP64_RCP || d->boundary_solver < FVS_BOUNDARY_EXACT ||
@@ -110,7 +110,9 @@ __device__ __forceinline__ void LoadComplete(const DevView& v, const int64_t* id
 // FluxMethod::ComputeNumericFluxes on one face per thread.  For MUSCL-Hancock
 // the stencil is the four cells face-2 .. face+1; rec_[0] is the right
 // half-step state of cell face-1 and rec_[1] the left one of cell face.
-template <int D, int DIR, bool LARR>
+// BASE: 0 plain HLLEM, 1 HLLEM (Larrouturou), 3 HLL with Einfeldt signals
+// (the fvs_base_method values).
+template <int D, int DIR, int BASE>
 __global__ __launch_bounds__(kBlock) void ViewFluxKernel(DevView fl, DevView st, double dt,
                                                          double dx, Gas<double> g,
                                                          int limiter, int muscl, int quirk) {
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void ViewFluxKernel(DevView fl, DevView st,
       c[DIR] += s - 2;
       LoadComplete<D>(st, c, u[s]);
     }
-    const double lh = 0.5 * dt / dx;
+    const double lh = DivRn(0.5 * dt, dx);
     double dummy[NQ];
     switch (limiter) {
     case kMinMod:
@@ -154,7 +156,10 @@ __global__ __launch_bounds__(kBlock) void ViewFluxKernel(DevView fl, DevView st,
     LoadComplete<D>(st, f, R);
   }
   double out[NC];
-  if constexpr (D == 3) {
+  constexpr bool LARR = BASE == FVS_BASE_HLLEM_LARROUTUROU;
+  if constexpr (BASE == FVS_BASE_HLL_EINFELDT) {
+    HllFlux<D, DIR>(out, L, R, g);
+  } else if constexpr (D == 3) {
     if (quirk) HllemFlux<D, DIR, LARR, true>(out, L, R, g);
     else HllemFlux<D, DIR, LARR, false>(out, L, R, g);
   } else {
@@ -187,7 +192,7 @@ __device__ __forceinline__ void BlockAtomicMin(unsigned long long key,
 // FluxMethod::ComputeStableDt: `faces` enumerates the first stencil cell of
 // every stencil inside the states box; the base method sees the two middle
 // cells (MUSCL) or both cells (first order).
-template <int D, int DIR>
+template <int D, int DIR, bool HLL>
 __global__ __launch_bounds__(kBlock) void ViewStableDtKernel(DevView st, DevView faces,
                                                              int offset, double dx,
                                                              Gas<double> g,
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void ViewStableDtKernel(DevView st, DevView
     LoadComplete<D>(st, c, L);
     c[DIR] += 1;
     LoadComplete<D>(st, c, R);
-    k = DtKey(HllemStableDt<D, DIR>(L, R, dx, g));
+    k = DtKey(HLL ? HllStableDt<D, DIR>(L, R, dx, g) : HllemStableDt<D, DIR>(L, R, dx, g));
   }
   BlockAtomicMin(k, key);
 }
@@ -378,14 +383,23 @@ int fvs_stable_dt(fvs_handle* h, const fvs_view* states, double dx, int dir,
   const unsigned nb = Blocks(Count(faces));
   hipStream_t s = h->stream;
   unsigned long long* key = h->scratch_keys;
+  const bool hll = h->desc.base == FVS_BASE_HLL_EINFELDT;
+#define FVS_VD(DD, DIR_)                                                                 \
+  hipLaunchKernelGGL((ViewStableDtKernel<DD, DIR_, false>), dim3(nb), dim3(kBlock), 0, s, st, \
+                     faces, offset, dx, g, key)
+#define FVS_VDH(DD, DIR_)                                                               \
+  hipLaunchKernelGGL((ViewStableDtKernel<DD, DIR_, true>), dim3(nb), dim3(kBlock), 0, s, st, \
+                     faces, offset, dx, g, key)
   if (D == 2) {
-    if (dir == 0) hipLaunchKernelGGL((ViewStableDtKernel<2, 0>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
-    else hipLaunchKernelGGL((ViewStableDtKernel<2, 1>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+    if (dir == 0) { if (hll) FVS_VDH(2, 0); else FVS_VD(2, 0); }
+    else { if (hll) FVS_VDH(2, 1); else FVS_VD(2, 1); }
   } else {
-    if (dir == 0) hipLaunchKernelGGL((ViewStableDtKernel<3, 0>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
-    else if (dir == 1) hipLaunchKernelGGL((ViewStableDtKernel<3, 1>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
-    else hipLaunchKernelGGL((ViewStableDtKernel<3, 2>), dim3(nb), dim3(kBlock), 0, s, st, faces, offset, dx, g, key);
+    if (dir == 0) { if (hll) FVS_VDH(3, 0); else FVS_VD(3, 0); }
+    else if (dir == 1) { if (hll) FVS_VDH(3, 1); else FVS_VD(3, 1); }
+    else { if (hll) FVS_VDH(3, 2); else FVS_VD(3, 2); }
   }
+#undef FVS_VD
+#undef FVS_VDH
   FVS_LAUNCH_CHECK();
   return LaunchKeysToDt(s, key, 1, dt_device);
 }
@@ -419,22 +433,32 @@ int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* st
                                  h->newton_stats + 2);
   }
   const Gas<double> g = GasOf(h->desc);
-  const bool larr = h->desc.base == FVS_BASE_HLLEM_LARROUTUROU;
+  const int base = h->desc.base;
   const unsigned nb = Blocks(Count(fl));
   hipStream_t s = h->stream;
   const int lim = h->desc.limiter, q = h->desc.quirks;
-#define FVS_VF(DD, DIR_, LA)                                                              \
-  hipLaunchKernelGGL((ViewFluxKernel<DD, DIR_, LA>), dim3(nb), dim3(kBlock), 0, s, fl, st, \
+#define FVS_VF1(DD, DIR_, B)                                                             \
+  hipLaunchKernelGGL((ViewFluxKernel<DD, DIR_, B>), dim3(nb), dim3(kBlock), 0, s, fl, st, \
                      dt, dx, g, lim, muscl, q)
+#define FVS_VF(DD, DIR_)                                                    \
+  do {                                                                      \
+    if (base == FVS_BASE_HLLEM_LARROUTUROU)                                 \
+      FVS_VF1(DD, DIR_, FVS_BASE_HLLEM_LARROUTUROU);                        \
+    else if (base == FVS_BASE_HLLEM)                                        \
+      FVS_VF1(DD, DIR_, FVS_BASE_HLLEM);                                    \
+    else                                                                    \
+      FVS_VF1(DD, DIR_, FVS_BASE_HLL_EINFELDT);                             \
+  } while (0)
   if (D == 2) {
-    if (dir == 0) { if (larr) FVS_VF(2, 0, true); else FVS_VF(2, 0, false); }
-    else { if (larr) FVS_VF(2, 1, true); else FVS_VF(2, 1, false); }
+    if (dir == 0) FVS_VF(2, 0);
+    else FVS_VF(2, 1);
   } else {
-    if (dir == 0) { if (larr) FVS_VF(3, 0, true); else FVS_VF(3, 0, false); }
-    else if (dir == 1) { if (larr) FVS_VF(3, 1, true); else FVS_VF(3, 1, false); }
-    else { if (larr) FVS_VF(3, 2, true); else FVS_VF(3, 2, false); }
+    if (dir == 0) FVS_VF(3, 0);
+    else if (dir == 1) FVS_VF(3, 1);
+    else FVS_VF(3, 2);
   }
 #undef FVS_VF
+#undef FVS_VF1
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
@@ -606,6 +630,10 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t 
     }
     return FVS_OK;
   }
+  if (h->desc.base == FVS_BASE_HLL_EINFELDT) {
+    SetError("fvs_patch_stable_dt has no HLL kernel (use fvs_stable_dt on views)");
+    return FVS_BAD_ARGUMENT;
+  }
   if (int rc = EnsurePartials(h, D * CflBlocks(D, nn))) return rc;
   return LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, h->partials, dt_dir_device, nullptr,
                         nullptr, 0, 0);
@@ -769,6 +797,10 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
   const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
   if (godunov && h->desc.reconstruction != FVS_RECON_NONE) {
     SetError("the fused level driver runs exact Godunov at first order");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (h->desc.base == FVS_BASE_HLL_EINFELDT) {
+    SetError("the fused level driver has no HLL kernel (patch operators only)");
     return FVS_BAD_ARGUMENT;
   }
   int64_t n[3] = {patch->n[0], patch->n[1], D == 3 ? patch->n[2] : 1};

@@ -17,6 +17,11 @@
 // mixed-precision form); the conservative update and the CFL values are
 // FP64.  Patches are 2D with two ghost layers; states are conservative
 // (rho, rho u, rho v, rho E), one pool per buffer.
+//
+// This file is compiled twice: the plain build holds the IEEE FP64 kernels
+// and the C entry points; fvs_batch_fast.hip defines FVS_BATCH_FAST_TU and
+// compiles the kernels of the tolerance-checked modes with multiply-add
+// contraction.
 #include "fvs_internal.hpp"
 #include "fvs_physics.hpp"
 
@@ -27,9 +32,7 @@
 #include <vector>
 
 namespace fvs {
-namespace {
-
-constexpr int kG = 2;
+namespace batch {
 
 struct PatchInfo {
   int64_t offset;  // element offset of cell (-2, -2), component 0
@@ -48,6 +51,21 @@ struct BatchArgs {
   int limiter;
   unsigned long long* partials;
 };
+
+// Batched sweeps of the tolerance-checked arithmetic modes (FVS_FP64_RCP,
+// FVS_FP32_FLUX), compiled with multiply-add contraction in
+// fvs_batch_fast.hip.
+void LaunchSweepFast(hipStream_t s, int dir, int shape, const BatchArgs& a, unsigned grid,
+                     int precision);
+
+} // namespace batch
+
+using batch::BatchArgs;
+using batch::PatchInfo;
+
+namespace {
+
+constexpr int kG = 2;
 
 template <class T>
 __device__ __forceinline__ Gas<T> GasT(const Gas<double>& g) {
@@ -131,7 +149,7 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
         z[k] = static_cast<T>(u0[k]);
         p[k] = static_cast<T>(up[k]);
       }
-      const T lh = static_cast<T>(0.5 * dt / P.dx);
+      const T lh = static_cast<T>(DivRn(0.5 * dt, P.dx));
       ReconstructCell<2, DIR, LIM, T, TA>(m, z, p, lh, gT, rl, rr);
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
@@ -156,7 +174,7 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
     __syncthreads();
     // update of owned cell c from faces c (this thread) and c + 1 (next)
     if (recon_ok && td >= 1 && td <= TD - 2 && c >= 0 && c < nd) {
-      const double dt_dx = dt / P.dx;
+      const double dt_dx = DivRn(dt, P.dx);
       const int ntx = DIR == 0 ? tx + 1 : tx, nty = DIR == 0 ? ty : ty + 1;
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
@@ -254,6 +272,21 @@ void LaunchSweepShape(hipStream_t s, int dir, int shape, const BatchArgs& a, int
 }
 
 } // namespace
+
+#ifdef FVS_BATCH_FAST_TU
+
+void batch::LaunchSweepFast(hipStream_t s, int dir, int shape, const BatchArgs& a,
+                            unsigned grid, int precision) {
+  if (precision == FVS_FP32_FLUX)
+    LaunchSweepShape<float, true>(s, dir, shape, a, a.limiter, grid);
+  else
+    LaunchSweepShape<double, true>(s, dir, shape, a, a.limiter, grid);
+}
+
+} // namespace fvs
+
+#else // FVS_BATCH_FAST_TU
+
 } // namespace fvs
 
 using namespace fvs;
@@ -326,10 +359,8 @@ int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int pr
   a.g.gm1 = b->h->desc.gamma - 1.0;
   a.limiter = b->h->desc.limiter;
   a.partials = b->partials;
-  if (precision == FVS_FP32_FLUX)
-    LaunchSweepShape<float, true>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
-  else if (precision == FVS_FP64_RCP)
-    LaunchSweepShape<double, true>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
+  if (precision != FVS_FP64)
+    batch::LaunchSweepFast(s, dir, b->shape[dir], a, b->grid[dir], precision);
   else
     LaunchSweepShape<double, false>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
   FVS_LAUNCH_CHECK();
@@ -628,3 +659,5 @@ int fvs_batch_tile_shape(fvs_batch* b, int dir, int* tx, int* ty, int* ntiles) {
 }
 
 } // extern "C"
+
+#endif // FVS_BATCH_FAST_TU

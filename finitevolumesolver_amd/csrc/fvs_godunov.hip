@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void GodunovSweepX(GodunovArgs a) {
   const int64_t rbase = static_cast<int64_t>(row) * nx;
   const Gas<double> g = GasOf(a);
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
-  const double dt_dx = dt / a.dx;
+  const double dt_dx = DivRn(dt, a.dx);
 
   double u[NQ];
   {
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const int64_t j1 = (j0 + a.seg < nd) ? j0 + a.seg : nd;
   const Gas<double> g = GasOf(a);
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
-  const double dt_dx = dt / a.dx;
+  const double dt_dx = DivRn(dt, a.dx);
   (void)nz;
 
   unsigned long long key = kInfKey, it_sum = 0, it_max = 0, nfail = 0, nfaces = 0;

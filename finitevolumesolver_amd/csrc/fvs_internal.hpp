@@ -87,6 +87,13 @@ int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg);
 int64_t CflBlocks(int dim, const int64_t n[3]);
 int LaunchSweep(const fvs_method_desc& desc, hipStream_t stream, int dir,
                 const SweepArgs& a);
+// Fused sweeps and CFL pass of the tolerance-checked arithmetic modes
+// (FVS_FP64_RCP, FVS_FP32_FLUX), compiled with multiply-add contraction.
+int LaunchSweepFast(int dim, int precision, bool larr, hipStream_t stream, int dir,
+                    const SweepArgs& a);
+int LaunchPatchCflFast(int dim, unsigned gx, unsigned gy, hipStream_t stream,
+                       const double* const* cons, const int64_t n[3], const double dx[3],
+                       double gamma, double gm1inv, unsigned long long* partials, int* halt);
 // Reduces a sweep's partial minima: writes the value to out (nullable) and,
 // with a control block, applies the split step's CFL check.
 int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials,

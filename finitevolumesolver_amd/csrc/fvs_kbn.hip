@@ -88,7 +88,7 @@ __device__ __forceinline__ void RotateMomentum(double* m, const double* a, const
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double sq = (sk[i][0] * sk[0][j] + sk[i][1] * sk[1][j]) + sk[i][2] * sk[2][j];
-        r[i][j] = ((i == j ? 1.0 : 0.0) + sk[i][j]) + sq / den;
+        r[i][j] = ((i == j ? 1.0 : 0.0) + sk[i][j]) + DivRn(sq, den);
       }
   }
   double out[D];
@@ -115,13 +115,13 @@ __device__ void HllemRiemannState(double* sol, const double* left, const double*
     R[k] = right[k];
   }
   const double Msq = 1.0;
-  const double Minv = 1.0 / sqrt(Msq);
+  const double Minv = DivRn(1.0, sqrt(Msq));
   const double rhoL = L[0], rhoR = R[0];
   double vL_[D], vR_[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    vL_[i] = L[M + i] / rhoL;
-    vR_[i] = R[M + i] / rhoR;
+    vL_[i] = DivRn(L[M + i], rhoL);
+    vR_[i] = DivRn(R[M + i], rhoR);
   }
   double eKinRL = 0.0, eKinRR = 0.0;
   if (larrouturou) {
@@ -141,26 +141,26 @@ __device__ void HllemRiemannState(double* sol, const double* left, const double*
   double fluxL[D + 2], fluxR[D + 2];
   PhysFlux<D, 0>(fluxL, L);
   PhysFlux<D, 0>(fluxR, R);
-  const double uL = L[M + d] / rhoL;
-  const double uR = R[M + d] / rhoR;
+  const double uL = DivRn(L[M + d], rhoL);
+  const double uR = DivRn(R[M + d], rhoR);
   const double aL = L[D + 3], aR = R[D + 3];
   const double rhoEL = L[D + 1], rhoER = R[D + 1];
   const double pL = L[D + 2], pR = R[D + 2];
-  const double hL = (rhoEL + pL) / rhoL;
-  const double hR = (rhoER + pR) / rhoR;
+  const double hL = DivRn(rhoEL + pL, rhoL);
+  const double hR = DivRn(rhoER + pR, rhoR);
   const double sqRhoL = sqrt(rhoL);
   const double sqRhoR = sqrt(rhoR);
   const double sqRhoSum = sqRhoL + sqRhoR;
   double roeU[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    const double vL = L[M + i] / rhoL;
-    const double vR = R[M + i] / rhoR;
-    roeU[i] = (sqRhoL * vL + sqRhoR * vR) / sqRhoSum;
+    const double vL = DivRn(L[M + i], rhoL);
+    const double vR = DivRn(R[M + i], rhoR);
+    roeU[i] = DivRn(sqRhoL * vL + sqRhoR * vR, sqRhoSum);
   }
   const double roeU0 = roeU[d];
-  const double roeH = (sqRhoL * hL + sqRhoR * hR) / sqRhoSum;
-  const double roeGamma = (sqRhoL * g.gamma + sqRhoR * g.gamma) / sqRhoSum;
+  const double roeH = DivRn(sqRhoL * hL + sqRhoR * hR, sqRhoSum);
+  const double roeGamma = DivRn(sqRhoL * g.gamma + sqRhoR * g.gamma, sqRhoSum);
   const double gm1 = roeGamma - 1.0;
   const double sqU = SqNormScalar<D>(roeU);
   const double roeA2 = gm1 * (roeH - 0.5 * Msq * sqU);
@@ -178,10 +178,10 @@ __device__ void HllemRiemannState(double* sol, const double* left, const double*
   const double db_positive = int(db <= 0) + int(db > 0) * db;
   double w[D + 2];
 #pragma unroll
-  for (int k = 0; k < D + 2; ++k) w[k] = (bR * R[k] - bL * L[k] + fluxL[k] - fluxR[k]) / db_positive;
+  for (int k = 0; k < D + 2; ++k) w[k] = DivRn(bR * R[k] - bL * L[k] + fluxL[k] - fluxR[k], db_positive);
   const double squaredNormRoeU_half = 0.5 * (Msq * sqU);
   const double u_bar = 0.5 * (sR + sL);
-  const double delta = roeA / (roeA + fabs(u_bar));
+  const double delta = DivRn(roeA, roeA + fabs(u_bar));
   const double deltaRho = rhoR - rhoL;
   double dRhoU[D];
 #pragma unroll
@@ -193,10 +193,10 @@ __device__ void HllemRiemannState(double* sol, const double* left, const double*
   const int upwind = (u_bar > 0.0);
   if constexpr (D == 2) {
     constexpr int ix = 0, iy = 1;
-    const double l21 = gm1 / roeA2 * (roeH - Msq * sqU);
-    const double l22 = gm1 / roeA2 * Msq * roeU[0];
-    const double l23 = gm1 / roeA2 * roeU[1];
-    const double l24 = gm1 / roeA2 * (-1.0);
+    const double l21 = DivRn(gm1, roeA2) * (roeH - Msq * sqU);
+    const double l22 = DivRn(gm1, roeA2) * Msq * roeU[0];
+    const double l23 = DivRn(gm1, roeA2) * roeU[1];
+    const double l24 = DivRn(gm1, roeA2) * (-1.0);
     const double alpha_2 = l21 * deltaRho + l22 * dRhoU[0] + l23 * dRhoU[1] + l24 * deltaRhoE;
     const double l31 = -roeU[iy];
     const double l33 = 1;
@@ -215,11 +215,11 @@ __device__ void HllemRiemannState(double* sol, const double* left, const double*
     }
   } else {
     constexpr int ix = 0, iy = 1, iz = 2;
-    const double l21 = gm1 / roeA2 * (roeH - Msq * sqU);
-    const double l22 = gm1 / roeA2 * Msq * roeU[0];
-    const double l23 = gm1 / roeA2 * roeU[1];
-    const double l24 = gm1 / roeA2 * roeU[2];
-    const double l25 = gm1 / roeA2 * (-1.0);
+    const double l21 = DivRn(gm1, roeA2) * (roeH - Msq * sqU);
+    const double l22 = DivRn(gm1, roeA2) * Msq * roeU[0];
+    const double l23 = DivRn(gm1, roeA2) * roeU[1];
+    const double l24 = DivRn(gm1, roeA2) * roeU[2];
+    const double l25 = DivRn(gm1, roeA2) * (-1.0);
     const double l31 = -roeU[iy];
     const double l33 = 1;
     const double l41 = -roeU[iz];
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void RegularFluxKernel(DevView fl, DevView 
       }
       const bool mL = a[0] > 0.0 && a[1] > 0.0 && a[2] > 0.0;
       const bool mR = a[1] > 0.0 && a[2] > 0.0 && a[3] > 0.0;
-      const double lh = 0.5 * fp.dt / fp.dx;
+      const double lh = DivRn(0.5 * fp.dt, fp.dx);
       double dummy[NQ];
       switch (fp.limiter) {
       case kMinMod:
@@ -573,12 +573,12 @@ __global__ __launch_bounds__(kBlock) void KbnUpdateKernel(UpdateViews v, double 
   const double betaL = *v.beta.at(c, 0);
   const double betaR = *v.beta.at(r, 0);
   const double center = *v.centroids.at(c, dir);
-  const double betaL_us_over_alpha = *v.us.at(c, 0) / alpha;
-  const double betaL_sL_over_alpha = *v.sl.at(c, 0) / alpha;
-  const double dLg = Clamp01((*v.sr.at(c, 0) * (0.5 + center)) / alpha);
-  const double betaR_us_over_alpha = *v.us.at(r, 0) / alpha;
-  const double betaR_sR_over_alpha = *v.sr.at(r, 0) / alpha;
-  const double dRg = Clamp01((*v.sl.at(r, 0) * (0.5 - center)) / alpha);
+  const double betaL_us_over_alpha = DivRn(*v.us.at(c, 0), alpha);
+  const double betaL_sL_over_alpha = DivRn(*v.sl.at(c, 0), alpha);
+  const double dLg = Clamp01(DivRn(*v.sr.at(c, 0) * (0.5 + center), alpha));
+  const double betaR_us_over_alpha = DivRn(*v.us.at(r, 0), alpha);
+  const double betaR_sR_over_alpha = DivRn(*v.sr.at(r, 0), alpha);
+  const double dRg = Clamp01(DivRn(*v.sl.at(r, 0) * (0.5 - center), alpha));
   for (int k = 0; k < ncomp; ++k) {
     double* next = v.scratch.at(c, k);
     const double prev = *next;
@@ -627,7 +627,7 @@ __global__ __launch_bounds__(kBlock) void KbnUpdate2Kernel(DevView dest, DevView
       next = prev + dt_over_dx * (fL - fR);
     } else {
       const double dB = bL - bR;
-      next = prev + dt_over_dx / volfrac * (bL * fL - bR * fR - dB * *fb.at(c, k));
+      next = prev + DivRn(dt_over_dx, volfrac) * (bL * fL - bR * fR - dB * *fb.at(c, k));
     }
     *dest.at(c, k) = next;
   }
@@ -669,13 +669,13 @@ __global__ __launch_bounds__(kBlock) void FillCutCellKernel(DevView beta, DevVie
   const double sRight = dBetaR - doubly;
   betaM = betaM > 0 ? betaM : 1.0;
   *us.at(f, 0) = unshielded;
-  *us.at(f, 1) = unshielded / betaM;
+  *us.at(f, 1) = DivRn(unshielded, betaM);
   *sl.at(f, 0) = sLeft;
-  *sl.at(f, 1) = sLeft / betaM;
+  *sl.at(f, 1) = DivRn(sLeft, betaM);
   *sr.at(f, 0) = sRight;
-  *sr.at(f, 1) = sRight / betaM;
+  *sr.at(f, 1) = DivRn(sRight, betaM);
   *ds.at(f, 0) = doubly;
-  *ds.at(f, 1) = doubly / betaM;
+  *ds.at(f, 1) = DivRn(doubly, betaM);
 }
 
 template <int D>
@@ -747,7 +747,8 @@ int CheckKbn(const fvs_handle* h, int dir, bool needs_hllem) {
     SetError("direction %d out of range", dir);
     return FVS_BAD_ARGUMENT;
   }
-  if (needs_hllem && h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+  if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
+      (needs_hllem && h->desc.base == FVS_BASE_GODUNOV_EXACT)) {
     SetError("the cut-cell regular fluxes use an HLLEM base method");
     return FVS_BAD_ARGUMENT;
   }
@@ -1287,7 +1288,7 @@ __global__ __launch_bounds__(kBlock) void LevelFluxKernel(LevelState st, LevelGe
       dt = HllemStableDt<2, DIR>(u[1], u[2], dx, gas);
       const bool mL = a[0] > 0.0 && a[1] > 0.0 && a[2] > 0.0;
       const bool mR = a[1] > 0.0 && a[2] > 0.0 && a[3] > 0.0;
-      const double lh = 0.5 * (*dt_ptr) / dx;
+      const double lh = DivRn(0.5 * (*dt_ptr), dx);
       double L[6], R[6], dummy[6];
       MaskedRecon<2, DIR, LIM>(u[0], u[1], u[2], mL, lh, gas, dummy, L);
       MaskedRecon<2, DIR, LIM>(u[1], u[2], u[3], mR, lh, gas, R, dummy);
@@ -1361,9 +1362,9 @@ __device__ __forceinline__ FaceFrac Fractions(double bl, double bm, double br) {
   f.sl = dBetaL - doubly;
   f.sr = dBetaR - doubly;
   const double b = bm > 0 ? bm : 1.0;
-  f.us_rel = f.us / b;
-  f.sl_rel = f.sl / b;
-  f.sr_rel = f.sr / b;
+  f.us_rel = DivRn(f.us, b);
+  f.sl_rel = DivRn(f.sl, b);
+  f.sr_rel = DivRn(f.sr, b);
   return f;
 }
 
@@ -1387,7 +1388,7 @@ __global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, Level
     for (int k = 0; k < 4; ++k) out.u[k][c] = 0.0;
     return;
   }
-  const double dt_over_dx = (*dt_ptr) / dx;
+  const double dt_over_dx = DivRn(*dt_ptr, dx);
   const int64_t fx = nx + (DIR == 0), fy = ny + (DIR == 1), nf = fx * fy;
   const int64_t fL = j * fx + i;                 // face i (left) of cell (i, j)
   const int64_t fR = fL + (DIR == 0 ? 1 : fx);   // face i + 1 (right)
@@ -1427,17 +1428,17 @@ __global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, Level
     double next;
     if (second) {
       next = (bL == bR) ? prev + dt_over_dx * (Fs_l - Fs_r)
-                        : prev + dt_over_dx / alpha * (bL * Fs_l - bR * Fs_r - (bL - bR) * fbc);
+                        : prev + DivRn(dt_over_dx, alpha) * (bL * Fs_l - bR * Fs_r - (bL - bR) * fbc);
     } else if (bL == bR) {
       next = prev + dt_over_dx * (Fs_l - Fs_r);
     } else if (bL > bR) {
-      const double dLg = Clamp01((fl.sr * (0.5 + center)) / alpha);
-      next = prev + dt_over_dx * (fl.us / alpha) * (Fl - Fs_r) +
-             dt_over_dx * (fl.sl / alpha) * (sL_l - Fs_r) + dt_over_dx * dLg * (Fl - fbc);
+      const double dLg = Clamp01(DivRn(fl.sr * (0.5 + center), alpha));
+      next = prev + dt_over_dx * DivRn(fl.us, alpha) * (Fl - Fs_r) +
+             dt_over_dx * DivRn(fl.sl, alpha) * (sL_l - Fs_r) + dt_over_dx * dLg * (Fl - fbc);
     } else {
-      const double dRg = Clamp01((fr.sl * (0.5 - center)) / alpha);
-      next = prev + dt_over_dx * (fr.us / alpha) * (Fs_l - Fr) +
-             dt_over_dx * (fr.sr / alpha) * (Fs_l - sR_r) + dt_over_dx * dRg * (fbc - Fr);
+      const double dRg = Clamp01(DivRn(fr.sl * (0.5 - center), alpha));
+      next = prev + dt_over_dx * DivRn(fr.us, alpha) * (Fs_l - Fr) +
+             dt_over_dx * DivRn(fr.sr, alpha) * (Fs_l - sR_r) + dt_over_dx * dRg * (fbc - Fr);
     }
     res[k] = next;
   }
@@ -1508,7 +1509,8 @@ int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
     return FVS_BAD_ARGUMENT;
   }
   *out = nullptr;
-  if (h->desc.dim != 2 || h->desc.base == FVS_BASE_GODUNOV_EXACT) {
+  if (h->desc.dim != 2 || h->desc.base == FVS_BASE_GODUNOV_EXACT ||
+      h->desc.base == FVS_BASE_HLL_EINFELDT) {
     SetError("the cut-cell level is 2D with an HLLEM base method");
     return FVS_BAD_ARGUMENT;
   }

@@ -18,7 +18,7 @@
 // which is what Eigen 3.3's scalar and packet min/max reproduce.
 //
 // Every division and square root goes through an arithmetic policy A:
-//   IeeeArith<T>  correctly rounded a / b and sqrt (the reference's
+//   IeeeArith<T>  correctly rounded a / b (DivRn) and sqrt (the reference's
 //                 arithmetic; the FP64 kernels are bit-exact with it);
 //   FastArith<T>  a shared reciprocal per denominator from the hardware
 //                 estimate (v_rcp_f64 refined by two Newton steps), quotients
@@ -45,6 +45,27 @@ __device__ __forceinline__ float rsqrt_ieee(float x) { return __builtin_sqrtf(x)
 __device__ __forceinline__ double rabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float rabs(float x) { return __builtin_fabsf(x); }
 
+// Correctly rounded a / b in FP64.  The compiler's division sequence
+// (v_div_scale, v_rcp_f64, Newton steps, v_div_fmas, v_div_fixup) returns a
+// faithful quotient that can miss the nearest double when a / b lies within
+// about 2^-53 ulp of a rounding midpoint; e.g. 0x1.6666666666663p+1 /
+// 0x1.ffffffffffffbp+0 (a Roe-averaged gamma) gives 0x1.6666666666666p+0
+// instead of ...667p+0.  The exact remainders of the quotient and of its
+// neighbour towards a / b decide between the two (a quotient of two doubles
+// is never exactly a midpoint).  Zero, infinite and NaN quotients pass
+// through unchanged.
+// q: a quotient within one ulp of a / b.
+__device__ __forceinline__ double RoundQuotient(double a, double b, double q) {
+  const double r = __builtin_fma(-q, b, a);
+  const bool down = (r < 0.0) != (b < 0.0) ? !(q < 0.0) : (q < 0.0);
+  const double q2 = __longlong_as_double(__double_as_longlong(q) + (down ? -1ll : 1ll));
+  const double r2 = __builtin_fma(-q2, b, a);
+  const bool regular = q != 0.0 && __builtin_fabs(q) < __builtin_huge_val();
+  return (regular && __builtin_fabs(r2) < __builtin_fabs(r)) ? q2 : q;
+}
+__device__ __forceinline__ double DivRn(double a, double b) { return RoundQuotient(a, b, a / b); }
+__device__ __forceinline__ float DivRn(float a, float b) { return a / b; }
+
 template <class T> struct Gas {
   T gamma;  // 1.4
   T gm1inv; // 1 / (gamma - 1), evaluated in double on the host
@@ -57,12 +78,12 @@ template <class T> struct IeeeArith {
     T b;
   };
   static __device__ __forceinline__ Inv inv(T b) { return Inv{b}; }
-  static __device__ __forceinline__ T div(T a, const Inv& i) { return a / i.b; }
-  static __device__ __forceinline__ T quot(T a, T b) { return a / b; }
+  static __device__ __forceinline__ T div(T a, const Inv& i) { return DivRn(a, i.b); }
+  static __device__ __forceinline__ T quot(T a, T b) { return DivRn(a, b); }
   static __device__ __forceinline__ T sqrt(T x) { return rsqrt_ieee(x); }
   // e_int / (1 / (gamma - 1))
   static __device__ __forceinline__ T pressure(T e_int, const Gas<T>& g) {
-    return e_int / g.gm1inv;
+    return DivRn(e_int, g.gm1inv);
   }
 };
 
@@ -473,6 +494,70 @@ __device__ __forceinline__ T HllemStableDt(const T* L, const T* R, T dx,
   const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
   const T maxS = rmax(rabs(sL), rabs(sR));
   return A::quot(dx, maxS);
+}
+
+// EinfeldtSignalVelocities<PerfectGas<D>> (array form,
+// src/equations/perfect_gas/EinfeldtSignalVelocities.hpp:64-97): signal
+// speeds from the Roe averages of the normal velocity and the enthalpy; the
+// Roe sound speed uses the normal velocity only.  irL / irR are the
+// reciprocals of the densities.
+template <int D, int DIR, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void EinfeldtSignals(const T* L, const typename A::Inv& irL,
+                                                const T* R, const typename A::Inv& irR,
+                                                const Gas<T>& g, T& sL, T& sR) {
+  using I = Idx<D>;
+  const T sqRhoL = A::sqrt(L[I::kRho]);
+  const T sqRhoR = A::sqrt(R[I::kRho]);
+  const T uL = A::div(L[I::kMom + DIR], irL);
+  const T uR = A::div(R[I::kMom + DIR], irR);
+  const T oosqRhoSum = A::quot(T(1), sqRhoL + sqRhoR);
+  const T roeU = (uL * sqRhoL + uR * sqRhoR) * oosqRhoSum;
+  const T hL = A::div(L[I::kE] + L[I::kP], irL);
+  const T hR = A::div(R[I::kE] + R[I::kP], irR);
+  const T roeH = (hL * sqRhoL + hR * sqRhoR) * oosqRhoSum;
+  const T roeA2 = (g.gamma - T(1)) * (roeH - T(0.5) * roeU * roeU);
+  const T roeA = A::sqrt(roeA2);
+  sL = rmin(uL - L[I::kA], roeU - roeA);
+  sR = rmax(roeU + roeA, uR + R[I::kA]);
+}
+
+// Hll<PerfectGas<D>, EinfeldtSignalVelocities>::ComputeNumericFlux (array
+// form, include/fub/flux_method/HllMethod.hpp:260-293).
+template <int D, int DIR, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void HllFlux(T* flux, const T* L, const typename A::Inv& irL,
+                                        const T* R, const typename A::Inv& irR,
+                                        const Gas<T>& g) {
+  using I = Idx<D>;
+  T s0, s1;
+  EinfeldtSignals<D, DIR, T, A>(L, irL, R, irR, g, s0, s1);
+  T fL[I::kCons], fR[I::kCons];
+  PhysFlux<D, DIR, T, A>(fL, L, irL);
+  PhysFlux<D, DIR, T, A>(fR, R, irR);
+  const T sL = rmin(s0, T(0));
+  const T sR = rmax(s1, T(0));
+  const T sLsR = sL * sR;
+  const T ds = sR - sL;
+  const typename A::Inv ids = A::inv((ds > T(0)) ? ds : T(1));
+#pragma unroll
+  for (int k = 0; k < I::kCons; ++k)
+    flux[k] = A::div(sR * fL[k] - sL * fR[k] + sLsR * (R[k] - L[k]), ids);
+}
+
+template <int D, int DIR, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ void HllFlux(T* flux, const T* L, const T* R, const Gas<T>& g) {
+  HllFlux<D, DIR, T, A>(flux, L, A::inv(L[Idx<D>::kRho]), R, A::inv(R[Idx<D>::kRho]), g);
+}
+
+// Hll::ComputeStableDt (array form, include/fub/flux_method/HllMethod.hpp:337-347):
+// dx / max(0, |sL|, |sR|) accumulated left to right.
+template <int D, int DIR, class T, class A = IeeeArith<T>>
+__device__ __forceinline__ T HllStableDt(const T* L, const T* R, T dx, const Gas<T>& g) {
+  T s0, s1;
+  EinfeldtSignals<D, DIR, T, A>(L, A::inv(L[Idx<D>::kRho]), R, A::inv(R[Idx<D>::kRho]), g, s0,
+                                s1);
+  T m = rmax(T(0), rabs(s0));
+  m = rmax(m, rabs(s1));
+  return A::quot(dx, m);
 }
 
 // Stable dt of a face whose two states are the same state Q (boundary face

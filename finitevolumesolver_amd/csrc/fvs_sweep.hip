@@ -21,6 +21,13 @@
 // Reductions (CFL minima) write one partial per workgroup and a one-block
 // finalize kernel reduces them: no contended device-scope atomics, and the
 // result does not depend on workgroup scheduling.
+//
+// This file is compiled twice.  The plain build holds the IEEE FP64 kernels
+// (no fused multiply-adds: bit equality with the reference) and every host
+// entry point; fvs_sweep_fast.hip defines FVS_SWEEP_FAST_TU and compiles the
+// kernels of the tolerance-checked modes (refined-reciprocal FP64, FP32 flux)
+// with multiply-add contraction, reached through LaunchSweepFast and
+// LaunchPatchCflFast.
 #include "fvs_internal.hpp"
 #include "fvs_physics.hpp"
 
@@ -252,8 +259,8 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   const bool owned = lane >= 2 && lane < 2 + kXOwned;
 
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
-  const T lh = static_cast<T>(0.5 * dt / a.dx);
-  const double dt_dx = dt / a.dx;
+  const T lh = static_cast<T>(DivRn(0.5 * dt, a.dx));
+  const double dt_dx = DivRn(dt, a.dx);
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
 
@@ -399,8 +406,8 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
   const int64_t j1 = (j0 + a.seg < nd) ? j0 + a.seg : nd;
 
   const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
-  const T lh = static_cast<T>(0.5 * dt / a.dx);
-  const double dt_dx = dt / a.dx;
+  const T lh = static_cast<T>(DivRn(0.5 * dt, a.dx));
+  const double dt_dx = DivRn(dt, a.dx);
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
 
@@ -872,19 +879,24 @@ int LaunchTyped(hipStream_t s, int dir, const SweepArgs& a) {
 // FP64: IEEE arithmetic (bit-exact with the reference); FP64 reciprocal:
 // double with refined hardware reciprocals; FP32 flux: float flux arithmetic
 // with the single-precision estimates.
+#ifdef FVS_SWEEP_FAST_TU
 template <int D>
 int LaunchPrecision(int precision, bool larr, hipStream_t s, int dir, const SweepArgs& a) {
   if (precision == FVS_FP32_FLUX) {
     return larr ? LaunchTyped<D, true, float, true>(s, dir, a)
                 : LaunchTyped<D, false, float, true>(s, dir, a);
   }
-  if (precision == FVS_FP64_RCP) {
-    return larr ? LaunchTyped<D, true, double, true>(s, dir, a)
-                : LaunchTyped<D, false, double, true>(s, dir, a);
-  }
+  return larr ? LaunchTyped<D, true, double, true>(s, dir, a)
+              : LaunchTyped<D, false, double, true>(s, dir, a);
+}
+#else
+template <int D>
+int LaunchPrecision(int precision, bool larr, hipStream_t s, int dir, const SweepArgs& a) {
+  if (precision != FVS_FP64) return LaunchSweepFast(D, precision, larr, s, dir, a);
   return larr ? LaunchTyped<D, true, double, false>(s, dir, a)
               : LaunchTyped<D, false, double, false>(s, dir, a);
 }
+#endif
 
 // Compute units of the current device (queried once per device).
 int DeviceCus() {
@@ -902,6 +914,33 @@ int DeviceCus() {
 }
 
 } // namespace
+
+#ifdef FVS_SWEEP_FAST_TU
+
+int LaunchSweepFast(int dim, int precision, bool larr, hipStream_t s, int dir,
+                    const SweepArgs& a) {
+  return dim == 2 ? LaunchPrecision<2>(precision, larr, s, dir, a)
+                  : LaunchPrecision<3>(precision, larr, s, dir, a);
+}
+
+int LaunchPatchCflFast(int dim, unsigned gx, unsigned gy, hipStream_t s,
+                       const double* const* cons, const int64_t n[3], const double dx[3],
+                       double gamma, double gm1inv, unsigned long long* partials, int* halt) {
+  const dim3 grid(gx, gy);
+  if (dim == 2) {
+    hipLaunchKernelGGL(PatchCfl2Kernel<true>, grid, dim3(kBlock), 0, s, cons[0], cons[1],
+                       cons[2], cons[3], n[0], n[1], dx[0], dx[1], gamma, gm1inv, partials,
+                       halt);
+  } else {
+    hipLaunchKernelGGL(PatchCfl3Kernel<true>, grid, dim3(64 * kC3Rows), 0, s, cons[0], cons[1],
+                       cons[2], cons[3], cons[4], n[0], n[1], n[2], dx[0], dx[1], dx[2], gamma,
+                       gm1inv, Cfl3Tiles(n[0], n[1]), partials, halt);
+  }
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+#else // FVS_SWEEP_FAST_TU
 
 int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
   const int64_t nz = dim == 3 ? n[2] : 1;
@@ -959,17 +998,19 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
   const int64_t gy = (nm + kCflSeg - 1) / kCflSeg;
   if (gx <= 0 || gx > 0x7fffffff || gy <= 0 || gy > 65535) return FVS_BAD_ARGUMENT;
   const double gm1inv = 1.0 / (desc.gamma - 1.0);
-  const bool fast = desc.precision != FVS_FP64;
   const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
-  if (D == 2) {
-    auto* k = fast ? PatchCfl2Kernel<true> : PatchCfl2Kernel<false>;
-    hipLaunchKernelGGL(k, grid, dim3(kBlock), 0, s, cons[0], cons[1], cons[2], cons[3], n[0],
-                       n[1], dx[0], dx[1], desc.gamma, gm1inv, partials, halt);
+  if (desc.precision != FVS_FP64) {
+    if (int rc = LaunchPatchCflFast(D, grid.x, grid.y, s, cons, n, dx, desc.gamma, gm1inv,
+                                    partials, halt))
+      return rc;
+  } else if (D == 2) {
+    hipLaunchKernelGGL(PatchCfl2Kernel<false>, grid, dim3(kBlock), 0, s, cons[0], cons[1],
+                       cons[2], cons[3], n[0], n[1], dx[0], dx[1], desc.gamma, gm1inv,
+                       partials, halt);
   } else {
-    auto* k = fast ? PatchCfl3Kernel<true> : PatchCfl3Kernel<false>;
-    hipLaunchKernelGGL(k, grid, dim3(64 * kC3Rows), 0, s, cons[0], cons[1], cons[2], cons[3],
-                       cons[4], n[0], n[1], n[2], dx[0], dx[1], dx[2], desc.gamma, gm1inv,
-                       Cfl3Tiles(n[0], n[1]), partials, halt);
+    hipLaunchKernelGGL(PatchCfl3Kernel<false>, grid, dim3(64 * kC3Rows), 0, s, cons[0],
+                       cons[1], cons[2], cons[3], cons[4], n[0], n[1], n[2], dx[0], dx[1],
+                       dx[2], desc.gamma, gm1inv, Cfl3Tiles(n[0], n[1]), partials, halt);
   }
   FVS_LAUNCH_CHECK();
   hipLaunchKernelGGL(FinalizeCflKernel, dim3(1), dim3(kRedBlock), 0, s, partials, gx * gy, D,
@@ -1010,5 +1051,7 @@ int LaunchResetKeys(hipStream_t s, unsigned long long* keys, int n, unsigned int
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
+
+#endif // FVS_SWEEP_FAST_TU
 
 } // namespace fvs

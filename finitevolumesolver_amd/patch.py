@@ -115,7 +115,8 @@ class _Handle:
 
 
 _BASES = {"HLLEM": L.BASE_HLLEM_LARROUTUROU, "HLLEM_Larrouturou": L.BASE_HLLEM_LARROUTUROU,
-          "HLLEM_plain": L.BASE_HLLEM, "Godunov": L.BASE_GODUNOV_EXACT}
+          "HLLEM_plain": L.BASE_HLLEM, "Godunov": L.BASE_GODUNOV_EXACT,
+          "HLL": L.BASE_HLL_EINFELDT, "HLL_Einfeldt": L.BASE_HLL_EINFELDT}
 _LIMITERS = {"MinMod": L.LIMITER_MINMOD, "VanLeer": L.LIMITER_VANLEER,
              "NoLimiter": L.LIMITER_NONE, "NoGradient": L.LIMITER_NO_GRADIENT}
 

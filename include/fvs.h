@@ -46,11 +46,17 @@ enum fvs_status {
  * FVS_BASE_HLLEM: perfect_gas::Hllem<Eq, false>.
  * FVS_BASE_GODUNOV_EXACT: GodunovMethod<Eq, ExactRiemannSolver<Eq>>
  *   (include/fub/flux_method/GodunovMethod.hpp:130-199,
- *    src/equations/perfect_gas/ExactRiemannSolver.hpp:34-353). */
+ *    src/equations/perfect_gas/ExactRiemannSolver.hpp:34-353).
+ * FVS_BASE_HLL_EINFELDT: HllMethod<Eq, EinfeldtSignalVelocities<Eq>>
+ *   (include/fub/flux_method/HllMethod.hpp:159-347,
+ *    src/equations/perfect_gas/EinfeldtSignalVelocities.hpp:24-97); the
+ *   patch operators (fvs_numeric_fluxes, fvs_stable_dt) only: the fused
+ *   sweeps and the level drivers reject it. */
 enum fvs_base_method {
   FVS_BASE_HLLEM_LARROUTUROU = 0,
   FVS_BASE_HLLEM = 1,
-  FVS_BASE_GODUNOV_EXACT = 2
+  FVS_BASE_GODUNOV_EXACT = 2,
+  FVS_BASE_HLL_EINFELDT = 3
 };
 
 /* Reconstruction wrapped around the base flux.

@@ -355,6 +355,76 @@ inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
   return (bR * pL - bL * pR) / db_positive;
 }
 
+// EinfeldtSignalVelocities<PerfectGas<D>>, array form
+// (src/equations/perfect_gas/EinfeldtSignalVelocities.hpp:64-97; the scalar
+// form at :25-62 performs the same operations in the same order).  Roe
+// averages of the normal velocity and of the enthalpy only; the Roe sound
+// speed uses the normal velocity alone.
+template <int D>
+inline void EinfeldtSignals(const Gas& g, const State<D>& L, const State<D>& R, int dir,
+                            double s[2]) {
+  const double rhoL = L.rho;
+  const double rhoR = R.rho;
+  const double rhoUL = L.mom[dir];
+  const double rhoUR = R.mom[dir];
+  const double aL = L.a;
+  const double aR = R.a;
+  const double sqRhoL = std::sqrt(rhoL);
+  const double sqRhoR = std::sqrt(rhoR);
+  const double uL = rhoUL / rhoL;
+  const double uR = rhoUR / rhoR;
+  const double oosqRhoSum = 1.0 / (sqRhoL + sqRhoR);
+  auto roeAvg = [&](double xL, double xR) { return (xL * sqRhoL + xR * sqRhoR) * oosqRhoSum; };
+  const double roeU = roeAvg(uL, uR);
+  const double hL = (L.E + L.p) / rhoL;
+  const double hR = (R.E + R.p) / rhoR;
+  const double roeH = roeAvg(hL, hR);
+  const double roeA2 = (g.gamma - 1.0) * (roeH - 0.5 * roeU * roeU);
+  const double roeA = std::sqrt(roeA2);
+  const double sL1 = uL - aL;
+  const double sL2 = roeU - roeA;
+  const double sR1 = roeU + roeA;
+  const double sR2 = uR + aR;
+  s[0] = std::min(sL1, sL2);
+  s[1] = std::max(sR1, sR2);
+}
+
+// Hll<PerfectGas<D>, EinfeldtSignalVelocities>::ComputeNumericFlux, array
+// form (include/fub/flux_method/HllMethod.hpp:260-293).  Returns p*.  The
+// scalar form (:191-222) differs only where the array form guards a
+// non-positive wave-speed difference (then 1).
+template <int D>
+inline double HllFlux(const Gas& g, Cons<D>& nf, const State<D>& L, const State<D>& R,
+                      int dir) {
+  double s[2];
+  EinfeldtSignals<D>(g, L, R, dir, s);
+  Cons<D> fL, fR;
+  Flux<D>(fL, L, dir);
+  Flux<D>(fR, R, dir);
+  const double sL = std::min(s[0], 0.0);
+  const double sR = std::max(s[1], 0.0);
+  const double sLsR = sL * sR;
+  double ds = sR - sL;
+  ds = (ds > 0.0) ? ds : 1.0;
+  for (int k = 0; k < D + 2; ++k) {
+    C(nf, k) = (sR * C(fL, k) - sL * C(fR, k) + sLsR * (C(R, k) - C(L, k))) / ds;
+  }
+  return (sR * L.p - sL * R.p) / ds;
+}
+
+// Hll::ComputeStableDt, array form (include/fub/flux_method/HllMethod.hpp:337-347):
+// dx / max(0, |sL|, |sR|) accumulated left to right.
+template <int D>
+inline double HllStableDt(const Gas& g, const State<D>& L, const State<D>& R, double dx,
+                          int dir) {
+  double s[2];
+  EinfeldtSignals<D>(g, L, R, dir, s);
+  double m = 0.0;
+  m = std::max(m, std::abs(s[0]));
+  m = std::max(m, std::abs(s[1]));
+  return dx / m;
+}
+
 // Hllem::ComputeStableDt, array form (src/equations/perfect_gas/HllemMethod.hpp:1152-1216).
 template <int D>
 inline double HllemStableDt(const Gas& g, const State<D>& L, const State<D>& R,

@@ -40,7 +40,7 @@ struct Method {
   int width() const { return recon == FVS_RECON_MUSCL_HANCOCK ? 2 : 1; }
 };
 
-// Base two-state flux (Hllem array path or Godunov scalar path).
+// Base two-state flux (Hllem or Hll array path, or Godunov scalar path).
 template <int D>
 void BaseFlux(const Method& m, Cons<D>& f, const State<D>& L, const State<D>& R,
               int dir, int* newton_fail) {
@@ -49,6 +49,8 @@ void BaseFlux(const Method& m, Cons<D>& f, const State<D>& L, const State<D>& R,
     MiddleState ms = SolveRiemannProblem<D>(m.gas, sol, L, R, dir, -1.0, m.tol, m.max_it);
     if (!ms.converged && newton_fail) *newton_fail = 1;
     Flux<D>(f, sol, dir);
+  } else if (m.base == FVS_BASE_HLL_EINFELDT) {
+    HllFlux<D>(m.gas, f, L, R, dir);
   } else {
     HllemFlux<D>(m.gas, f, L, R, dir, m.base == FVS_BASE_HLLEM_LARROUTUROU, m.quirk3d);
   }
@@ -60,6 +62,7 @@ double BaseStableDt(const Method& m, const State<D>& L, const State<D>& R, doubl
   if (m.base == FVS_BASE_GODUNOV_EXACT) {
     return GodunovStableDt<D>(m.gas, L, R, dx, dir, m.tol, m.max_it);
   }
+  if (m.base == FVS_BASE_HLL_EINFELDT) return HllStableDt<D>(m.gas, L, R, dx, dir);
   return HllemStableDt<D>(m.gas, L, R, dx, dir);
 }
 

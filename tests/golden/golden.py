@@ -18,7 +18,11 @@ reference):
   cutcell_2d             FillCutCellData, IsCutCell, masked regular fluxes,
                          boundary fluxes (HLLEM boundary solver), stabilised
                          fluxes and the h-box update (TimeIntegrator) on a
-                         12 x 12 patch cut by a disk and a wedge.
+                         12 x 12 patch cut by a disk and a wedge;
+  hll_simd_2d            HLL (Einfeldt signals) fluxes and stable dt on the
+                         inputs of the reference's own HLL test
+                         (tests/test.simd_FluxMethod.cpp:54-186): one
+                         MUSCL-Hancock(HLL) face and a 16 x 16 patch in X and Y.
 """
 from __future__ import annotations
 
@@ -237,11 +241,81 @@ def cutcell_outputs(inp: dict) -> dict:
     return out
 
 
+def hll_inputs() -> dict:
+    """The inputs of the reference's tests/test.simd_FluxMethod.cpp:54-186:
+    one MUSCL-Hancock(HLL) face between two pairs of gases at rest
+    (rho = 1, E = 2 | E = 1; dt = dx = 1), and a 16 x 16 patch with
+    rho = 1, u = 0, E = 1 + sin(x^2 + y^2) / 2 at x = i dx, y = j dx
+    (dx = 0.0125), grown by one cell in the sweep direction (X and Y)."""
+    import math
+    inp = {"single": np.array([[1.0, 0.0, 0.0, 2.0], [1.0, 0.0, 0.0, 2.0],
+                               [1.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 1.0]]),
+           "dx": 0.0125}
+    for d in range(2):
+        lo = [0, 0]
+        hi = [15, 15]
+        lo[d] -= 1
+        hi[d] += 1
+        ny, nx = hi[1] - lo[1] + 1, hi[0] - lo[0] + 1
+        cons = np.zeros((4, 1, ny, nx))
+        cons[0] = 1.0
+        for j in range(ny):
+            for i in range(nx):
+                x, y = (lo[0] + i) * 0.0125, (lo[1] + j) * 0.0125
+                cons[3, 0, j, i] = 1.0 + 0.5 * math.sin(x * x + y * y)
+        inp[f"patch{d}"] = cons
+        inp[f"patch_lo{d}"] = np.array([lo[0], lo[1], 0], np.int64)
+    return inp
+
+
+def hll_outputs(inp: dict) -> dict:
+    """Oracle HLL (Einfeldt signals) fluxes and stable dt on those inputs."""
+    out = {}
+    desc = fvs.make_desc(fvs.PerfectGas(dim=2), base="HLL", reconstruction="MusclHancock")
+    st = np.zeros((4, 6))
+    for s_, q in enumerate(inp["single"]):
+        st[s_, :4] = q
+        st[s_, 4] = q[3] * 0.4  # p = (E - 0) / (1 / (gamma - 1)) is exact here
+    comp = np.zeros((6, 1, 1, 4))
+    comp[:4, 0, 0, :] = inp["single"].T
+    cv = O.host_view(comp)
+    O.load().oracle_complete_view(2, 1.4, C.byref(cv), C.byref(cv))
+    stencil = np.ascontiguousarray(comp[:, 0, 0, :].T)  # (4 states, 6 components)
+    f = np.zeros(4)
+    rc = O.load().oracle_face_flux(C.byref(desc), 0, 1.0, 1.0, 1,
+                                   stencil.ctypes.data_as(C.POINTER(C.c_double)),
+                                   f.ctypes.data_as(C.POINTER(C.c_double)))
+    assert rc == 0
+    out["single_flux"] = f
+    out["single_stencil"] = stencil
+    first = fvs.make_desc(fvs.PerfectGas(dim=2), base="HLL", reconstruction="None")
+    dx = float(inp["dx"])
+    for d in range(2):
+        cons, lo = inp[f"patch{d}"], tuple(int(v) for v in inp[f"patch_lo{d}"])
+        comp = np.zeros((6,) + cons.shape[1:])
+        comp[:4] = cons
+        cv = O.host_view(comp, lo)
+        O.load().oracle_complete_view(2, 1.4, C.byref(cv), C.byref(cv))
+        out[f"complete{d}"] = comp
+        fshape = [4, 1, 16, 16]
+        fshape[3 - d] += 1
+        flux = np.zeros(fshape)
+        dtc = C.c_double()
+        O.load().oracle_stable_dt(C.byref(first), C.byref(cv), dx, d, C.byref(dtc))
+        rc = O.load().oracle_numeric_fluxes(C.byref(first), C.byref(O.host_view(flux)),
+                                            C.byref(cv), dtc.value, dx, d)
+        assert rc == 0
+        out[f"flux{d}"] = flux
+        out[f"dt{d}"] = np.array([dtc.value])
+    return out
+
+
 CASES = {
     "fluxes_2d": (lambda: flux_inputs(2), lambda i: flux_outputs(2, i)),
     "fluxes_3d": (lambda: flux_inputs(3), lambda i: flux_outputs(3, i)),
     "sweep_2d": (sweep_inputs, sweep_outputs),
     "cutcell_2d": (cutcell_inputs, cutcell_outputs),
+    "hll_simd_2d": (hll_inputs, hll_outputs),
 }
 
 

@@ -128,3 +128,28 @@ def test_gpu_cutcell_matches_golden(cuda):
         np.testing.assert_array_equal(t_u.cpu().numpy(), want[f"update{d}"])
     _note(f"cut-cell golden vectors: {int(flags.sum())} cut cells, fractions, fluxes and "
           f"updates bit-exact")
+
+
+def test_gpu_hll_matches_golden(cuda):
+    """HLL with Einfeldt signals on the inputs of the reference's own HLL test
+    (tests/test.simd_FluxMethod.cpp:54-186), bit for bit."""
+    import torch
+    inp, want, _ = G.load_case("hll_simd_2d")
+    dx = inp["dx"]
+    first = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLL", reconstruction="None")
+    for d in range(2):
+        lo = tuple(int(v) for v in inp[f"patch_lo{d}"])
+        states = PatchView(_dev(want[f"complete{d}"], cuda), lo)
+        ref = want[f"flux{d}"]
+        sdt = first.ComputeStableDt(states, dx, d).item()
+        assert sdt == float(want[f"dt{d}"][0])
+        t_f = torch.zeros(ref.shape, dtype=torch.float64, device=cuda)
+        first.ComputeNumericFluxes(PatchView(t_f, (0, 0, 0)), states, sdt, dx, d)
+        np.testing.assert_array_equal(t_f.cpu().numpy(), ref, err_msg=f"HLL dir {d}")
+    muscl = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLL", reconstruction="MusclHancock")
+    st = want["single_stencil"]  # (4 cells, 6 components) along x
+    t_st = _dev(np.ascontiguousarray(st.T.reshape(6, 1, 1, 4)), cuda)
+    t_f = torch.zeros((4, 1, 1, 1), dtype=torch.float64, device=cuda)
+    muscl.ComputeNumericFluxes(PatchView(t_f, (0, 0, 0)), PatchView(t_st, (-2, 0, 0)), 1.0, 1.0, 0)
+    np.testing.assert_array_equal(t_f.cpu().numpy().reshape(4), want["single_flux"])
+    _note("HLL (Einfeldt) golden vectors: 16 x 16 X and Y fluxes, stable dt, MUSCL face bit-exact")
