@@ -63,7 +63,18 @@ __device__ __forceinline__ double RoundQuotient(double a, double b, double q) {
   const bool regular = q != 0.0 && __builtin_fabs(q) < __builtin_huge_val();
   return (regular && __builtin_fabs(r2) < __builtin_fabs(r)) ? q2 : q;
 }
-__device__ __forceinline__ double DivRn(double a, double b) { return RoundQuotient(a, b, a / b); }
+// Measured on gfx950 with quotients within ~2^-50 ulp of a midpoint, the
+// misrounded ones all have denominators whose significand lies within a few
+// ulps of 2 (none among denominators farther than 2^-40 from 2, nor near 1 or
+// uniformly drawn); the correction therefore runs only for significands of at
+// least 2 - 2^-20 (high 20 mantissa bits all set), a branch that waves skip
+// when none of their lanes needs it.
+__device__ __forceinline__ double DivRn(double a, double b) {
+  const double q = a / b;
+  const int hi = static_cast<int>(__double_as_longlong(b) >> 32);
+  if (__builtin_expect((hi & 0xFFFFF) == 0xFFFFF, 0)) return RoundQuotient(a, b, q);
+  return q;
+}
 __device__ __forceinline__ float DivRn(float a, float b) { return a / b; }
 
 template <class T> struct Gas {
@@ -81,6 +92,9 @@ template <class T> struct IeeeArith {
   static __device__ __forceinline__ T div(T a, const Inv& i) { return DivRn(a, i.b); }
   static __device__ __forceinline__ T quot(T a, T b) { return DivRn(a, b); }
   static __device__ __forceinline__ T sqrt(T x) { return rsqrt_ieee(x); }
+  // std::min / std::max argument semantics (signed zeros and NaN as the reference)
+  static __device__ __forceinline__ T min(T a, T b) { return rmin(a, b); }
+  static __device__ __forceinline__ T max(T a, T b) { return rmax(a, b); }
   // e_int / (1 / (gamma - 1))
   static __device__ __forceinline__ T pressure(T e_int, const Gas<T>& g) {
     return DivRn(e_int, g.gm1inv);
@@ -115,6 +129,10 @@ template <> struct FastArith<double> {
     s = __builtin_fma(d, h, s);
     return (x == 0.0) ? x : s;
   }
+  // hardware minimum / maximum (one instruction; may differ from std::min in
+  // the sign of a zero result or when an operand is NaN)
+  static __device__ __forceinline__ double min(double a, double b) { return __builtin_fmin(a, b); }
+  static __device__ __forceinline__ double max(double a, double b) { return __builtin_fmax(a, b); }
   static __device__ __forceinline__ double pressure(double e_int, const Gas<double>& g) {
     return e_int * g.gm1;
   }
@@ -131,6 +149,8 @@ template <> struct FastArith<float> {
     return a * __builtin_amdgcn_rcpf(b);
   }
   static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  static __device__ __forceinline__ float min(float a, float b) { return __builtin_fminf(a, b); }
+  static __device__ __forceinline__ float max(float a, float b) { return __builtin_fmaxf(a, b); }
   static __device__ __forceinline__ float pressure(float e_int, const Gas<float>& g) {
     return e_int * g.gm1;
   }
@@ -202,7 +222,7 @@ __device__ __forceinline__ T LimitedSlope(T qL, T qM, T qR) {
   const T sL = qM - qL;
   const T sR = qR - qM;
   if constexpr (LIM == kMinMod) {
-    return (sR > T(0)) ? rmax(T(0), rmin(sL, sR)) : rmin(T(0), rmax(sL, sR));
+    return (sR > T(0)) ? A::max(T(0), A::min(sL, sR)) : A::min(T(0), A::max(sL, sR));
   } else if constexpr (LIM == kVanLeer) {
     const T r = T(2) * sL * sR;
     return (r > T(0)) ? A::quot(r, sL + sR) : T(0);
@@ -327,15 +347,15 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const typename A:
     const T squaredNormRoeU_half = T(0.5) * squaredNormRoeU;
     const T roeA2 = gm1 * (roeH - squaredNormRoeU_half);
     const T roeA = A::sqrt(roeA2);
-    const T maxA = rmax(aL, aR);
+    const T maxA = A::max(aL, aR);
     const T sL1 = uLd - maxA;
     const T sL2 = roeU - roeA;
     const T sR1 = roeU + roeA;
     const T sR2 = uRd + maxA;
-    const T sL = rmin(sL1, sL2);
-    const T sR = rmax(sR1, sR2);
-    const T bL = rmin(sL, T(0));
-    const T bR = rmax(sR, T(0));
+    const T sL = A::min(sL1, sL2);
+    const T sR = A::max(sR1, sR2);
+    const T bL = A::min(sL, T(0));
+    const T bR = A::max(sR, T(0));
     const T bLbR = bL * bR;
     const T db = bR - bL;
     const T db_positive = (db > T(0)) ? db : T(1);
@@ -399,11 +419,11 @@ __device__ __forceinline__ void HllemFlux(T* flux, const T* L, const typename A:
     const T squaredNormRoeU_half = T(0.5) * squaredNormRoeU;
     const T roeA2 = gm1 * (roeH - squaredNormRoeU_half);
     const T roeA = A::sqrt(roeA2);
-    const T maxA = rmax(aL, aR);
-    const T sL = rmin(uL[DIR] - maxA, roeU[DIR] - roeA);
-    const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
-    const T bL = rmin(sL, T(0));
-    const T bR = rmax(sR, T(0));
+    const T maxA = A::max(aL, aR);
+    const T sL = A::min(uL[DIR] - maxA, roeU[DIR] - roeA);
+    const T sR = A::max(roeU[DIR] + roeA, uR[DIR] + maxA);
+    const T bL = A::min(sL, T(0));
+    const T bR = A::max(sR, T(0));
     const T bLbR = bL * bR;
     const T db = bR - bL;
     const T db_positive = (db > T(0)) ? db : T(1);
@@ -489,10 +509,10 @@ __device__ __forceinline__ T HllemStableDt(const T* L, const T* R, T dx,
   for (int i = 1; i < D; ++i) sq += roeU[i] * roeU[i];
   const T roeA2 = g.gm1 * (roeH - T(0.5) * sq);
   const T roeA = A::sqrt(roeA2);
-  const T maxA = rmax(L[I::kA], R[I::kA]);
-  const T sL = rmin(uL[DIR] - maxA, roeU[DIR] - roeA);
-  const T sR = rmax(roeU[DIR] + roeA, uR[DIR] + maxA);
-  const T maxS = rmax(rabs(sL), rabs(sR));
+  const T maxA = A::max(L[I::kA], R[I::kA]);
+  const T sL = A::min(uL[DIR] - maxA, roeU[DIR] - roeA);
+  const T sR = A::max(roeU[DIR] + roeA, uR[DIR] + maxA);
+  const T maxS = A::max(rabs(sL), rabs(sR));
   return A::quot(dx, maxS);
 }
 
@@ -517,8 +537,8 @@ __device__ __forceinline__ void EinfeldtSignals(const T* L, const typename A::In
   const T roeH = (hL * sqRhoL + hR * sqRhoR) * oosqRhoSum;
   const T roeA2 = (g.gamma - T(1)) * (roeH - T(0.5) * roeU * roeU);
   const T roeA = A::sqrt(roeA2);
-  sL = rmin(uL - L[I::kA], roeU - roeA);
-  sR = rmax(roeU + roeA, uR + R[I::kA]);
+  sL = A::min(uL - L[I::kA], roeU - roeA);
+  sR = A::max(roeU + roeA, uR + R[I::kA]);
 }
 
 // Hll<PerfectGas<D>, EinfeldtSignalVelocities>::ComputeNumericFlux (array
@@ -533,8 +553,8 @@ __device__ __forceinline__ void HllFlux(T* flux, const T* L, const typename A::I
   T fL[I::kCons], fR[I::kCons];
   PhysFlux<D, DIR, T, A>(fL, L, irL);
   PhysFlux<D, DIR, T, A>(fR, R, irR);
-  const T sL = rmin(s0, T(0));
-  const T sR = rmax(s1, T(0));
+  const T sL = A::min(s0, T(0));
+  const T sR = A::max(s1, T(0));
   const T sLsR = sL * sR;
   const T ds = sR - sL;
   const typename A::Inv ids = A::inv((ds > T(0)) ? ds : T(1));
@@ -555,8 +575,8 @@ __device__ __forceinline__ T HllStableDt(const T* L, const T* R, T dx, const Gas
   T s0, s1;
   EinfeldtSignals<D, DIR, T, A>(L, A::inv(L[Idx<D>::kRho]), R, A::inv(R[Idx<D>::kRho]), g, s0,
                                 s1);
-  T m = rmax(T(0), rabs(s0));
-  m = rmax(m, rabs(s1));
+  T m = A::max(T(0), rabs(s0));
+  m = A::max(m, rabs(s1));
   return A::quot(dx, m);
 }
 
@@ -580,9 +600,9 @@ __device__ __forceinline__ T HllemStableDtUniform(const T* Q, T dx, const Gas<T>
   const T roeA2 = g.gm1 * (h - T(0.5) * sq);
   const T roeA = A::sqrt(roeA2);
   const T a = Q[I::kA];
-  const T sL = rmin(u[DIR] - a, u[DIR] - roeA);
-  const T sR = rmax(u[DIR] + roeA, u[DIR] + a);
-  const T maxS = rmax(rabs(sL), rabs(sR));
+  const T sL = A::min(u[DIR] - a, u[DIR] - roeA);
+  const T sR = A::max(u[DIR] + roeA, u[DIR] + a);
+  const T maxS = A::max(rabs(sL), rabs(sR));
   return A::quot(dx, maxS);
 }
 

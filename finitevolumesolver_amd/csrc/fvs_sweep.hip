@@ -157,10 +157,10 @@ __device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R,
   for (int i = 1; i < D; ++i) sq += roeU[i] * roeU[i];
   const double roeA2 = gm1 * (roeH - 0.5 * sq);
   const double roeA = A::sqrt(roeA2);
-  const double maxA = rmax(L.a, R.a);
-  const double sL = rmin(L.u[DIR] - maxA, roeU[DIR] - roeA);
-  const double sR = rmax(roeU[DIR] + roeA, R.u[DIR] + maxA);
-  const double maxS = rmax(rabs(sL), rabs(sR));
+  const double maxA = A::max(L.a, R.a);
+  const double sL = A::min(L.u[DIR] - maxA, roeU[DIR] - roeA);
+  const double sR = A::max(roeU[DIR] + roeA, R.u[DIR] + maxA);
+  const double maxS = A::max(rabs(sL), rabs(sR));
   return A::quot(dx, maxS);
 }
 
