@@ -21,11 +21,12 @@
 //   IeeeArith<T>  correctly rounded a / b (DivRn) and sqrt (the reference's
 //                 arithmetic; the FP64 kernels are bit-exact with it);
 //   FastArith<T>  a shared reciprocal per denominator from the hardware
-//                 estimate (v_rcp_f64 refined by two Newton steps), quotients
+//                 estimate (v_rcp_f64 refined by one Newton step), quotients
 //                 as products, and sqrt from v_rsq_f64 refined by one
 //                 Goldschmidt step and one residual correction.  Each result
-//                 is within a few units in the last place of the correctly
-//                 rounded one; a zero denominator yields NaN instead of inf.
+//                 is within about ten units in the last place of the
+//                 correctly rounded one; a zero denominator yields NaN
+//                 instead of inf.
 //                 The float form uses the single-precision estimates as is.
 // Sharing a reciprocal changes nothing under IeeeArith (Inv keeps the
 // denominator and every quotient is still a / b).
@@ -109,10 +110,10 @@ template <> struct FastArith<double> {
     double y;
   };
   static __device__ __forceinline__ Inv inv(double b) {
+    // v_rcp_f64 is good to ~2^-24 (2.5e8 ulp measured); one Newton step
+    // brings it to <= 11 ulp (mean 0.6 ulp over 4M random operands)
     double y = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-b, y, 1.0);
+    const double e = __builtin_fma(-b, y, 1.0);
     y = __builtin_fma(y, e, y);
     return Inv{y};
   }
