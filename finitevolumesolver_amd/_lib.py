@@ -115,6 +115,7 @@ SIGNATURES = {
     "fvs_level_advance": (_I, [_P, _I, _D, _I, _PD, C.POINTER(_I), C.POINTER(_I)]),
     "fvs_level_time": (_I, [_P, _PD, _PI64]),
     "fvs_level_set_timing": (_I, [_P, _I]),
+    "fvs_level_set_graphs": (_I, [_P, _I]),
     "fvs_level_kernel_times": (_I, [_P, _PD, _PI64]),
     "fvs_split_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P]),
     "fvs_patch_stable_dt": (_I, [_P, C.POINTER(_P), _PI64, _PD, _P]),

@@ -738,6 +738,15 @@ struct fvs_level {
   std::vector<hipEvent_t> spare;
   double t_ms[FVS_LEVEL_TIMERS] = {};
   int64_t t_n[FVS_LEVEL_TIMERS] = {};
+  // Captured steps (HIP graphs), one per (start buffer, cycle parity, network).
+  int use_graphs = 1;
+  hipStream_t cap = nullptr;  // capture stream (default priority)
+  struct StepGraph {
+    int cur, parity, next;
+    const fvs_mlp* net;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
 };
 
 namespace {
@@ -822,6 +831,9 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
   L->h = h;
   L->patch = *patch;
   for (int d = 0; d < 3; ++d) L->n[d] = n[d];
+  // The launch geometry queries the device's CU count once; do it here, not
+  // inside a step capture.
+  (void)SweepBlocks(D, 0, n, 32);
   L->ncell = n[0] * n[1] * n[2];
   const size_t bytes = static_cast<size_t>(L->ncell) * (D + 2) * 3 * sizeof(double);
   if (hipMalloc(&L->block, bytes) != hipSuccess) {
@@ -874,7 +886,18 @@ int fvs_level_destroy(fvs_level* L) {
     (void)hipEventDestroy(m.b);
   }
   for (hipEvent_t e : L->spare) (void)hipEventDestroy(e);
+  for (const auto& g : L->graphs) (void)hipGraphExecDestroy(g.exec);
+  if (L->cap) (void)hipStreamDestroy(L->cap);
   delete L;
+  return FVS_OK;
+}
+
+int fvs_level_set_graphs(fvs_level* L, int on) {
+  if (!L) {
+    SetError("null level");
+    return FVS_BAD_ARGUMENT;
+  }
+  L->use_graphs = on ? 1 : 0;
   return FVS_OK;
 }
 
@@ -959,6 +982,132 @@ int fvs_level_time(fvs_level* L, double* t, int64_t* cycles) {
   return FVS_OK;
 }
 
+// One time step of the level (CFL pass, the split sweeps with their CFL
+// checks, step end) launched on the handle's stream from buffer `cur`;
+// *next receives the buffer holding the result.  The launch sequence
+// depends only on (cur, cycle parity, network), so it can be captured once
+// per combination into a HIP graph.
+static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
+                           const fvs_mlp* net, int* next) {
+  fvs_handle* h = L->h;
+  const int D = h->desc.dim;
+  const bool strang = L->patch.splitting == FVS_SPLIT_STRANG;
+  const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
+  int rc = FVS_OK;
+  // MakeSplitDirections (SplittingMethod.hpp:115-126), subcycle {0, 1}.
+  const int is_odd = static_cast<int>(cyc % 2);
+  int dirs[3];
+  for (int i = 0; i < D; ++i) dirs[i] = is_odd ? (D - 1 - i) : i;
+  int seq[6];
+  int ns = 0;
+  for (int i = 0; i < D; ++i) seq[ns++] = dirs[i];
+  if (strang)
+    for (int i = D - 1; i >= 0; --i) seq[ns++] = dirs[i];
+  const double* cin[5];
+  for (int q = 0; q < D + 2; ++q) cin[q] = L->buf[cur][q];
+  if (godunov) {
+    const int64_t nb = GodunovCflBlocks(D, L->n);
+    rc = Timed(L, s, FVS_TIMER_CFL, [&] {
+      return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, nullptr, L->partials,
+                              &L->ctl->halt, net);
+    });
+    for (int d = 0; d < D && rc == FVS_OK; ++d) {
+      rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
+                                 h->newton_stats, L->ctl);
+    }
+    if (rc == FVS_OK) rc = LaunchStepDt(s, L->ctl, D, seq[0], strang);
+  } else {
+    rc = Timed(L, s, FVS_TIMER_CFL, [&] {
+      return LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
+                            &L->ctl->halt, seq[0], strang);
+    });
+  }
+  if (rc) return rc;
+  const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
+  int src = b0, dst = b1;
+  for (int w = 0; w < ns; ++w) {
+    SweepArgs a;
+    double* outp[5];
+    for (int q = 0; q < D + 2; ++q) outp[q] = L->buf[dst][q];
+    const double* inp[5];
+    for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
+    if (godunov) {
+      const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, 32);
+      rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] {
+        return LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
+                                  &L->ctl->split_dt, 0.0, nullptr, L->partials,
+                                  &L->ctl->halt, 1, net);
+      });
+      if (rc == FVS_OK)
+        rc = LaunchGodunovFinalize(s, L->partials, nb, &L->ctl->last_check, h->newton_stats,
+                                   L->ctl);
+      if (rc == FVS_OK && w > 0) rc = LaunchCheckValue(s, L->ctl, &L->ctl->last_check);
+      if (rc) break;
+      src = dst;
+      dst = (dst == b1) ? b2 : b1;
+      continue;
+    }
+    rc = FillSweepArgs(h, inp, outp, L->n, L->patch.dx[seq[w]], &a);
+    if (rc) break;
+    a.dt_ptr = &L->ctl->split_dt;
+    a.halt = &L->ctl->halt;
+    a.check = (w > 0);
+    a.partials = L->partials;
+    rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] { return LaunchSweep(h->desc, s, seq[w], a); });
+    if (rc) break;
+    if (a.check) {
+      rc = LaunchFinalizeCheck(s, L->partials, SweepBlocks(D, seq[w], L->n, a.seg), nullptr,
+                               L->ctl);
+      if (rc) break;
+    }
+    src = dst;
+    dst = (dst == b1) ? b2 : b1;
+  }
+  if (rc) return rc;
+  *next = src;
+  return LaunchStepEnd(s, L->ctl);
+}
+
+// The step as a replayed HIP graph, captured on first use per key on the
+// level's own capture stream (the handle's stream may be the null stream,
+// which cannot capture) and launched on the handle's stream.
+static int LaunchLevelStepGraph(fvs_level* L, int cur, int64_t cyc, const fvs_mlp* net, int* next) {
+  hipStream_t s = L->h->stream;
+  const int parity = static_cast<int>(cyc % 2);
+  for (const auto& g : L->graphs) {
+    if (g.cur == cur && g.parity == parity && g.net == net) {
+      *next = g.next;
+      FVS_HIP_CHECK(hipGraphLaunch(g.exec, s));
+      return FVS_OK;
+    }
+  }
+  fvs_level::StepGraph g;
+  g.cur = cur;
+  g.parity = parity;
+  g.net = net;
+  if (!L->cap) FVS_HIP_CHECK(hipStreamCreateWithFlags(&L->cap, hipStreamNonBlocking));
+  FVS_HIP_CHECK(hipStreamBeginCapture(L->cap, hipStreamCaptureModeThreadLocal));
+  const int rc = LaunchLevelStep(L, L->cap, cur, cyc, net, &g.next);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(L->cap, &graph);
+  if (rc != FVS_OK || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != FVS_OK) return rc;
+    SetError("step capture failed: %s", hipGetErrorString(ec));
+    return FVS_HIP_ERROR;
+  }
+  const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    SetError("step graph instantiation failed: %s", hipGetErrorString(ei));
+    return FVS_HIP_ERROR;
+  }
+  L->graphs.push_back(g);
+  *next = g.next;
+  FVS_HIP_CHECK(hipGraphLaunch(g.exec, s));
+  return FVS_OK;
+}
+
 int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
                       double* dt_log, int* steps_done, int* failures) {
   if (!L || max_steps < 0 || batch < 1) {
@@ -967,8 +1116,6 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
   }
   fvs_handle* h = L->h;
   hipStream_t s = h->stream;
-  const int D = h->desc.dim;
-  const bool strang = L->patch.splitting == FVS_SPLIT_STRANG;
   const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
   // With a network attached, the Godunov kernels infer each face's guess
   // inline (no guess arrays, no extra pass over the state).
@@ -976,15 +1123,18 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
   FVS_HIP_CHECK(hipStreamSynchronize(s));
+  double* dlog = nullptr;
+  if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
   c.final_time = final_time;
   c.cfl = L->patch.cfl;
   c.halt = 0;
+  c.dt_log = dlog;
+  c.dt_log_base = c.cycle;
+  c.dt_log_len = max_steps;
   FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
   const int64_t cycle0 = c.cycle;
   int64_t cycle = c.cycle;
   int nfail = 0;
-  double* dlog = nullptr;
-  if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
   int rc = FVS_OK;
   std::vector<int> start_buf;
   while (cycle - cycle0 < max_steps) {
@@ -994,78 +1144,11 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
     for (int sidx = 0; sidx < k && rc == FVS_OK; ++sidx) {
       start_buf[sidx] = cur;
       const int64_t cyc = cycle + sidx;
-      // MakeSplitDirections (SplittingMethod.hpp:115-126), subcycle {0, 1}.
-      const int is_odd = static_cast<int>(cyc % 2);
-      int dirs[3];
-      for (int i = 0; i < D; ++i) dirs[i] = is_odd ? (D - 1 - i) : i;
-      int seq[6];
-      int ns = 0;
-      for (int i = 0; i < D; ++i) seq[ns++] = dirs[i];
-      if (strang)
-        for (int i = D - 1; i >= 0; --i) seq[ns++] = dirs[i];
-      const double* cin[5];
-      for (int q = 0; q < D + 2; ++q) cin[q] = L->buf[cur][q];
-      if (godunov) {
-        const int64_t nb = GodunovCflBlocks(D, L->n);
-        rc = Timed(L, s, FVS_TIMER_CFL, [&] {
-          return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, nullptr, L->partials,
-                                  &L->ctl->halt, net);
-        });
-        for (int d = 0; d < D && rc == FVS_OK; ++d) {
-          rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
-                                     h->newton_stats, L->ctl);
-        }
-        if (rc == FVS_OK) rc = LaunchStepDt(s, L->ctl, D, seq[0], strang);
-      } else {
-        rc = Timed(L, s, FVS_TIMER_CFL, [&] {
-          return LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,
-                                &L->ctl->halt, seq[0], strang);
-        });
-      }
+      int next = cur;
+      rc = (L->timing || !L->use_graphs) ? LaunchLevelStep(L, s, cur, cyc, net, &next)
+                                         : LaunchLevelStepGraph(L, cur, cyc, net, &next);
       if (rc) break;
-      const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
-      int src = b0, dst = b1;
-      for (int w = 0; w < ns; ++w) {
-        SweepArgs a;
-        double* outp[5];
-        for (int q = 0; q < D + 2; ++q) outp[q] = L->buf[dst][q];
-        const double* inp[5];
-        for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
-        if (godunov) {
-          const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, 32);
-          rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] {
-            return LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
-                                      &L->ctl->split_dt, 0.0, nullptr, L->partials,
-                                      &L->ctl->halt, 1, net);
-          });
-          if (rc == FVS_OK)
-            rc = LaunchGodunovFinalize(s, L->partials, nb, &L->ctl->last_check, h->newton_stats,
-                                       L->ctl);
-          if (rc == FVS_OK && w > 0) rc = LaunchCheckValue(s, L->ctl, &L->ctl->last_check);
-          if (rc) break;
-          src = dst;
-          dst = (dst == b1) ? b2 : b1;
-          continue;
-        }
-        rc = FillSweepArgs(h, inp, outp, L->n, L->patch.dx[seq[w]], &a);
-        if (rc) break;
-        a.dt_ptr = &L->ctl->split_dt;
-        a.halt = &L->ctl->halt;
-        a.check = (w > 0);
-        a.partials = L->partials;
-        rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] { return LaunchSweep(h->desc, s, seq[w], a); });
-        if (rc) break;
-        if (a.check) {
-          rc = LaunchFinalizeCheck(s, L->partials, SweepBlocks(D, seq[w], L->n, a.seg), nullptr,
-                                   L->ctl);
-          if (rc) break;
-        }
-        src = dst;
-        dst = (dst == b1) ? b2 : b1;
-      }
-      if (rc) break;
-      cur = src;
-      rc = LaunchStepEnd(s, L->ctl, dlog, cycle0, max_steps);
+      cur = next;
     }
     if (rc) break;
     FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));

@@ -80,6 +80,9 @@ struct Ctl {
   int64_t cycle;
   double dt_dir[3];   // step-start stable dt per direction (decoded)
   double last_check;
+  double* dt_log;       // optional log of the steps' dt (set per advance call)
+  int64_t dt_log_base;  // cycle of dt_log[0]
+  int64_t dt_log_len;
 };
 
 // Number of workgroups (= partial minima) of a sweep / of the CFL pass.
@@ -138,8 +141,7 @@ int LaunchMiddleState(int dim, hipStream_t s, const double* L, const double* R, 
 // with the first split step's CFL check.
 int LaunchFinalizeCfl(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
                       int dim, double* out, Ctl* ctl, int first_dir, int strang);
-int LaunchStepEnd(hipStream_t s, Ctl* ctl, double* dt_log, int64_t dt_log_base,
-                  int dt_log_len);
+int LaunchStepEnd(hipStream_t s, Ctl* ctl);
 int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang);
 int LaunchCheckValue(hipStream_t s, Ctl* ctl, const double* value);
 int LaunchKeysToDt(hipStream_t s, unsigned long long* keys, int n, double* out);

@@ -17,6 +17,7 @@
 // with the reference's operation order (the divisions by small volume
 // fractions in particular).
 #include "fvs_internal.hpp"
+#include "fvs_lanes.hpp"
 #include "fvs_physics.hpp"
 #include "fvs_riemann.hpp"
 #include "fvs_views.hpp"
@@ -473,6 +474,10 @@ __global__ __launch_bounds__(kBlock) void RegularFluxKernel(DevView fl, DevView 
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) *fl.at(f, k) = out[k];
+}
+
+__device__ __forceinline__ unsigned long long KeyMin(unsigned long long a, unsigned long long b) {
+  return b < a ? b : a;
 }
 
 __device__ __forceinline__ unsigned long long WaveMinKey(unsigned long long k) {
@@ -1258,50 +1263,6 @@ __global__ __launch_bounds__(kBlock) void LevelDtKernel(LevelState st, LevelGeom
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
-// Regular (masked MUSCL-Hancock + HLLEM) flux of every face 0..n of DIR and
-// its masked CFL value (input state of the sweep).
-template <int DIR, int LIM, bool LARR>
-__global__ __launch_bounds__(kBlock) void LevelFluxKernel(LevelState st, LevelGeom g, int64_t nx,
-                                                          int64_t ny, double dx,
-                                                          const double* dt_ptr, Gas<double> gas,
-                                                          double* F, unsigned long long* partials,
-                                                          const int* halt) {
-  if (*halt) return;
-  const int64_t fx = nx + (DIR == 0), fy = ny + (DIR == 1);
-  const int64_t nf = fx * fy;
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  unsigned long long key = 0x7ff0000000000000ull;
-  if (q < nf) {
-    const int64_t i = q % fx, j = q / fx;
-    const double beta = g.beta_at(DIR, i, j);
-    double out[4] = {0.0, 0.0, 0.0, 0.0};
-    double dt = DBL_MAX;
-    if (beta > 0.0) {
-      double u[4][6];
-      double a[4];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int64_t c = g.cell(i + (DIR == 0) * (s - 2), j + (DIR == 1) * (s - 2));
-        CellComplete(st, g, c, gas, u[s]);
-        a[s] = g.alpha[c];
-      }
-      dt = HllemStableDt<2, DIR>(u[1], u[2], dx, gas);
-      const bool mL = a[0] > 0.0 && a[1] > 0.0 && a[2] > 0.0;
-      const bool mR = a[1] > 0.0 && a[2] > 0.0 && a[3] > 0.0;
-      const double lh = DivRn(0.5 * (*dt_ptr), dx);
-      double L[6], R[6], dummy[6];
-      MaskedRecon<2, DIR, LIM>(u[0], u[1], u[2], mL, lh, gas, dummy, L);
-      MaskedRecon<2, DIR, LIM>(u[1], u[2], u[3], mR, lh, gas, R, dummy);
-      HllemFlux<2, DIR, LARR, false>(out, L, R, gas);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) F[k * nf + q] = out[k];
-    key = DtKey(dt);
-  }
-  const unsigned long long m = BlockMinKey(key);
-  if (threadIdx.x == 0) partials[blockIdx.x] = m;
-}
-
 // Reference states of the cut cells (list) from the step-start state.
 __global__ __launch_bounds__(kBlock) void LevelRefsKernel(LevelState st, LevelGeom g,
                                                           const int32_t* cut, int ncut,
@@ -1368,30 +1329,18 @@ __device__ __forceinline__ FaceFrac Fractions(double bl, double bm, double br) {
   return f;
 }
 
-// h-box update (TimeIntegrator, or TimeIntegrator2 when `second`) of every
-// interior cell; output buffer `out` (interior only).
+// h-box update (TimeIntegrator, or TimeIntegrator2 when `second`) of the
+// uncovered interior cell (i, j) from the regular fluxes Fl / Fr of its two
+// faces in DIR; the boundary fluxes fb are read only where a shielded
+// fraction or unequal face fractions need them.  Returns the masked result
+// (the zero state unless the density is positive).
 template <int DIR>
-__global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, LevelState out,
-                                                            LevelGeom g, int64_t nx, int64_t ny,
-                                                            const double* F, const double* fb,
-                                                            const double* dt_ptr, double dx,
-                                                            int second, const int* halt) {
-  if (*halt) return;
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (q >= nx * ny) return;
-  const int64_t i = q % nx, j = q / nx;
+__device__ __forceinline__ void KbnUpdateCell(const LevelState& in, const LevelGeom& g,
+                                              const double* fb, int64_t i, int64_t j,
+                                              const double* Fl, const double* Fr,
+                                              double dt_over_dx, int second, double* res) {
   const int64_t c = g.cell(i, j);
   const double alpha = g.alpha[c];
-  if (alpha == 0.0) {
-    // covered cells: zero after the masked reconstruction (both integrators)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) out.u[k][c] = 0.0;
-    return;
-  }
-  const double dt_over_dx = DivRn(*dt_ptr, dx);
-  const int64_t fx = nx + (DIR == 0), fy = ny + (DIR == 1), nf = fx * fy;
-  const int64_t fL = j * fx + i;                 // face i (left) of cell (i, j)
-  const int64_t fR = fL + (DIR == 0 ? 1 : fx);   // face i + 1 (right)
   const int di = DIR == 0, dj = DIR == 1;
   const double bLL = g.beta_at(DIR, i - di, j - dj);
   const double bL = g.beta_at(DIR, i, j);
@@ -1410,20 +1359,17 @@ __global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, Level
   const double dL_r = affR ? Clamp01(0.5 - xc[c]) : 0.5;
   const double dR_r = affR ? Clamp01(0.5 + xc[cr]) : 0.5;
   const double center = xc[c];
-  double res[4];
   for (int k = 0; k < 4; ++k) {
-    const double Fl = F[k * nf + fL];
-    const double Fr = F[k * nf + fR];
     const double fbl = affL ? fb[k * plane + cl] : 0.0;
     const double fbc = (affL || affR || bL != bR) ? fb[k * plane + c] : 0.0;
     const double fbr = affR ? fb[k * plane + cr] : 0.0;
     // stabilised fluxes and the shielded ones of both faces
-    const double sL_l = dL_l * Fl + (1.0 - dL_l) * fbl;
-    const double sR_l = dR_l * Fl + (1.0 - dR_l) * fbc;
-    const double Fs_l = fl.us_rel * Fl + fl.sl_rel * sL_l + fl.sr_rel * sR_l;
-    const double sL_r = dL_r * Fr + (1.0 - dL_r) * fbc;
-    const double sR_r = dR_r * Fr + (1.0 - dR_r) * fbr;
-    const double Fs_r = fr.us_rel * Fr + fr.sl_rel * sL_r + fr.sr_rel * sR_r;
+    const double sL_l = dL_l * Fl[k] + (1.0 - dL_l) * fbl;
+    const double sR_l = dR_l * Fl[k] + (1.0 - dR_l) * fbc;
+    const double Fs_l = fl.us_rel * Fl[k] + fl.sl_rel * sL_l + fl.sr_rel * sR_l;
+    const double sL_r = dL_r * Fr[k] + (1.0 - dL_r) * fbc;
+    const double sR_r = dR_r * Fr[k] + (1.0 - dR_r) * fbr;
+    const double Fs_r = fr.us_rel * Fr[k] + fr.sl_rel * sL_r + fr.sr_rel * sR_r;
     const double prev = in.u[k][c];
     double next;
     if (second) {
@@ -1433,12 +1379,12 @@ __global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, Level
       next = prev + dt_over_dx * (Fs_l - Fs_r);
     } else if (bL > bR) {
       const double dLg = Clamp01(DivRn(fl.sr * (0.5 + center), alpha));
-      next = prev + dt_over_dx * DivRn(fl.us, alpha) * (Fl - Fs_r) +
-             dt_over_dx * DivRn(fl.sl, alpha) * (sL_l - Fs_r) + dt_over_dx * dLg * (Fl - fbc);
+      next = prev + dt_over_dx * DivRn(fl.us, alpha) * (Fl[k] - Fs_r) +
+             dt_over_dx * DivRn(fl.sl, alpha) * (sL_l - Fs_r) + dt_over_dx * dLg * (Fl[k] - fbc);
     } else {
       const double dRg = Clamp01(DivRn(fr.sl * (0.5 - center), alpha));
-      next = prev + dt_over_dx * DivRn(fr.us, alpha) * (Fs_l - Fr) +
-             dt_over_dx * DivRn(fr.sr, alpha) * (Fs_l - sR_r) + dt_over_dx * dRg * (fbc - Fr);
+      next = prev + dt_over_dx * DivRn(fr.us, alpha) * (Fs_l - Fr[k]) +
+             dt_over_dx * DivRn(fr.sr, alpha) * (Fs_l - sR_r) + dt_over_dx * dRg * (fbc - Fr[k]);
     }
     res[k] = next;
   }
@@ -1446,7 +1392,155 @@ __global__ __launch_bounds__(kBlock) void LevelUpdateKernel(LevelState in, Level
   // the zero state
   const bool keep = res[0] > 0.0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) out.u[k][c] = keep ? res[k] : 0.0;
+  for (int k = 0; k < 4; ++k) res[k] = keep ? res[k] : 0.0;
+}
+
+// One fused cut-cell split step in X (the regular masked MUSCL-Hancock +
+// HLLEM fluxes, their masked CFL values and the h-box update of
+// LevelFluxKernel + LevelUpdateKernel's former two passes): a wave covers 64
+// consecutive cells of one row, lanes 2..61 own theirs; each lane completes
+// and reconstructs its cell once, neighbours come from DPP lane shifts, the
+// flux of a lane's left face is shifted to its left neighbour as that one's
+// right-face flux.  Ghost columns (two per side) are filled beforehand.
+constexpr int kKbnOwned = 60;
+
+template <int LIM, bool LARR>
+__global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelState out,
+                                                          LevelGeom g, int64_t nx, int64_t ny,
+                                                          double dx, const double* dt_ptr,
+                                                          Gas<double> gas, const double* fb,
+                                                          int second,
+                                                          unsigned long long* partials,
+                                                          const int* halt) {
+  if (*halt) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t nseg = (nx + kKbnOwned - 1) / kKbnOwned;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  const bool wave_ok = wave < nseg * ny;
+  const int64_t w = wave_ok ? wave : 0;
+  const int64_t j = w / nseg;
+  const int64_t i = (w % nseg) * kKbnOwned + lane - 2;  // -2 .. nx + 61
+  const int64_t ic = i < nx + 1 ? i : nx + 1;            // stay inside the ghost ring
+  const int64_t c = g.cell(ic, j);
+  double q[6];
+  CellComplete(in, g, c, gas, q);
+  const double alpha = g.alpha[c];
+  double qm[6], qp[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    qm[k] = ShiftUp(q[k]);
+    qp[k] = ShiftDown(q[k]);
+  }
+  // Every lane must take part in a DPP shift (an inactive source lane yields
+  // the reader's own value): shift first, combine without short-circuit.
+  const double alpha_m = ShiftUp(alpha), alpha_p = ShiftDown(alpha);
+  const bool mask = (alpha_m > 0.0) & (alpha > 0.0) & (alpha_p > 0.0);
+  const double dt = *dt_ptr;
+  const double lh = DivRn(0.5 * dt, dx);
+  double recL[6], recR[6];
+  MaskedRecon<2, 0, LIM>(qm, q, qp, mask, lh, gas, recL, recR);
+  double recRm[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) recRm[k] = ShiftUp(recR[k]);
+  // left face of the lane's cell: faces 0 .. nx, lanes 2 .. 62
+  const bool face_lane = wave_ok && lane >= 2 && lane <= 2 + kKbnOwned && i >= 0 && i <= nx;
+  const double beta = g.beta_at(0, ic, j);
+  double F[4] = {0.0, 0.0, 0.0, 0.0};
+  unsigned long long key = 0x7ff0000000000000ull;
+  if (face_lane && beta > 0.0) {
+    HllemFlux<2, 0, LARR, false>(F, recRm, recL, gas);
+    key = DtKey(HllemStableDt<2, 0>(qm, q, dx, gas));
+  }
+  double Fr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) Fr[k] = ShiftDown(F[k]);
+  if (wave_ok && lane >= 2 && lane < 2 + kKbnOwned && i >= 0 && i < nx) {
+    double res[4] = {0.0, 0.0, 0.0, 0.0};
+    if (alpha != 0.0) KbnUpdateCell<0>(in, g, fb, i, j, F, Fr, DivRn(dt, dx), second, res);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
+  }
+  const unsigned long long m = BlockMinKey(key);
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+// The same step in Y: one thread per interior column marching a segment of
+// rows with a register window of three completed cells, the previous
+// cell's right half-step state and the previous face's flux.
+constexpr int kKbnSeg = 16;
+
+template <int LIM, bool LARR>
+__global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelState out,
+                                                          LevelGeom g, int64_t nx, int64_t ny,
+                                                          double dx, const double* dt_ptr,
+                                                          Gas<double> gas, const double* fb,
+                                                          int second,
+                                                          unsigned long long* partials,
+                                                          const int* halt) {
+  if (*halt) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const bool col_ok = i < nx;
+  const int64_t ic = col_ok ? i : nx - 1;
+  const int64_t j0 = static_cast<int64_t>(blockIdx.y) * kKbnSeg;
+  const int64_t j1 = j0 + kKbnSeg < ny ? j0 + kKbnSeg : ny;
+  const double dt = *dt_ptr;
+  const double lh = DivRn(0.5 * dt, dx);
+  const double dt_over_dx = DivRn(dt, dx);
+  double qm[6], q0[6], qp[6];
+  double am, a0, ap;
+  auto fetch = [&](int64_t jj, double* q, double& a) {
+    const int64_t c = g.cell(ic, jj);
+    CellComplete(in, g, c, gas, q);
+    a = g.alpha[c];
+  };
+  fetch(j0 - 2, qm, am);
+  fetch(j0 - 1, q0, a0);
+  fetch(j0, qp, ap);
+  double recRm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double Fm[4] = {0.0, 0.0, 0.0, 0.0};
+  unsigned long long key = 0x7ff0000000000000ull;
+  for (int64_t t = j0 - 1; t <= j1; ++t) {
+    // q0 is cell t, qm cell t - 1, qp cell t + 1
+    double recL[6], recR[6];
+    MaskedRecon<2, 1, LIM>(qm, q0, qp, am > 0.0 && a0 > 0.0 && ap > 0.0, lh, gas, recL, recR);
+    if (t >= j0) {
+      // face t between cells t - 1 and t
+      double F[4] = {0.0, 0.0, 0.0, 0.0};
+      if (g.beta_at(1, ic, t) > 0.0) {
+        HllemFlux<2, 1, LARR, false>(F, recRm, recL, gas);
+        if (col_ok) key = KeyMin(key, DtKey(HllemStableDt<2, 1>(qm, q0, dx, gas)));
+      }
+      if (t - 1 >= j0 && col_ok) {
+        const int64_t c = g.cell(i, t - 1);
+        double res[4] = {0.0, 0.0, 0.0, 0.0};
+        if (g.alpha[c] != 0.0) KbnUpdateCell<1>(in, g, fb, i, t - 1, Fm, F, dt_over_dx, second, res);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) Fm[k] = F[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      recRm[k] = recR[k];
+      qm[k] = q0[k];
+      q0[k] = qp[k];
+    }
+    am = a0;
+    a0 = ap;
+    if (t + 2 <= j1 + 1) fetch(t + 2, qp, ap);
+  }
+  const unsigned long long m = BlockMinKey(key);
+  if (threadIdx.x == 0) partials[static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x] = m;
+}
+
+// Workgroups (= partial minima) of a fused cut-cell sweep.
+int64_t KbnSweepBlocks(int dir, int64_t nx, int64_t ny) {
+  if (dir == 0) {
+    const int64_t waves = ((nx + kKbnOwned - 1) / kKbnOwned) * ny;
+    return (waves + kBlock / 64 - 1) / (kBlock / 64);
+  }
+  return ((nx + kBlock - 1) / kBlock) * ((ny + kKbnSeg - 1) / kKbnSeg);
 }
 
 // Masked reconstruction of the interior at set_state: covered cells and
@@ -1633,7 +1727,12 @@ int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
       if (hipMemsetAsync(L->buf[b][k], 0, ncg * sizeof(double), s) != hipSuccess)
         return fail(FVS_HIP_ERROR);
   L->nblocks = (std::max((nx + 1) * ny, nx * (ny + 1)) + kBlock - 1) / kBlock;
-  if (hipMalloc(&L->partials, 2 * L->nblocks * sizeof(unsigned long long)) != hipSuccess)
+  const int64_t npart = std::max({2 * L->nblocks, KbnSweepBlocks(0, nx, ny), KbnSweepBlocks(1, nx, ny)});
+  if (npart > 0x7fffffff || (ny + kKbnSeg - 1) / kKbnSeg > 65535) {
+    SetError("cut-cell patch too large for the sweep grid");
+    return fail(FVS_BAD_ARGUMENT);
+  }
+  if (hipMalloc(&L->partials, npart * sizeof(unsigned long long)) != hipSuccess)
     return fail(FVS_HIP_ERROR);
   Ctl c{};
   c.cfl = patch->cfl;
@@ -1734,39 +1833,33 @@ static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
     FVS_LAUNCH_CHECK();
   }
   const double dx = L->patch.dx[dir];
-  const int64_t nf = (L->nx + (dir == 0)) * (L->ny + (dir == 1));
-  const unsigned nb = Blocks(nf);
   const bool larr = h->desc.base == FVS_BASE_HLLEM_LARROUTUROU;
   const double* dtp = &L->ctl->split_dt;
-#define FVS_LF(DR, LM, LA)                                                                    \
-  hipLaunchKernelGGL((LevelFluxKernel<DR, LM, LA>), dim3(nb), dim3(kBlock), 0, s, in, L->lg,  \
-                     L->nx, L->ny, dx, dtp, gas, L->F, L->partials, halt)
-#define FVS_LF_LIM(DR, LA)                                                     \
-  switch (h->desc.limiter) {                                                   \
-  case kMinMod: FVS_LF(DR, kMinMod, LA); break;                                \
-  case kVanLeer: FVS_LF(DR, kVanLeer, LA); break;                              \
-  case kNoLimiter: FVS_LF(DR, kNoLimiter, LA); break;                          \
-  default: FVS_LF(DR, kNoGradient, LA); break;                                 \
+  const int64_t nb = KbnSweepBlocks(dir, L->nx, L->ny);
+  const dim3 grid = dir == 0 ? dim3(static_cast<unsigned>(nb))
+                             : dim3(static_cast<unsigned>((L->nx + kBlock - 1) / kBlock),
+                                    static_cast<unsigned>((L->ny + kKbnSeg - 1) / kKbnSeg));
+#define FVS_KS(K, LM, LA)                                                                 \
+  hipLaunchKernelGGL((K<LM, LA>), grid, dim3(kBlock), 0, s, in, out, L->lg, L->nx, L->ny, \
+                     dx, dtp, gas, L->fb, L->integrator, L->partials, halt)
+#define FVS_KS_LIM(K, LA)                                  \
+  switch (h->desc.limiter) {                               \
+  case kMinMod: FVS_KS(K, kMinMod, LA); break;             \
+  case kVanLeer: FVS_KS(K, kVanLeer, LA); break;           \
+  case kNoLimiter: FVS_KS(K, kNoLimiter, LA); break;       \
+  default: FVS_KS(K, kNoGradient, LA); break;              \
   }
   if (dir == 0) {
-    if (larr) { FVS_LF_LIM(0, true) } else { FVS_LF_LIM(0, false) }
+    if (larr) { FVS_KS_LIM(KbnSweepXKernel, true) } else { FVS_KS_LIM(KbnSweepXKernel, false) }
   } else {
-    if (larr) { FVS_LF_LIM(1, true) } else { FVS_LF_LIM(1, false) }
+    if (larr) { FVS_KS_LIM(KbnSweepYKernel, true) } else { FVS_KS_LIM(KbnSweepYKernel, false) }
   }
-#undef FVS_LF_LIM
-#undef FVS_LF
+#undef FVS_KS_LIM
+#undef FVS_KS
   FVS_LAUNCH_CHECK();
   if (check) {
     if (int rc = LaunchFinalizeCheck(s, L->partials, nb, nullptr, L->ctl)) return rc;
   }
-  const unsigned ub = Blocks(L->nx * L->ny);
-  if (dir == 0)
-    hipLaunchKernelGGL(LevelUpdateKernel<0>, dim3(ub), dim3(kBlock), 0, s, in, out, L->lg, L->nx,
-                       L->ny, L->F, L->fb, dtp, dx, L->integrator, halt);
-  else
-    hipLaunchKernelGGL(LevelUpdateKernel<1>, dim3(ub), dim3(kBlock), 0, s, in, out, L->lg, L->nx,
-                       L->ny, L->F, L->fb, dtp, dx, L->integrator, halt);
-  FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
 
@@ -1787,15 +1880,18 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
   FVS_HIP_CHECK(hipStreamSynchronize(s));
+  double* dlog = nullptr;
+  if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
   c.final_time = final_time;
   c.cfl = L->patch.cfl;
   c.halt = 0;
+  c.dt_log = dlog;
+  c.dt_log_base = c.cycle;
+  c.dt_log_len = max_steps;
   FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
   const int64_t cycle0 = c.cycle;
   int64_t cycle = c.cycle;
   int nfail = 0;
-  double* dlog = nullptr;
-  if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
   int rc = FVS_OK;
   std::vector<int> start_buf;
   const int* halt = &L->ctl->halt;
@@ -1845,7 +1941,7 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
       }
       if (rc) break;
       cur = src;
-      rc = LaunchStepEnd(s, L->ctl, dlog, cycle0, max_steps);
+      rc = LaunchStepEnd(s, L->ctl);
     }
     if (rc) break;
     FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));

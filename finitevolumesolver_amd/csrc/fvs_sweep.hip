@@ -29,6 +29,7 @@
 // with multiply-add contraction, reached through LaunchSweepFast and
 // LaunchPatchCflFast.
 #include "fvs_internal.hpp"
+#include "fvs_lanes.hpp"
 #include "fvs_physics.hpp"
 
 namespace fvs {
@@ -54,30 +55,6 @@ __device__ __forceinline__ unsigned long long WaveMinKey(unsigned long long k) {
   return k;
 }
 
-// Whole-wave shifts by one lane (DPP wave_shr:1 / wave_shl:1): lane i gets
-// lane i - 1 (ShiftUp) or lane i + 1 (ShiftDown); the end lane keeps its own
-// value, as __shfl_up / __shfl_down(x, 1) do.  A DPP move is one VALU
-// instruction with no LDS round trip.
-__device__ __forceinline__ int DppUp(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, 0x138, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int DppDown(int x) {
-  return __builtin_amdgcn_update_dpp(x, x, 0x130, 0xf, 0xf, false);
-}
-template <bool UP> __device__ __forceinline__ double Shift(double x) {
-  const int64_t v = __double_as_longlong(x);
-  const int lo = static_cast<int>(v);
-  const int hi = static_cast<int>(v >> 32);
-  const int rlo = UP ? DppUp(lo) : DppDown(lo);
-  const int rhi = UP ? DppUp(hi) : DppDown(hi);
-  return __longlong_as_double((static_cast<int64_t>(rhi) << 32) | static_cast<uint32_t>(rlo));
-}
-template <bool UP> __device__ __forceinline__ float Shift(float x) {
-  const int v = __float_as_int(x);
-  return __int_as_float(UP ? DppUp(v) : DppDown(v));
-}
-template <class T> __device__ __forceinline__ T ShiftUp(T x) { return Shift<true>(x); }
-template <class T> __device__ __forceinline__ T ShiftDown(T x) { return Shift<false>(x); }
 
 // Loads and stores as a wave-uniform base pointer plus a 32-bit byte offset:
 // the base lives in scalar registers, the offset in one vector register shared
@@ -321,7 +298,10 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   };
 
   unsigned long long key = kInfKey;
-  int64_t c = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  // The chunk index is wave-uniform (a scalar register), so the chunk loop is
+  // a uniform branch and its DPP shifts always run with the whole wave.
+  int64_t c = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) +
+              __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   place(c < nchunks ? c : 0);
   Slot cur = locate(c < nchunks ? c : 0);
   double u[NC];
@@ -837,11 +817,10 @@ __global__ void CheckValueKernel(Ctl* ctl, const double* value) {
   }
 }
 
-__global__ void StepEndKernel(Ctl* ctl, double* dt_log, int64_t dt_log_base,
-                              int dt_log_len) {
+__global__ void StepEndKernel(Ctl* ctl) {
   if (ctl->halt) return;
-  const int64_t slot = ctl->cycle - dt_log_base;
-  if (dt_log && slot >= 0 && slot < dt_log_len) dt_log[slot] = ctl->dt;
+  const int64_t slot = ctl->cycle - ctl->dt_log_base;
+  if (ctl->dt_log && slot >= 0 && slot < ctl->dt_log_len) ctl->dt_log[slot] = ctl->dt;
   ctl->t = ctl->t + ctl->dt;
   ctl->cycle += 1;
   ctl->have_failure = 0;
@@ -1051,10 +1030,8 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
   return FVS_OK;
 }
 
-int LaunchStepEnd(hipStream_t s, Ctl* ctl, double* dt_log, int64_t dt_log_base,
-                  int dt_log_len) {
-  hipLaunchKernelGGL(StepEndKernel, dim3(1), dim3(1), 0, s, ctl, dt_log, dt_log_base,
-                     dt_log_len);
+int LaunchStepEnd(hipStream_t s, Ctl* ctl) {
+  hipLaunchKernelGGL(StepEndKernel, dim3(1), dim3(1), 0, s, ctl);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }

@@ -349,6 +349,11 @@ class SinglePatchLevel:
         """Brackets every kernel launch of advance() with HIP events."""
         L.check(self._lib.fvs_level_set_timing(self.h, 1 if on else 0))
 
+    def set_graphs(self, on: bool = True) -> None:
+        """Replay captured HIP graphs of the time steps (default) or launch
+        every kernel eagerly."""
+        L.check(self._lib.fvs_level_set_graphs(self.h, 1 if on else 0))
+
     def kernel_times(self) -> dict:
         """{class: (summed ms, launches)} since the last call (see set_timing)."""
         ms = (C.c_double * L.LEVEL_TIMERS)()

@@ -274,6 +274,12 @@ enum fvs_level_timer {
 int fvs_level_set_timing(fvs_level* level, int on);
 int fvs_level_kernel_times(fvs_level* level, double* ms, int64_t* count);
 
+/* Capture each distinct time step of the level (per start buffer, cycle
+ * parity and attached network) once into a HIP graph and replay it (on by
+ * default; the timing mode launches eagerly).  No counterpart in the
+ * reference, whose steps are host loops; results are identical either way. */
+int fvs_level_set_graphs(fvs_level* level, int on);
+
 /* One fused split sweep (flux + update + completion + CFL check of the
  * input) from cons_in to cons_out (distinct arrays, D+2 components of
  * n[0]*n[1]*n[2] each, x fastest, transmissive boundaries).  Writes the

@@ -224,3 +224,39 @@ def test_level_final_time_and_cfl_failure(cuda):
             assert abs(t - final_time) < 1e-15 and done < 12
         else:
             assert fails > 0
+
+
+@pytest.mark.parametrize("case", ["2d_strang", "3d_godunov", "godunov_base"])
+def test_level_graph_replay_matches_eager(cuda, case):
+    """The level's captured step graphs, replayed over several batches with
+    host synchronisation in between, give the eager launches' states and dt
+    sequence bit for bit."""
+    torch = _torch()
+    from finitevolumesolver_amd.initial_data import circular_riemann_2d, fourier_modes_3d
+    if case == "2d_strang":
+        cons, dx = circular_riemann_2d(48)
+        fm = fvs.FluxMethod(fvs.PerfectGas(dim=2))
+        shape, split = (48, 48), "Strang"
+    elif case == "3d_godunov":
+        cons, dx = fourier_modes_3d(20)
+        fm = fvs.FluxMethod(fvs.PerfectGas(dim=3), precision="fp64_rcp")
+        shape, split = (20, 20, 20), "Godunov"
+    else:
+        cons, dx = circular_riemann_2d(40)
+        fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="Godunov", reconstruction="None")
+        shape, split = (40, 40), "Godunov"
+    levels = []
+    for graphs in (True, False):
+        lvl = fvs.SinglePatchLevel(fm, shape, dx, splitting=split, cfl=0.8)
+        lvl.set_graphs(graphs)
+        lvl.set_state(torch.from_numpy(cons).to(cuda))
+        levels.append(lvl)
+    for batch in range(4):
+        outs = []
+        for lvl in levels:
+            done, fails, dts = lvl.advance(5, batch=5, return_dts=True)
+            assert done == 5 and fails == 0
+            outs.append((lvl.get_state().cpu().numpy(), dts))
+        assert outs[0][1] == outs[1][1], (case, batch)
+        assert np.array_equal(outs[0][0], outs[1][0]), (case, batch)
+    print(f"{case}: 4 batches of 5 graph-replayed steps equal the eager ones", flush=True)
