@@ -539,7 +539,7 @@ int fvs_transmissive_fill(fvs_handle* h, const fvs_view* states, const int64_t d
 
 
 static int FillSweepArgs(fvs_handle* h, const double* const* in, double* const* out,
-                         const int64_t n[3], double dx, SweepArgs* a) {
+                         const int64_t n[3], double dx, int dir, SweepArgs* a) {
   const int D = h->desc.dim;
   std::memset(a, 0, sizeof *a);
   for (int c = 0; c < D + 2; ++c) {
@@ -563,7 +563,7 @@ static int FillSweepArgs(fvs_handle* h, const double* const* in, double* const* 
   a->limiter = h->desc.limiter;
   a->muscl = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK;
   a->quirk3d = h->desc.quirks;
-  a->seg = 32;
+  a->seg = PencilSeg(h->desc.dim, dir, a->n);
   return FVS_OK;
 }
 
@@ -576,7 +576,7 @@ int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* 
   }
   if (int rc = CheckDir(h, dir)) return rc;
   SweepArgs a;
-  if (int rc = FillSweepArgs(h, cons_in, cons_out, n, dx, &a)) return rc;
+  if (int rc = FillSweepArgs(h, cons_in, cons_out, n, dx, dir, &a)) return rc;
   for (int c = 0; c < h->desc.dim + 2; ++c) {
     if (cons_in[c] == cons_out[c]) {
       SetError("fused sweep needs distinct input and output arrays");
@@ -665,7 +665,7 @@ int fvs_godunov_sweep(fvs_handle* h, const double* const* cons_in, double* const
       return FVS_BAD_ARGUMENT;
     }
   }
-  const int64_t nb = GodunovSweepBlocks(D, dir, nn, 32);
+  const int64_t nb = GodunovSweepBlocks(D, dir, nn, PencilSeg(D, dir, nn));
   if (int rc = EnsurePartials(h, 5 * nb)) return rc;
   if (int rc = LaunchGodunovSweep(h->desc, h->stream, dir, cons_in, cons_out, nn, dx, nullptr, dt,
                                   guess_device, h->partials, nullptr, cons_out != nullptr))
@@ -848,10 +848,10 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
     return FVS_HIP_ERROR;
   }
   int64_t np = D * CflBlocks(D, n);
-  for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, SweepBlocks(D, d, n, 32));
+  for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, SweepBlocks(D, d, n, PencilSeg(D, d, n)));
   if (godunov) {
     np = std::max<int64_t>(np, 5 * D * GodunovCflBlocks(D, n));
-    for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, 5 * GodunovSweepBlocks(D, d, n, 32));
+    for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, 5 * GodunovSweepBlocks(D, d, n, PencilSeg(D, d, n)));
   }
   if (hipMalloc(&L->partials, np * sizeof(unsigned long long)) != hipSuccess) {
     (void)hipFree(L->block);
@@ -1032,7 +1032,7 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
     const double* inp[5];
     for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
     if (godunov) {
-      const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, 32);
+      const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, PencilSeg(D, seq[w], L->n));
       rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] {
         return LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
                                   &L->ctl->split_dt, 0.0, nullptr, L->partials,
@@ -1047,7 +1047,7 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
       dst = (dst == b1) ? b2 : b1;
       continue;
     }
-    rc = FillSweepArgs(h, inp, outp, L->n, L->patch.dx[seq[w]], &a);
+    rc = FillSweepArgs(h, inp, outp, L->n, L->patch.dx[seq[w]], seq[w], &a);
     if (rc) break;
     a.dt_ptr = &L->ctl->split_dt;
     a.halt = &L->ctl->halt;

@@ -732,7 +732,7 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
   if (net) MlpOperands(net, &a.tiles, &a.bias);
   a.partials = partials;
   a.halt = halt;
-  a.seg = 32;
+  a.seg = PencilSeg(D, dir, a.n);
   a.update = update;
   const bool mixed = desc.precision == FVS_FP32_FLUX;
   const bool learned = a.tiles != nullptr;

@@ -547,8 +547,10 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
   const int64_t xs = active ? x : nx - 1;
   const int64_t nm = ny;
   const int64_t ms = nx;
-  const int64_t m0 = static_cast<int64_t>(blockIdx.y) * kCflSeg;
-  const int64_t m1 = (m0 + kCflSeg < nm) ? m0 + kCflSeg : nm;
+  // rows per segment follow the launch (fewer for small patches)
+  const int64_t seg = (nm + gridDim.y - 1) / gridDim.y;
+  const int64_t m0 = static_cast<int64_t>(blockIdx.y) * seg;
+  const int64_t m1 = (m0 + seg < nm) ? m0 + seg : nm;
   const int64_t rowoff = 0;
   const double dxs[2] = {dx0, dx1};
 
@@ -968,10 +970,29 @@ int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
   return bx * nseg;
 }
 
+// Row segments of the 2D CFL pass: 32 rows, fewer when the patch is too
+// small to give the device about 1,024 workgroups (down to 4 rows).
+int64_t Cfl2Segments(int64_t nx, int64_t ny) {
+  const int64_t tiles = (nx + kTX - 1) / kTX;
+  int64_t seg = (ny * tiles + 1023) / 1024;
+  seg = seg < 4 ? 4 : (seg > kCflSeg ? kCflSeg : seg);
+  return (ny + seg - 1) / seg;
+}
+
+int PencilSeg(int dim, int dir, const int64_t n[3]) {
+  if (dir == 0) return 32;
+  const int64_t npencil = dir == 1 ? n[0] * (dim == 3 ? n[2] : 1) : n[0] * n[1];
+  const int64_t bx = (npencil + kBlock - 1) / kBlock;
+  const int64_t nseg = (1024 + bx - 1) / bx;
+  int64_t seg = (n[dir] + nseg - 1) / nseg;
+  seg = seg < 4 ? 4 : (seg > 32 ? 32 : seg);
+  return static_cast<int>(seg);
+}
+
 int64_t CflBlocks(int dim, const int64_t n[3]) {
   if (dim == 3) return Cfl3GridX(n[0], n[1]) * ((n[2] + kCflSeg - 1) / kCflSeg);
   const int64_t tiles = (n[0] + kTX - 1) / kTX;
-  return tiles * ((n[1] + kCflSeg - 1) / kCflSeg);
+  return tiles * Cfl2Segments(n[0], n[1]);
 }
 
 int LaunchSweep(const fvs_method_desc& desc, hipStream_t s, int dir, const SweepArgs& a) {
@@ -1006,7 +1027,7 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
   const int D = desc.dim;
   const int64_t gx = D == 3 ? Cfl3GridX(n[0], n[1]) : (n[0] + kTX - 1) / kTX;
   const int64_t nm = D == 3 ? n[2] : n[1];
-  const int64_t gy = (nm + kCflSeg - 1) / kCflSeg;
+  const int64_t gy = D == 3 ? (nm + kCflSeg - 1) / kCflSeg : Cfl2Segments(n[0], n[1]);
   if (gx <= 0 || gx > 0x7fffffff || gy <= 0 || gy > 65535) return FVS_BAD_ARGUMENT;
   const double gm1inv = 1.0 / (desc.gamma - 1.0);
   const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
