@@ -1407,8 +1407,8 @@ __device__ __forceinline__ void KbnUpdateCell(const LevelState& in, const LevelG
 }
 
 // One fused cut-cell split step in X (the regular masked MUSCL-Hancock +
-// HLLEM fluxes, their masked CFL values and the h-box update of
-// LevelFluxKernel + LevelUpdateKernel's former two passes): a wave covers 64
+// HLLEM fluxes, their masked CFL values and the h-box update in one pass
+// over the level, with no face-flux arrays in memory): a wave covers 64
 // consecutive cells of one row, lanes 2..61 own theirs; each lane completes
 // and reconstructs its cell once, neighbours come from DPP lane shifts, the
 // flux of a lane's left face is shifted to its left neighbour as that one's
