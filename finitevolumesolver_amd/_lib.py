@@ -174,6 +174,13 @@ SIGNATURES = {
     "fvs_batch_cfl": (_P, [_P]),
     "fvs_batch_read_cfl": (_I, [_P, _P]),
     "fvs_batch_set_shape": (_I, [_P, _I, _I]),
+    "fvs_amr_create": (_I, [_P, _I, _P, _P, _P, _I, C.POINTER(_P)]),
+    "fvs_amr_destroy": (_I, [_P]),
+    "fvs_amr_fill_ghosts": (_I, [_P, _I, _I]),
+    "fvs_amr_average_down": (_I, [_P, _I]),
+    "fvs_amr_advance": (_I, [_P, _D]),
+    "fvs_amr_level_cfl": (_I, [_P, _P]),
+    "fvs_amr_stable_dt": (_I, [_P, _PD]),
 }
 
 _lib = None

@@ -1,0 +1,118 @@
+"""2D AMR hierarchy over batched levels (fvs_amr_* of include/fvs.h).
+
+Mirrors the reference's subcycled hierarchy step (SubcycleFineFirstSolver::
+AdvanceHierarchy, include/fub/solver/SubcycleFineFirstSolver.hpp:114-175,
+driven by RunSimulation, include/fub/RunSimulation.hpp:86-191) with the
+AMReX operations the IntegratorContext calls for it
+(src/AMReX/IntegratorContext.cpp:380-533): coarse-fine ghost fill with
+piecewise-constant interpolation, flux-register reflux and average_down.
+Every level is a PatchBatch; the ghost, register and averaging link lists
+are built once in the library and every operation is a launch on the
+handle's stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .batch import GHOSTS, PatchBatch
+from .patch import FluxMethod
+
+
+class AmrHierarchy:
+    def __init__(self, flux_method: FluxMethod, domain, layouts, dx0: float, ratio: int = 2):
+        """domain: (nx, ny) cells of level 0; layouts: per level a list of
+        patches (i0, j0, nx, ny) in that level's cell indices; level l has
+        cell width dx0 / ratio**l."""
+        self._lib = L.load()
+        self.flux_method = flux_method
+        self.ratio = int(ratio)
+        self.domain = (int(domain[0]), int(domain[1]))
+        self.layouts = [[tuple(int(v) for v in p) for p in lay] for lay in layouts]
+        self.dx = [dx0 / self.ratio ** l for l in range(len(layouts))]
+        self.levels = [PatchBatch(flux_method, [(p[2], p[3]) for p in lay], self.dx[l])
+                       for l, lay in enumerate(self.layouts)]
+        lo = np.ascontiguousarray(
+            np.array([(p[0], p[1]) for lay in self.layouts for p in lay], dtype=np.int32).ravel())
+        dom = np.array(self.domain, dtype=np.int64)
+        handles = (C.c_void_p * len(self.levels))(*[b.b.value for b in self.levels])
+        a = C.c_void_p()
+        L.check(self._lib.fvs_amr_create(flux_method.handle, len(self.levels), handles,
+                                         lo.ctypes.data, dom.ctypes.data, self.ratio, C.byref(a)))
+        self.a = a
+
+    def __del__(self):
+        a = getattr(self, "a", None)
+        if a is not None and a.value:
+            self._lib.fvs_amr_destroy(a)
+            self.a = None
+
+    @property
+    def nlevels(self) -> int:
+        return len(self.levels)
+
+    def cell_centres(self, level: int, p: int):
+        """x, y of the interior cell centres of patch p (domain [0, nx dx0] x
+        [0, ny dx0] shifted by the caller)."""
+        i0, j0, nx, ny = self.layouts[level][p]
+        dx = self.dx[level]
+        x = (i0 + np.arange(nx) + 0.5) * dx
+        y = (j0 + np.arange(ny) + 0.5) * dx
+        return np.meshgrid(x, y)
+
+    def set_state(self, cons_of) -> None:
+        """cons_of(level, p) -> (4, ny, nx) conservative interior of patch p;
+        written to pool 0 of every level (ghosts are filled by the steps)."""
+        for l, b in enumerate(self.levels):
+            pool = b.new_pool()
+            for p in range(len(b.sizes)):
+                v = b.patch_view(pool, p)
+                v[:, GHOSTS:-GHOSTS, GHOSTS:-GHOSTS] = torch.as_tensor(
+                    np.asarray(cons_of(l, p), dtype=np.float64), device=pool.device)
+            b.upload(pool, 0)
+        torch.cuda.synchronize()
+
+    def interiors(self, level: int) -> list:
+        """Host copies (4, ny, nx) of every patch interior of a level (pool 0)."""
+        b = self.levels[level]
+        pool = b.download(0).cpu()
+        return [b.patch_view(pool, p)[:, GHOSTS:-GHOSTS, GHOSTS:-GHOSTS].numpy().copy()
+                for p in range(len(b.sizes))]
+
+    def fill_ghosts(self, level: int, which: int = 0) -> None:
+        L.check(self._lib.fvs_amr_fill_ghosts(self.a, level, which))
+
+    def average_down(self, fine_level: int) -> None:
+        L.check(self._lib.fvs_amr_average_down(self.a, fine_level))
+
+    def stable_dt(self) -> float:
+        out = C.c_double()
+        L.check(self._lib.fvs_amr_stable_dt(self.a, C.byref(out)))
+        return out.value
+
+    def advance(self, dt: float) -> None:
+        L.check(self._lib.fvs_amr_advance(self.a, float(dt)))
+
+    def level_cfl(self) -> np.ndarray:
+        out = np.zeros(self.nlevels)
+        L.check(self._lib.fvs_amr_level_cfl(self.a, out.ctypes.data))
+        return out
+
+    def run(self, steps: int, cfl: float = 0.8) -> list:
+        """RunSimulation-style loop: dt = cfl * stable dt, one hierarchy step,
+        then the per-level check of the sweeps' stable dt against the level's
+        own time step (the reference's TimeStepTooLarge).  Returns the dts."""
+        dts = []
+        for _ in range(steps):
+            dt = cfl * self.stable_dt()
+            self.advance(dt)
+            got = self.level_cfl()
+            for l in range(self.nlevels):
+                if got[l] < dt / self.ratio ** l:
+                    raise RuntimeError(f"level {l}: stable dt {got[l]:.6e} below its time step "
+                                       f"{dt / self.ratio ** l:.6e}")
+            dts.append(dt)
+        return dts

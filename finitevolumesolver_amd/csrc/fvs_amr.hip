@@ -1,0 +1,581 @@
+// AMR hierarchy over batched levels (2D): coarse-fine ghost fill,
+// flux-register reflux, conservative averaging and fine-first subcycling.
+//
+// The reference advances a hierarchy with SubcycleFineFirstSolver::
+// AdvanceLevel (include/fub/solver/SubcycleFineFirstSolver.hpp:114-175):
+// reset the coarse-fine register of the next finer level, advance that
+// level `ratio` times with dt / ratio, advance this level
+// (DimensionalSplitLevelIntegrator::AdvanceLevelNonRecursively,
+// include/fub/solver/DimensionalSplitLevelIntegrator.hpp:221-276, with the
+// split order of MakeSplitDirections, include/fub/split_method/
+// SplittingMethod.hpp:115-126), reflux and average the finer level down.
+// Ghost layers come from IntegratorContext::FillGhostLayerTwoLevels
+// (src/AMReX/IntegratorContext.cpp:380-418: amrex::FillPatchTwoLevels with
+// piecewise-constant interpolation) and the register from
+// AccumulateCoarseFineFluxes / ApplyFluxCorrection / CoarsenConservatively
+// (:451-533: amrex::FluxRegister, amrex::average_down).
+//
+// Here every copy, interpolation, register face and averaged cell is
+// resolved once on the host into flat link lists when the hierarchy is
+// created (the analogue of AMReX's cached copy plans); each operation is
+// then one gather-style launch over its list, ordered on the handle's
+// stream with the batched sweeps.  Patches of level l >= 1 must be aligned
+// to the refinement ratio and properly nested: every ghost cell of a fine
+// patch (clamped to the domain) lies in a fine patch or above a coarse
+// patch's interior, and every coarse cell that receives a reflux
+// correction belongs to a coarse patch.
+
+#include "fvs_batch_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+using namespace fvs;
+using batch::kG;
+using batch::PatchInfo;
+using batch::RegFace;
+
+namespace {
+
+// Copy of all four components from one pool cell to another.
+struct GhostLink {
+  int64_t dst, src;
+  int32_t dcs, scs;  // component strides of the destination and source patches
+};
+
+// One coarse cell covered by a fine patch: the mean of its ratio^2 fine cells.
+struct AvgLink {
+  int64_t dst, src;  // coarse cell; fine cell at the block's low corner
+  int32_t dcs, scs, sldx, pad;
+};
+
+__global__ void GatherKernel(const GhostLink* links, int n, double* dst,
+                             const double* src) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const GhostLink L = links[q];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dst[L.dst + k * int64_t(L.dcs)] = src[L.src + k * int64_t(L.scs)];
+}
+
+// amrex::average_down for cell data: c = sum over the fine block (rows
+// outer, x inner, starting from zero), coarse = (1 / ratio^2) * c.
+__global__ void AvgDownKernel(const AvgLink* links, int n, double* coarse, const double* fine,
+                              int ratio, double volfrac) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const AvgLink L = links[q];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double c = 0.0;
+    for (int jj = 0; jj < ratio; ++jj)
+      for (int ii = 0; ii < ratio; ++ii) c = c + fine[L.src + k * int64_t(L.scs) + jj * int64_t(L.sldx) + ii];
+    coarse[L.dst + k * int64_t(L.dcs)] = volfrac * c;
+  }
+}
+
+// Reflux of one (direction, side) group of register entries: the coarse
+// cell beside each coarse face takes -reg / V (cell left of / below the
+// face) or +reg / V (right of / above), V the coarse cell volume.  Within a
+// group no two entries share a coarse cell.
+__global__ void RefluxKernel(const RegFace* faces, int first, int count, int nfaces,
+                             const double* reg, const PatchInfo* patches, double* pool, int dir) {
+  const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q0 >= count) return;
+  const int q = first + q0;
+  const RegFace F = faces[q];
+  const PatchInfo P = patches[F.cp];
+  const int i = F.ci - (F.side == 0 && dir == 0 ? 1 : 0);
+  const int j = F.cj - (F.side == 0 && dir == 1 ? 1 : 0);
+  const int64_t ldx = P.nx + 2 * kG, cs = ldx * (P.ny + 2 * kG);
+  const int64_t at = P.offset + (int64_t(j) + kG) * ldx + (int64_t(i) + kG);
+  const double vol = P.dx * P.dx;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double corr = DivRn(reg[k * int64_t(nfaces) + q], vol);
+    pool[at + k * cs] = F.side == 0 ? pool[at + k * cs] - corr : pool[at + k * cs] + corr;
+  }
+}
+
+__global__ void FillKernel(double* p, int64_t n, double v) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = v;
+}
+
+unsigned Blocks(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+} // namespace
+
+struct fvs_amr {
+  fvs_handle* h = nullptr;
+  int nlevels = 0;
+  int ratio = 2;
+  int64_t n0[2] = {0, 0};
+  std::vector<fvs_batch*> lv;
+  struct Level {
+    GhostLink* self = nullptr;  // sources in the same level
+    int nself = 0;
+    GhostLink* coarse = nullptr;  // sources in the next coarser level (pool 0)
+    int ncoarse = 0;
+    AvgLink* avg = nullptr;  // this level onto the next coarser one
+    int navg = 0;
+    RegFace* reg = nullptr;  // register between this level and the coarser one
+    int nreg = 0;
+    int group[5] = {0, 0, 0, 0, 0};  // entries of (dir 0, side 0), (0, 1), (1, 0), (1, 1)
+    double* regval = nullptr;        // 4 x nreg
+    int cfl_base = 0;                // first CFL slot of this level's sweeps
+  };
+  std::vector<Level> L;
+  double* cfl = nullptr;  // one slot per sweep of a coarse step, then 2 per level
+  int ncfl = 0;
+  int64_t cycle = 0;
+};
+
+namespace {
+
+template <class T>
+int UploadVec(const std::vector<T>& v, T** dst) {
+  *dst = nullptr;
+  if (v.empty()) return FVS_OK;
+  FVS_HIP_CHECK(hipMalloc(dst, v.size() * sizeof(T)));
+  FVS_HIP_CHECK(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FVS_OK;
+}
+
+// Patch index covering each cell of a level (-1: none).
+struct Owner {
+  int64_t nx = 0, ny = 0;
+  std::vector<int32_t> id;
+  int32_t at(int64_t i, int64_t j) const {
+    if (i < 0 || j < 0 || i >= nx || j >= ny) return -1;
+    return id[j * nx + i];
+  }
+};
+
+int64_t CellOffset(const PatchInfo& P, int64_t i, int64_t j) {
+  return P.offset + (j + kG) * (P.nx + 2 * kG) + (i + kG);
+}
+int32_t CompStride(const PatchInfo& P) { return (P.nx + 2 * kG) * (P.ny + 2 * kG); }
+
+int Build(fvs_amr* A, const int32_t* lo) {
+  const int r = A->ratio;
+  std::vector<Owner> own(A->nlevels);
+  std::vector<std::vector<int32_t>> los(A->nlevels);
+  int64_t base = 0;
+  for (int l = 0; l < A->nlevels; ++l) {
+    fvs_batch* b = A->lv[l];
+    int64_t scale = 1;
+    for (int k = 0; k < l; ++k) scale *= r;
+    Owner& O = own[l];
+    O.nx = A->n0[0] * scale;
+    O.ny = A->n0[1] * scale;
+    O.id.assign(static_cast<size_t>(O.nx * O.ny), -1);
+    los[l].assign(lo + 2 * base, lo + 2 * (base + b->count));
+    base += b->count;
+    for (int64_t p = 0; p < b->count; ++p) {
+      const PatchInfo& P = b->host_patches[p];
+      const int64_t i0 = los[l][2 * p], j0 = los[l][2 * p + 1];
+      if (i0 < 0 || j0 < 0 || i0 + P.nx > O.nx || j0 + P.ny > O.ny) {
+        SetError("level %d patch %lld lies outside the level's domain", l, (long long)p);
+        return FVS_BAD_ARGUMENT;
+      }
+      if (l > 0 && (i0 % r || j0 % r || P.nx % r || P.ny % r)) {
+        SetError("level %d patch %lld is not aligned to the refinement ratio", l, (long long)p);
+        return FVS_BAD_ARGUMENT;
+      }
+      for (int64_t j = j0; j < j0 + P.ny; ++j)
+        for (int64_t i = i0; i < i0 + P.nx; ++i) {
+          int32_t& o = O.id[j * O.nx + i];
+          if (o >= 0) {
+            SetError("level %d patches %d and %lld overlap", l, o, (long long)p);
+            return FVS_BAD_ARGUMENT;
+          }
+          o = static_cast<int32_t>(p);
+        }
+    }
+  }
+  for (int64_t c : own[0].id)
+    if (c < 0) {
+      SetError("level 0 patches must cover the domain");
+      return FVS_BAD_ARGUMENT;
+    }
+  int slot = 0;
+  for (int l = 0; l < A->nlevels; ++l) {
+    fvs_batch* b = A->lv[l];
+    const auto& hp = b->host_patches;
+    const Owner& O = own[l];
+    fvs_amr::Level& LV = A->L[l];
+    int64_t sub = 1;
+    for (int k = 0; k < l; ++k) sub *= r;
+    LV.cfl_base = slot;
+    slot += static_cast<int>(2 * sub);
+    std::vector<GhostLink> self, coarse;
+    // ghost layers: clamp to the domain (transmissive edges), then the
+    // level's own patches, else the coarser level's cell above it
+    for (int64_t p = 0; p < b->count; ++p) {
+      const PatchInfo& P = hp[p];
+      const int64_t i0 = los[l][2 * p], j0 = los[l][2 * p + 1];
+      for (int64_t j = -kG; j < P.ny + kG; ++j)
+        for (int64_t i = -kG; i < P.nx + kG; ++i) {
+          if (i >= 0 && i < P.nx && j >= 0 && j < P.ny) continue;
+          const int64_t I = std::clamp<int64_t>(i0 + i, 0, O.nx - 1);
+          const int64_t J = std::clamp<int64_t>(j0 + j, 0, O.ny - 1);
+          GhostLink g;
+          g.dst = CellOffset(P, i, j);
+          g.dcs = CompStride(P);
+          const int32_t q = O.at(I, J);
+          if (q >= 0) {
+            const PatchInfo& Q = hp[q];
+            g.src = CellOffset(Q, I - los[l][2 * q], J - los[l][2 * q + 1]);
+            g.scs = CompStride(Q);
+            self.push_back(g);
+            continue;
+          }
+          const int32_t cq = l > 0 ? own[l - 1].at(I / r, J / r) : -1;
+          if (cq < 0) {
+            SetError("level %d patch %lld: ghost cell (%lld, %lld) has no source (patches "
+                     "not properly nested)", l, (long long)p, (long long)(i0 + i),
+                     (long long)(j0 + j));
+            return FVS_BAD_ARGUMENT;
+          }
+          const PatchInfo& Q = A->lv[l - 1]->host_patches[cq];
+          g.src = CellOffset(Q, I / r - los[l - 1][2 * cq], J / r - los[l - 1][2 * cq + 1]);
+          g.scs = CompStride(Q);
+          coarse.push_back(g);
+        }
+    }
+    std::vector<AvgLink> avg;
+    std::vector<RegFace> reg[4];
+    if (l > 0) {
+      const Owner& C = own[l - 1];
+      const auto& cp = A->lv[l - 1]->host_patches;
+      const auto& clo = los[l - 1];
+      for (int64_t p = 0; p < b->count; ++p) {
+        const PatchInfo& P = hp[p];
+        const int64_t i0 = los[l][2 * p], j0 = los[l][2 * p + 1];
+        for (int64_t cj = j0 / r; cj < (j0 + P.ny) / r; ++cj)
+          for (int64_t ci = i0 / r; ci < (i0 + P.nx) / r; ++ci) {
+            const int32_t q = C.at(ci, cj);
+            if (q < 0) {
+              SetError("level %d patch %lld is not above a level %d patch", l, (long long)p, l - 1);
+              return FVS_BAD_ARGUMENT;
+            }
+            AvgLink a;
+            a.dst = CellOffset(cp[q], ci - clo[2 * q], cj - clo[2 * q + 1]);
+            a.dcs = CompStride(cp[q]);
+            a.src = CellOffset(P, ci * r - i0, cj * r - j0);
+            a.scs = CompStride(P);
+            a.sldx = P.nx + 2 * kG;
+            a.pad = 0;
+            avg.push_back(a);
+          }
+        // register entries on the four sides
+        for (int dir = 0; dir < 2; ++dir)
+          for (int side = 0; side < 2; ++side) {
+            const int64_t nd = dir == 0 ? P.nx : P.ny, nt = dir == 0 ? P.ny : P.nx;
+            const int64_t d0 = dir == 0 ? i0 : j0, t0 = dir == 0 ? j0 : i0;
+            const int64_t Nd = dir == 0 ? O.nx : O.ny;
+            const int64_t face = side == 0 ? d0 : d0 + nd;  // fine face index along dir
+            if (face == 0 || face == Nd) continue;         // domain edge
+            const int64_t out = side == 0 ? face - 1 : face;  // fine cell outside the patch
+            for (int64_t ct = t0 / r; ct < (t0 + nt) / r; ++ct) {
+              const int64_t ft = ct * r;
+              const int64_t oi = dir == 0 ? out : ft, oj = dir == 0 ? ft : out;
+              if (O.at(oi, oj) >= 0) continue;  // fine-fine face
+              const int64_t cface = face / r;
+              const int64_t cc = side == 0 ? cface - 1 : cface;  // corrected coarse cell
+              const int64_t cci = dir == 0 ? cc : ct, ccj = dir == 0 ? ct : cc;
+              const int32_t q = C.at(cci, ccj);
+              if (q < 0) {
+                SetError("level %d patch %lld: coarse cell (%lld, %lld) beside it is in no "
+                         "level %d patch", l, (long long)p, (long long)cci, (long long)ccj, l - 1);
+                return FVS_BAD_ARGUMENT;
+              }
+              RegFace f;
+              f.fp = static_cast<int32_t>(p);
+              const int64_t fin = side == 0 ? 0 : nd;  // fine cell right of / above the face
+              f.fi = static_cast<int32_t>(dir == 0 ? fin : ft - i0);
+              f.fj = static_cast<int32_t>(dir == 0 ? ft - j0 : fin);
+              f.cp = q;
+              const int64_t cb_d = cface, cb_t = ct;  // coarse cell right of / above the face
+              f.ci = static_cast<int32_t>((dir == 0 ? cb_d : cb_t) - clo[2 * q]);
+              f.cj = static_cast<int32_t>((dir == 0 ? cb_t : cb_d) - clo[2 * q + 1]);
+              f.side = side;
+              f.pad = 0;
+              reg[2 * dir + side].push_back(f);
+            }
+          }
+      }
+    }
+    std::vector<RegFace> all;
+    for (int g = 0; g < 4; ++g) {
+      LV.group[g] = static_cast<int>(all.size());
+      all.insert(all.end(), reg[g].begin(), reg[g].end());
+    }
+    LV.group[4] = static_cast<int>(all.size());
+    LV.nself = static_cast<int>(self.size());
+    LV.ncoarse = static_cast<int>(coarse.size());
+    LV.navg = static_cast<int>(avg.size());
+    LV.nreg = static_cast<int>(all.size());
+    if (int rc = UploadVec(self, &LV.self)) return rc;
+    if (int rc = UploadVec(coarse, &LV.coarse)) return rc;
+    if (int rc = UploadVec(avg, &LV.avg)) return rc;
+    if (int rc = UploadVec(all, &LV.reg)) return rc;
+    if (LV.nreg) FVS_HIP_CHECK(hipMalloc(&LV.regval, 4 * sizeof(double) * LV.nreg));
+  }
+  A->ncfl = slot + 2 * A->nlevels;
+  FVS_HIP_CHECK(hipMalloc(&A->cfl, A->ncfl * sizeof(double)));
+  return FVS_OK;
+}
+
+int FillGhosts(fvs_amr* A, int l, int which) {
+  fvs_amr::Level& LV = A->L[l];
+  fvs_batch* b = A->lv[l];
+  hipStream_t s = A->h->stream;
+  if (LV.nself) {
+    hipLaunchKernelGGL(GatherKernel, dim3(Blocks(LV.nself)), dim3(256), 0, s, LV.self,
+                       LV.nself, b->pool[which], b->pool[which]);
+    FVS_LAUNCH_CHECK();
+  }
+  if (LV.ncoarse) {
+    hipLaunchKernelGGL(GatherKernel, dim3(Blocks(LV.ncoarse)), dim3(256), 0, s, LV.coarse,
+                       LV.ncoarse, b->pool[which], A->lv[l - 1]->pool[0]);
+    FVS_LAUNCH_CHECK();
+  }
+  return FVS_OK;
+}
+
+int SetDt(fvs_amr* A, int l, double dt) {
+  hipLaunchKernelGGL(FillKernel, dim3(1), dim3(64), 0, A->h->stream, A->lv[l]->dts, int64_t(1), dt);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+// Register contributions of fine level f (fine = 1: its faces, taken from its
+// pool src) or of its coarser level (fine = 0, coarse pool src) for the
+// sweep of direction dir.
+int AddRegister(fvs_amr* A, int f, int dir, int fine, int src) {
+  fvs_amr::Level& LV = A->L[f];
+  const int first = LV.group[2 * dir], count = LV.group[2 * dir + 2] - first;
+  if (count == 0) return FVS_OK;
+  fvs_batch* b = A->lv[fine ? f : f - 1];
+  batch::RegArgs a;
+  a.faces = LV.reg;
+  a.first = first;
+  a.count = count;
+  a.nfaces = LV.nreg;
+  a.patches = b->patches;
+  a.pool = b->pool[src];
+  a.dt = b->dts;
+  a.reg = LV.regval;
+  a.g.gamma = A->h->desc.gamma;
+  a.g.gm1inv = 1.0 / (A->h->desc.gamma - 1.0);
+  a.g.gm1 = A->h->desc.gamma - 1.0;
+  a.limiter = A->h->desc.limiter;
+  a.ratio = fine ? A->ratio : 1;
+  a.fine = fine;
+  a.sign = fine ? 1.0 : -1.0;
+  batch::LaunchRegFlux(A->h->stream, dir, a, A->h->desc.precision);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int Reflux(fvs_amr* A, int f) {
+  fvs_amr::Level& LV = A->L[f];
+  fvs_batch* c = A->lv[f - 1];
+  for (int g = 0; g < 4; ++g) {
+    const int first = LV.group[g], count = LV.group[g + 1] - first;
+    if (!count) continue;
+    hipLaunchKernelGGL(RefluxKernel, dim3(Blocks(count)), dim3(256), 0, A->h->stream, LV.reg,
+                       first, count, LV.nreg, LV.regval, c->patches, c->pool[0], g / 2);
+    FVS_LAUNCH_CHECK();
+  }
+  return FVS_OK;
+}
+
+int AverageDown(fvs_amr* A, int f) {
+  fvs_amr::Level& LV = A->L[f];
+  if (!LV.navg) return FVS_OK;
+  const double volfrac = 1.0 / static_cast<double>(A->ratio * A->ratio);
+  hipLaunchKernelGGL(AvgDownKernel, dim3(Blocks(LV.navg)), dim3(256), 0, A->h->stream, LV.avg,
+                     LV.navg, A->lv[f - 1]->pool[0], A->lv[f]->pool[0], A->ratio, volfrac);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+// SubcycleFineFirstSolver::AdvanceLevel with the level's split step; the
+// level's state is in pool 0 before and after.  sub = (index, count) of
+// this level's subcycle within its parent's step; slot = this level's
+// next CFL slot.
+int AdvanceLevel(fvs_amr* A, int l, double dt, int sub_first, std::vector<int>& slot) {
+  if (int rc = SetDt(A, l, dt)) return rc;
+  if (int rc = FillGhosts(A, l, 0)) return rc;
+  const bool finer = l + 1 < A->nlevels;
+  if (finer) {
+    fvs_amr::Level& F = A->L[l + 1];
+    if (F.nreg) {
+      hipLaunchKernelGGL(FillKernel, dim3(Blocks(4 * int64_t(F.nreg))), dim3(256), 0,
+                         A->h->stream, F.regval, 4 * int64_t(F.nreg), 0.0);
+      FVS_LAUNCH_CHECK();
+    }
+    for (int s = 0; s < A->ratio; ++s) {
+      if (int rc = AdvanceLevel(A, l + 1, dt / A->ratio, s, slot)) return rc;
+    }
+  }
+  // MakeSplitDirections: reversed order when (cycle % 2 + subcycle) % 2 == 1
+  const bool rev = ((A->cycle % 2) + sub_first) % 2 == 1;
+  const int dirs[2] = {rev ? 1 : 0, rev ? 0 : 1};
+  int src = 0;
+  for (int w = 0; w < 2; ++w) {
+    const int d = dirs[w];
+    if (w > 0) {
+      if (int rc = FillGhosts(A, l, src)) return rc;
+    }
+    if (finer) {
+      if (int rc = AddRegister(A, l + 1, d, 0, src)) return rc;
+    }
+    if (l > 0) {
+      if (int rc = AddRegister(A, l, d, 1, src)) return rc;
+    }
+    if (int rc = batch::Sweep(A->lv[l], A->h->stream, d, src, A->cfl + slot[l]++)) return rc;
+    src = 1 - src;
+  }
+  if (finer) {
+    if (int rc = Reflux(A, l + 1)) return rc;
+    if (int rc = AverageDown(A, l + 1)) return rc;
+  }
+  return FVS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int fvs_amr_create(fvs_handle* h, int nlevels, fvs_batch* const* levels, const int32_t* lo,
+                   const int64_t* domain, int ratio, fvs_amr** out) {
+  if (!h || nlevels < 1 || !levels || !lo || !domain || !out || ratio < 2) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  *out = nullptr;
+  if (domain[0] < 1 || domain[1] < 1) {
+    SetError("bad domain size");
+    return FVS_BAD_ARGUMENT;
+  }
+  fvs_amr* A = new (std::nothrow) fvs_amr();
+  if (!A) {
+    SetError("out of host memory");
+    return FVS_BAD_ARGUMENT;
+  }
+  A->h = h;
+  A->nlevels = nlevels;
+  A->ratio = ratio;
+  A->n0[0] = domain[0];
+  A->n0[1] = domain[1];
+  A->lv.assign(levels, levels + nlevels);
+  A->L.resize(nlevels);
+  for (int l = 0; l < nlevels; ++l) {
+    if (!levels[l] || levels[l]->h != h) {
+      delete A;
+      SetError("level %d: missing batch or batch of another handle", l);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  if (int rc = Build(A, lo)) {
+    fvs_amr_destroy(A);
+    return rc;
+  }
+  *out = A;
+  return FVS_OK;
+}
+
+int fvs_amr_destroy(fvs_amr* A) {
+  if (!A) return FVS_OK;
+  if (A->h) (void)hipStreamSynchronize(A->h->stream);
+  for (auto& LV : A->L) {
+    void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  }
+  if (A->cfl) (void)hipFree(A->cfl);
+  delete A;
+  return FVS_OK;
+}
+
+int fvs_amr_fill_ghosts(fvs_amr* A, int level, int which) {
+  if (!A || level < 0 || level >= A->nlevels || (which != 0 && which != 1)) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  return FillGhosts(A, level, which);
+}
+
+int fvs_amr_average_down(fvs_amr* A, int fine_level) {
+  if (!A || fine_level < 1 || fine_level >= A->nlevels) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  return AverageDown(A, fine_level);
+}
+
+int fvs_amr_advance(fvs_amr* A, double dt) {
+  if (!A || !(dt > 0.0)) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  std::vector<int> slot(A->nlevels);
+  for (int l = 0; l < A->nlevels; ++l) slot[l] = A->L[l].cfl_base;
+  if (int rc = AdvanceLevel(A, 0, dt, 0, slot)) return rc;
+  A->cycle += 1;
+  return FVS_OK;
+}
+
+int fvs_amr_level_cfl(fvs_amr* A, double* host) {
+  if (!A || !host) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  std::vector<double> v(A->ncfl);
+  FVS_HIP_CHECK(hipMemcpyAsync(v.data(), A->cfl, A->ncfl * sizeof(double),
+                               hipMemcpyDeviceToHost, A->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(A->h->stream));
+  for (int l = 0; l < A->nlevels; ++l) {
+    const int n = (l + 1 < A->nlevels ? A->L[l + 1].cfl_base : A->ncfl - 2 * A->nlevels) -
+                  A->L[l].cfl_base;
+    double m = __builtin_huge_val();
+    for (int k = 0; k < n; ++k) m = std::min(m, v[A->L[l].cfl_base + k]);
+    host[l] = m;
+  }
+  return FVS_OK;
+}
+
+int fvs_amr_stable_dt(fvs_amr* A, double* dt_host) {
+  if (!A || !dt_host) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int base = A->ncfl - 2 * A->nlevels;
+  for (int l = 0; l < A->nlevels; ++l) {
+    if (int rc = FillGhosts(A, l, 0)) return rc;
+    if (int rc = SetDt(A, l, 0.0)) return rc;
+    // sweeps into the scratch pool 1 for their CFL minima only
+    for (int d = 0; d < 2; ++d)
+      if (int rc = batch::Sweep(A->lv[l], A->h->stream, d, 0, A->cfl + base + 2 * l + d)) return rc;
+  }
+  std::vector<double> v(2 * A->nlevels);
+  FVS_HIP_CHECK(hipMemcpyAsync(v.data(), A->cfl + base, v.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, A->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(A->h->stream));
+  double m = __builtin_huge_val();
+  double scale = 1.0;
+  for (int l = 0; l < A->nlevels; ++l) {
+    m = std::min(m, scale * std::min(v[2 * l], v[2 * l + 1]));
+    scale *= A->ratio;
+  }
+  *dt_host = m;
+  return FVS_OK;
+}
+
+} // extern "C"

@@ -22,8 +22,7 @@
 // and the C entry points; fvs_batch_fast.hip defines FVS_BATCH_FAST_TU and
 // compiles the kernels of the tolerance-checked modes with multiply-add
 // contraction.
-#include "fvs_internal.hpp"
-#include "fvs_physics.hpp"
+#include "fvs_batch_internal.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -32,40 +31,11 @@
 #include <vector>
 
 namespace fvs {
-namespace batch {
-
-struct PatchInfo {
-  int64_t offset;  // element offset of cell (-2, -2), component 0
-  int32_t nx, ny;
-  double dx;
-};
-
-struct BatchArgs {
-  const PatchInfo* patches;
-  const int4* tiles;  // (patch, start along the sweep, start across, 0)
-  int ntiles;
-  const double* in;
-  double* out;
-  const double* dt;
-  Gas<double> g;
-  int limiter;
-  unsigned long long* partials;
-};
-
-// Batched sweeps of the tolerance-checked arithmetic modes (FVS_FP64_RCP,
-// FVS_FP32_FLUX), compiled with multiply-add contraction in
-// fvs_batch_fast.hip.
-void LaunchSweepFast(hipStream_t s, int dir, int shape, const BatchArgs& a, unsigned grid,
-                     int precision);
-
-} // namespace batch
-
 using batch::BatchArgs;
 using batch::PatchInfo;
+using batch::kG;
 
 namespace {
-
-constexpr int kG = 2;
 
 template <class T>
 __device__ __forceinline__ Gas<T> GasT(const Gas<double>& g) {
@@ -271,9 +241,74 @@ void LaunchSweepShape(hipStream_t s, int dir, int shape, const BatchArgs& a, int
   }
 }
 
+
+// Flux-register contributions: one thread per register entry, the entry's
+// faces in order (the fine faces along the coarse face, or the coarse face),
+// each face flux computed exactly as BatchSweepKernel computes it (the two
+// adjacent cells' MUSCL-Hancock half-step states, then HLLEM) from the same
+// pool and time step, so the register holds the fluxes the sweeps applied.
+template <int DIR, int LIM, class T, bool FAST>
+__global__ __launch_bounds__(256) void RegFluxKernel(batch::RegArgs a) {
+  using TA = typename Policies<T, FAST>::TA;
+  constexpr int NQ = 6, NC = 4;
+  const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q0 >= a.count) return;
+  const int q = a.first + q0;
+  const batch::RegFace F = a.faces[q];
+  const PatchInfo P = a.patches[a.fine ? F.fp : F.cp];
+  const int i = a.fine ? F.fi : F.ci, j = a.fine ? F.fj : F.cj;
+  const int64_t ldx = P.nx + 2 * kG, cs = ldx * (P.ny + 2 * kG);
+  const int64_t step = DIR == 0 ? 1 : ldx, across = DIR == 0 ? ldx : 1;
+  const double dt = *a.dt;
+  const T lh = static_cast<T>(DivRn(0.5 * dt, P.dx));
+  const double w = P.dx * dt;
+  const Gas<T> gT = GasT<T>(a.g);
+  double acc[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) acc[k] = a.reg[k * a.nfaces + q];
+  for (int m = 0; m < a.ratio; ++m) {
+    const int64_t b = P.offset + (int64_t(j) + kG) * ldx + (int64_t(i) + kG) + m * across;
+    T s0[NQ], s1[NQ], s2[NQ], s3[NQ];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      s0[k] = static_cast<T>(a.pool[b - 2 * step + k * cs]);
+      s1[k] = static_cast<T>(a.pool[b - step + k * cs]);
+      s2[k] = static_cast<T>(a.pool[b + k * cs]);
+      s3[k] = static_cast<T>(a.pool[b + step + k * cs]);
+    }
+    T rlA[NQ], rrA[NQ], rlB[NQ], rrB[NQ], f[NC];
+    ReconstructCell<2, DIR, LIM, T, TA>(s0, s1, s2, lh, gT, rlA, rrA);
+    ReconstructCell<2, DIR, LIM, T, TA>(s1, s2, s3, lh, gT, rlB, rrB);
+    HllemFlux<2, DIR, true, false, T, TA>(f, rrA, rlB, gT);
+#pragma unroll
+    for (int k = 0; k < NC; ++k) acc[k] = acc[k] + a.sign * (w * static_cast<double>(f[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) a.reg[k * a.nfaces + q] = acc[k];
+}
+
+template <int DIR, class T, bool FAST>
+void LaunchRegFluxT(hipStream_t s, const batch::RegArgs& a) {
+  const dim3 grid((a.count + 255) / 256);
+  switch (a.limiter) {
+  case kMinMod: hipLaunchKernelGGL((RegFluxKernel<DIR, kMinMod, T, FAST>), grid, dim3(256), 0, s, a); break;
+  case kVanLeer: hipLaunchKernelGGL((RegFluxKernel<DIR, kVanLeer, T, FAST>), grid, dim3(256), 0, s, a); break;
+  case kNoLimiter: hipLaunchKernelGGL((RegFluxKernel<DIR, kNoLimiter, T, FAST>), grid, dim3(256), 0, s, a); break;
+  default: hipLaunchKernelGGL((RegFluxKernel<DIR, kNoGradient, T, FAST>), grid, dim3(256), 0, s, a); break;
+  }
+}
+
 } // namespace
 
 #ifdef FVS_BATCH_FAST_TU
+
+void batch::LaunchRegFluxFast(hipStream_t s, int dir, const batch::RegArgs& a, int precision) {
+  if (precision == FVS_FP32_FLUX) {
+    if (dir == 0) LaunchRegFluxT<0, float, true>(s, a); else LaunchRegFluxT<1, float, true>(s, a);
+  } else {
+    if (dir == 0) LaunchRegFluxT<0, double, true>(s, a); else LaunchRegFluxT<1, double, true>(s, a);
+  }
+}
 
 void batch::LaunchSweepFast(hipStream_t s, int dir, int shape, const BatchArgs& a,
                             unsigned grid, int precision) {
@@ -287,29 +322,21 @@ void batch::LaunchSweepFast(hipStream_t s, int dir, int shape, const BatchArgs& 
 
 #else // FVS_BATCH_FAST_TU
 
+void batch::LaunchRegFlux(hipStream_t s, int dir, const batch::RegArgs& a, int precision) {
+  if (a.count <= 0) return;
+  if (precision != FVS_FP64) {
+    batch::LaunchRegFluxFast(s, dir, a, precision);
+  } else if (dir == 0) {
+    LaunchRegFluxT<0, double, false>(s, a);
+  } else {
+    LaunchRegFluxT<1, double, false>(s, a);
+  }
+}
+
 } // namespace fvs
 
 using namespace fvs;
 
-struct fvs_batch {
-  fvs_handle* h;
-  int64_t count;
-  std::vector<PatchInfo> host_patches;
-  PatchInfo* patches;
-  int64_t pool_size;
-  double* pool[2];
-  int shape[2];                 // chosen tile shape per direction
-  int4* tiles[2];               // tile list per direction
-  int ntiles[2];
-  unsigned grid[2];
-  unsigned long long* partials;
-  int64_t partials_cap;
-  double* dts;                  // device: [0] dt of the sweeps, [1..2] CFL results
-  hipStream_t cap_stream;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
-  int cus;
-};
 
 namespace {
 
@@ -368,6 +395,10 @@ int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int pr
 }
 
 } // namespace
+
+int fvs::batch::Sweep(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out) {
+  return Launch(b, s, dir, src, dt_out, b->h->desc.precision);
+}
 
 extern "C" {
 

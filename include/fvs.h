@@ -522,6 +522,48 @@ double* fvs_batch_cfl(fvs_batch* batch);
 /* Host copy of the two CFL minima (synchronises the handle's stream). */
 int fvs_batch_read_cfl(fvs_batch* batch, double* host2);
 
+/* ---- AMR hierarchy over batched levels (2D) -------------------------------
+ * Levels 0..nlevels-1 are batches of one handle; level l has dx / ratio^l
+ * and a domain of domain[0] x domain[1] cells times ratio^l.  lo: host, the
+ * lower corner (i0, j0) of every patch of every level, level 0 first, in
+ * the level's cell indices.  Level 0 must cover the domain; patches of
+ * finer levels must be aligned to the ratio and properly nested (ghost
+ * layers clamped to the domain lie in same-level patches or above
+ * coarser-level patches, and the coarse cells beside each patch belong to
+ * coarser patches).  Replaces the AMReX machinery the reference drives
+ * through IntegratorContext (src/AMReX/IntegratorContext.cpp:380-533):
+ * FillGhostLayerTwoLevels / FillGhostLayerSingleLevel (FillPatchTwoLevels
+ * with piecewise-constant interpolation; transmissive domain edges),
+ * AccumulateCoarseFineFluxes + ApplyFluxCorrection (FluxRegister) and
+ * CoarsenConservatively (average_down). */
+typedef struct fvs_amr fvs_amr;
+int fvs_amr_create(fvs_handle* h, int nlevels, fvs_batch* const* levels, const int32_t* lo,
+                   const int64_t* domain, int ratio, fvs_amr** out);
+int fvs_amr_destroy(fvs_amr* amr);
+/* Ghost layers of every patch of level `level`, pool `which`: same-level
+ * patches first, else the coarser level's cell above (pool 0). */
+int fvs_amr_fill_ghosts(fvs_amr* amr, int level, int which);
+/* Cells of level fine_level - 1 (pool 0) under level fine_level's patches
+ * become the mean of their ratio^2 fine cells (pool 0). */
+int fvs_amr_average_down(fvs_amr* amr, int fine_level);
+/* One coarse time step dt of the hierarchy (SubcycleFineFirstSolver::
+ * AdvanceHierarchy, include/fub/solver/SubcycleFineFirstSolver.hpp:114-175):
+ * level l+1 advances ratio times with dt / ratio before level l; each level
+ * step is the two split sweeps (order per MakeSplitDirections,
+ * include/fub/split_method/SplittingMethod.hpp:115-126) with ghost layers
+ * refilled before each sweep; then reflux and average-down onto level l.
+ * States live in pool 0 of each level between steps; pool 1 is scratch. */
+int fvs_amr_advance(fvs_amr* amr, double dt);
+/* Per level, the minimum stable dt over the sweeps of the last advance
+ * (host, nlevels values; synchronises).  A level's value below its own
+ * time step dt / ratio^l is the reference's TimeStepTooLarge. */
+int fvs_amr_level_cfl(fvs_amr* amr, double* host);
+/* Stable coarse dt of the current state: min over levels of ratio^l times
+ * the level's minimum HLLEM stable dt over both directions
+ * (SubcycleFineFirstSolver::ComputeStableDt, :101-112), without the CFL
+ * factor.  Uses pool 1 of each level as scratch; synchronises. */
+int fvs_amr_stable_dt(fvs_amr* amr, double* dt_host);
+
 #ifdef __cplusplus
 }
 #endif

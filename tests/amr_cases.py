@@ -1,0 +1,87 @@
+"""Shared cases of the AMR hierarchy tests: a small three-level hierarchy
+and the per-patch oracle sweep the AMR oracle composes."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+import oracle_lib as O
+
+# Level 0: two patches covering a 32 x 32 domain on [-1, 1]^2.  Level 1
+# (64 x 64): patches touching along x = 32, one isolated, one at the
+# domain corner.  Level 2 (128 x 128): properly nested patches, one pair
+# sharing a face (x = 60), one spanning two level-1 patches, one on the
+# domain edge y = 0.
+DOMAIN = (32, 32)
+DX0 = 2.0 / 32
+LAYOUTS = [
+    [(0, 0, 16, 32), (16, 0, 16, 32)],
+    [(16, 16, 16, 24), (32, 16, 16, 16), (8, 44, 24, 16), (48, 0, 16, 16)],
+    [(36, 36, 24, 40), (60, 36, 16, 24), (100, 0, 24, 16)],
+]
+
+
+def oracle_sweep(desc):
+    """sweep(cons, dt, dx, d) of one 2D patch with two ghost layers: the
+    oracle's FluxMethod + UpdateConservatively over the interior rows
+    (src/HyperbolicPatchIntegrator.cpp:35-154) and the minimum HLLEM stable
+    dt of the faces 0..n of those rows."""
+    lib = O.load()
+
+    def sweep(cons, dt, dx, d):
+        ny, nx = cons.shape[1] - 4, cons.shape[2] - 4
+        comp = O.complete_from_cons(np.ascontiguousarray(cons).reshape(4, -1), 2)
+        comp = comp.reshape(6, 1, ny + 4, nx + 4)
+        sv = O.host_view(np.ascontiguousarray(comp), (-2, -2, 0))
+        f = np.zeros((4, 1, ny + (d == 1), nx + (d == 0)))
+        lib.oracle_numeric_fluxes(C.byref(desc), C.byref(O.host_view(f)), C.byref(sv), dt, dx, d)
+        prev = np.ascontiguousarray(cons[:, 2:-2, 2:-2].reshape(4, 1, ny, nx))
+        nxt = prev.copy()
+        lib.oracle_update_conservatively(C.byref(O.host_view(nxt)), C.byref(O.host_view(prev)),
+                                         C.byref(O.host_view(f)), dt, dx, d)
+        c = comp[:, 0, 2:-2, :] if d == 0 else comp[:, 0, :, 2:-2]
+        if d == 0:
+            L, R = c[:, :, 1:nx + 2], c[:, :, 2:nx + 3]
+        else:
+            L, R = c[:, 1:ny + 2, :], c[:, 2:ny + 3, :]
+        dts = O.hllem_stable_dt(np.ascontiguousarray(L.reshape(6, -1)),
+                                np.ascontiguousarray(R.reshape(6, -1)), 2, d, dx)
+        return nxt[:, 0], f[:, 0], float(dts.min())
+
+    return sweep
+
+
+def centres(layout_patch, level, ratio=2):
+    i0, j0, nx, ny = layout_patch
+    dx = DX0 / ratio ** level
+    x = -1.0 + (i0 + np.arange(nx) + 0.5) * dx
+    y = -1.0 + (j0 + np.arange(ny) + 0.5) * dx
+    return np.meshgrid(x, y)
+
+
+def cons_from(rho, u, v, p, gamma=1.4):
+    return np.stack([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)])
+
+
+def moving_state(level, p, layouts=LAYOUTS):
+    """Smooth blob with a swirling velocity field."""
+    x, y = centres(layouts[level][p], level)
+    r2 = (x - 0.1) ** 2 + (y + 0.05) ** 2
+    rho = 1.0 + 0.4 * np.exp(-r2 / 0.08)
+    u = 0.3 * np.sin(np.pi * x) * np.cos(np.pi * y)
+    v = -0.2 * np.cos(np.pi * x) * np.sin(np.pi * y)
+    p_ = 1.0 + 0.5 * np.exp(-r2 / 0.05)
+    return cons_from(rho, u, v, p_)
+
+
+def resting_blob(level, p, layouts=LAYOUTS):
+    """Compactly supported density and pressure bump at rest (radius 0.5):
+    no flux reaches the domain edges within a few steps, so the composite
+    integrals of mass and energy are invariants of the hierarchy step."""
+    x, y = centres(layouts[level][p], level)
+    s = np.maximum(0.0, 1.0 - (x * x + y * y) / 0.25) ** 2
+    rho = 1.0 + 0.5 * s
+    p_ = 1.0 + 0.8 * s
+    z = np.zeros_like(x)
+    return cons_from(rho, z, z, p_)
