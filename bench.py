@@ -454,6 +454,60 @@ def bench_amr(steps: int, torch, fvs) -> dict:
     return res
 
 
+def bench_amr_hierarchy(steps: int, torch, fvs) -> dict:
+    """End-to-end subcycled hierarchy step of the 2D shock-bubble problem:
+    level 0 512^2, level 1 the 300 seeded patches, level 2 nested inside them
+    (ratio 2).  One coarse step = ghost fills (same-level copies and
+    piecewise-constant coarse data), level 2 four times and level 1 twice
+    with their flux-register contributions, level 0 once, reflux and
+    average_down; the sweeps use the autotuned tile shapes.  Cell-updates
+    count each level's cells times its sweeps per coarse step."""
+    import numpy as np
+    from finitevolumesolver_amd.amr import AmrHierarchy
+    from finitevolumesolver_amd.initial_data import (amr_nested_levels, cons_from_prim,
+                                                     shock_bubble_state)
+    layouts = amr_nested_levels()
+    res = {"workload": "2D shock-bubble AMR hierarchy, 3 levels (512^2, 300 patches, "
+                       f"{len(layouts[2])} nested patches), ratio 2, fine-first subcycling "
+                       "with ghost fill, reflux and average_down"}
+    for prec in ("fp64", "fp32"):
+        fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
+                            precision=prec)
+        h = AmrHierarchy(fm, (512, 512), layouts, 2.0 / 512, 2)
+
+        def state(l, p):
+            X, Y = h.cell_centres(l, p)
+            rho, u, v, pr = shock_bubble_state(X - 1.0, Y - 1.0)
+            return cons_from_prim(rho, [u, v], pr).reshape(4, X.shape[0], X.shape[1])
+
+        h.set_state(state)
+        for f in (2, 1):
+            h.average_down(f)
+        for b in h.levels:
+            for d in (0, 1):
+                b.tune(d, reps=3)
+        dt = 0.8 * h.stable_dt()
+        h.advance(dt)  # warm-up
+        ok = all(h.level_cfl()[l] >= dt / 2 ** l for l in range(3))
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(steps):
+            h.advance(dt)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / steps
+        cells = [sum(p[2] * p[3] for p in lay) for lay in layouts]
+        updates = sum(2.0 * c * 2 ** l for l, c in enumerate(cells))
+        res[prec] = {"coarse_step_ms": round(ms, 4), "cells_per_level": cells,
+                     "cell_updates_per_coarse_step": int(updates),
+                     "value": round(updates / (ms * 1e-3) / 1e9, 4), "unit": "Gcell-updates/s",
+                     "dt": dt, "cfl_ok": bool(ok)}
+        del h
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,7 +527,7 @@ def main() -> int:
     ap.add_argument("--cutcell-steps", type=int, default=20)
     ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--godunov-epochs", type=int, default=12)
-    ap.add_argument("--only", choices=["godunov", "c1", "cutcell", "amr"],
+    ap.add_argument("--only", choices=["godunov", "c1", "cutcell", "amr", "amr_hier"],
                     help="run one variant alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -494,7 +548,8 @@ def main() -> int:
                                                 epochs=args.godunov_epochs),
                "c1": lambda: bench_c1(torch, fvs),
                "cutcell": lambda: bench_cutcell(2048, 1024, args.cutcell_steps, torch, fvs),
-               "amr": lambda: bench_amr(args.amr_steps, torch, fvs)}[args.only]
+               "amr": lambda: bench_amr(args.amr_steps, torch, fvs),
+               "amr_hier": lambda: bench_amr_hierarchy(args.amr_steps, torch, fvs)}[args.only]
         print(json.dumps({args.only: run()}), flush=True)
         return 0
     from finitevolumesolver_amd.initial_data import fourier_modes_3d_device
@@ -595,6 +650,7 @@ def main() -> int:
         torch.cuda.empty_cache()
         log("[bench] variant: batched AMR levels")
         variants["batched_amr"] = bench_amr(args.amr_steps, torch, fvs)
+        variants["amr_hierarchy"] = bench_amr_hierarchy(args.amr_steps, torch, fvs)
         result["variants"] = variants
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle)")

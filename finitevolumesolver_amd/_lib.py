@@ -181,6 +181,7 @@ SIGNATURES = {
     "fvs_amr_advance": (_I, [_P, _D]),
     "fvs_amr_level_cfl": (_I, [_P, _P]),
     "fvs_amr_stable_dt": (_I, [_P, _PD]),
+    "fvs_amr_set_graphs": (_I, [_P, _I]),
 }
 
 _lib = None

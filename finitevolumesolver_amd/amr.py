@@ -88,6 +88,11 @@ class AmrHierarchy:
     def average_down(self, fine_level: int) -> None:
         L.check(self._lib.fvs_amr_average_down(self.a, fine_level))
 
+    def set_graphs(self, on: bool = True) -> None:
+        """Replay each coarse step from a captured HIP graph (default) or
+        launch it eagerly."""
+        L.check(self._lib.fvs_amr_set_graphs(self.a, 1 if on else 0))
+
     def stable_dt(self) -> float:
         out = C.c_double()
         L.check(self._lib.fvs_amr_stable_dt(self.a, C.byref(out)))

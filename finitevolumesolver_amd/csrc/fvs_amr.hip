@@ -51,11 +51,17 @@ struct AvgLink {
   int32_t dcs, scs, sldx, pad;
 };
 
-__global__ void GatherKernel(const GhostLink* links, int n, double* dst,
-                             const double* src) {
+// One launch per ghost fill: links [0, nself) copy from the level's own
+// pool, links [nself, nself + ncoarse) from the coarser level's pool 0.
+// Every ghost cell appears in exactly one link and no link reads a ghost
+// cell, so the copies are independent.
+__global__ void GatherKernel(const GhostLink* self, int nself, const GhostLink* coarse,
+                             int ncoarse, double* dst, const double* csrc) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const GhostLink L = links[q];
+  if (q >= nself + ncoarse) return;
+  const bool own = q < nself;
+  const GhostLink L = own ? self[q] : coarse[q - nself];
+  const double* src = own ? dst : csrc;
 #pragma unroll
   for (int k = 0; k < 4; ++k) dst[L.dst + k * int64_t(L.dcs)] = src[L.src + k * int64_t(L.scs)];
 }
@@ -76,26 +82,52 @@ __global__ void AvgDownKernel(const AvgLink* links, int n, double* coarse, const
   }
 }
 
-// Reflux of one (direction, side) group of register entries: the coarse
-// cell beside each coarse face takes -reg / V (cell left of / below the
-// face) or +reg / V (right of / above), V the coarse cell volume.  Within a
-// group no two entries share a coarse cell.
-__global__ void RefluxKernel(const RegFace* faces, int first, int count, int nfaces,
-                             const double* reg, const PatchInfo* patches, double* pool, int dir) {
-  const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q0 >= count) return;
-  const int q = first + q0;
-  const RegFace F = faces[q];
-  const PatchInfo P = patches[F.cp];
-  const int i = F.ci - (F.side == 0 && dir == 0 ? 1 : 0);
-  const int j = F.cj - (F.side == 0 && dir == 1 ? 1 : 0);
-  const int64_t ldx = P.nx + 2 * kG, cs = ldx * (P.ny + 2 * kG);
-  const int64_t at = P.offset + (int64_t(j) + kG) * ldx + (int64_t(i) + kG);
-  const double vol = P.dx * P.dx;
+// Reflux: the coarse cell beside each coarse face takes -reg / V (cell left
+// of / below the face) or +reg / V (right of / above), V the coarse cell
+// volume.  One thread per corrected cell applies its entries in the order
+// of the (direction, side) groups: (x, low), (x, high), (y, low), (y, high).
+struct RefluxCell {
+  int64_t at;        // coarse pool offset of the cell, component 0
+  int32_t cs;        // its patch's component stride
+  int32_t first, count;  // its entries in the ordered entry list
+  double vol;
+};
+
+__global__ void RefluxKernel(const RefluxCell* cells, int ncells, const int32_t* entries,
+                             const RegFace* faces, int nfaces, const double* reg, double* pool) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncells) return;
+  const RefluxCell C = cells[q];
+  double u[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const double corr = DivRn(reg[k * int64_t(nfaces) + q], vol);
-    pool[at + k * cs] = F.side == 0 ? pool[at + k * cs] - corr : pool[at + k * cs] + corr;
+  for (int k = 0; k < 4; ++k) u[k] = pool[C.at + k * int64_t(C.cs)];
+  for (int e = 0; e < C.count; ++e) {
+    const int f = entries[C.first + e];
+    const bool low = faces[f].side == 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double corr = DivRn(reg[k * int64_t(nfaces) + f], C.vol);
+      u[k] = low ? u[k] - corr : u[k] + corr;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pool[C.at + k * int64_t(C.cs)] = u[k];
+}
+
+// The levels' time steps of one coarse step: level l + 1 takes the level
+// l step divided by the ratio (correctly rounded, as the host divides).
+struct LevelDts {
+  double* dst[8];
+  int n;
+  int ratio;
+};
+
+__global__ void LevelDtKernel(LevelDts d, const double* dtc) {
+  if (threadIdx.x != 0) return;
+  double dt = *dtc;
+  for (int l = 0; l < d.n; ++l) {
+    *d.dst[l] = dt;
+    dt = DivRn(dt, static_cast<double>(d.ratio));
   }
 }
 
@@ -125,12 +157,22 @@ struct fvs_amr {
     int nreg = 0;
     int group[5] = {0, 0, 0, 0, 0};  // entries of (dir 0, side 0), (0, 1), (1, 0), (1, 1)
     double* regval = nullptr;        // 4 x nreg
+    RefluxCell* rcells = nullptr;    // coarse cells corrected by this register
+    int nrcells = 0;
+    int32_t* rentries = nullptr;     // their entries, per cell in group order
     int cfl_base = 0;                // first CFL slot of this level's sweeps
   };
   std::vector<Level> L;
   double* cfl = nullptr;  // one slot per sweep of a coarse step, then 2 per level
   int ncfl = 0;
   int64_t cycle = 0;
+  double* dtc = nullptr;  // device: the coarse time step of the current advance
+  hipStream_t s = nullptr;  // stream the operations are enqueued on
+  // whole coarse steps captured per cycle parity (the split order flips)
+  int use_graphs = 1;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<int64_t> gens;  // the levels' tile-list generations the graphs were captured with
 };
 
 namespace {
@@ -315,6 +357,38 @@ int Build(fvs_amr* A, const int32_t* lo) {
       all.insert(all.end(), reg[g].begin(), reg[g].end());
     }
     LV.group[4] = static_cast<int>(all.size());
+    std::vector<RefluxCell> rc;
+    std::vector<int32_t> rent;
+    if (l > 0) {
+      // entries per corrected coarse cell, in entry (= group) order
+      const auto& cp = A->lv[l - 1]->host_patches;
+      std::vector<std::pair<int64_t, int32_t>> key;
+      for (size_t e = 0; e < all.size(); ++e) {
+        const RegFace& f = all[e];
+        const int dir = static_cast<int>(e) < LV.group[2] ? 0 : 1;
+        const int i = f.ci - (f.side == 0 && dir == 0 ? 1 : 0);
+        const int j = f.cj - (f.side == 0 && dir == 1 ? 1 : 0);
+        key.emplace_back(CellOffset(cp[f.cp], i, j), static_cast<int32_t>(e));
+      }
+      std::stable_sort(key.begin(), key.end(),
+                       [](const auto& a, const auto& b) { return a.first < b.first; });
+      for (size_t k = 0; k < key.size();) {
+        size_t m = k;
+        RefluxCell c;
+        c.at = key[k].first;
+        c.first = static_cast<int32_t>(rent.size());
+        const RegFace& f = all[key[k].second];
+        c.cs = CompStride(cp[f.cp]);
+        c.vol = cp[f.cp].dx * cp[f.cp].dx;
+        while (m < key.size() && key[m].first == key[k].first) rent.push_back(key[m++].second);
+        c.count = static_cast<int32_t>(m - k);
+        rc.push_back(c);
+        k = m;
+      }
+    }
+    LV.nrcells = static_cast<int>(rc.size());
+    if (int rc2 = UploadVec(rc, &LV.rcells)) return rc2;
+    if (int rc2 = UploadVec(rent, &LV.rentries)) return rc2;
     LV.nself = static_cast<int>(self.size());
     LV.ncoarse = static_cast<int>(coarse.size());
     LV.navg = static_cast<int>(avg.size());
@@ -327,28 +401,25 @@ int Build(fvs_amr* A, const int32_t* lo) {
   }
   A->ncfl = slot + 2 * A->nlevels;
   FVS_HIP_CHECK(hipMalloc(&A->cfl, A->ncfl * sizeof(double)));
+  FVS_HIP_CHECK(hipMalloc(&A->dtc, sizeof(double)));
+  FVS_HIP_CHECK(hipStreamCreateWithFlags(&A->cap, hipStreamNonBlocking));
   return FVS_OK;
 }
 
 int FillGhosts(fvs_amr* A, int l, int which) {
   fvs_amr::Level& LV = A->L[l];
   fvs_batch* b = A->lv[l];
-  hipStream_t s = A->h->stream;
-  if (LV.nself) {
-    hipLaunchKernelGGL(GatherKernel, dim3(Blocks(LV.nself)), dim3(256), 0, s, LV.self,
-                       LV.nself, b->pool[which], b->pool[which]);
-    FVS_LAUNCH_CHECK();
-  }
-  if (LV.ncoarse) {
-    hipLaunchKernelGGL(GatherKernel, dim3(Blocks(LV.ncoarse)), dim3(256), 0, s, LV.coarse,
-                       LV.ncoarse, b->pool[which], A->lv[l - 1]->pool[0]);
-    FVS_LAUNCH_CHECK();
-  }
+  const int n = LV.nself + LV.ncoarse;
+  if (!n) return FVS_OK;
+  hipLaunchKernelGGL(GatherKernel, dim3(Blocks(n)), dim3(256), 0, A->s, LV.self, LV.nself,
+                     LV.coarse, LV.ncoarse, b->pool[which],
+                     l > 0 ? A->lv[l - 1]->pool[0] : nullptr);
+  FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
 
 int SetDt(fvs_amr* A, int l, double dt) {
-  hipLaunchKernelGGL(FillKernel, dim3(1), dim3(64), 0, A->h->stream, A->lv[l]->dts, int64_t(1), dt);
+  hipLaunchKernelGGL(FillKernel, dim3(1), dim3(64), 0, A->s, A->lv[l]->dts, int64_t(1), dt);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
@@ -377,21 +448,17 @@ int AddRegister(fvs_amr* A, int f, int dir, int fine, int src) {
   a.ratio = fine ? A->ratio : 1;
   a.fine = fine;
   a.sign = fine ? 1.0 : -1.0;
-  batch::LaunchRegFlux(A->h->stream, dir, a, A->h->desc.precision);
+  batch::LaunchRegFlux(A->s, dir, a, A->h->desc.precision);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
 
 int Reflux(fvs_amr* A, int f) {
   fvs_amr::Level& LV = A->L[f];
-  fvs_batch* c = A->lv[f - 1];
-  for (int g = 0; g < 4; ++g) {
-    const int first = LV.group[g], count = LV.group[g + 1] - first;
-    if (!count) continue;
-    hipLaunchKernelGGL(RefluxKernel, dim3(Blocks(count)), dim3(256), 0, A->h->stream, LV.reg,
-                       first, count, LV.nreg, LV.regval, c->patches, c->pool[0], g / 2);
-    FVS_LAUNCH_CHECK();
-  }
+  if (!LV.nrcells) return FVS_OK;
+  hipLaunchKernelGGL(RefluxKernel, dim3(Blocks(LV.nrcells)), dim3(256), 0, A->s, LV.rcells,
+                     LV.nrcells, LV.rentries, LV.reg, LV.nreg, LV.regval, A->lv[f - 1]->pool[0]);
+  FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
 
@@ -399,7 +466,7 @@ int AverageDown(fvs_amr* A, int f) {
   fvs_amr::Level& LV = A->L[f];
   if (!LV.navg) return FVS_OK;
   const double volfrac = 1.0 / static_cast<double>(A->ratio * A->ratio);
-  hipLaunchKernelGGL(AvgDownKernel, dim3(Blocks(LV.navg)), dim3(256), 0, A->h->stream, LV.avg,
+  hipLaunchKernelGGL(AvgDownKernel, dim3(Blocks(LV.navg)), dim3(256), 0, A->s, LV.avg,
                      LV.navg, A->lv[f - 1]->pool[0], A->lv[f]->pool[0], A->ratio, volfrac);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
@@ -409,19 +476,18 @@ int AverageDown(fvs_amr* A, int f) {
 // level's state is in pool 0 before and after.  sub = (index, count) of
 // this level's subcycle within its parent's step; slot = this level's
 // next CFL slot.
-int AdvanceLevel(fvs_amr* A, int l, double dt, int sub_first, std::vector<int>& slot) {
-  if (int rc = SetDt(A, l, dt)) return rc;
+int AdvanceLevel(fvs_amr* A, int l, int sub_first, std::vector<int>& slot) {
   if (int rc = FillGhosts(A, l, 0)) return rc;
   const bool finer = l + 1 < A->nlevels;
   if (finer) {
     fvs_amr::Level& F = A->L[l + 1];
     if (F.nreg) {
-      hipLaunchKernelGGL(FillKernel, dim3(Blocks(4 * int64_t(F.nreg))), dim3(256), 0,
-                         A->h->stream, F.regval, 4 * int64_t(F.nreg), 0.0);
+      hipLaunchKernelGGL(FillKernel, dim3(Blocks(4 * int64_t(F.nreg))), dim3(256), 0, A->s,
+                         F.regval, 4 * int64_t(F.nreg), 0.0);
       FVS_LAUNCH_CHECK();
     }
     for (int s = 0; s < A->ratio; ++s) {
-      if (int rc = AdvanceLevel(A, l + 1, dt / A->ratio, s, slot)) return rc;
+      if (int rc = AdvanceLevel(A, l + 1, s, slot)) return rc;
     }
   }
   // MakeSplitDirections: reversed order when (cycle % 2 + subcycle) % 2 == 1
@@ -439,7 +505,7 @@ int AdvanceLevel(fvs_amr* A, int l, double dt, int sub_first, std::vector<int>& 
     if (l > 0) {
       if (int rc = AddRegister(A, l, d, 1, src)) return rc;
     }
-    if (int rc = batch::Sweep(A->lv[l], A->h->stream, d, src, A->cfl + slot[l]++)) return rc;
+    if (int rc = batch::Sweep(A->lv[l], A->s, d, src, A->cfl + slot[l]++)) return rc;
     src = 1 - src;
   }
   if (finer) {
@@ -449,13 +515,27 @@ int AdvanceLevel(fvs_amr* A, int l, double dt, int sub_first, std::vector<int>& 
   return FVS_OK;
 }
 
+// One coarse step on stream A->s: the levels' time steps from *A->dtc, then
+// the recursive level steps.
+int EnqueueStep(fvs_amr* A) {
+  LevelDts d;
+  d.n = A->nlevels;
+  d.ratio = A->ratio;
+  for (int l = 0; l < A->nlevels; ++l) d.dst[l] = A->lv[l]->dts;
+  hipLaunchKernelGGL(LevelDtKernel, dim3(1), dim3(64), 0, A->s, d, A->dtc);
+  FVS_LAUNCH_CHECK();
+  std::vector<int> slot(A->nlevels);
+  for (int l = 0; l < A->nlevels; ++l) slot[l] = A->L[l].cfl_base;
+  return AdvanceLevel(A, 0, 0, slot);
+}
+
 } // namespace
 
 extern "C" {
 
 int fvs_amr_create(fvs_handle* h, int nlevels, fvs_batch* const* levels, const int32_t* lo,
                    const int64_t* domain, int ratio, fvs_amr** out) {
-  if (!h || nlevels < 1 || !levels || !lo || !domain || !out || ratio < 2) {
+  if (!h || nlevels < 1 || nlevels > 8 || !levels || !lo || !domain || !out || ratio < 2) {
     SetError("bad argument");
     return FVS_BAD_ARGUMENT;
   }
@@ -494,8 +574,13 @@ int fvs_amr_create(fvs_handle* h, int nlevels, fvs_batch* const* levels, const i
 int fvs_amr_destroy(fvs_amr* A) {
   if (!A) return FVS_OK;
   if (A->h) (void)hipStreamSynchronize(A->h->stream);
+  if (A->cap) (void)hipStreamSynchronize(A->cap);
+  for (hipGraphExec_t& e : A->exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (A->cap) (void)hipStreamDestroy(A->cap);
+  if (A->dtc) (void)hipFree(A->dtc);
   for (auto& LV : A->L) {
-    void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval};
+    void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval, LV.rcells, LV.rentries};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -509,6 +594,7 @@ int fvs_amr_fill_ghosts(fvs_amr* A, int level, int which) {
     SetError("bad argument");
     return FVS_BAD_ARGUMENT;
   }
+  A->s = A->h->stream;
   return FillGhosts(A, level, which);
 }
 
@@ -517,6 +603,7 @@ int fvs_amr_average_down(fvs_amr* A, int fine_level) {
     SetError("bad argument");
     return FVS_BAD_ARGUMENT;
   }
+  A->s = A->h->stream;
   return AverageDown(A, fine_level);
 }
 
@@ -525,10 +612,56 @@ int fvs_amr_advance(fvs_amr* A, double dt) {
     SetError("bad argument");
     return FVS_BAD_ARGUMENT;
   }
-  std::vector<int> slot(A->nlevels);
-  for (int l = 0; l < A->nlevels; ++l) slot[l] = A->L[l].cfl_base;
-  if (int rc = AdvanceLevel(A, 0, dt, 0, slot)) return rc;
+  hipStream_t hs = A->h->stream;
+  FVS_HIP_CHECK(hipMemcpyAsync(A->dtc, &dt, sizeof dt, hipMemcpyHostToDevice, hs));
+  if (!A->use_graphs) {
+    A->s = hs;
+    if (int rc = EnqueueStep(A)) return rc;
+  } else {
+    // a level whose tile lists changed (autotuning, a new shape) invalidates
+    // the captured steps, which hold its old lists
+    std::vector<int64_t> gens(A->nlevels);
+    for (int l = 0; l < A->nlevels; ++l) gens[l] = A->lv[l]->gen;
+    if (gens != A->gens) {
+      FVS_HIP_CHECK(hipStreamSynchronize(hs));
+      for (hipGraphExec_t& e : A->exec)
+        if (e) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+      A->gens = gens;
+    }
+    hipGraphExec_t& exec = A->exec[A->cycle % 2];
+    if (!exec) {
+      // capture on the hierarchy's own stream (the handle's may be the null
+      // stream, which cannot capture); nothing runs while capturing
+      A->s = A->cap;
+      FVS_HIP_CHECK(hipStreamBeginCapture(A->cap, hipStreamCaptureModeThreadLocal));
+      const int rc = EnqueueStep(A);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(A->cap, &g);
+      A->s = hs;
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      FVS_HIP_CHECK(e);
+      const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      FVS_HIP_CHECK(ie);
+    }
+    FVS_HIP_CHECK(hipGraphLaunch(exec, hs));
+  }
   A->cycle += 1;
+  return FVS_OK;
+}
+
+int fvs_amr_set_graphs(fvs_amr* A, int on) {
+  if (!A) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  A->use_graphs = on ? 1 : 0;
   return FVS_OK;
 }
 
@@ -557,12 +690,13 @@ int fvs_amr_stable_dt(fvs_amr* A, double* dt_host) {
     return FVS_BAD_ARGUMENT;
   }
   const int base = A->ncfl - 2 * A->nlevels;
+  A->s = A->h->stream;
   for (int l = 0; l < A->nlevels; ++l) {
     if (int rc = FillGhosts(A, l, 0)) return rc;
     if (int rc = SetDt(A, l, 0.0)) return rc;
     // sweeps into the scratch pool 1 for their CFL minima only
     for (int d = 0; d < 2; ++d)
-      if (int rc = batch::Sweep(A->lv[l], A->h->stream, d, 0, A->cfl + base + 2 * l + d)) return rc;
+      if (int rc = batch::Sweep(A->lv[l], A->s, d, 0, A->cfl + base + 2 * l + d)) return rc;
   }
   std::vector<double> v(2 * A->nlevels);
   FVS_HIP_CHECK(hipMemcpyAsync(v.data(), A->cfl + base, v.size() * sizeof(double),

@@ -366,6 +366,7 @@ int Upload(fvs_batch* b, int dir, int shape_idx) {
   FVS_HIP_CHECK(hipMemcpy(b->tiles[dir], tiles.data(), tiles.size() * sizeof(int4),
                           hipMemcpyHostToDevice));
   b->ntiles[dir] = static_cast<int>(tiles.size());
+  b->gen += 1;
   b->shape[dir] = shape_idx;
   // persistent grid: a few workgroups per CU, never more than tiles
   const unsigned want = static_cast<unsigned>(b->cus * 8);

@@ -89,6 +89,7 @@ struct fvs_batch {
   hipGraph_t graph;
   hipGraphExec_t exec;
   int cus;
+  int64_t gen;                  // bumped whenever the tile lists change
 };
 
 namespace fvs {

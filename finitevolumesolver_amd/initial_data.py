@@ -317,6 +317,22 @@ def amr_patch_layout(level: int, count: int = 300, base: int = 512, ratio: int =
     return out, dx
 
 
+def amr_nested_levels(base: int = 512, ratio: int = 2, count: int = 300, shrink: int = 4):
+    """Properly nested three-level layout of the 2D shock-bubble hierarchy:
+    level 0 one base^2 patch, level 1 the `count` seeded patches of
+    amr_patch_layout(1), level 2 every level-1 patch shrunk by `shrink`
+    level-1 cells per side and refined (kept when both sides stay >= 8), so
+    that each level-2 patch's ghost layers and the level-1 cells beside it
+    lie inside its parent.  Returns per level a list of (i0, j0, nx, ny)."""
+    l1, _ = amr_patch_layout(1, count, base, ratio)
+    l2 = []
+    for i0, j0, nx, ny in l1:
+        a, b = nx - 2 * shrink, ny - 2 * shrink
+        if a >= 8 and b >= 8:
+            l2.append(((i0 + shrink) * ratio, (j0 + shrink) * ratio, a * ratio, b * ratio))
+    return [[(0, 0, base, base)], l1, l2]
+
+
 def shock_bubble_state(x, y, mach: float = 1.5, x_shock: float = -0.5, r_bubble: float = 0.2,
                        rho_bubble: float = 0.138, gamma: float = 1.4):
     """Primitive state (rho, u, v, p) of the 2D shock-bubble problem at points

@@ -523,7 +523,7 @@ double* fvs_batch_cfl(fvs_batch* batch);
 int fvs_batch_read_cfl(fvs_batch* batch, double* host2);
 
 /* ---- AMR hierarchy over batched levels (2D) -------------------------------
- * Levels 0..nlevels-1 are batches of one handle; level l has dx / ratio^l
+ * Levels 0..nlevels-1 (at most 8) are batches of one handle; level l has dx / ratio^l
  * and a domain of domain[0] x domain[1] cells times ratio^l.  lo: host, the
  * lower corner (i0, j0) of every patch of every level, level 0 first, in
  * the level's cell indices.  Level 0 must cover the domain; patches of
@@ -563,6 +563,11 @@ int fvs_amr_level_cfl(fvs_amr* amr, double* host);
  * (SubcycleFineFirstSolver::ComputeStableDt, :101-112), without the CFL
  * factor.  Uses pool 1 of each level as scratch; synchronises. */
 int fvs_amr_stable_dt(fvs_amr* amr, double* dt_host);
+/* Capture each coarse step (per cycle parity) once into a HIP graph and
+ * replay it (on by default; results are identical either way).  The step's
+ * dt is read from device memory, so one graph serves every dt; retuning a
+ * level's tiles triggers a new capture.  No counterpart in the reference. */
+int fvs_amr_set_graphs(fvs_amr* amr, int on);
 
 #ifdef __cplusplus
 }
