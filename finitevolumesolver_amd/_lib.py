@@ -24,6 +24,7 @@ BASE_GODUNOV_EXACT = 2
 BASE_HLL_EINFELDT = 3
 RECON_NONE = 0
 RECON_MUSCL_HANCOCK = 1
+RECON_MUSCL_HANCOCK_PRIM = 2
 LIMITER_MINMOD = 0
 LIMITER_VANLEER = 1
 LIMITER_NONE = 2

@@ -74,11 +74,16 @@ int CheckDesc(const fvs_method_desc* d) {
     return FVS_BAD_ARGUMENT;
   }
   if (d->base < 0 || d->base > FVS_BASE_HLL_EINFELDT || d->reconstruction < 0 ||
-      d->reconstruction > FVS_RECON_MUSCL_HANCOCK || d->limiter < 0 ||
+      d->reconstruction > FVS_RECON_MUSCL_HANCOCK_PRIM || d->limiter < 0 ||
       d->limiter > FVS_LIMITER_NO_GRADIENT || d->precision < 0 ||
       d->precision > FVS_FP64_RCP || d->boundary_solver < FVS_BOUNDARY_EXACT ||
       d->boundary_solver > FVS_BOUNDARY_HLLEM) {
     SetError("method enum out of range");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (d->reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM &&
+      (d->base != FVS_BASE_HLLEM_LARROUTUROU || d->precision != FVS_FP64)) {
+    SetError("MusclHancockPrim takes the Larrouturou HLLEM base in FP64");
     return FVS_BAD_ARGUMENT;
   }
   return FVS_OK;
@@ -115,14 +120,22 @@ __device__ __forceinline__ void LoadComplete(const DevView& v, const int64_t* id
 template <int D, int DIR, int BASE>
 __global__ __launch_bounds__(kBlock) void ViewFluxKernel(DevView fl, DevView st, double dt,
                                                          double dx, Gas<double> g,
-                                                         int limiter, int muscl, int quirk) {
+                                                         int limiter, int recon, int quirk) {
   constexpr int NC = D + 2, NQ = D + 4;
   const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (q >= fl.count()) return;
   int64_t f[3];
   fl.index(q, f);
   double L[NQ], R[NQ];
-  if (muscl) {
+  if (recon == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+    double u[4][NQ];
+    for (int s = 0; s < 4; ++s) {
+      int64_t c[3] = {f[0], f[1], f[2]};
+      c[DIR] += s - 2;
+      LoadComplete<D>(st, c, u[s]);
+    }
+    MusclHancockPrimStates<D, DIR>(u, DivRn(dt, dx), g, L, R);
+  } else if (recon == FVS_RECON_MUSCL_HANCOCK) {
     double u[4][NQ];
     for (int s = 0; s < 4; ++s) {
       int64_t c[3] = {f[0], f[1], f[2]};
@@ -371,7 +384,7 @@ int fvs_stable_dt(fvs_handle* h, const fvs_view* states, double dx, int dir,
   }
   DevView st;
   if (int rc = MakeDevView(states, D + 4, &st)) return rc;
-  const int S = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK ? 4 : 2;
+  const int S = h->desc.reconstruction != FVS_RECON_NONE ? 4 : 2;
   DevView faces = st;
   faces.ext[dir] -= (S - 1);
   if (faces.ext[dir] <= 0) {
@@ -415,8 +428,8 @@ int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* st
   DevView fl, st;
   if (int rc = MakeDevView(fluxes, D + 2, &fl)) return rc;
   if (int rc = MakeDevView(states, D + 4, &st)) return rc;
-  const int muscl = h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK;
-  const int w = muscl ? 2 : 1;
+  const int recon = h->desc.reconstruction;
+  const int w = recon != FVS_RECON_NONE ? 2 : 1;
   int64_t need_lo[3], need_hi[3];
   for (int d = 0; d < 3; ++d) {
     need_lo[d] = fluxes->lo[d];
@@ -439,7 +452,7 @@ int fvs_numeric_fluxes(fvs_handle* h, const fvs_view* fluxes, const fvs_view* st
   const int lim = h->desc.limiter, q = h->desc.quirks;
 #define FVS_VF1(DD, DIR_, B)                                                             \
   hipLaunchKernelGGL((ViewFluxKernel<DD, DIR_, B>), dim3(nb), dim3(kBlock), 0, s, fl, st, \
-                     dt, dx, g, lim, muscl, q)
+                     dt, dx, g, lim, recon, q)
 #define FVS_VF(DD, DIR_)                                                    \
   do {                                                                      \
     if (base == FVS_BASE_HLLEM_LARROUTUROU)                                 \
@@ -575,6 +588,10 @@ int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* 
     return FVS_BAD_ARGUMENT;
   }
   if (int rc = CheckDir(h, dir)) return rc;
+  if (h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+    SetError("MusclHancockPrim runs through the patch operators only");
+    return FVS_BAD_ARGUMENT;
+  }
   SweepArgs a;
   if (int rc = FillSweepArgs(h, cons_in, cons_out, n, dx, dir, &a)) return rc;
   for (int c = 0; c < h->desc.dim + 2; ++c) {
@@ -630,8 +647,9 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t 
     }
     return FVS_OK;
   }
-  if (h->desc.base == FVS_BASE_HLL_EINFELDT) {
-    SetError("fvs_patch_stable_dt has no HLL kernel (use fvs_stable_dt on views)");
+  if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
+      h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+    SetError("fvs_patch_stable_dt has no HLL or MusclHancockPrim kernel (use fvs_stable_dt on views)");
     return FVS_BAD_ARGUMENT;
   }
   if (int rc = EnsurePartials(h, D * CflBlocks(D, nn))) return rc;
@@ -808,8 +826,9 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
     SetError("the fused level driver runs exact Godunov at first order");
     return FVS_BAD_ARGUMENT;
   }
-  if (h->desc.base == FVS_BASE_HLL_EINFELDT) {
-    SetError("the fused level driver has no HLL kernel (patch operators only)");
+  if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
+      h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+    SetError("the fused level driver has no HLL or MusclHancockPrim kernel (patch operators only)");
     return FVS_BAD_ARGUMENT;
   }
   int64_t n[3] = {patch->n[0], patch->n[1], D == 3 ? patch->n[2] : 1};

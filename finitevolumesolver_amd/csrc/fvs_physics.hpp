@@ -234,6 +234,70 @@ __device__ __forceinline__ T LimitedSlope(T qL, T qM, T qR) {
   }
 }
 
+// Slope of MusclHancockPrim (src/equations/perfect_gas/MusclHancockPrimMethod.hpp:68-81,
+// the limiter lambda of the array path): sigma(r) * (qR - qL) / 2 with r the
+// smaller positive one-sided ratio, sigma = 2r / (1 + r) below 1 and
+// (r + 1) / 2 above; differences below 1e-12 give a zero slope.  FP64, IEEE.
+__device__ __forceinline__ double PrimSlope(double qL, double qM, double qR) {
+  const double dq = 0.5 * (qR - qL);
+  const bool relevant = __builtin_fabs(dq) > 1e-12;
+  const double den = relevant ? dq : 1.0;
+  const double rL = relevant ? DivRn(qM - qL, den) : 0.0;
+  const double rR = relevant ? DivRn(qR - qM, den) : 0.0;
+  const double r = (rL > 0.0 && rR > 0.0) ? rmin(rL, rR) : 0.0;
+  const double sigma = r < 1.0 ? DivRn(2.0 * r, 1.0 + r) : 0.5 * (r + 1.0);
+  return sigma * dq;
+}
+
+// MusclHancockPrim::ComputeNumericFlux, array path (:83-176): primitive
+// slopes of (rho, u, p) in cells 1 and 2 of the complete stencil u[0..3],
+// the characteristic half-step evolution of the primitives with
+// lambda = dt / dx, and the complete states L (cell 1, at the face) and R
+// (cell 2) that the HLLEM base takes.  FP64, IEEE, in the reference's order.
+template <int D, int DIR>
+__device__ __forceinline__ void MusclHancockPrimStates(const double (*u)[D + 4], double lambda,
+                                                       const Gas<double>& g, double* L,
+                                                       double* R) {
+  double rho[4], p[4], v[4][D];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    rho[s] = u[s][0];
+    p[s] = u[s][D + 2];
+#pragma unroll
+    for (int i = 0; i < D; ++i) v[s][i] = DivRn(u[s][1 + i], rho[s]);
+  }
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int c = 1 + side;  // the cell whose face state is built
+    const double a = u[c][D + 3];
+    const double drho = PrimSlope(rho[c - 1], rho[c], rho[c + 1]);
+    const double dp = PrimSlope(p[c - 1], p[c], p[c + 1]);
+    double du[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) du[i] = PrimSlope(v[c - 1][i], v[c][i], v[c + 1][i]);
+    double rr, rp, ru[D];
+    if (side == 0) {
+      rr = rho[c] + 0.5 * (drho - lambda * (v[c][DIR] * drho + rho[c] * du[DIR]));
+      rp = p[c] + 0.5 * (dp - lambda * (rho[c] * a * a * du[DIR] + v[c][DIR] * dp));
+#pragma unroll
+      for (int i = 0; i < D; ++i) ru[i] = v[c][i] + 0.5 * (du[i] - lambda * (v[c][DIR] * du[i]));
+    } else {
+      rr = rho[c] - 0.5 * (drho + lambda * (v[c][DIR] * drho + rho[c] * du[DIR]));
+      rp = p[c] - 0.5 * (dp + lambda * (rho[c] * a * a * du[DIR] + v[c][DIR] * dp));
+#pragma unroll
+      for (int i = 0; i < D; ++i) ru[i] = v[c][i] - 0.5 * (du[i] + lambda * (v[c][DIR] * du[i]));
+    }
+    ru[DIR] = ru[DIR] - DivRn(0.5 * lambda * dp, rho[c]);
+    double* w = side == 0 ? L : R;
+    w[0] = rr;
+#pragma unroll
+    for (int i = 0; i < D; ++i) w[1 + i] = rr * ru[i];
+    w[D + 2] = rp;
+    w[D + 3] = __builtin_sqrt(DivRn(g.gamma * rp, rr));
+    w[D + 1] = 0.5 * rr * SqNormArray<D>(ru) + rp * g.gm1inv;
+  }
+}
+
 // MUSCL-Hancock reconstruction of one cell from its conservative neighbours
 // um (left), u0 (centre), up (right).  Produces the complete half-step
 // states at the cell's left face (recL = rec_[1] of that face) and right

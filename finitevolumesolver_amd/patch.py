@@ -129,8 +129,12 @@ def make_desc(equation: PerfectGas, base="HLLEM", reconstruction="MusclHancock",
     d.dim = equation.dim
     d.gamma = equation.gamma
     d.base = _BASES[base] if isinstance(base, str) else int(base)
-    d.reconstruction = (L.RECON_MUSCL_HANCOCK if reconstruction in ("MusclHancock", 1)
-                        else L.RECON_NONE)
+    if reconstruction in ("MusclHancock", 1):
+        d.reconstruction = L.RECON_MUSCL_HANCOCK
+    elif reconstruction in ("MusclHancockPrim", 2):
+        d.reconstruction = L.RECON_MUSCL_HANCOCK_PRIM
+    else:
+        d.reconstruction = L.RECON_NONE
     d.limiter = _LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
     if precision in ("fp32", "fp32_flux", 1):
         d.precision = L.FP32_FLUX
@@ -164,7 +168,7 @@ class FluxMethod:
         return self._h.h
 
     def GetStencilWidth(self) -> int:
-        return 2 if self.reconstruction in ("MusclHancock", 1) else 1
+        return 2 if self.reconstruction in ("MusclHancock", 1, "MusclHancockPrim", 2) else 1
 
     def GetStencilSize(self) -> int:
         return 2 * self.GetStencilWidth()

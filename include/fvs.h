@@ -62,8 +62,18 @@ enum fvs_base_method {
 /* Reconstruction wrapped around the base flux.
  * FVS_RECON_NONE: FluxMethod<Base> (stencil width 1).
  * FVS_RECON_MUSCL_HANCOCK: MusclHancockMethod<Eq, Base, Limiter>
- *   (include/fub/flux_method/MusclHancockMethod.hpp:277-571, stencil width 2). */
-enum fvs_reconstruction { FVS_RECON_NONE = 0, FVS_RECON_MUSCL_HANCOCK = 1 };
+ *   (include/fub/flux_method/MusclHancockMethod.hpp:277-571, stencil width 2).
+ * FVS_RECON_MUSCL_HANCOCK_PRIM: perfect_gas::MusclHancockPrim<Dim>
+ *   (src/equations/perfect_gas/MusclHancockPrimMethod.hpp:59-176, stencil
+ *   width 2): limited primitive slopes, characteristic half step, HLLEM
+ *   (Larrouturou) on the two face states; its own limiter (the limiter field
+ *   is ignored).  Requires FVS_BASE_HLLEM_LARROUTUROU and FVS_FP64; patch
+ *   operators (fvs_numeric_fluxes, fvs_stable_dt) only. */
+enum fvs_reconstruction {
+  FVS_RECON_NONE = 0,
+  FVS_RECON_MUSCL_HANCOCK = 1,
+  FVS_RECON_MUSCL_HANCOCK_PRIM = 2
+};
 
 /* Slope limiters, include/fub/flux_method/MusclHancockMethod.hpp:31-187. */
 enum fvs_limiter {

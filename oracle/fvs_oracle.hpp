@@ -360,6 +360,76 @@ inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
 // form at :25-62 performs the same operations in the same order).  Roe
 // averages of the normal velocity and of the enthalpy only; the Roe sound
 // speed uses the normal velocity alone.
+// MusclHancockPrim<Dim>::ComputeNumericFlux, array path
+// (src/equations/perfect_gas/MusclHancockPrimMethod.hpp:62-176): the
+// limiter lambda (:68-81) on rho, u and p of cells 1 and 2 of the stencil,
+// the half-step evolution of the primitives (:114-141), the complete face
+// states (:148-168); the caller applies Hllem<PerfectGas<Dim>, true>.
+inline double PrimLimiter(double qL, double qM, double qR) {
+  const double delta_q = 0.5 * (qR - qL);
+  const bool is_relevant = std::fabs(delta_q) > 1e-12;
+  const double delta_q_if_relevant = is_relevant ? delta_q : 1.0;
+  const double rL = is_relevant ? (qM - qL) / delta_q_if_relevant : 0.0;
+  const double rR = is_relevant ? (qR - qM) / delta_q_if_relevant : 0.0;
+  const double r = (rL > 0.0 && rR > 0.0) ? std::min(rL, rR) : 0.0;
+  const double r_1 = 2 * r / (1.0 + r);
+  const double r_2 = 0.5 * (r + 1.0);
+  const double sigma = r < 1.0 ? r_1 : r_2;
+  return sigma * delta_q;
+}
+
+template <int D>
+inline void MusclHancockPrimStates(const Gas& g, const State<D>* st, double dt, double dx,
+                                   int d, State<D> w[2]) {
+  double rho[4], p[4], u[4][D];
+  for (int s = 0; s < 4; ++s) {
+    rho[s] = st[s].rho;
+    p[s] = st[s].p;
+    for (int i = 0; i < D; ++i) u[s][i] = st[s].mom[i] / st[s].rho;
+  }
+  const double lambda = dt / dx;
+  // left face state from cell 1
+  const double a_l = st[1].a;
+  const double drho_l = PrimLimiter(rho[0], rho[1], rho[2]);
+  const double dp_l = PrimLimiter(p[0], p[1], p[2]);
+  double du_l[D];
+  for (int i = 0; i < D; ++i) du_l[i] = PrimLimiter(u[0][i], u[1][i], u[2][i]);
+  // right face state from cell 2
+  const double a_r = st[2].a;
+  const double drho_r = PrimLimiter(rho[1], rho[2], rho[3]);
+  const double dp_r = PrimLimiter(p[1], p[2], p[3]);
+  double du_r[D];
+  for (int i = 0; i < D; ++i) du_r[i] = PrimLimiter(u[1][i], u[2][i], u[3][i]);
+
+  const double rec_rho_l =
+      rho[1] + 0.5 * (drho_l - lambda * (u[1][d] * drho_l + rho[1] * du_l[d]));
+  const double rec_p_l =
+      p[1] + 0.5 * (dp_l - lambda * (rho[1] * a_l * a_l * du_l[d] + u[1][d] * dp_l));
+  double rec_u_l[D];
+  for (int i = 0; i < D; ++i)
+    rec_u_l[i] = u[1][i] + 0.5 * (du_l[i] - lambda * (u[1][d] * du_l[i]));
+  rec_u_l[d] -= 0.5 * lambda * dp_l / rho[1];
+
+  const double rec_rho_r =
+      rho[2] - 0.5 * (drho_r + lambda * (u[2][d] * drho_r + rho[2] * du_r[d]));
+  const double rec_p_r =
+      p[2] - 0.5 * (dp_r + lambda * (rho[2] * a_r * a_r * du_r[d] + u[2][d] * dp_r));
+  double rec_u_r[D];
+  for (int i = 0; i < D; ++i)
+    rec_u_r[i] = u[2][i] - 0.5 * (du_r[i] + lambda * (u[2][d] * du_r[i]));
+  rec_u_r[d] -= 0.5 * lambda * dp_r / rho[2];
+
+  const double rr[2] = {rec_rho_l, rec_rho_r}, rp[2] = {rec_p_l, rec_p_r};
+  const double* ru[2] = {rec_u_l, rec_u_r};
+  for (int s = 0; s < 2; ++s) {
+    w[s].rho = rr[s];
+    for (int i = 0; i < D; ++i) w[s].mom[i] = rr[s] * ru[s][i];
+    w[s].p = rp[s];
+    w[s].a = std::sqrt(g.gamma * rp[s] / rr[s]);
+    w[s].E = 0.5 * rr[s] * SquaredNormArray<D>(ru[s]) + rp[s] * g.gm1inv;
+  }
+}
+
 template <int D>
 inline void EinfeldtSignals(const Gas& g, const State<D>& L, const State<D>& R, int dir,
                             double s[2]) {

@@ -37,7 +37,7 @@ struct Method {
         tol(d->newton_tolerance > 0 ? d->newton_tolerance : 1e-7),
         max_it(d->max_newton_iterations > 0 ? d->max_newton_iterations : 5000),
         seq(d->base == FVS_BASE_GODUNOV_EXACT) {}
-  int width() const { return recon == FVS_RECON_MUSCL_HANCOCK ? 2 : 1; }
+  int width() const { return recon != FVS_RECON_NONE ? 2 : 1; }
 };
 
 // Base two-state flux (Hllem or Hll array path, or Godunov scalar path).
@@ -109,7 +109,11 @@ void MusclHancockFlux(const Method& m, Cons<D>& flux, const State<D> st[4],
 template <int D>
 void FaceFlux(const Method& m, Cons<D>& flux, const State<D>* st, double dt,
               double dx, int dir, int* newton_fail) {
-  if (m.recon == FVS_RECON_MUSCL_HANCOCK) {
+  if (m.recon == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+    State<D> w[2];
+    MusclHancockPrimStates<D>(m.gas, st, dt, dx, dir, w);
+    HllemFlux<D>(m.gas, flux, w[0], w[1], dir, true, m.quirk3d);
+  } else if (m.recon == FVS_RECON_MUSCL_HANCOCK) {
     MusclHancockFlux<D>(m, flux, st, dt, dx, dir, newton_fail);
   } else {
     BaseFlux<D>(m, flux, st[0], st[1], dir, newton_fail);
@@ -119,7 +123,9 @@ void FaceFlux(const Method& m, Cons<D>& flux, const State<D>* st, double dt,
 template <int D>
 double FaceStableDt(const Method& m, const State<D>* st, double dx, int dir) {
   // MusclHancock::ComputeStableDt uses the middle two cells (MusclHancockMethod.hpp:214-218).
-  if (m.recon == FVS_RECON_MUSCL_HANCOCK) {
+  // MusclHancockPrim::ComputeStableDt: Hllem on the middle two
+  // (src/equations/perfect_gas/MusclHancockPrimMethod.hpp:40-46).
+  if (m.recon != FVS_RECON_NONE) {
     return BaseStableDt<D>(m, st[1], st[2], dx, dir);
   }
   return BaseStableDt<D>(m, st[0], st[1], dx, dir);
