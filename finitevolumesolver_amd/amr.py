@@ -88,6 +88,22 @@ class AmrHierarchy:
     def average_down(self, fine_level: int) -> None:
         L.check(self._lib.fvs_amr_average_down(self.a, fine_level))
 
+    def tag_cells(self, level: int, tol_rho: float, tol_p: float) -> list:
+        """Refinement tags (ScalarGradientDetector with the density and
+        pressure projections) of every patch of a level: (ny, nx) uint8
+        arrays."""
+        sizes = self.levels[level].sizes
+        n = sum(nx * ny for nx, ny in sizes)
+        dev = torch.zeros(max(n, 1), dtype=torch.uint8, device=self.levels[level].device)
+        L.check(self._lib.fvs_amr_tag_cells(self.a, level, float(tol_rho), float(tol_p),
+                                            dev.data_ptr()))
+        host = dev.cpu().numpy()
+        out, o = [], 0
+        for nx, ny in sizes:
+            out.append(host[o:o + nx * ny].reshape(ny, nx).copy())
+            o += nx * ny
+        return out
+
     def set_graphs(self, on: bool = True) -> None:
         """Replay each coarse step from a captured HIP graph (default) or
         launch it eagerly."""
@@ -121,3 +137,48 @@ class AmrHierarchy:
                                        f"{dt / self.ratio ** l:.6e}")
             dts.append(dt)
         return dts
+
+
+def tile_layout(domain, layout, tags, level: int, tile: int = 8, ratio: int = 2,
+                margin: int = 1, max_run: int = 8) -> list:
+    """Patches of level + 1 from the tags of a level's patches, on a fixed
+    tile lattice (the GPU analogue of clustering the tagged cells into
+    boxes): a tile (tile x tile cells of the level, aligned to multiples of
+    `tile`) is refined when it holds a tagged cell and, grown by `margin`
+    cells and clamped to the domain, lies inside the level's patches (proper
+    nesting: the finer patches' ghost layers and the coarse cells beside them
+    then have coarse data).  Tiles kept in one tile row merge into runs of
+    at most `max_run` tiles.  Returns (i0, j0, nx, ny) in level + 1 indices."""
+    nx_l = domain[0] * ratio ** level
+    ny_l = domain[1] * ratio ** level
+    owner = np.zeros((ny_l, nx_l), dtype=bool)
+    tagged = np.zeros((ny_l, nx_l), dtype=bool)
+    for (i0, j0, nx, ny), t in zip(layout, tags):
+        owner[j0:j0 + ny, i0:i0 + nx] = True
+        tagged[j0:j0 + ny, i0:i0 + nx] |= t.astype(bool)
+    tx, ty = -(-nx_l // tile), -(-ny_l // tile)
+    keep = np.zeros((ty, tx), dtype=bool)
+    for b in range(ty):
+        for a in range(tx):
+            x0, y0 = a * tile, b * tile
+            if not tagged[y0:y0 + tile, x0:x0 + tile].any():
+                continue
+            gx0, gy0 = max(x0 - margin, 0), max(y0 - margin, 0)
+            gx1, gy1 = min(x0 + tile + margin, nx_l), min(y0 + tile + margin, ny_l)
+            if x0 + tile > nx_l or y0 + tile > ny_l:
+                continue
+            keep[b, a] = owner[gy0:gy1, gx0:gx1].all()
+    out = []
+    f = tile * ratio
+    for b in range(ty):
+        a = 0
+        while a < tx:
+            if not keep[b, a]:
+                a += 1
+                continue
+            run = 1
+            while a + run < tx and keep[b, a + run] and run < max_run:
+                run += 1
+            out.append((a * f, b * f, run * f, f))
+            a += run
+    return out

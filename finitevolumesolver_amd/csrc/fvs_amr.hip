@@ -138,6 +138,53 @@ __global__ void FillKernel(double* p, int64_t n, double v) {
 
 unsigned Blocks(int64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
+// Density and pressure of a conservative cell, pressure through the array
+// path's CompleteFromCons.
+__device__ __forceinline__ void RhoP(const double* pool, int64_t at, int64_t cs,
+                                     const Gas<double>& g, double* rho, double* p) {
+  double q[6];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = pool[at + k * cs];
+  (void)CompleteFromCons<2, double>(q, g);
+  *rho = q[0];
+  *p = q[4];
+}
+
+// ScalarGradientDetector::TagCellsForRefinement
+// (include/fub/tagging_method/GradientDetector.hpp:150-194) with the
+// projections density and pressure: a cell is tagged when, in either
+// direction, a neighbour differs and |x_M - x_N| / (|x_M| + |x_N|) exceeds
+// the projection's tolerance.  One thread per interior cell (blockIdx.y =
+// patch); tags (one byte per interior cell, patches in order, x fastest)
+// are overwritten.
+__device__ __forceinline__ bool Jump(double xL, double xM, double xR, double tol) {
+  if (!(xM != xR || xM != xL)) return false;
+  const double left = DivRn(__builtin_fabs(xM - xL), __builtin_fabs(xM) + __builtin_fabs(xL));
+  const double right = DivRn(__builtin_fabs(xM - xR), __builtin_fabs(xM) + __builtin_fabs(xR));
+  return left > tol || right > tol;
+}
+
+__global__ void TagKernel(const PatchInfo* patches, const int64_t* tag_off, const double* pool,
+                          Gas<double> g, double tol_rho, double tol_p, uint8_t* tags) {
+  const PatchInfo P = patches[blockIdx.y];
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= int64_t(P.nx) * P.ny) return;
+  const int64_t i = q % P.nx, j = q / P.nx;
+  const int64_t ldx = P.nx + 2 * kG, cs = ldx * (P.ny + 2 * kG);
+  const int64_t at = P.offset + (j + kG) * ldx + (i + kG);
+  double r[5], p[5];  // centre, x - 1, x + 1, y - 1, y + 1
+  const int64_t nb[5] = {at, at - 1, at + 1, at - ldx, at + ldx};
+#pragma unroll
+  for (int s = 0; s < 5; ++s) RhoP(pool, nb[s], cs, g, &r[s], &p[s]);
+  bool tag = false;
+  // directions in order, density before pressure (the condition order)
+  tag |= Jump(r[1], r[0], r[2], tol_rho);
+  tag |= Jump(p[1], p[0], p[2], tol_p);
+  tag |= Jump(r[3], r[0], r[4], tol_rho);
+  tag |= Jump(p[3], p[0], p[4], tol_p);
+  tags[tag_off[blockIdx.y] + q] = tag ? 1 : 0;
+}
+
 } // namespace
 
 struct fvs_amr {
@@ -161,6 +208,9 @@ struct fvs_amr {
     int nrcells = 0;
     int32_t* rentries = nullptr;     // their entries, per cell in group order
     int cfl_base = 0;                // first CFL slot of this level's sweeps
+    int64_t* tag_off = nullptr;      // first tag of each patch
+    int64_t ntags = 0;
+    int64_t max_cells = 0;
   };
   std::vector<Level> L;
   double* cfl = nullptr;  // one slot per sweep of a coarse step, then 2 per level
@@ -386,6 +436,13 @@ int Build(fvs_amr* A, const int32_t* lo) {
         k = m;
       }
     }
+    std::vector<int64_t> toff(b->count);
+    for (int64_t p = 0; p < b->count; ++p) {
+      toff[p] = LV.ntags;
+      LV.ntags += int64_t(hp[p].nx) * hp[p].ny;
+      LV.max_cells = std::max<int64_t>(LV.max_cells, int64_t(hp[p].nx) * hp[p].ny);
+    }
+    if (int rc2 = UploadVec(toff, &LV.tag_off)) return rc2;
     LV.nrcells = static_cast<int>(rc.size());
     if (int rc2 = UploadVec(rc, &LV.rcells)) return rc2;
     if (int rc2 = UploadVec(rent, &LV.rentries)) return rc2;
@@ -580,7 +637,8 @@ int fvs_amr_destroy(fvs_amr* A) {
   if (A->cap) (void)hipStreamDestroy(A->cap);
   if (A->dtc) (void)hipFree(A->dtc);
   for (auto& LV : A->L) {
-    void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval, LV.rcells, LV.rentries};
+    void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval, LV.rcells, LV.rentries,
+                  LV.tag_off};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -653,6 +711,30 @@ int fvs_amr_advance(fvs_amr* A, double dt) {
     FVS_HIP_CHECK(hipGraphLaunch(exec, hs));
   }
   A->cycle += 1;
+  return FVS_OK;
+}
+
+int fvs_amr_tag_cells(fvs_amr* A, int level, double tol_rho, double tol_p, uint8_t* tags) {
+  if (!A || level < 0 || level >= A->nlevels || !tags || !(tol_rho >= 0.0) || !(tol_p >= 0.0)) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  A->s = A->h->stream;
+  if (int rc = FillGhosts(A, level, 0)) return rc;
+  fvs_batch* b = A->lv[level];
+  const fvs_amr::Level& LV = A->L[level];
+  if (b->count > 65535) {
+    SetError("tagging: more than 65535 patches in a level");
+    return FVS_BAD_ARGUMENT;
+  }
+  Gas<double> g;
+  g.gamma = A->h->desc.gamma;
+  g.gm1inv = 1.0 / (A->h->desc.gamma - 1.0);
+  g.gm1 = A->h->desc.gamma - 1.0;
+  hipLaunchKernelGGL(TagKernel, dim3(Blocks(LV.max_cells), static_cast<unsigned>(b->count)),
+                     dim3(256), 0, A->s, b->patches, LV.tag_off, b->pool[0], g, tol_rho, tol_p,
+                     tags);
+  FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
 

@@ -578,6 +578,13 @@ int fvs_amr_stable_dt(fvs_amr* amr, double* dt_host);
  * dt is read from device memory, so one graph serves every dt; retuning a
  * level's tiles triggers a new capture.  No counterpart in the reference. */
 int fvs_amr_set_graphs(fvs_amr* amr, int on);
+/* Refinement tags of a level's current state (pool 0, ghosts refilled
+ * first): ScalarGradientDetector::TagCellsForRefinement
+ * (include/fub/tagging_method/GradientDetector.hpp:150-194) with the
+ * density and pressure projections and their tolerances.  tags: device,
+ * one byte per interior cell (1 tagged), patches in level order, x fastest
+ * within a patch (sum of nx * ny bytes). */
+int fvs_amr_tag_cells(fvs_amr* amr, int level, double tol_rho, double tol_p, uint8_t* tags);
 
 #ifdef __cplusplus
 }

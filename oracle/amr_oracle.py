@@ -221,3 +221,28 @@ class AmrOracle:
         """Sum over level 0 of U * cell area (the composite integral after
         average_down)."""
         return sum(u[:, G:-G, G:-G].sum(axis=(1, 2)) for u in self.U[0]) * self.dx[0] ** 2
+
+
+def gradient_tags(rho, p, tol_rho, tol_p):
+    """ScalarGradientDetector::TagCellsForRefinement
+    (include/fub/tagging_method/GradientDetector.hpp:150-194) with the
+    density and pressure projections, on one patch: rho and p with two
+    ghost layers (ny + 4, nx + 4); returns (ny, nx) tags of the interior.
+    A cell is tagged when along x or y a neighbour differs and
+    |x_M - x_N| / (|x_M| + |x_N|) exceeds the tolerance on either side."""
+    ny, nx = rho.shape[0] - 2 * G, rho.shape[1] - 2 * G
+    tags = np.zeros((ny, nx), dtype=np.uint8)
+    for j in range(ny):
+        for i in range(nx):
+            jj, ii = j + G, i + G
+            t = False
+            for (dj, di) in ((0, 1), (1, 0)):
+                for x, tol in ((rho, tol_rho), (p, tol_p)):
+                    xL, xM, xR = x[jj - dj, ii - di], x[jj, ii], x[jj + dj, ii + di]
+                    if xM != xR or xM != xL:
+                        with np.errstate(invalid="ignore", divide="ignore"):
+                            left = abs(xM - xL) / (abs(xM) + abs(xL))
+                            right = abs(xM - xR) / (abs(xM) + abs(xR))
+                        t = t or left > tol or right > tol
+            tags[j, i] = 1 if t else 0
+    return tags

@@ -101,3 +101,30 @@ def test_amr_rejects_bad_layouts(cuda):
     for lay in bad:
         with pytest.raises(L.FvsError):
             AmrHierarchy(fm, A.DOMAIN, lay, A.DX0, 2)
+
+
+def test_amr_tags_match_oracle_and_regrid_nests(cuda):
+    """GPU gradient tags of level 1 equal the oracle's on the same state
+    (ghosts filled), and the tile regrid built from them is a layout the
+    hierarchy accepts as its level 2."""
+    import oracle_lib as O
+    from amr_oracle import gradient_tags
+    from finitevolumesolver_amd.amr import tile_layout
+    _note("amr tagging: moving state, level 1")
+    fm, h, o = _pair(A.moving_state)
+    tags = h.tag_cells(1, 0.004, 0.004)
+    b = h.levels[1]
+    pool = b.download(0).cpu()
+    total = 0
+    for p, (nx, ny) in enumerate(b.sizes):
+        cons = b.patch_view(pool, p).numpy()
+        comp = O.complete_from_cons(np.ascontiguousarray(cons).reshape(4, -1), 2)
+        comp = comp.reshape(6, ny + 4, nx + 4)
+        ref = gradient_tags(comp[0], comp[4], 0.004, 0.004)
+        np.testing.assert_array_equal(tags[p], ref, err_msg=f"patch {p}")
+        total += int(ref.sum())
+    assert total > 0
+    new2 = tile_layout(A.DOMAIN, A.LAYOUTS[1], tags, level=1, tile=4, ratio=2, margin=1)
+    _note(f"  {total} tagged cells, {len(new2)} level-2 patches from 4x4 tiles")
+    assert new2
+    AmrHierarchy(fm, A.DOMAIN, [A.LAYOUTS[0], A.LAYOUTS[1], new2], A.DX0, 2)
