@@ -184,6 +184,7 @@ SIGNATURES = {
     "fvs_amr_stable_dt": (_I, [_P, _PD]),
     "fvs_amr_set_graphs": (_I, [_P, _I]),
     "fvs_amr_tag_cells": (_I, [_P, _I, _D, _D, _P]),
+    "fvs_amr_transfer": (_I, [_P, _P]),
 }
 
 _lib = None

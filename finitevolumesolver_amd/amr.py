@@ -104,6 +104,27 @@ class AmrHierarchy:
             o += nx * ny
         return out
 
+    def transfer_from(self, old: "AmrHierarchy") -> None:
+        """Moves old's state onto this hierarchy's layout (same domain,
+        ratio and number of levels): old same-level cells where they exist,
+        else this hierarchy's coarser cell above (piecewise constant)."""
+        L.check(self._lib.fvs_amr_transfer(self.a, old.a))
+
+    def regrid_finest(self, tol_rho: float, tol_p: float, tile: int = 8,
+                      margin: int = 1) -> "AmrHierarchy":
+        """A new hierarchy whose finest level is rebuilt from the gradient
+        tags of the level below it (tile_layout), with the state moved onto
+        it.  The other levels keep their patches."""
+        lv = self.nlevels - 2
+        tags = self.tag_cells(lv, tol_rho, tol_p)
+        fine = tile_layout(self.domain, self.layouts[lv], tags, lv, tile, self.ratio, margin)
+        if not fine:
+            raise ValueError("no tagged tile nests inside the level below; nothing to refine")
+        new = AmrHierarchy(self.flux_method, self.domain, self.layouts[:-1] + [fine],
+                           self.dx[0], self.ratio)
+        new.transfer_from(self)
+        return new
+
     def set_graphs(self, on: bool = True) -> None:
         """Replay each coarse step from a captured HIP graph (default) or
         launch it eagerly."""

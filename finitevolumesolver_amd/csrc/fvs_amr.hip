@@ -131,6 +131,14 @@ __global__ void LevelDtKernel(LevelDts d, const double* dtc) {
   }
 }
 
+__global__ void CopyKernel(const GhostLink* links, int n, double* dst, const double* src) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const GhostLink L = links[q];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) dst[L.dst + k * int64_t(L.dcs)] = src[L.src + k * int64_t(L.scs)];
+}
+
 __global__ void FillKernel(double* p, int64_t n, double v) {
   const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (q < n) p[q] = v;
@@ -213,6 +221,7 @@ struct fvs_amr {
     int64_t max_cells = 0;
   };
   std::vector<Level> L;
+  std::vector<std::vector<int32_t>> lo;  // per level, (i0, j0) of every patch
   double* cfl = nullptr;  // one slot per sweep of a coarse step, then 2 per level
   int ncfl = 0;
   int64_t cycle = 0;
@@ -245,6 +254,21 @@ struct Owner {
     return id[j * nx + i];
   }
 };
+
+Owner OwnerOf(const fvs_amr* A, int l) {
+  Owner O;
+  int64_t scale = 1;
+  for (int k = 0; k < l; ++k) scale *= A->ratio;
+  O.nx = A->n0[0] * scale;
+  O.ny = A->n0[1] * scale;
+  O.id.assign(static_cast<size_t>(O.nx * O.ny), -1);
+  const auto& hp = A->lv[l]->host_patches;
+  for (size_t p = 0; p < hp.size(); ++p)
+    for (int64_t j = A->lo[l][2 * p + 1]; j < A->lo[l][2 * p + 1] + hp[p].ny; ++j)
+      for (int64_t i = A->lo[l][2 * p]; i < A->lo[l][2 * p] + hp[p].nx; ++i)
+        O.id[j * O.nx + i] = static_cast<int32_t>(p);
+  return O;
+}
 
 int64_t CellOffset(const PatchInfo& P, int64_t i, int64_t j) {
   return P.offset + (j + kG) * (P.nx + 2 * kG) + (i + kG);
@@ -288,6 +312,7 @@ int Build(fvs_amr* A, const int32_t* lo) {
         }
     }
   }
+  A->lo = los;
   for (int64_t c : own[0].id)
     if (c < 0) {
       SetError("level 0 patches must cover the domain");
@@ -735,6 +760,71 @@ int fvs_amr_tag_cells(fvs_amr* A, int level, double tol_rho, double tol_p, uint8
                      dim3(256), 0, A->s, b->patches, LV.tag_off, b->pool[0], g, tol_rho, tol_p,
                      tags);
   FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int fvs_amr_transfer(fvs_amr* D, fvs_amr* S) {
+  if (!D || !S || D == S || D->h != S->h || D->nlevels != S->nlevels || D->ratio != S->ratio ||
+      D->n0[0] != S->n0[0] || D->n0[1] != S->n0[1]) {
+    SetError("transfer needs two hierarchies of one handle with the same domain, ratio and "
+             "number of levels");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int r = D->ratio;
+  hipStream_t s = D->h->stream;
+  for (int l = 0; l < D->nlevels; ++l) {
+    const Owner old = OwnerOf(S, l);
+    const Owner crs = l > 0 ? OwnerOf(D, l - 1) : Owner();
+    const auto& dp = D->lv[l]->host_patches;
+    const auto& sp = S->lv[l]->host_patches;
+    std::vector<GhostLink> from_old, from_coarse;
+    for (size_t p = 0; p < dp.size(); ++p) {
+      const int64_t i0 = D->lo[l][2 * p], j0 = D->lo[l][2 * p + 1];
+      for (int64_t j = 0; j < dp[p].ny; ++j)
+        for (int64_t i = 0; i < dp[p].nx; ++i) {
+          const int64_t I = i0 + i, J = j0 + j;
+          GhostLink g;
+          g.dst = CellOffset(dp[p], i, j);
+          g.dcs = CompStride(dp[p]);
+          const int32_t q = old.at(I, J);
+          if (q >= 0) {
+            g.src = CellOffset(sp[q], I - S->lo[l][2 * q], J - S->lo[l][2 * q + 1]);
+            g.scs = CompStride(sp[q]);
+            from_old.push_back(g);
+            continue;
+          }
+          const int32_t cq = l > 0 ? crs.at(I / r, J / r) : -1;
+          if (cq < 0) {
+            SetError("transfer: level %d cell (%lld, %lld) has no source", l, (long long)I,
+                     (long long)J);
+            return FVS_BAD_ARGUMENT;
+          }
+          const PatchInfo& C = D->lv[l - 1]->host_patches[cq];
+          g.src = CellOffset(C, I / r - D->lo[l - 1][2 * cq], J / r - D->lo[l - 1][2 * cq + 1]);
+          g.scs = CompStride(C);
+          from_coarse.push_back(g);
+        }
+    }
+    GhostLink *a = nullptr, *b = nullptr;
+    int rc = UploadVec(from_old, &a);
+    if (!rc) rc = UploadVec(from_coarse, &b);
+    if (!rc && !from_old.empty())
+      hipLaunchKernelGGL(CopyKernel, dim3(Blocks(from_old.size())), dim3(256), 0, s, a,
+                         static_cast<int>(from_old.size()), D->lv[l]->pool[0], S->lv[l]->pool[0]);
+    if (!rc && !from_coarse.empty())
+      hipLaunchKernelGGL(CopyKernel, dim3(Blocks(from_coarse.size())), dim3(256), 0, s, b,
+                         static_cast<int>(from_coarse.size()), D->lv[l]->pool[0],
+                         D->lv[l - 1]->pool[0]);
+    if (!rc && hipGetLastError() != hipSuccess) rc = FVS_HIP_ERROR;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = FVS_HIP_ERROR;
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (rc) {
+      if (rc == FVS_HIP_ERROR) SetError("transfer: HIP error");
+      return rc;
+    }
+  }
+  D->cycle = S->cycle;
   return FVS_OK;
 }
 

@@ -585,6 +585,14 @@ int fvs_amr_set_graphs(fvs_amr* amr, int on);
  * one byte per interior cell (1 tagged), patches in level order, x fastest
  * within a patch (sum of nx * ny bytes). */
 int fvs_amr_tag_cells(fvs_amr* amr, int level, double tol_rho, double tol_p, uint8_t* tags);
+/* State of hierarchy src moved onto hierarchy dst (same handle, domain,
+ * ratio and number of levels; typically dst has regridded finer levels),
+ * level by level into pool 0: a cell takes src's same-level cell where src
+ * has one, otherwise dst's coarser-level cell above it (piecewise
+ * constant), the fill after GriddingAlgorithm::RegridAllFinerlevels
+ * (src/AMReX/GriddingAlgorithm.cpp:86-92, FillPatchTwoLevels with
+ * pc_interp).  Synchronises; allocates its link lists temporarily. */
+int fvs_amr_transfer(fvs_amr* dst, fvs_amr* src);
 
 #ifdef __cplusplus
 }

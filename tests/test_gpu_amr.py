@@ -128,3 +128,36 @@ def test_amr_tags_match_oracle_and_regrid_nests(cuda):
     _note(f"  {total} tagged cells, {len(new2)} level-2 patches from 4x4 tiles")
     assert new2
     AmrHierarchy(fm, A.DOMAIN, [A.LAYOUTS[0], A.LAYOUTS[1], new2], A.DX0, 2)
+
+
+def test_amr_regrid_transfer(cuda):
+    """Regrid of the finest level from level-1 tags: levels 0 and 1 keep
+    their state bit for bit; a new level-2 cell holds the old level-2 cell
+    where the old layout had one, else the level-1 cell above it; the new
+    hierarchy then steps."""
+    _note("amr regrid: moving state, one step, regrid level 2, one step")
+    fm, h, o = _pair(A.moving_state)
+    h.advance(0.8 * h.stable_dt())
+    new = h.regrid_finest(0.004, 0.004, tile=4)
+    _note(f"  new level 2: {len(new.layouts[2])} patches")
+    for l in (0, 1):
+        for a, b in zip(h.interiors(l), new.interiors(l)):
+            np.testing.assert_array_equal(a, b)
+    old2 = h.interiors(2)
+    lvl1 = h.interiors(1)
+    for p, got in enumerate(new.interiors(2)):
+        i0, j0, nx, ny = new.layouts[2][p]
+        for j in range(ny):
+            for i in range(nx):
+                I, J = i0 + i, j0 + j
+                src = None
+                for q, (a, b, w, hh) in enumerate(h.layouts[2]):
+                    if a <= I < a + w and b <= J < b + hh:
+                        src = old2[q][:, J - b, I - a]
+                if src is None:
+                    for q, (a, b, w, hh) in enumerate(h.layouts[1]):
+                        if a <= I // 2 < a + w and b <= J // 2 < b + hh:
+                            src = lvl1[q][:, J // 2 - b, I // 2 - a]
+                np.testing.assert_array_equal(got[:, j, i], src)
+    new.advance(0.8 * new.stable_dt())
+    assert np.all(np.isfinite(new.level_cfl()))
