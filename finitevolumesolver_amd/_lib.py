@@ -25,6 +25,7 @@ BASE_HLL_EINFELDT = 3
 RECON_NONE = 0
 RECON_MUSCL_HANCOCK = 1
 RECON_MUSCL_HANCOCK_PRIM = 2
+RECON_MUSCL_HANCOCK_CHARACTERISTIC = 3
 LIMITER_MINMOD = 0
 LIMITER_VANLEER = 1
 LIMITER_NONE = 2

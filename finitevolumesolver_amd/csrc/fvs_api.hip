@@ -74,16 +74,17 @@ int CheckDesc(const fvs_method_desc* d) {
     return FVS_BAD_ARGUMENT;
   }
   if (d->base < 0 || d->base > FVS_BASE_HLL_EINFELDT || d->reconstruction < 0 ||
-      d->reconstruction > FVS_RECON_MUSCL_HANCOCK_PRIM || d->limiter < 0 ||
+      d->reconstruction > FVS_RECON_MUSCL_HANCOCK_CHARACTERISTIC || d->limiter < 0 ||
       d->limiter > FVS_LIMITER_NO_GRADIENT || d->precision < 0 ||
       d->precision > FVS_FP64_RCP || d->boundary_solver < FVS_BOUNDARY_EXACT ||
       d->boundary_solver > FVS_BOUNDARY_HLLEM) {
     SetError("method enum out of range");
     return FVS_BAD_ARGUMENT;
   }
-  if (d->reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM &&
+  if ((d->reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM ||
+       d->reconstruction == FVS_RECON_MUSCL_HANCOCK_CHARACTERISTIC) &&
       (d->base != FVS_BASE_HLLEM_LARROUTUROU || d->precision != FVS_FP64)) {
-    SetError("MusclHancockPrim takes the Larrouturou HLLEM base in FP64");
+    SetError("MusclHancockPrim and MusclHancockCharacteristic take the Larrouturou HLLEM base in FP64");
     return FVS_BAD_ARGUMENT;
   }
   return FVS_OK;
@@ -127,14 +128,20 @@ __global__ __launch_bounds__(kBlock) void ViewFluxKernel(DevView fl, DevView st,
   int64_t f[3];
   fl.index(q, f);
   double L[NQ], R[NQ];
-  if (recon == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+  if (recon == FVS_RECON_MUSCL_HANCOCK_PRIM || recon == FVS_RECON_MUSCL_HANCOCK_CHARACTERISTIC) {
     double u[4][NQ];
     for (int s = 0; s < 4; ++s) {
       int64_t c[3] = {f[0], f[1], f[2]};
       c[DIR] += s - 2;
       LoadComplete<D>(st, c, u[s]);
     }
-    MusclHancockPrimStates<D, DIR>(u, DivRn(dt, dx), g, L, R);
+    if (recon == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+      MusclHancockPrimStates<D, DIR>(u, DivRn(dt, dx), g, L, R);
+    } else {
+      const double lambda = DivRn(dt, dx);
+      CharacteristicFaceState<D, DIR>(u, lambda, 0, g, L);
+      CharacteristicFaceState<D, DIR>(u + 1, lambda, 1, g, R);
+    }
   } else if (recon == FVS_RECON_MUSCL_HANCOCK) {
     double u[4][NQ];
     for (int s = 0; s < 4; ++s) {
@@ -588,7 +595,7 @@ int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* 
     return FVS_BAD_ARGUMENT;
   }
   if (int rc = CheckDir(h, dir)) return rc;
-  if (h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+  if (h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM) {
     SetError("MusclHancockPrim runs through the patch operators only");
     return FVS_BAD_ARGUMENT;
   }
@@ -648,7 +655,7 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t 
     return FVS_OK;
   }
   if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
-      h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+      h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM) {
     SetError("fvs_patch_stable_dt has no HLL or MusclHancockPrim kernel (use fvs_stable_dt on views)");
     return FVS_BAD_ARGUMENT;
   }
@@ -827,7 +834,7 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
     return FVS_BAD_ARGUMENT;
   }
   if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
-      h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+      h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM) {
     SetError("the fused level driver has no HLL or MusclHancockPrim kernel (patch operators only)");
     return FVS_BAD_ARGUMENT;
   }

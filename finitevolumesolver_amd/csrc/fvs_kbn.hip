@@ -753,7 +753,7 @@ int CheckKbn(const fvs_handle* h, int dir, bool needs_hllem) {
     return FVS_BAD_ARGUMENT;
   }
   if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
-      h->desc.reconstruction == FVS_RECON_MUSCL_HANCOCK_PRIM ||
+      h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM ||
       (needs_hllem && h->desc.base == FVS_BASE_GODUNOV_EXACT)) {
     SetError("the cut-cell regular fluxes use an HLLEM base method");
     return FVS_BAD_ARGUMENT;

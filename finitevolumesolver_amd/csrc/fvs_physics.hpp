@@ -298,6 +298,74 @@ __device__ __forceinline__ void MusclHancockPrimStates(const double (*u)[D + 4],
   }
 }
 
+// MusclHancockCharacteristic, array path
+// (src/equations/perfect_gas/MusclHancockCharacteristicMethod.hpp:26-285):
+// the face state of cell u[1] of the complete stencil u[0..2] on its right
+// (side 0) or left (side 1) face: VanLeer-limited wave amplitudes
+// (minus, zero, plus, v, w) of the primitive differences, advanced along
+// their characteristic speeds over half a step, recombined, added to the
+// cell's primitives and completed.  Velocities have three rows (zero beyond
+// D); ix = DIR, iy = ix + 1, iz = iy + 1 (mod 3).  FP64, IEEE.
+__device__ __forceinline__ double VanLeerPair(double sL, double sR) {
+  const double r = 2.0 * sL * sR;
+  return r > 0.0 ? DivRn(r, sL + sR) : 0.0;
+}
+
+template <int D, int DIR>
+__device__ __forceinline__ void CharacteristicFaceState(const double (*u)[D + 4], double lambda,
+                                                        int side, const Gas<double>& g,
+                                                        double* w) {
+  constexpr int ix = DIR, iy = (ix + 1) % 3, iz = (iy + 1) % 3;
+  const double rho = u[1][0];
+  const double un = DivRn(u[1][1 + DIR], rho);
+  const double c = u[1][D + 3];
+  const double rhoc = rho * c;
+  const double ooc2 = DivRn(1.0, c * c);
+  double pr[3], pp[3], pv[3][3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    pr[s] = u[s][0];
+    pp[s] = u[s][D + 2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pv[s][d] = d < D ? DivRn(u[s][1 + d], u[s][0]) : 0.0;
+  }
+  double amp[2][5];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double dp = pp[k + 1] - pp[k];
+    const double du = pv[k + 1][ix] - pv[k][ix];
+    const double drho = pr[k + 1] - pr[k];
+    amp[k][0] = dp - rhoc * du;
+    amp[k][1] = drho - ooc2 * dp;
+    amp[k][2] = dp + rhoc * du;
+    amp[k][3] = pv[k + 1][iy] - pv[k][iy];
+    amp[k][4] = pv[k + 1][iz] - pv[k][iz];
+  }
+  const double sign = side == 0 ? 1.0 : -1.0;
+  double a[5];
+  const double speed[5] = {un - c, un, un + c, un, un};
+#pragma unroll
+  for (int m = 0; m < 5; ++m)
+    a[m] = sign * 0.5 * VanLeerPair(amp[0][m], amp[1][m]) * (1.0 - sign * lambda * speed[m]);
+  double dpres = 0.5 * (a[0] + a[2]);
+  double drho = ooc2 * dpres + a[1];
+  double dvel[3];
+  dvel[ix] = DivRn(0.5 * (a[2] - a[0]), rhoc);
+  dvel[iy] = a[3];
+  dvel[iz] = a[4];
+  drho += rho;
+#pragma unroll
+  for (int d = 0; d < D; ++d) dvel[d] += DivRn(u[1][1 + d], u[1][0]);
+  dpres += u[1][D + 2];
+  w[0] = drho;
+#pragma unroll
+  for (int d = 0; d < D; ++d) w[1 + d] = drho * dvel[d];
+  w[D + 2] = dpres;
+  const double e_kin = DivRn(0.5 * SqNormArray<D>(w + 1), drho);
+  w[D + 1] = e_kin + dpres * g.gm1inv;
+  w[D + 3] = __builtin_sqrt(DivRn(g.gamma * dpres, drho));
+}
+
 // MUSCL-Hancock reconstruction of one cell from its conservative neighbours
 // um (left), u0 (centre), up (right).  Produces the complete half-step
 // states at the cell's left face (recL = rec_[1] of that face) and right

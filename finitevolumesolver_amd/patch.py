@@ -133,6 +133,8 @@ def make_desc(equation: PerfectGas, base="HLLEM", reconstruction="MusclHancock",
         d.reconstruction = L.RECON_MUSCL_HANCOCK
     elif reconstruction in ("MusclHancockPrim", 2):
         d.reconstruction = L.RECON_MUSCL_HANCOCK_PRIM
+    elif reconstruction in ("MusclHancockCharacteristic", 3):
+        d.reconstruction = L.RECON_MUSCL_HANCOCK_CHARACTERISTIC
     else:
         d.reconstruction = L.RECON_NONE
     d.limiter = _LIMITERS[limiter] if isinstance(limiter, str) else int(limiter)
@@ -168,7 +170,8 @@ class FluxMethod:
         return self._h.h
 
     def GetStencilWidth(self) -> int:
-        return 2 if self.reconstruction in ("MusclHancock", 1, "MusclHancockPrim", 2) else 1
+        return 2 if self.reconstruction in ("MusclHancock", 1, "MusclHancockPrim", 2,
+                                            "MusclHancockCharacteristic", 3) else 1
 
     def GetStencilSize(self) -> int:
         return 2 * self.GetStencilWidth()

@@ -68,11 +68,17 @@ enum fvs_base_method {
  *   width 2): limited primitive slopes, characteristic half step, HLLEM
  *   (Larrouturou) on the two face states; its own limiter (the limiter field
  *   is ignored).  Requires FVS_BASE_HLLEM_LARROUTUROU and FVS_FP64; patch
- *   operators (fvs_numeric_fluxes, fvs_stable_dt) only. */
+ *   operators (fvs_numeric_fluxes, fvs_stable_dt) only.
+ * FVS_RECON_MUSCL_HANCOCK_CHARACTERISTIC: perfect_gas::
+ *   MusclHancockCharacteristic<Rank> with its default VanLeer limiter
+ *   (src/equations/perfect_gas/MusclHancockCharacteristicMethod.hpp:26-320):
+ *   limited wave amplitudes advanced along the characteristics over half a
+ *   step, then HLLEM (Larrouturou).  Same restrictions as the PRIM form. */
 enum fvs_reconstruction {
   FVS_RECON_NONE = 0,
   FVS_RECON_MUSCL_HANCOCK = 1,
-  FVS_RECON_MUSCL_HANCOCK_PRIM = 2
+  FVS_RECON_MUSCL_HANCOCK_PRIM = 2,
+  FVS_RECON_MUSCL_HANCOCK_CHARACTERISTIC = 3
 };
 
 /* Slope limiters, include/fub/flux_method/MusclHancockMethod.hpp:31-187. */

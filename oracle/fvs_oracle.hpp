@@ -430,6 +430,76 @@ inline void MusclHancockPrimStates(const Gas& g, const State<D>* st, double dt, 
   }
 }
 
+// MusclHancockCharacteristic<Rank>, array path
+// (src/equations/perfect_gas/MusclHancockCharacteristicMethod.hpp:26-285):
+// wave amplitudes of the primitive differences (ComputeAmplitudes, :49-65),
+// limited by VanLeer's array operator (include/fub/flux_method/
+// MusclHancockMethod.hpp:144-149), advanced by the characteristic speeds
+// over half a step (DoTimeStep, :170-180), recombined into primitive
+// differences (Recombine, :76-84), added to the cell and completed
+// (Reconstruct, :238-259; PerfectGas::CompleteFromPrim array form,
+// src/equations/PerfectGas.cpp:160-174).  Velocities have three rows
+// (zero beyond the dimension); ix = dir, iy = ix + 1, iz = iy + 1 (mod 3).
+inline double VanLeerPair(double sL, double sR) {
+  const double r = 2.0 * sL * sR;
+  return r > 0.0 ? r / (sL + sR) : 0.0;
+}
+
+template <int D>
+inline void CharacteristicFaceState(const Gas& g, const State<D>* s3, double lambda,
+                                    int is_right_side, int dir, State<D>& rec) {
+  const State<D>& q = s3[1];
+  const double rho = q.rho;
+  const double u = q.mom[dir] / rho;
+  const double c = q.a;
+  const double rhoc = rho * c;
+  const double ooc2 = 1.0 / (c * c);
+  const int ix = dir, iy = (ix + 1) % 3, iz = (iy + 1) % 3;
+  double pr[3], pp[3], pv[3][3];
+  for (int s = 0; s < 3; ++s) {
+    pr[s] = s3[s].rho;
+    pp[s] = s3[s].p;
+    for (int d = 0; d < 3; ++d) pv[s][d] = d < D ? s3[s].mom[d] / s3[s].rho : 0.0;
+  }
+  double amp[2][5];  // (left, right) x (minus, zero, plus, v, w)
+  for (int k = 0; k < 2; ++k) {
+    const int a = k, b = k + 1;
+    const double dp = pp[b] - pp[a];
+    const double du = pv[b][ix] - pv[a][ix];
+    const double drho = pr[b] - pr[a];
+    amp[k][0] = dp - rhoc * du;
+    amp[k][1] = drho - ooc2 * dp;
+    amp[k][2] = dp + rhoc * du;
+    amp[k][3] = pv[b][iy] - pv[a][iy];
+    amp[k][4] = pv[b][iz] - pv[a][iz];
+  }
+  double slope[5];
+  for (int m = 0; m < 5; ++m) slope[m] = VanLeerPair(amp[0][m], amp[1][m]);
+  const double sign = 1 - 2 * is_right_side;
+  double ampl[5];
+  ampl[0] = sign * 0.5 * slope[0] * (1.0 - sign * lambda * (u - c));
+  ampl[1] = sign * 0.5 * slope[1] * (1.0 - sign * lambda * u);
+  ampl[2] = sign * 0.5 * slope[2] * (1.0 - sign * lambda * (u + c));
+  ampl[3] = sign * 0.5 * slope[3] * (1.0 - sign * lambda * u);
+  ampl[4] = sign * 0.5 * slope[4] * (1.0 - sign * lambda * u);
+  double dpres = 0.5 * (ampl[0] + ampl[2]);
+  double drho = ooc2 * dpres + ampl[1];
+  double dvel[3];
+  dvel[ix] = 0.5 * (ampl[2] - ampl[0]) / rhoc;
+  dvel[iy] = ampl[3];
+  dvel[iz] = ampl[4];
+  drho += rho;
+  for (int d = 0; d < D; ++d) dvel[d] += (q.mom[d] / q.rho);
+  dpres += q.p;
+  rec = State<D>{};
+  rec.rho = drho;
+  for (int d = 0; d < D; ++d) rec.mom[d] = drho * dvel[d];
+  rec.p = dpres;
+  const double e_kin = 0.5 * SquaredNormArray<D>(rec.mom) / rec.rho;
+  rec.E = e_kin + dpres * g.gm1inv;
+  rec.a = std::sqrt(g.gamma * rec.p / rec.rho);
+}
+
 template <int D>
 inline void EinfeldtSignals(const Gas& g, const State<D>& L, const State<D>& R, int dir,
                             double s[2]) {

@@ -109,7 +109,13 @@ void MusclHancockFlux(const Method& m, Cons<D>& flux, const State<D> st[4],
 template <int D>
 void FaceFlux(const Method& m, Cons<D>& flux, const State<D>* st, double dt,
               double dx, int dir, int* newton_fail) {
-  if (m.recon == FVS_RECON_MUSCL_HANCOCK_PRIM) {
+  if (m.recon == FVS_RECON_MUSCL_HANCOCK_CHARACTERISTIC) {
+    State<D> w[2];
+    const double lambda = dt / dx;
+    CharacteristicFaceState<D>(m.gas, st, lambda, 0, dir, w[0]);
+    CharacteristicFaceState<D>(m.gas, st + 1, lambda, 1, dir, w[1]);
+    HllemFlux<D>(m.gas, flux, w[0], w[1], dir, true, m.quirk3d);
+  } else if (m.recon == FVS_RECON_MUSCL_HANCOCK_PRIM) {
     State<D> w[2];
     MusclHancockPrimStates<D>(m.gas, st, dt, dx, dir, w);
     HllemFlux<D>(m.gas, flux, w[0], w[1], dir, true, m.quirk3d);

@@ -93,7 +93,8 @@ def test_patch_stable_dt_matches_oracle(cuda, D):
 
 
 @pytest.mark.parametrize("D", [2, 3])
-@pytest.mark.parametrize("recon", ["MusclHancock", "None", "MusclHancockPrim"])
+@pytest.mark.parametrize("recon", ["MusclHancock", "None", "MusclHancockPrim",
+                                   "MusclHancockCharacteristic"])
 def test_view_operators_bitexact(cuda, D, recon):
     """fvs_numeric_fluxes / fvs_stable_dt / fvs_update_conservatively /
     fvs_complete_from_cons / fvs_transmissive_fill on strided views with

@@ -74,3 +74,36 @@ def test_stable_dt_is_hllem_of_middle_cells():
     assert rc == 0
     L, R = (np.ascontiguousarray(s[:, None]) for s in st[1:3])
     assert out.value == O.hllem_stable_dt(L, R, D, 0, 0.1)[0]
+
+
+# MusclHancockCharacteristic (src/equations/perfect_gas/
+# MusclHancockCharacteristicMethod.hpp:26-320): with uniform velocity and
+# pressure only the entropy wave carries a jump, so the face density is
+# rho_1 + s (1 - lambda u) / 2 with s the VanLeer mean 2 dl dr / (dl + dr)
+# of the two density differences (linear data: the difference itself).
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_characteristic_constant_state_gives_physical_flux(D):
+    desc = make_desc(PerfectGas(dim=D), reconstruction="MusclHancockCharacteristic")
+    q = _complete(1.3, [0.4, -0.2, 0.1][:D], 2.1)
+    for d in range(D):
+        f, rc = O.face_flux(desc, _stencil([q] * 4), d, 0.01, 0.1)
+        assert rc == 0
+        ref = O.physical_flux(np.ascontiguousarray(q[:, None]), D, d)[:, 0]
+        np.testing.assert_allclose(f[:, 0], ref, rtol=1e-14, atol=1e-15)
+
+
+@pytest.mark.parametrize("rhos,slope", [
+    ((1.0, 1.1, 1.2, 1.3), 0.1),                  # linear
+    ((1.0, 1.1, 1.4, 1.0), 2 * 0.1 * 0.3 / 0.4),  # VanLeer mean of 0.1 and 0.3
+    ((1.0, 1.2, 1.1, 5.0), 0.0),                  # extremum
+])
+def test_characteristic_supersonic_entropy_wave(rhos, slope):
+    desc = make_desc(PerfectGas(dim=2), reconstruction="MusclHancockCharacteristic")
+    u, p, dt, dx = 3.0, 1.0, 0.01, 0.1
+    st = [_complete(r, [u, 0.0], p) for r in rhos]
+    f, rc = O.face_flux(desc, _stencil(st), 0, dt, dx)
+    assert rc == 0
+    rho_face = rhos[1] + 0.5 * slope * (1.0 - dt / dx * u)
+    np.testing.assert_allclose(f[0, 0], rho_face * u, rtol=1e-14)
+    np.testing.assert_allclose(f[1, 0], rho_face * u * u + p, rtol=1e-14)
