@@ -696,7 +696,10 @@ int fvs_amr_advance(fvs_amr* A, double dt) {
     return FVS_BAD_ARGUMENT;
   }
   hipStream_t hs = A->h->stream;
-  FVS_HIP_CHECK(hipMemcpyAsync(A->dtc, &dt, sizeof dt, hipMemcpyHostToDevice, hs));
+  // the step's dt goes to device memory as a kernel argument (no host
+  // buffer has to outlive the call); captured steps read it from there
+  hipLaunchKernelGGL(FillKernel, dim3(1), dim3(64), 0, hs, A->dtc, int64_t(1), dt);
+  FVS_LAUNCH_CHECK();
   if (!A->use_graphs) {
     A->s = hs;
     if (int rc = EnqueueStep(A)) return rc;
