@@ -143,22 +143,33 @@ class AmrHierarchy:
         L.check(self._lib.fvs_amr_level_cfl(self.a, out.ctypes.data))
         return out
 
-    def run(self, steps: int, cfl: float = 0.8) -> list:
-        """RunSimulation-style loop: dt = cfl * stable dt, one hierarchy step,
-        then the per-level check of the sweeps' stable dt against the level's
-        own time step (the reference's TimeStepTooLarge).  Returns the dts."""
+    def run(self, steps: int, cfl: float = 0.8, max_restarts: int = 64) -> list:
+        """RunSimulation-style loop (include/fub/RunSimulation.hpp:86-191):
+        dt = cfl * stable dt, one hierarchy step, then each level's sweeps'
+        stable dt checked against the level's own time step.  A failed check
+        is the reference's TimeStepTooLarge: the step is undone (the state
+        before it is kept on the device, like the reference's data /
+        scratch pair) and retried with cfl times the stable dt the sweeps
+        found; more than max_restarts failures in a row raise
+        L.TimeStepTooLarge.  Returns the accepted dts."""
         dts = []
         for _ in range(steps):
             dt = cfl * self.stable_dt()
-            self.advance(dt)
-            got = self.level_cfl()
-            for l in range(self.nlevels):
-                if got[l] < dt / self.ratio ** l:
-                    raise RuntimeError(f"level {l}: stable dt {got[l]:.6e} below its time step "
-                                       f"{dt / self.ratio ** l:.6e}")
+            saved = [b.download(0) for b in self.levels]
+            for attempt in range(max_restarts + 1):
+                self.advance(dt)
+                got = self.level_cfl()
+                limit = min(got[l] * self.ratio ** l for l in range(self.nlevels))
+                if all(got[l] >= dt / self.ratio ** l for l in range(self.nlevels)):
+                    break
+                for b, pool in zip(self.levels, saved):
+                    b.upload(pool, 0)
+                dt = cfl * limit
+            else:
+                raise L.TimeStepTooLarge(L.FVS_TIME_STEP_TOO_LARGE,
+                                         f"{max_restarts} restarts of one coarse step")
             dts.append(dt)
         return dts
-
 
 def tile_layout(domain, layout, tags, level: int, tile: int = 8, ratio: int = 2,
                 margin: int = 1, max_run: int = 8) -> list:

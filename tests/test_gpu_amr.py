@@ -161,3 +161,21 @@ def test_amr_regrid_transfer(cuda):
                 np.testing.assert_array_equal(got[:, j, i], src)
     new.advance(0.8 * new.stable_dt())
     assert np.all(np.isfinite(new.level_cfl()))
+
+
+def test_amr_run_restarts_on_cfl_failure(cuda):
+    """With cfl 1.6 every attempt of a step exceeds a level's stable dt (as
+    in the reference, a cfl above 1 never passes the check): run() undoes
+    each attempt, retries with 1.6 times the stable dt the sweeps found, and
+    after max_restarts raises TimeStepTooLarge with the state of before the
+    step restored bit for bit."""
+    _note("amr restart: moving state, cfl 1.6, 3 restarts")
+    fm, h, o = _pair(A.moving_state)
+    before = [h.interiors(l) for l in range(h.nlevels)]
+    with pytest.raises(L.TimeStepTooLarge):
+        h.run(1, cfl=1.6, max_restarts=3)
+    for l in range(h.nlevels):
+        for a, b in zip(before[l], h.interiors(l)):
+            np.testing.assert_array_equal(a, b)
+    # and the hierarchy steps normally afterwards
+    assert len(h.run(1, cfl=0.8)) == 1
