@@ -532,7 +532,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the patch shards as z-slabs with 2-plane halos; see DESIGN.md).",
+              "(the patch shards as z-slabs with 2-plane halos).",
               file=sys.stderr)
         return 2
 

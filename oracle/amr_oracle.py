@@ -1,6 +1,6 @@
 """CPU restatement of the 2D AMR hierarchy step: test infrastructure only.
 
-Only tests (and the benchmark's CPU baseline) use this module.  It restates
+Only tests use this module.  It restates
 the composition the reference drives for a subcycled hierarchy and checks
 the device library against it; nothing in the product calls it.
 
