@@ -531,7 +531,7 @@ def main() -> int:
                     help="run one variant alone and print its JSON (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: there is no multi-GPU path "
               "(the patch shards as z-slabs with 2-plane halos).",
               file=sys.stderr)
         return 2
