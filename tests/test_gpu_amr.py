@@ -179,3 +179,49 @@ def test_amr_run_restarts_on_cfl_failure(cuda):
             np.testing.assert_array_equal(a, b)
     # and the hierarchy steps normally afterwards
     assert len(h.run(1, cfl=0.8)) == 1
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64_rcp", 1e-12), ("fp32", 1e-6)])
+def test_amr_tolerance_modes(cuda, precision, tol):
+    """The refined-reciprocal FP64 and the FP32-flux forms of the hierarchy
+    step against the FP64 oracle over three steps (same dts): relative L1 of
+    density and energy per level within tol (1e-12 and 1e-6; momentum, whose
+    sums are small, is reported);
+    composite mass and energy of the resting bump conserved to tol."""
+    _note(f"amr {precision}: moving state, 3 steps")
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
+                        precision=precision)
+    for state in (A.moving_state, A.resting_blob):
+        h = AmrHierarchy(fm, A.DOMAIN, A.LAYOUTS, A.DX0, 2)
+        h.set_state(state)
+        ref = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock")
+        o = AmrOracle(A.DOMAIN, A.LAYOUTS, A.DX0, 2, A.oracle_sweep(ref._h.desc))
+        for l, lay in enumerate(A.LAYOUTS):
+            for p in range(len(lay)):
+                o.U[l][p][:, 2:-2, 2:-2] = state(l, p)
+        for f in (2, 1):
+            h.average_down(f)
+            o.average_down(f)
+        before = o.total()
+        for _ in range(3):
+            dt = 0.8 * o.stable_dt()
+            h.advance(dt)
+            o.advance(dt)
+        worst = 0.0
+        mom = 0.0
+        for l in range(h.nlevels):
+            for g, p in zip(h.interiors(l), range(len(A.LAYOUTS[l]))):
+                r = o.U[l][p][:, 2:-2, 2:-2]
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    err = np.abs(g - r).sum(axis=(1, 2)) / np.abs(r).sum(axis=(1, 2))
+                worst = max(worst, float(err[[0, 3]].max()))
+                m = err[[1, 2]]
+                mom = max(mom, float(m[np.isfinite(m)].max(initial=0.0)))
+        _note(f"  {state.__name__}: worst relative L1 density/energy {worst:.3e}, "
+              f"momentum {mom:.3e}")
+        assert worst <= tol
+        if state is A.resting_blob:
+            tot = sum(u.sum(axis=(1, 2)) for u in h.interiors(0)) * A.DX0 ** 2
+            rel = np.abs(tot[[0, 3]] - before[[0, 3]]) / np.abs(before[[0, 3]])
+            _note(f"  conservation of mass and energy {rel}")
+            assert rel.max() <= tol
