@@ -125,6 +125,16 @@ class AmrHierarchy:
         new.transfer_from(self)
         return new
 
+    def write_hdf5(self, path, time: float, cycle: int, xlower=(0.0, 0.0)) -> None:
+        """Appends (or starts) the HDF5 record of the composite state on the
+        finest level's domain (finitevolumesolver_amd.output)."""
+        from . import output
+        fab = output.composite(self.domain, self.layouts,
+                               [self.interiors(l) for l in range(self.nlevels)], self.ratio,
+                               self.flux_method.equation.gamma)
+        dx = self.dx[-1]
+        output.write_hdf5(path, fab, xlower, (dx, dx), time, cycle)
+
     def set_graphs(self, on: bool = True) -> None:
         """Replay each coarse step from a captured HIP graph (default) or
         launch it eagerly."""

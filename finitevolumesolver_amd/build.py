@@ -80,6 +80,26 @@ def build_library() -> Path:
     return LIB
 
 
+HDF5_PREFIX = Path("/opt/conda")
+HDF5_LIB = LIBDIR / "libfvs_hdf5.so"
+
+
+def build_hdf5_writer() -> Path | None:
+    """Host-only HDF5 output library (io/fvs_hdf5.cpp) against the HDF5 C
+    library of the image; skipped when its headers are absent."""
+    if not (HDF5_PREFIX / "include" / "hdf5.h").exists():
+        print("[build] no HDF5 headers: libfvs_hdf5.so not built", flush=True)
+        return None
+    LIBDIR.mkdir(exist_ok=True)
+    tmp = HDF5_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{ROOT / 'include'}",
+          f"-I{HDF5_PREFIX / 'include'}", str(PKG / "io" / "fvs_hdf5.cpp"),
+          f"-L{HDF5_PREFIX / 'lib'}", f"-Wl,-rpath,{HDF5_PREFIX / 'lib'}", "-lhdf5",
+          "-o", str(tmp)])
+    os.replace(tmp, HDF5_LIB)
+    return HDF5_LIB
+
+
 def build_oracle() -> Path:
     """Compiles the CPU oracle (test infrastructure) with its own recipe."""
     _run(["make", "-C", str(ROOT / "oracle"), "-j4"])

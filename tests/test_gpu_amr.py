@@ -225,3 +225,23 @@ def test_amr_tolerance_modes(cuda, precision, tol):
             rel = np.abs(tot[[0, 3]] - before[[0, 3]]) / np.abs(before[[0, 3]])
             _note(f"  conservation of mass and energy {rel}")
             assert rel.max() <= tol
+
+
+def test_amr_write_hdf5_roundtrip_frame(cuda, tmp_path):
+    """The hierarchy's HDF5 record: two frames of the composite, the second
+    after a step; the file's frame count and times follow the calls."""
+    import shutil
+    import subprocess
+    from finitevolumesolver_amd import output
+    if not output.LIB_PATH.exists():
+        pytest.skip("libfvs_hdf5.so not built")
+    fm, h, o = _pair(A.moving_state)
+    path = tmp_path / "amr.h5"
+    h.write_hdf5(path, 0.0, 0, xlower=(-1.0, -1.0))
+    dt = 0.8 * h.stable_dt()
+    h.advance(dt)
+    h.write_hdf5(path, dt, 1, xlower=(-1.0, -1.0))
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if shutil.which(h5dump):
+        head = subprocess.run([h5dump, "-H", str(path)], capture_output=True, text=True).stdout
+        assert "( 2, 6, 128, 128 )" in head
