@@ -125,13 +125,19 @@ class AmrHierarchy:
         new.transfer_from(self)
         return new
 
-    def write_hdf5(self, path, time: float, cycle: int, xlower=(0.0, 0.0)) -> None:
+    def write_hdf5(self, path, time: float, cycle: int, xlower=(0.0, 0.0), box=None) -> None:
         """Appends (or starts) the HDF5 record of the composite state on the
-        finest level's domain (finitevolumesolver_amd.output)."""
+        finest level's domain (finitevolumesolver_amd.output), or on the
+        finest-level cell box ((i0, j0), (i1, j1)) inclusive, as the
+        reference's WriteHdf5RestrictedToBox (src/AMReX/output/
+        WriteHdf5Impl.cpp:100-175) does; xlower stays the domain's."""
         from . import output
         fab = output.composite(self.domain, self.layouts,
                                [self.interiors(l) for l in range(self.nlevels)], self.ratio,
                                self.flux_method.equation.gamma)
+        if box is not None:
+            (i0, j0), (i1, j1) = box
+            fab = fab[:, j0:j1 + 1, i0:i1 + 1]
         dx = self.dx[-1]
         output.write_hdf5(path, fab, xlower, (dx, dx), time, cycle)
 

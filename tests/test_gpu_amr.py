@@ -233,8 +233,10 @@ def test_amr_write_hdf5_roundtrip_frame(cuda, tmp_path):
     import shutil
     import subprocess
     from finitevolumesolver_amd import output
-    if not output.LIB_PATH.exists():
-        pytest.skip("libfvs_hdf5.so not built")
+    try:
+        output._load()
+    except (ImportError, OSError) as e:
+        pytest.skip(f"HDF5 writer unavailable: {e}")
     fm, h, o = _pair(A.moving_state)
     path = tmp_path / "amr.h5"
     h.write_hdf5(path, 0.0, 0, xlower=(-1.0, -1.0))
