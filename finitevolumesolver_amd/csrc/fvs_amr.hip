@@ -114,6 +114,27 @@ __global__ void RefluxKernel(const RefluxCell* cells, int ncells, const int32_t*
   for (int k = 0; k < 4; ++k) pool[C.at + k * int64_t(C.cs)] = u[k];
 }
 
+// Per-sweep stable dt of one coarse step: block q reduces the partial
+// minima (keys = bit patterns of positive doubles) of sweep q.
+__global__ __launch_bounds__(256) void SlotMinKernel(const unsigned long long* partials,
+                                                     int cap, double* out) {
+  __shared__ unsigned long long s[4];
+  const unsigned long long* p = partials + static_cast<int64_t>(blockIdx.x) * cap;
+  unsigned long long k = 0x7ff0000000000000ull;
+  for (int i = threadIdx.x; i < cap; i += blockDim.x) k = p[i] < k ? p[i] : k;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(k, off, 64);
+    k = o < k ? o : k;
+  }
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = s[0];
+    for (int w = 1; w < 4; ++w) m = s[w] < m ? s[w] : m;
+    out[blockIdx.x] = __longlong_as_double(static_cast<long long>(m));
+  }
+}
+
 // The levels' time steps of one coarse step: level l + 1 takes the level
 // l step divided by the ratio (correctly rounded, as the host divides).
 struct LevelDts {
@@ -226,6 +247,8 @@ struct fvs_amr {
   int ncfl = 0;
   int64_t cycle = 0;
   double* dtc = nullptr;  // device: the coarse time step of the current advance
+  unsigned long long* slot_partials = nullptr;  // per sweep of a step: pcap minima
+  int pcap = 0;
   hipStream_t s = nullptr;  // stream the operations are enqueued on
   // whole coarse steps captured per cycle parity (the split order flips)
   int use_graphs = 1;
@@ -484,6 +507,8 @@ int Build(fvs_amr* A, const int32_t* lo) {
   A->ncfl = slot + 2 * A->nlevels;
   FVS_HIP_CHECK(hipMalloc(&A->cfl, A->ncfl * sizeof(double)));
   FVS_HIP_CHECK(hipMalloc(&A->dtc, sizeof(double)));
+  A->pcap = A->lv[0]->cus * 8;  // the batched sweeps' largest grid
+  FVS_HIP_CHECK(hipMalloc(&A->slot_partials, sizeof(unsigned long long) * A->pcap * slot));
   FVS_HIP_CHECK(hipStreamCreateWithFlags(&A->cap, hipStreamNonBlocking));
   return FVS_OK;
 }
@@ -587,7 +612,14 @@ int AdvanceLevel(fvs_amr* A, int l, int sub_first, std::vector<int>& slot) {
     if (l > 0) {
       if (int rc = AddRegister(A, l, d, 1, src)) return rc;
     }
-    if (int rc = batch::Sweep(A->lv[l], A->s, d, src, A->cfl + slot[l]++)) return rc;
+    if (A->lv[l]->grid[d] > static_cast<unsigned>(A->pcap)) {
+      SetError("batched sweep grid exceeds the per-sweep minima buffer");
+      return FVS_BAD_ARGUMENT;
+    }
+    const int q = slot[l]++;
+    if (int rc = batch::SweepPartials(A->lv[l], A->s, d, src,
+                                      A->slot_partials + static_cast<int64_t>(q) * A->pcap))
+      return rc;
     src = 1 - src;
   }
   if (finer) {
@@ -606,9 +638,19 @@ int EnqueueStep(fvs_amr* A) {
   for (int l = 0; l < A->nlevels; ++l) d.dst[l] = A->lv[l]->dts;
   hipLaunchKernelGGL(LevelDtKernel, dim3(1), dim3(64), 0, A->s, d, A->dtc);
   FVS_LAUNCH_CHECK();
+  // minima of sweeps with smaller grids must not see an earlier step's
+  const int nsweeps = A->ncfl - 2 * A->nlevels;
+  const int64_t nkeys = static_cast<int64_t>(A->pcap) * nsweeps;
+  hipLaunchKernelGGL(FillKernel, dim3(Blocks(nkeys)), dim3(256), 0, A->s,
+                     reinterpret_cast<double*>(A->slot_partials), nkeys, __builtin_huge_val());
+  FVS_LAUNCH_CHECK();
   std::vector<int> slot(A->nlevels);
   for (int l = 0; l < A->nlevels; ++l) slot[l] = A->L[l].cfl_base;
-  return AdvanceLevel(A, 0, 0, slot);
+  if (int rc = AdvanceLevel(A, 0, 0, slot)) return rc;
+  hipLaunchKernelGGL(SlotMinKernel, dim3(nsweeps), dim3(256), 0, A->s, A->slot_partials, A->pcap,
+                     A->cfl);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
 }
 
 } // namespace
@@ -661,6 +703,7 @@ int fvs_amr_destroy(fvs_amr* A) {
     if (e) (void)hipGraphExecDestroy(e);
   if (A->cap) (void)hipStreamDestroy(A->cap);
   if (A->dtc) (void)hipFree(A->dtc);
+  if (A->slot_partials) (void)hipFree(A->slot_partials);
   for (auto& LV : A->L) {
     void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval, LV.rcells, LV.rentries,
                   LV.tag_off};

@@ -374,7 +374,10 @@ int Upload(fvs_batch* b, int dir, int shape_idx) {
   return FVS_OK;
 }
 
-int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int precision) {
+// The sweep kernel alone, its per-workgroup minima written to `partials`
+// (b->grid[dir] entries).
+int LaunchSweepOnly(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long long* partials,
+                    int precision) {
   BatchArgs a;
   a.patches = b->patches;
   a.tiles = b->tiles[dir];
@@ -386,12 +389,17 @@ int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int pr
   a.g.gm1inv = 1.0 / (b->h->desc.gamma - 1.0);
   a.g.gm1 = b->h->desc.gamma - 1.0;
   a.limiter = b->h->desc.limiter;
-  a.partials = b->partials;
+  a.partials = partials;
   if (precision != FVS_FP64)
     batch::LaunchSweepFast(s, dir, b->shape[dir], a, b->grid[dir], precision);
   else
     LaunchSweepShape<double, false>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
   FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int precision) {
+  if (int rc = LaunchSweepOnly(b, s, dir, src, b->partials, precision)) return rc;
   return LaunchFinalizeCheck(s, b->partials, b->grid[dir], dt_out, nullptr);
 }
 
@@ -399,6 +407,11 @@ int Launch(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out, int pr
 
 int fvs::batch::Sweep(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out) {
   return Launch(b, s, dir, src, dt_out, b->h->desc.precision);
+}
+
+int fvs::batch::SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src,
+                              unsigned long long* partials) {
+  return LaunchSweepOnly(b, s, dir, src, partials, b->h->desc.precision);
 }
 
 extern "C" {

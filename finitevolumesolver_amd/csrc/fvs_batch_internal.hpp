@@ -97,5 +97,8 @@ namespace batch {
 // One batched sweep of direction dir from pool src into pool 1 - src on
 // stream s; the minimum stable dt of its faces goes to *dt_out (nullable).
 int Sweep(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out);
+// The same sweep without its reduction: the per-workgroup minima (keys, at
+// most 8 per CU) go to `partials`.
+int SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long long* partials);
 } // namespace batch
 } // namespace fvs
