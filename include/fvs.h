@@ -551,7 +551,9 @@ int fvs_batch_read_cfl(fvs_batch* batch, double* host2);
  * FillGhostLayerTwoLevels / FillGhostLayerSingleLevel (FillPatchTwoLevels
  * with piecewise-constant interpolation; transmissive domain edges),
  * AccumulateCoarseFineFluxes + ApplyFluxCorrection (FluxRegister) and
- * CoarsenConservatively (average_down). */
+ * CoarsenConservatively (average_down).  All work goes to the handle's
+ * stream; calls on one hierarchy must not overlap (it keeps per-object
+ * launch state), and its levels' batches must outlive it. */
 typedef struct fvs_amr fvs_amr;
 int fvs_amr_create(fvs_handle* h, int nlevels, fvs_batch* const* levels, const int32_t* lo,
                    const int64_t* domain, int ratio, fvs_amr** out);
