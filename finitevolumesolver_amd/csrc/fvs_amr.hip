@@ -114,27 +114,6 @@ __global__ void RefluxKernel(const RefluxCell* cells, int ncells, const int32_t*
   for (int k = 0; k < 4; ++k) pool[C.at + k * int64_t(C.cs)] = u[k];
 }
 
-// Per-sweep stable dt of one coarse step: block q reduces the partial
-// minima (keys = bit patterns of positive doubles) of sweep q.
-__global__ __launch_bounds__(256) void SlotMinKernel(const unsigned long long* partials,
-                                                     int cap, double* out) {
-  __shared__ unsigned long long s[4];
-  const unsigned long long* p = partials + static_cast<int64_t>(blockIdx.x) * cap;
-  unsigned long long k = 0x7ff0000000000000ull;
-  for (int i = threadIdx.x; i < cap; i += blockDim.x) k = p[i] < k ? p[i] : k;
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(k, off, 64);
-    k = o < k ? o : k;
-  }
-  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = k;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = s[0];
-    for (int w = 1; w < 4; ++w) m = s[w] < m ? s[w] : m;
-    out[blockIdx.x] = __longlong_as_double(static_cast<long long>(m));
-  }
-}
-
 // The levels' time steps of one coarse step: level l + 1 takes the level
 // l step divided by the ratio (correctly rounded, as the host divides).
 struct LevelDts {
@@ -640,17 +619,12 @@ int EnqueueStep(fvs_amr* A) {
   FVS_LAUNCH_CHECK();
   // minima of sweeps with smaller grids must not see an earlier step's
   const int nsweeps = A->ncfl - 2 * A->nlevels;
-  const int64_t nkeys = static_cast<int64_t>(A->pcap) * nsweeps;
-  hipLaunchKernelGGL(FillKernel, dim3(Blocks(nkeys)), dim3(256), 0, A->s,
-                     reinterpret_cast<double*>(A->slot_partials), nkeys, __builtin_huge_val());
-  FVS_LAUNCH_CHECK();
+  if (int rc = batch::LaunchSlotMin(A->s, A->slot_partials, nsweeps, A->pcap, nullptr, true))
+    return rc;
   std::vector<int> slot(A->nlevels);
   for (int l = 0; l < A->nlevels; ++l) slot[l] = A->L[l].cfl_base;
   if (int rc = AdvanceLevel(A, 0, 0, slot)) return rc;
-  hipLaunchKernelGGL(SlotMinKernel, dim3(nsweeps), dim3(256), 0, A->s, A->slot_partials, A->pcap,
-                     A->cfl);
-  FVS_LAUNCH_CHECK();
-  return FVS_OK;
+  return batch::LaunchSlotMin(A->s, A->slot_partials, nsweeps, A->pcap, A->cfl, false);
 }
 
 } // namespace

@@ -409,6 +409,49 @@ int fvs::batch::Sweep(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_
   return Launch(b, s, dir, src, dt_out, b->h->desc.precision);
 }
 
+namespace {
+
+// Block q reduces the `cap` partial minima (keys: bit patterns of positive
+// doubles) of slot q into out[q].
+__global__ __launch_bounds__(256) void SlotMinKernel(const unsigned long long* partials, int cap,
+                                                     double* out) {
+  __shared__ unsigned long long s[4];
+  const unsigned long long* p = partials + static_cast<int64_t>(blockIdx.x) * cap;
+  unsigned long long k = 0x7ff0000000000000ull;
+  for (int i = threadIdx.x; i < cap; i += blockDim.x) k = p[i] < k ? p[i] : k;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(k, off, 64);
+    k = o < k ? o : k;
+  }
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = s[0];
+    for (int w = 1; w < 4; ++w) m = s[w] < m ? s[w] : m;
+    out[blockIdx.x] = __longlong_as_double(static_cast<long long>(m));
+  }
+}
+
+__global__ void FillKeysKernel(unsigned long long* p, int64_t n) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = 0x7ff0000000000000ull;
+}
+
+} // namespace
+
+int fvs::batch::LaunchSlotMin(hipStream_t s, unsigned long long* partials, int nslots, int cap,
+                              double* out, bool clear) {
+  if (clear) {
+    const int64_t n = static_cast<int64_t>(nslots) * cap;
+    hipLaunchKernelGGL(FillKeysKernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       s, partials, n);
+  } else {
+    hipLaunchKernelGGL(SlotMinKernel, dim3(nslots), dim3(256), 0, s, partials, cap, out);
+  }
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
 int fvs::batch::SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src,
                               unsigned long long* partials) {
   return LaunchSweepOnly(b, s, dir, src, partials, b->h->desc.precision);
@@ -470,7 +513,7 @@ int fvs_batch_create(fvs_handle* h, int64_t count, const int32_t* n, const doubl
       hipMemset(b->pool[1], 0, off * sizeof(double)) != hipSuccess ||
       hipMemset(b->dts, 0, 4 * sizeof(double)) != hipSuccess)
     return fail(FVS_HIP_ERROR);
-  b->partials_cap = int64_t(b->cus) * 8;
+  b->partials_cap = int64_t(b->cus) * 8 * 2;  // two slots for the captured step
   if (hipMalloc(&b->partials, b->partials_cap * sizeof(unsigned long long)) != hipSuccess)
     return fail(FVS_HIP_ERROR);
   for (int d = 0; d < 2; ++d)
@@ -620,12 +663,17 @@ int fvs_batch_capture(fvs_batch* b) {
   int rc = FVS_OK;
   hipLaunchKernelGGL(BatchGhostKernel, dim3(ggrid), dim3(256), 0, cs, b->patches,
                      static_cast<int>(b->count), b->pool[0]);
-  rc = Launch(b, cs, 0, 0, b->dts + 1, b->h->desc.precision);
+  // the two sweeps' minima go to two slots of the partials buffer, reduced
+  // by one launch at the end of the step
+  const int cap = b->cus * 8;
+  rc = batch::LaunchSlotMin(cs, b->partials, 2, cap, nullptr, true);
+  if (rc == FVS_OK) rc = LaunchSweepOnly(b, cs, 0, 0, b->partials, b->h->desc.precision);
   if (rc == FVS_OK) {
     hipLaunchKernelGGL(BatchGhostKernel, dim3(ggrid), dim3(256), 0, cs, b->patches,
                        static_cast<int>(b->count), b->pool[1]);
-    rc = Launch(b, cs, 1, 1, b->dts + 2, b->h->desc.precision);
+    rc = LaunchSweepOnly(b, cs, 1, 1, b->partials + cap, b->h->desc.precision);
   }
+  if (rc == FVS_OK) rc = batch::LaunchSlotMin(cs, b->partials, 2, cap, b->dts + 1, false);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(cs, &g);
   if (rc != FVS_OK || e != hipSuccess) {

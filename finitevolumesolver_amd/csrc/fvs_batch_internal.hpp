@@ -100,5 +100,10 @@ int Sweep(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out);
 // The same sweep without its reduction: the per-workgroup minima (keys, at
 // most 8 per CU) go to `partials`.
 int SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long long* partials);
+// Slots of `cap` partial minima: clear = true sets every key to +inf (before
+// sweeps whose grids may be smaller than cap), clear = false reduces slot q
+// into out[q].
+int LaunchSlotMin(hipStream_t s, unsigned long long* partials, int nslots, int cap, double* out,
+                  bool clear);
 } // namespace batch
 } // namespace fvs
