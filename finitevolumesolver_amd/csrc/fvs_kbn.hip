@@ -1264,6 +1264,74 @@ __global__ __launch_bounds__(kBlock) void LevelDtKernel(LevelState st, LevelGeom
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
+// Step-start masked HLLEM CFL of both directions in one pass: a workgroup
+// is a tile of 256 columns (column x0 - 1 + t; threads 1..255 own theirs)
+// marching kDt2Seg rows, each thread completing its cell once per row; the
+// x face to its left pairs it with the left neighbour through LDS, the y
+// face below with the previous row kept in registers.  Face values are
+// those of LevelDtKernel (DBL_MAX where beta is zero); the per-direction
+// minima go to partials[b] (x) and partials[nblocks + b] (y); blocks beyond
+// the tiles write +inf.  Ghost rows and columns must be filled.
+constexpr int kDt2Seg = 8;
+
+__global__ __launch_bounds__(kBlock) void LevelDt2Kernel(LevelState st, LevelGeom g, int64_t nx,
+                                                         int64_t ny, double dx0, double dx1,
+                                                         Gas<double> gas,
+                                                         unsigned long long* partials,
+                                                         int64_t nblocks, const int* halt) {
+  if (*halt) return;
+  constexpr int kOwn = kBlock - 1;
+  const int64_t tiles_x = (nx + 1 + kOwn - 1) / kOwn;
+  const int64_t segs = (ny + 1 + kDt2Seg - 1) / kDt2Seg;
+  const int64_t b = blockIdx.x;
+  unsigned long long kx = 0x7ff0000000000000ull, ky = 0x7ff0000000000000ull;
+  if (b < tiles_x * segs) {
+    __shared__ double s_q[6][kBlock];
+    const int t = threadIdx.x;
+    const int64_t i = (b % tiles_x) * kOwn - 1 + t;   // column, -1 .. nx (+ tile padding)
+    const int64_t j0 = (b / tiles_x) * kDt2Seg;        // first face row of the segment
+    const int64_t j1 = j0 + kDt2Seg < ny + 1 ? j0 + kDt2Seg : ny + 1;
+    const bool col_ok = i <= nx;
+    const int64_t ic = col_ok ? i : nx;
+    double prev[6];
+    CellComplete(st, g, g.cell(ic, j0 - 1), gas, prev);
+    for (int64_t j = j0; j < j1; ++j) {
+      double cur[6];
+      CellComplete(st, g, g.cell(ic, j), gas, cur);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_q[k][t] = cur[k];
+      __syncthreads();
+      if (t >= 1 && col_ok) {
+        // x face i between cells i - 1 and i (rows 0 .. ny - 1, faces 0 .. nx)
+        if (j < ny && i >= 0 && g.beta_at(0, i, j) > 0.0) {
+          double L[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) L[k] = s_q[k][t - 1];
+          kx = KeyMin(kx, DtKey(HllemStableDt<2, 0>(L, cur, dx0, gas)));
+        } else if (j < ny && i >= 0) {
+          kx = KeyMin(kx, DtKey(DBL_MAX));
+        }
+        // y face j between cells (i, j - 1) and (i, j) (columns 0 .. nx - 1)
+        if (i >= 0 && i < nx) {
+          const double v = g.beta_at(1, i, j) > 0.0 ? HllemStableDt<2, 1>(prev, cur, dx1, gas)
+                                                    : DBL_MAX;
+          ky = KeyMin(ky, DtKey(v));
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 6; ++k) prev[k] = cur[k];
+    }
+  }
+  const unsigned long long mx = BlockMinKey(kx);
+  __syncthreads();  // BlockMinKey's shared slots are reused
+  const unsigned long long my = BlockMinKey(ky);
+  if (threadIdx.x == 0) {
+    partials[b] = mx;
+    partials[nblocks + b] = my;
+  }
+}
+
 // Reference states of the cut cells (list) from the step-start state.
 __global__ __launch_bounds__(kBlock) void LevelRefsKernel(LevelState st, LevelGeom g,
                                                           const int32_t* cut, int ncut,
@@ -1930,13 +1998,19 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
         const int64_t nt = d == 0 ? L->ny + 2 * kG : L->nx + 2 * kG;
         hipLaunchKernelGGL(GhostFillKernel, dim3(Blocks(nt * 2 * kG)), dim3(kBlock), 0, s, st0,
                            L->nx, L->ny, L->ngx, d);
-        if (d == 0)
-          hipLaunchKernelGGL(LevelDtKernel<0>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
-                             L->lg, L->nx, L->ny, L->patch.dx[0], gas, L->partials, halt);
-        else
-          hipLaunchKernelGGL(LevelDtKernel<1>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
-                             L->lg, L->nx, L->ny, L->patch.dx[1], gas, L->partials + L->nblocks,
-                             halt);
+      }
+      const int64_t dt2_tiles = ((L->nx + 1 + kBlock - 2) / (kBlock - 1)) *
+                                ((L->ny + 1 + kDt2Seg - 1) / kDt2Seg);
+      if (dt2_tiles <= L->nblocks) {
+        hipLaunchKernelGGL(LevelDt2Kernel, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
+                           L->lg, L->nx, L->ny, L->patch.dx[0], L->patch.dx[1], gas, L->partials,
+                           L->nblocks, halt);
+      } else {  // very thin levels: one pass per direction
+        hipLaunchKernelGGL(LevelDtKernel<0>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
+                           L->lg, L->nx, L->ny, L->patch.dx[0], gas, L->partials, halt);
+        hipLaunchKernelGGL(LevelDtKernel<1>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
+                           L->lg, L->nx, L->ny, L->patch.dx[1], gas, L->partials + L->nblocks,
+                           halt);
       }
       rc = LaunchFinalizeCfl(s, L->partials, L->nblocks, 2, nullptr, L->ctl, seq[0], strang);
       if (rc) break;
