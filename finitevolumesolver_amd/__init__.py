@@ -9,12 +9,12 @@ from . import _lib
 from .patch import (CompleteFromCons, FluxMethod, HyperbolicPatchIntegrator, PatchView,
                     PerfectGas, SinglePatchLevel, TransmissiveBoundary, godunov_sweep,
                     make_desc, patch_stable_dt, split_sweep)
-from ._lib import FvsError, TimeStepTooLarge
+from ._lib import FvsError, NewtonNotConverged, TimeStepTooLarge
 from .newton_guess import LearnedNewtonGuess, exact_middle_state
 
 __all__ = [
     "CompleteFromCons", "FluxMethod", "HyperbolicPatchIntegrator", "PatchView", "PerfectGas",
     "SinglePatchLevel", "TransmissiveBoundary", "godunov_sweep", "make_desc", "patch_stable_dt",
     "split_sweep",
-    "FvsError", "TimeStepTooLarge", "LearnedNewtonGuess", "exact_middle_state",
+    "FvsError", "NewtonNotConverged", "TimeStepTooLarge", "LearnedNewtonGuess", "exact_middle_state",
 ]

@@ -184,6 +184,8 @@ SIGNATURES = {
     "fvs_amr_level_cfl": (_I, [_P, _P]),
     "fvs_amr_stable_dt": (_I, [_P, _PD]),
     "fvs_amr_set_graphs": (_I, [_P, _I]),
+    "fvs_amr_cycle": (_I, [_P, C.POINTER(C.c_int64)]),
+    "fvs_amr_set_cycle": (_I, [_P, C.c_int64]),
     "fvs_amr_tag_cells": (_I, [_P, _I, _D, _D, _P]),
     "fvs_amr_transfer": (_I, [_P, _P]),
 }
@@ -219,12 +221,19 @@ class TimeStepTooLarge(FvsError):
     """Mirror of fub::TimeStepTooLarge (include/fub/TimeStepError.hpp:71-74)."""
 
 
+class NewtonNotConverged(FvsError):
+    """The exception NewtonIteration throws at its iteration cap
+    (include/fub/NewtonIteration.hpp:44-50)."""
+
+
 def check(code: int) -> int:
     if code == FVS_OK:
         return code
     msg = load().fvs_last_error().decode()
     if code == FVS_TIME_STEP_TOO_LARGE:
         raise TimeStepTooLarge(code, msg)
+    if code == FVS_NEWTON_NOT_CONVERGED:
+        raise NewtonNotConverged(code, msg)
     raise FvsError(code, msg)
 
 

@@ -154,6 +154,17 @@ class AmrHierarchy:
     def advance(self, dt: float) -> None:
         L.check(self._lib.fvs_amr_advance(self.a, float(dt)))
 
+    @property
+    def cycle(self) -> int:
+        """Coarse steps taken (selects the next step's split order)."""
+        out = C.c_int64()
+        L.check(self._lib.fvs_amr_cycle(self.a, C.byref(out)))
+        return out.value
+
+    @cycle.setter
+    def cycle(self, value: int) -> None:
+        L.check(self._lib.fvs_amr_set_cycle(self.a, int(value)))
+
     def level_cfl(self) -> np.ndarray:
         out = np.zeros(self.nlevels)
         L.check(self._lib.fvs_amr_level_cfl(self.a, out.ctypes.data))
@@ -167,11 +178,15 @@ class AmrHierarchy:
         before it is kept on the device, like the reference's data /
         scratch pair) and retried with cfl times the stable dt the sweeps
         found; more than max_restarts failures in a row raise
-        L.TimeStepTooLarge.  Returns the accepted dts."""
+        L.TimeStepTooLarge.  Undoing a step restores the cycle count too
+        (the reference restores its whole solver, RunSimulation.hpp:151),
+        so the retry runs in the split order of the failed attempt.
+        Returns the accepted dts."""
         dts = []
         for _ in range(steps):
             dt = cfl * self.stable_dt()
             saved = [b.download(0) for b in self.levels]
+            cycle = self.cycle
             for attempt in range(max_restarts + 1):
                 self.advance(dt)
                 got = self.level_cfl()
@@ -180,6 +195,7 @@ class AmrHierarchy:
                     break
                 for b, pool in zip(self.levels, saved):
                     b.upload(pool, 0)
+                self.cycle = cycle
                 dt = cfl * limit
             else:
                 raise L.TimeStepTooLarge(L.FVS_TIME_STEP_TOO_LARGE,

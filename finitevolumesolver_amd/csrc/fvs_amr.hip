@@ -848,6 +848,24 @@ int fvs_amr_transfer(fvs_amr* D, fvs_amr* S) {
   return FVS_OK;
 }
 
+int fvs_amr_cycle(fvs_amr* A, int64_t* cycle) {
+  if (!A || !cycle) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  *cycle = A->cycle;
+  return FVS_OK;
+}
+
+int fvs_amr_set_cycle(fvs_amr* A, int64_t cycle) {
+  if (!A || cycle < 0) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  A->cycle = cycle;
+  return FVS_OK;
+}
+
 int fvs_amr_set_graphs(fvs_amr* A, int on) {
   if (!A) {
     SetError("bad argument");

@@ -735,6 +735,10 @@ int fvs_newton_stats(fvs_handle* h, uint64_t stats[5], int reset) {
   FVS_HIP_CHECK(hipStreamSynchronize(h->stream));
   for (int i = 0; i < 5; ++i) stats[i] = tmp[i];
   if (reset) FVS_HIP_CHECK(hipMemsetAsync(h->newton_stats, 0, sizeof tmp, h->stream));
+  if (tmp[2] > 0) {
+    SetError("%llu Riemann problems hit the Newton iteration cap", tmp[2]);
+    return FVS_NEWTON_NOT_CONVERGED;
+  }
   return FVS_OK;
 }
 
@@ -1160,9 +1164,19 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
   FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
   const int64_t cycle0 = c.cycle;
   int64_t cycle = c.cycle;
-  int nfail = 0;
+  int nfail = 0;        // restarts in this call
+  int consecutive = 0;  // restarts since the last completed step
   int rc = FVS_OK;
   std::vector<int> start_buf;
+  // Newton non-convergence (exact Godunov): the reference throws from
+  // NewtonIteration (include/fub/NewtonIteration.hpp:44-50) and the run
+  // ends; here the run ends at the batch whose steps hit the cap.
+  unsigned long long newton0[5] = {}, newton1[5] = {};
+  if (godunov) {
+    FVS_HIP_CHECK(hipMemcpyAsync(newton0, h->newton_stats, sizeof newton0,
+                                 hipMemcpyDeviceToHost, s));
+    FVS_HIP_CHECK(hipStreamSynchronize(s));
+  }
   while (cycle - cycle0 < max_steps) {
     const int k = static_cast<int>(std::min<int64_t>(batch, max_steps - (cycle - cycle0)));
     start_buf.assign(k + 1, 0);
@@ -1178,8 +1192,20 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
     }
     if (rc) break;
     FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
+    if (godunov)
+      FVS_HIP_CHECK(hipMemcpyAsync(newton1, h->newton_stats, sizeof newton1,
+                                   hipMemcpyDeviceToHost, s));
     FVS_HIP_CHECK(hipStreamSynchronize(s));
     const int64_t completed = c.cycle - cycle;
+    if (completed > 0) consecutive = 0;
+    if (godunov && newton1[2] > newton0[2]) {
+      L->cur = c.halt == 0 ? cur : start_buf[completed];
+      cycle = c.cycle;
+      SetError("%llu Riemann problems hit the Newton iteration cap (%d iterations)",
+               newton1[2] - newton0[2], h->desc.max_newton_iterations);
+      rc = FVS_NEWTON_NOT_CONVERGED;
+      break;
+    }
     if (c.halt == 0) {
       L->cur = cur;
       cycle = c.cycle;
@@ -1193,10 +1219,11 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
     // TimeStepTooLarge: restart the step with cfl * failure_dt
     // (RunSimulation.hpp:141-151).
     ++nfail;
+    ++consecutive;
     c.halt = 0;
     c.have_failure = 1;
     FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
-    if (nfail > 64) {
+    if (consecutive > 64) {
       SetError("more than 64 consecutive CFL failures");
       rc = FVS_TIME_STEP_TOO_LARGE;
       break;

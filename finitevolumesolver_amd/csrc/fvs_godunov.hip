@@ -143,6 +143,32 @@ __device__ __forceinline__ void GodunovFluxMixed(double* flux, const double* L, 
   for (int k = 0; k < D + 2; ++k) flux[k] = static_cast<double>(ff[k]);
 }
 
+// Newton from a caller's seed (learned or given), with the reference's
+// initial guess, the mean pressure, as the fallback.  A seed that is not a
+// finite positive pressure (an overflowed or underflowed network output) is
+// not tried; a run from the seed that does not converge, or ends on a
+// pressure that is not finite and non-negative, is redone from the mean.
+// Both count as a fallback.
+template <int D, int DIR, bool MIXED>
+__device__ __forceinline__ MiddleState SeededMiddleState(const double* L, const double* R,
+                                                         const Gas<double>& g,
+                                                         const NewtonParams& np, bool seeded,
+                                                         double seed, double mean, int* iters,
+                                                         int* fallback) {
+  const bool usable = seeded && seed > 0.0 && seed <= 1.7976931348623157e308;
+  MiddleState ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, usable ? seed : mean);
+  int it = ms.iterations;
+  int fb = seeded && !usable;
+  if (usable && !(ms.converged && ms.pM >= 0.0 && ms.pM <= 1.7976931348623157e308)) {
+    ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, mean);
+    it += ms.iterations;
+    fb = 1;
+  }
+  *iters = it;
+  *fallback = fb;
+  return ms;
+}
+
 // One face: Newton (shared by flux and CFL), flux, dt.  MIXED: the Newton
 // iteration stays FP64 (with exp/log powers) and the sample and flux are
 // float.
@@ -156,15 +182,9 @@ __device__ __forceinline__ void GodunovFace(const double* L, const double* R, do
   np.max_it = a.max_it;
   using I = Idx<D>;
   const double mean = 0.5 * (L[I::kP] + R[I::kP]);
-  MiddleState ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, has_guess ? guess : mean);
-  int it = ms.iterations;
-  int fallback = 0;
-  if (has_guess && !ms.converged) {
-    // The reference's initial guess is the fallback for a learned guess.
-    ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, mean);
-    it += ms.iterations;
-    fallback = 1;
-  }
+  int it, fallback;
+  const MiddleState ms =
+      SeededMiddleState<D, DIR, MIXED>(L, R, g, np, has_guess, guess, mean, &it, &fallback);
   if constexpr (MIXED) {
     GodunovFluxMixed<D, DIR>(F, L, R, ms, g);
   } else {
@@ -508,13 +528,11 @@ __device__ __forceinline__ void CflFace(const double* L, const double* R, const 
   const double mean = 0.5 * (L[D + 2] + R[D + 2]);
   const bool seeded = LEARNED || a.guess[DIR] != nullptr;
   const double x0 = LEARNED ? lg : a.guess[DIR] ? a.guess[DIR][fidx] : mean;
-  MiddleState ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, x0);
-  unsigned long long it = ms.iterations, fb = 0;
-  if (seeded && !ms.converged) {
-    ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, mean);
-    it += ms.iterations;
-    fb = 1;
-  }
+  int iters, fallback;
+  const MiddleState ms =
+      SeededMiddleState<D, DIR, MIXED>(L, R, g, np, seeded, x0, mean, &iters, &fallback);
+  const unsigned long long it = static_cast<unsigned long long>(iters);
+  const unsigned long long fb = static_cast<unsigned long long>(fallback);
   const double dt = GodunovDtFromMiddle<D, DIR>(L, R, ms.pM, g, a.dx[DIR]);
   acc[0] = KMin(acc[0], DtKey(dt));
   acc[1] += it;

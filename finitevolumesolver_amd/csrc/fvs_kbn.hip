@@ -1971,7 +1971,8 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
   FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
   const int64_t cycle0 = c.cycle;
   int64_t cycle = c.cycle;
-  int nfail = 0;
+  int nfail = 0;        // restarts in this call
+  int consecutive = 0;  // restarts since the last completed step
   int rc = FVS_OK;
   std::vector<int> start_buf;
   const int* halt = &L->ctl->halt;
@@ -2033,6 +2034,7 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
     FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));
     FVS_HIP_CHECK(hipStreamSynchronize(s));
     const int64_t completed = c.cycle - cycle;
+    if (completed > 0) consecutive = 0;
     if (c.halt == 0) {
       L->cur = cur;
       cycle = c.cycle;
@@ -2042,10 +2044,11 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
     cycle = c.cycle;
     if (c.halt == 2) break;
     ++nfail;
+    ++consecutive;
     c.halt = 0;
     c.have_failure = 1;
     FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
-    if (nfail > 64) {
+    if (consecutive > 64) {
       SetError("more than 64 consecutive CFL failures");
       rc = FVS_TIME_STEP_TOO_LARGE;
       break;

@@ -31,6 +31,7 @@
 #include "fvs_internal.hpp"
 #include "fvs_lanes.hpp"
 #include "fvs_physics.hpp"
+#include "fvs_xslots.hpp"
 
 namespace fvs {
 
@@ -217,9 +218,6 @@ template <class T> __device__ __forceinline__ Gas<T> MakeGas(const SweepArgs& a)
 // chunk while it computes the current one, so every wave keeps a load in
 // flight behind its FP64 work; no LDS, no barrier until the final CFL
 // reduction.
-constexpr int kXOwned = 60;
-constexpr int kXWavesPerCu = 16;
-
 template <int D, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   using TA = typename Policies<T, FAST>::TA;
@@ -241,55 +239,15 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
 
-  // Slot of this lane in chunk c.  Lanes outside the slot sequence take its
-  // nearest end, so that slots grow with the lane index and every lane's row
-  // is at least lane 0's (row offsets are unsigned and relative to lane 0's
-  // row, a wave spanning at most 64 / (nx + 2) + 2 rows).  The first chunk's
-  // (row, slot) comes from one division; later chunks advance it by the
-  // constant stride (nwaves chunks) with one compare, no division.
-  struct Slot {
-    bool valid;
-    uint32_t sl;
-    int64_t cell;
-    uint32_t rbase;
-    int64_t wbase;
-  };
-  const uint32_t step = static_cast<uint32_t>(nwaves * kXOwned);
-  const uint32_t step_rows = step / S, step_sl = step - step_rows * S;
-  uint32_t q_row = 0, q_sl = 0;  // (row, slot) of q = c * kXOwned + lane - 2, unclamped
-  auto place = [&](int64_t c) {
-    const int64_t q64 = c * kXOwned + lane - 2;
-    const uint32_t q = q64 < 0 ? 0u : static_cast<uint32_t>(q64 < static_cast<int64_t>(Q) ? q64 : Q - 1);
-    q_row = q / S;
-    q_sl = q - q_row * S;
-  };
+  // Slot of this lane in chunk c (fvs_xslots.hpp): the row base is lane 0's
+  // row, read across the wave.
+  XSlotWalker walk(nx, Q, nwaves, lane);
+  using Slot = XSlot;
   auto locate = [&](int64_t c) {
-    Slot s;
-    const int64_t q64 = c * kXOwned + lane - 2;
-    s.valid = q64 >= 0 && q64 < static_cast<int64_t>(Q);
+    bool valid;
     uint32_t row, sl;
-    if (s.valid) {
-      row = q_row;
-      sl = q_sl;
-    } else {
-      const uint32_t q = q64 < 0 ? 0u : Q - 1u;
-      row = q / S;
-      sl = q - row * S;
-    }
-    s.sl = sl;
-    s.cell = static_cast<int64_t>(s.sl) - 1;
-    const uint32_t row0 = __builtin_amdgcn_readfirstlane(row);
-    s.rbase = (row - row0) * static_cast<uint32_t>(nx);
-    s.wbase = static_cast<int64_t>(row0) * nx;
-    return s;
-  };
-  auto advance = [&]() {
-    q_sl += step_sl;
-    q_row += step_rows;
-    if (q_sl >= S) {
-      q_sl -= S;
-      q_row += 1;
-    }
+    walk.RowSlot(c, &valid, &row, &sl);
+    return walk.Make(valid, row, sl, __builtin_amdgcn_readfirstlane(row));
   };
   auto load = [&](const Slot& s, double* u) {
     const uint32_t off = 8u * (s.rbase + static_cast<uint32_t>(Clamp(s.cell, nx)));
@@ -302,18 +260,14 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   // a uniform branch and its DPP shifts always run with the whole wave.
   int64_t c = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) +
               __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  place(c < nchunks ? c : 0);
+  walk.Place(c < nchunks ? c : 0);
   Slot cur = locate(c < nchunks ? c : 0);
   double u[NC];
   if (c < nchunks) load(cur, u);
   for (; c < nchunks; c += nwaves) {
     // The next chunk's load is in flight while this chunk is computed.
     const int64_t cn = c + nwaves;
-    if (cn < nchunks && c * kXOwned + lane - 2 >= 0) {
-      advance();
-    } else {
-      place(cn < nchunks ? cn : c);
-    }
+    walk.Next(c, cn, nchunks);
     const Slot nxt = locate(cn < nchunks ? cn : c);
     double un[NC];
     if (cn < nchunks) load(nxt, un);

@@ -185,9 +185,19 @@ class FluxMethod:
     def newton_stats(self, reset: bool = False) -> dict:
         """Newton counters of the exact Riemann solver on this handle."""
         arr = (C.c_uint64 * 5)()
-        L.check(L.load().fvs_newton_stats(self.handle, arr, 1 if reset else 0))
+        rc = L.load().fvs_newton_stats(self.handle, arr, 1 if reset else 0)
+        if rc != L.FVS_NEWTON_NOT_CONVERGED:  # that status still fills the counters
+            L.check(rc)
         return {"iterations": arr[0], "max_iterations": arr[1],
                 "not_converged": arr[2], "faces": arr[3], "fallbacks": arr[4]}
+
+    def check_newton(self, reset: bool = False) -> None:
+        """Raises NewtonNotConverged when a Riemann problem solved on this
+        handle hit the Newton iteration cap, where the reference throws
+        (include/fub/NewtonIteration.hpp:44-50); the patch operators run
+        asynchronously, so their failures surface here."""
+        arr = (C.c_uint64 * 5)()
+        L.check(L.load().fvs_newton_stats(self.handle, arr, 1 if reset else 0))
 
     def ComputeStableDt(self, states: PatchView, dx: float, dir,
                         out: torch.Tensor | None = None) -> torch.Tensor:

@@ -581,6 +581,13 @@ int fvs_amr_level_cfl(fvs_amr* amr, double* host);
  * (SubcycleFineFirstSolver::ComputeStableDt, :101-112), without the CFL
  * factor.  Uses pool 1 of each level as scratch; synchronises. */
 int fvs_amr_stable_dt(fvs_amr* amr, double* dt_host);
+/* Coarse-step count of the hierarchy (it selects the split order of the
+ * next step, MakeSplitDirections).  A caller that undoes a step by
+ * restoring the levels' states restores the count with it, as the
+ * reference restores its whole solver from the backup
+ * (include/fub/RunSimulation.hpp:141-151). */
+int fvs_amr_cycle(fvs_amr* amr, int64_t* cycle);
+int fvs_amr_set_cycle(fvs_amr* amr, int64_t cycle);
 /* Capture each coarse step (per cycle parity) once into a HIP graph and
  * replay it (on by default; results are identical either way).  The step's
  * dt is read from device memory, so one graph serves every dt; retuning a

@@ -519,12 +519,17 @@ int Update(const fvs_view* scratch, const fvs_view* fus, const fvs_view* fst,
           const double betaR_us_over_alpha = At(us, r[0], r[1], r[2], 0) / alpha;
           const double betaR_sR_over_alpha = At(sr, r[0], r[1], r[2], 0) / alpha;
           const double dRg =
-              std::clamp((At(sl, r[0], r[1], r[2], 0) * (0.5 - center)) / alpha, 0.0, 1.0);
+              std::clamp((At(sl, r[0], r[1], r[2], 0) *
+                          (Planted(PLANT_HBOX_RIGHT_DISTANCE) ? 0.5 + center : 0.5 - center)) /
+                             alpha,
+                         0.0, 1.0);
           const double fRus = At(*fus, r[0], r[1], r[2], c);
           const double fRsR = At(*fsr, r[0], r[1], r[2], c);
           const double right_greater = prev + dt_over_dx * betaR_us_over_alpha * (fLs - fRus) +
                                        dt_over_dx * betaR_sR_over_alpha * (fLs - fRsR) +
-                                       dt_over_dx * dRg * (fB - fRus);
+                                       dt_over_dx * dRg *
+                                           (Planted(PLANT_HBOX_RIGHT_DB_SIGN) ? fRus - fB
+                                                                              : fB - fRus);
           *next = betaL == betaR ? regular : betaL > betaR ? left_greater : right_greater;
         }
   return FVS_OK;

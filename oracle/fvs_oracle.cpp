@@ -370,6 +370,14 @@ void SoaLoad(State<D>& q, const double* a, int64_t n, int64_t i, int ncomp) {
 
 extern "C" {
 
+// Switches a planted mistake on (enum Plant in fvs_oracle.hpp) or off (0);
+// returns the previous setting.  Only the oracle's own pinning tests use it.
+int oracle_plant(int which) {
+  const int prev = g_plant;
+  g_plant = which;
+  return prev;
+}
+
 int oracle_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);

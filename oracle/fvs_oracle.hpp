@@ -19,6 +19,25 @@
 
 namespace fvs_oracle {
 
+// Planted mistakes.  Tests that pin the oracle switch one of these on
+// (oracle_plant) to show that their closed form notices it, then switch it
+// off again; 0 (the default) runs the restated algorithm unchanged.
+enum Plant {
+  PLANT_NONE = 0,
+  PLANT_MUSCL_HALF_STEP_SIGN = 1,  // rec = q -/+ (dt/2dx)(F(qL) - F(qR))
+  PLANT_MUSCL_HALF_STEP_DROP = 2,  // rec = q (no half step)
+  PLANT_HLLEM_NO_ANTIDIFFUSION = 3,  // b delta alpha_2 = 0
+  PLANT_HLLEM_L21 = 4,             // l21 with |roeU|^2 / 2 instead of |roeU|^2
+  PLANT_HLLEM_L2U_SIGN = 5,        // velocity coefficients of l2 negated
+  PLANT_HLLEM_DELTA = 6,           // delta = roeA / (roeA + 2 |u_bar|)
+  PLANT_EXACT_FAN_LEFT_U = 7,      // left fan u = 2/(g+1) (aL - (g-1)/2 uL)
+  PLANT_EXACT_FAN_RIGHT_RHO = 8,   // right fan density exponent 2/g
+  PLANT_HBOX_RIGHT_DISTANCE = 9,   // beta_L < beta_R: 0.5 + centre
+  PLANT_HBOX_RIGHT_DB_SIGN = 10,   // beta_L < beta_R: F_R,us - F_B
+};
+inline int g_plant = PLANT_NONE;
+inline bool Planted(int which) { return g_plant == which; }
+
 // PerfectGas<Dim> parameters (include/fub/equations/PerfectGas.hpp:216-222):
 // gamma = 1.4, gamma_minus_1_inv = 1 / (gamma - 1) evaluated at run time.
 struct Gas {
@@ -272,8 +291,11 @@ inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
 
   const double u_bar = 0.5 * (sR + sL);
   const double u_bar_abs = std::abs(u_bar);
-  const double delta = roeA / (roeA + u_bar_abs);
-  const double b = bLbR / db_positive;
+  const double delta = Planted(PLANT_HLLEM_DELTA) ? roeA / (roeA + 2.0 * u_bar_abs)
+                                                  : roeA / (roeA + u_bar_abs);
+  const double b = Planted(PLANT_HLLEM_NO_ANTIDIFFUSION) ? 0.0 : bLbR / db_positive;
+  const double l21_norm = Planted(PLANT_HLLEM_L21) ? squaredNormRoeU_half : squaredNormRoeU;
+  const double l2u = Planted(PLANT_HLLEM_L2U_SIGN) ? -1.0 : 1.0;
 
   const double deltaRho = rhoR - rhoL;
   double deltaRhoU[D];
@@ -282,8 +304,8 @@ inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
   const double gm1_over_roeA2 = gm1 / roeA2;
 
   if constexpr (D == 1) {
-    const double l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
-    const double l22 = gm1_over_roeA2 * Msq * roeU[0];
+    const double l21 = gm1_over_roeA2 * (roeH - l21_norm);
+    const double l22 = l2u * gm1_over_roeA2 * Msq * roeU[0];
     const double l23 = gm1_over_roeA2 * (-1.0);
     const double alpha_2 = l21 * deltaRho + l22 * deltaRhoU[0] + l23 * deltaRhoE;
     const double b_delta_alpha_2 = b * delta * alpha_2;
@@ -293,9 +315,9 @@ inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
   } else if constexpr (D == 2) {
     const int ix = dir;
     const int iy = (ix == 0);
-    const double l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
-    const double l22 = gm1_over_roeA2 * Msq * roeU[0];
-    const double l23 = gm1_over_roeA2 * roeU[1];
+    const double l21 = gm1_over_roeA2 * (roeH - l21_norm);
+    const double l22 = l2u * gm1_over_roeA2 * Msq * roeU[0];
+    const double l23 = l2u * gm1_over_roeA2 * roeU[1];
     const double l24 = -gm1_over_roeA2;
     const double alpha_2 = l21 * deltaRho + l22 * deltaRhoU[0] +
                            l23 * deltaRhoU[1] + l24 * deltaRhoE;
@@ -320,10 +342,10 @@ inline double HllemFlux(const Gas& g, Cons<D>& flux, const State<D>& stL,
     const int ix = dir;
     const int iy = (ix + 1) % 3;
     const int iz = (iy + 1) % 3;
-    const double l21 = gm1_over_roeA2 * (roeH - squaredNormRoeU);
-    const double l22 = gm1_over_roeA2 * Msq * roeU[0];
-    const double l23 = gm1_over_roeA2 * roeU[1];
-    const double l24 = gm1_over_roeA2 * roeU[2];
+    const double l21 = gm1_over_roeA2 * (roeH - l21_norm);
+    const double l22 = l2u * gm1_over_roeA2 * Msq * roeU[0];
+    const double l23 = l2u * gm1_over_roeA2 * roeU[1];
+    const double l24 = l2u * gm1_over_roeA2 * roeU[2];
     const double l25 = -gm1_over_roeA2;
     const double alpha_2 = l21 * deltaRho + l22 * deltaRhoU[0] +
                            l23 * deltaRhoU[1] + l24 * deltaRhoU[2] +
@@ -798,7 +820,8 @@ inline MiddleState SolveRiemannProblem(const Gas& eq, State<D>& state,
           from_prim(state);
         } else {
           state.rho = left.rho * std::pow(2.0 / gp1 + g_ratio * uL / aL, 2.0 / gm1);
-          const double u = 2.0 / gp1 * (aL + 0.5 * gm1 * uL);
+          const double u = Planted(PLANT_EXACT_FAN_LEFT_U) ? 2.0 / gp1 * (aL - 0.5 * gm1 * uL)
+                                                           : 2.0 / gp1 * (aL + 0.5 * gm1 * uL);
           set_momentum(state, state.rho * u, left);
           state.p = pL * std::pow(2.0 / gp1 + g_ratio * uL / aL, 2.0 * g / gm1);
           from_prim(state);
@@ -833,7 +856,8 @@ inline MiddleState SolveRiemannProblem(const Gas& eq, State<D>& state,
           state.p = pM;
           from_prim(state);
         } else {
-          state.rho = right.rho * std::pow(2.0 / gp1 - g_ratio * uR / aR, 2.0 / gm1);
+          state.rho = right.rho * std::pow(2.0 / gp1 - g_ratio * uR / aR,
+                                           Planted(PLANT_EXACT_FAN_RIGHT_RHO) ? 2.0 / g : 2.0 / gm1);
           const double u = 2.0 / gp1 * (-aR + 0.5 * gm1 * uR);
           set_momentum(state, state.rho * u, right);
           state.p = pR * std::pow(2.0 / gp1 - g_ratio * uR / aR, 2.0 * g / gm1);

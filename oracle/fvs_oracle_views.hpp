@@ -72,7 +72,9 @@ double BaseStableDt(const Method& m, const State<D>& L, const State<D>& R, doubl
 template <int D>
 void MusclHancockFlux(const Method& m, Cons<D>& flux, const State<D> st[4],
                       double dt, double dx, int dir, int* newton_fail) {
-  const double lambda_half = 0.5 * dt / dx;
+  const double lambda_half = Planted(PLANT_MUSCL_HALF_STEP_SIGN)   ? -(0.5 * dt / dx)
+                             : Planted(PLANT_MUSCL_HALF_STEP_DROP) ? 0.0
+                                                                   : 0.5 * dt / dx;
   State<D> rec[2];
   for (int side = 0; side < 2; ++side) {
     const State<D>* s3 = st + side; // first<3>() or last<3>()

@@ -85,3 +85,32 @@ def resting_blob(level, p, layouts=LAYOUTS):
     p_ = 1.0 + 0.8 * s
     z = np.zeros_like(x)
     return cons_from(rho, z, z, p_)
+
+
+def oracle_run(o, steps, cfl, max_restarts=64, restore_cycle=True):
+    """RunSimulation's loop (include/fub/RunSimulation.hpp:86-191) over the
+    AMR oracle: dt = cfl * stable dt, one coarse step, each level's sweep
+    minima checked against the level's own time step; a failed attempt is
+    undone (state and, as the reference restores its whole solver from the
+    backup at :151, the cycle count) and retried with cfl times the stable dt
+    the sweeps found.  Returns (accepted dts, attempts per step)."""
+    import copy
+    dts, attempts = [], []
+    for _ in range(steps):
+        dt = cfl * o.stable_dt()
+        saved, cycle = copy.deepcopy(o.U), o.cycle
+        for attempt in range(max_restarts + 1):
+            o.advance(dt)
+            got = o.level_cfl
+            if all(got[l] >= dt / o.r ** l for l in range(len(o.U))):
+                break
+            limit = min(got[l] * o.r ** l for l in range(len(o.U)))
+            o.U = copy.deepcopy(saved)
+            if restore_cycle:
+                o.cycle = cycle
+            dt = cfl * limit
+        else:
+            raise RuntimeError("too many restarts")
+        dts.append(dt)
+        attempts.append(attempt + 1)
+    return dts, attempts

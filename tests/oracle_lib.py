@@ -28,6 +28,7 @@ def load():
         I, D, I64, P = C.c_int, C.c_double, C.c_int64, C.c_void_p
         sig = {
             "oracle_set_threads": (I, [I]),
+            "oracle_plant": (I, [I]),
             "oracle_complete_from_cons": (I, [I, D, I64, P, P]),
             "oracle_complete_from_prim": (I, [I, D, I64, P, P, P, P]),
             "oracle_physical_flux": (I, [I, I, I64, P, P]),
@@ -75,6 +76,34 @@ def load():
             fn.argtypes = args
         _oracle = lib
     return _oracle
+
+
+class planted:
+    """Context manager: runs the oracle with one planted mistake (the codes
+    of enum Plant in oracle/fvs_oracle.hpp), restoring the clean oracle on
+    exit, so a pinning test can show that its closed form notices it."""
+
+    MUSCL_HALF_STEP_SIGN = 1
+    MUSCL_HALF_STEP_DROP = 2
+    HLLEM_NO_ANTIDIFFUSION = 3
+    HLLEM_L21 = 4
+    HLLEM_L2U_SIGN = 5
+    HLLEM_DELTA = 6
+    EXACT_FAN_LEFT_U = 7
+    EXACT_FAN_RIGHT_RHO = 8
+    HBOX_RIGHT_DISTANCE = 9
+    HBOX_RIGHT_DB_SIGN = 10
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        load().oracle_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_plant(0)
+        return False
 
 
 def ptr(a: np.ndarray):

@@ -125,3 +125,21 @@ def test_tile_layout_nests_and_covers():
     for j, i in zip(*np.nonzero(tags[0][8:12, 4:6])):
         I, J = 2 * (i0 + 4 + i), 2 * (j0 + 8 + j)
         assert any(a <= I < a + w and b <= J < b + h for a, b, w, h in out)
+
+
+def test_retry_keeps_the_split_order_of_the_failed_attempt():
+    """cfl 0.99 on the resting bump: the first attempt of the first two
+    coarse steps fails the CFL check and the retry passes.  Restoring the
+    cycle with the state (as the reference restores its solver) and not
+    restoring it give different results, so the retry's split order is
+    observable."""
+    runs = {}
+    for restore in (True, False):
+        o = _oracle(A.resting_blob)
+        dts, attempts = A.oracle_run(o, 2, 0.99, restore_cycle=restore)
+        runs[restore] = (o, dts, attempts)
+    o1, dts1, att1 = runs[True]
+    o0, dts0, att0 = runs[False]
+    assert att1[0] == 2 and att1 == att0
+    assert o1.cycle == 2
+    assert any(not np.array_equal(a, b) for a, b in zip(o1.U[2], o0.U[2]))

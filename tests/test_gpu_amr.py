@@ -181,6 +181,23 @@ def test_amr_run_restarts_on_cfl_failure(cuda):
     assert len(h.run(1, cfl=0.8)) == 1
 
 
+def test_amr_run_retry_keeps_split_order(cuda):
+    """cfl 0.99 on the resting bump: the first attempt of each of the first
+    two coarse steps fails the CFL check and its retry passes.  run()
+    restores the cycle count with the state, so the retry keeps the failed
+    attempt's split order: dts, attempts and every level equal the oracle's
+    run loop bit for bit (which differs when the cycle is not restored,
+    tests/test_amr_oracle.py)."""
+    _note("amr retry: resting bump, cfl 0.99, 2 steps")
+    fm, h, o = _pair(A.resting_blob)
+    ref_dts, attempts = A.oracle_run(o, 2, 0.99)
+    assert attempts[0] == 2
+    dts = h.run(2, cfl=0.99)
+    assert dts == ref_dts
+    assert h.cycle == o.cycle == 2
+    _compare(h, o, "after two steps with retries")
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64_rcp", 1e-12), ("fp32", 1e-6)])
 def test_amr_tolerance_modes(cuda, precision, tol):
     """The refined-reciprocal FP64 and the FP32-flux forms of the hierarchy

@@ -24,6 +24,21 @@ def _note(msg):
     print(msg, flush=True)
 
 
+def _one_pass_cfl(nx, ny):
+    """The level's choice of step-start CFL kernel: the one-pass kernel
+    (255 owned columns by 8-row segments per workgroup) when its tiles fit
+    the level's workgroup count (one per 256 faces of the larger direction),
+    else one pass per direction."""
+    tiles = -(-(nx + 1) // 255) * -(-(ny + 1) // 8)
+    nblocks = -(-max((nx + 1) * ny, nx * (ny + 1)) // 256)
+    return tiles <= nblocks
+
+
+def test_cfl_kernel_choice_covers_both_paths():
+    assert _one_pass_cfl(160, 80) and _one_pass_cfl(121, 61)
+    assert not _one_pass_cfl(16, 512)
+
+
 @pytest.mark.parametrize("solver,integrator,splitting,shape", [
     ("hllem", "TimeIntegrator", "Godunov", (160, 80)),
     ("hllem", "TimeIntegrator2", "Godunov", (160, 80)),
@@ -32,10 +47,14 @@ def _note(msg):
     # ragged: the last 60-cell row segment holds one cell, the last
     # 16-row column segment 13 rows
     ("hllem", "TimeIntegrator", "Godunov", (121, 61)),
+    # narrow: too few cells per workgroup for the one-pass step-start CFL
+    # kernel, so the two per-direction passes run
+    ("hllem", "TimeIntegrator", "Godunov", (16, 512)),
 ])
 def test_cutcell_level_matches_oracle(cuda, solver, integrator, splitting, shape):
     import torch
     (nx, ny), steps = shape, 24
+    _note(f"shape {shape}: {'one-pass' if _one_pass_cfl(nx, ny) else 'per-direction'} CFL")
     geom = channel_geometry(nx, ny, perturb=True)
     cons, dx = shock_channel(nx, ny, x_shock=-0.4)
     fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
