@@ -121,7 +121,12 @@ def cpu_baseline(n: int, steps: int, precision: str) -> dict:
     import finitevolumesolver_amd as fvs
     from finitevolumesolver_amd.initial_data import fourier_modes_3d
     from finitevolumesolver_amd.patch import make_desc
-    threads = int(os.environ.get("FVS_CPU_THREADS", min(os.cpu_count() or 1, 16)))
+    # Every CPU this process may run on, unless the job's thread share
+    # (OMP_NUM_THREADS, set by the GPU host for its per-GPU CPU share) is
+    # smaller; FVS_CPU_THREADS overrides both.
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", affinity) or affinity)
+    threads = int(os.environ.get("FVS_CPU_THREADS", min(affinity, share)))
     threads = O.load().oracle_set_threads(threads)
     cons, dx = fourier_modes_3d(n)
     desc = make_desc(fvs.PerfectGas(3))
@@ -151,7 +156,8 @@ def cpu_baseline(n: int, steps: int, precision: str) -> dict:
                       f"({updates / 1e6:.1f} M cell-updates, {el:.1f} s, {threads} threads)",
             "single_core": {"value": n1 ** 3 * 3 * len(d1) / el1 / 1e9,
                             "sample": f"{n1}^3 patch, {len(d1)} steps, 1 thread, {el1:.1f} s"},
-            "cpu_model": model}
+            "cpu_model": model, "host_cpus": os.cpu_count(), "affinity_cpus": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def l1_vs_oracle(precision: str, torch) -> dict:

@@ -43,6 +43,7 @@ class AmrHierarchy:
         L.check(self._lib.fvs_amr_create(flux_method.handle, len(self.levels), handles,
                                          lo.ctypes.data, dom.ctypes.data, self.ratio, C.byref(a)))
         self.a = a
+        self.time = 0.0  # time point of every level after a whole coarse step
 
     def __del__(self):
         a = getattr(self, "a", None)
@@ -125,6 +126,45 @@ class AmrHierarchy:
         new.transfer_from(self)
         return new
 
+    def regrid_all(self, base: int, tol_rho: float, tol_p: float, tile: int = 8,
+                   margin: int = 1) -> "AmrHierarchy":
+        """GriddingAlgorithm::RegridAllFinerlevels(base)
+        (src/AMReX/GriddingAlgorithm.cpp:86-108): every level finer than
+        `base` is rebuilt.  Level l + 1 (l = base .. finest - 1) is laid out
+        from the gradient tags of the current level l (tile_layout), nested
+        in the new level l; levels 0 .. base keep their patches.  Returns a
+        new hierarchy with the state moved onto it (transfer_from); the
+        number of levels stays (a level whose tiles all drop keeps the tiles
+        of its old layout that still nest, or raises when none does)."""
+        if not 0 <= base < self.nlevels - 1:
+            raise ValueError(f"base level {base} has no finer level to regrid")
+        tags = {lv: self.tag_cells(lv, tol_rho, tol_p) for lv in range(base, self.nlevels - 1)}
+        layouts = [list(lay) for lay in self.layouts[:base + 1]]
+        for lv in range(base, self.nlevels - 1):
+            fine = tile_layout(self.domain, self.layouts[lv], tags[lv], lv, tile, self.ratio,
+                               margin, owner_layout=layouts[lv])
+            if not fine:
+                ext = (self.domain[0] * self.ratio ** lv, self.domain[1] * self.ratio ** lv)
+                fine = [p for p in self.layouts[lv + 1]
+                        if nests(p, layouts[lv], ext, self.ratio, margin)]
+            if not fine:
+                raise ValueError(f"no patch of level {lv + 1} nests inside the new level {lv}")
+            layouts.append(fine)
+        new = AmrHierarchy(self.flux_method, self.domain, layouts, self.dx[0], self.ratio)
+        new.transfer_from(self)
+        new.cycle = self.cycle
+        new.time = self.time
+        return new
+
+    def _adopt(self, new: "AmrHierarchy") -> None:
+        """Takes over new's levels and library hierarchy (with this
+        hierarchy's graph setting); new keeps (and frees) the old ones."""
+        for k in ("a", "layouts", "levels", "dx", "time"):
+            mine, theirs = getattr(self, k), getattr(new, k)
+            setattr(self, k, theirs)
+            setattr(new, k, mine)
+        self.set_graphs(getattr(self, "_graphs", True))
+
     def write_hdf5(self, path, time: float, cycle: int, xlower=(0.0, 0.0), box=None) -> None:
         """Appends (or starts) the HDF5 record of the composite state on the
         finest level's domain (finitevolumesolver_amd.output), or on the
@@ -145,6 +185,7 @@ class AmrHierarchy:
         """Replay each coarse step from a captured HIP graph (default) or
         launch it eagerly."""
         L.check(self._lib.fvs_amr_set_graphs(self.a, 1 if on else 0))
+        self._graphs = bool(on)
 
     def stable_dt(self) -> float:
         out = C.c_double()
@@ -153,6 +194,7 @@ class AmrHierarchy:
 
     def advance(self, dt: float) -> None:
         L.check(self._lib.fvs_amr_advance(self.a, float(dt)))
+        self.time += float(dt)
 
     @property
     def cycle(self) -> int:
@@ -170,7 +212,8 @@ class AmrHierarchy:
         L.check(self._lib.fvs_amr_level_cfl(self.a, out.ctypes.data))
         return out
 
-    def run(self, steps: int, cfl: float = 0.8, max_restarts: int = 64) -> list:
+    def run(self, steps: int, cfl: float = 0.8, max_restarts: int = 64,
+            regrid_every: int = 0, tol_rho: float = 0.1, tol_p: float = 0.1) -> list:
         """RunSimulation-style loop (include/fub/RunSimulation.hpp:86-191):
         dt = cfl * stable dt, one hierarchy step, then each level's sweeps'
         stable dt checked against the level's own time step.  A failed check
@@ -181,12 +224,20 @@ class AmrHierarchy:
         L.TimeStepTooLarge.  Undoing a step restores the cycle count too
         (the reference restores its whole solver, RunSimulation.hpp:151),
         so the retry runs in the split order of the failed attempt.
+        With regrid_every = k > 0, the levels finer than 0 are rebuilt from
+        gradient tags (regrid_all(0, tol_rho, tol_p)) before every k-th
+        coarse step after the first, as the reference's PreAdvanceLevel
+        regrids all finer levels at the start of a coarse step
+        (src/AMReX/IntegratorContext.cpp:600-626; its regrid_frequency
+        option, include/fub/AMReX/IntegratorContext.hpp:54).
         Returns the accepted dts."""
         dts = []
-        for _ in range(steps):
+        for step in range(steps):
+            if regrid_every > 0 and step > 0 and step % regrid_every == 0:
+                self._adopt(self.regrid_all(0, tol_rho, tol_p))
             dt = cfl * self.stable_dt()
             saved = [b.download(0) for b in self.levels]
-            cycle = self.cycle
+            cycle, time = self.cycle, self.time
             for attempt in range(max_restarts + 1):
                 self.advance(dt)
                 got = self.level_cfl()
@@ -195,7 +246,7 @@ class AmrHierarchy:
                     break
                 for b, pool in zip(self.levels, saved):
                     b.upload(pool, 0)
-                self.cycle = cycle
+                self.cycle, self.time = cycle, time
                 dt = cfl * limit
             else:
                 raise L.TimeStepTooLarge(L.FVS_TIME_STEP_TOO_LARGE,
@@ -203,8 +254,27 @@ class AmrHierarchy:
             dts.append(dt)
         return dts
 
+def nests(patch, coarse_layout, extent, ratio: int = 2, margin: int = 1) -> bool:
+    """Whether a patch (i0, j0, nx, ny) of level l + 1, grown by `margin`
+    cells of level l and clamped to level l's extent (nx_l, ny_l), lies
+    inside the patches of level l (coarse_layout): the nesting rule
+    tile_layout keeps."""
+    i0, j0, nx, ny = patch
+    ci0 = max(i0 // ratio - margin, 0)
+    cj0 = max(j0 // ratio - margin, 0)
+    ci1 = min(-(-(i0 + nx) // ratio) + margin, extent[0])
+    cj1 = min(-(-(j0 + ny) // ratio) + margin, extent[1])
+    cover = np.zeros((cj1 - cj0, ci1 - ci0), dtype=bool)
+    for (a0, b0, an, bn) in coarse_layout:
+        x0, x1 = max(a0, ci0), min(a0 + an, ci1)
+        y0, y1 = max(b0, cj0), min(b0 + bn, cj1)
+        if x0 < x1 and y0 < y1:
+            cover[y0 - cj0:y1 - cj0, x0 - ci0:x1 - ci0] = True
+    return bool(cover.all())
+
+
 def tile_layout(domain, layout, tags, level: int, tile: int = 8, ratio: int = 2,
-                margin: int = 1, max_run: int = 8) -> list:
+                margin: int = 1, max_run: int = 8, owner_layout=None) -> list:
     """Patches of level + 1 from the tags of a level's patches, on a fixed
     tile lattice (the GPU analogue of clustering the tagged cells into
     boxes): a tile (tile x tile cells of the level, aligned to multiples of
@@ -212,14 +282,18 @@ def tile_layout(domain, layout, tags, level: int, tile: int = 8, ratio: int = 2,
     cells and clamped to the domain, lies inside the level's patches (proper
     nesting: the finer patches' ghost layers and the coarse cells beside them
     then have coarse data).  Tiles kept in one tile row merge into runs of
-    at most `max_run` tiles.  Returns (i0, j0, nx, ny) in level + 1 indices."""
+    at most `max_run` tiles.  The tags belong to the patches of `layout`;
+    nesting is checked against `owner_layout` (default: the same layout),
+    the level's new patches when it is being regridded too.  Returns
+    (i0, j0, nx, ny) in level + 1 indices."""
     nx_l = domain[0] * ratio ** level
     ny_l = domain[1] * ratio ** level
     owner = np.zeros((ny_l, nx_l), dtype=bool)
     tagged = np.zeros((ny_l, nx_l), dtype=bool)
     for (i0, j0, nx, ny), t in zip(layout, tags):
-        owner[j0:j0 + ny, i0:i0 + nx] = True
         tagged[j0:j0 + ny, i0:i0 + nx] |= t.astype(bool)
+    for (i0, j0, nx, ny) in (layout if owner_layout is None else owner_layout):
+        owner[j0:j0 + ny, i0:i0 + nx] = True
     tx, ty = -(-nx_l // tile), -(-ny_l // tile)
     keep = np.zeros((ty, tx), dtype=bool)
     for b in range(ty):

@@ -1082,6 +1082,7 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
     a.dt_ptr = &L->ctl->split_dt;
     a.halt = &L->ctl->halt;
     a.check = (w > 0);
+    a.bound_check = 1;
     a.partials = L->partials;
     rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] { return LaunchSweep(h->desc, s, seq[w], a); });
     if (rc) break;

@@ -62,6 +62,9 @@ struct SweepArgs {
   int muscl;              // 1: MUSCL-Hancock, 0: first order
   int quirk3d;
   int check;              // compute the CFL check of the input in this sweep
+  int bound_check;        // the check decides pass / fail only (level driver):
+                          // waves whose faces pass a cheap upper bound of
+                          // their signal speeds skip the exact speeds
   int seg;                // pencil segment length (Y/Z sweeps)
   unsigned long long* partials;  // per-workgroup minimum of the check
   int* halt;              // nonzero: skip all work (nullable)

@@ -15,7 +15,11 @@
 // One thread per face or cell; the boundary problems run one thread per cell
 // and return at once for cells that are not cut.  All arithmetic is FP64
 // with the reference's operation order (the divisions by small volume
-// fractions in particular).
+// fractions in particular), except in the FP32-flux form of the fused level
+// sweeps: there the masked MUSCL-Hancock states and the HLLEM fluxes are
+// float, while the CFL values, the boundary fluxes, the stabilised fluxes,
+// the h-box update with its divisions by the volume fraction and the
+// accumulation into the state stay FP64.
 #include "fvs_internal.hpp"
 #include "fvs_lanes.hpp"
 #include "fvs_physics.hpp"
@@ -355,56 +359,57 @@ __global__ __launch_bounds__(kBlock) void BoundaryFluxKernel(DevView bflux, DevV
 }
 
 // PerfectGas::CompleteFromCons, masked array form: the zero state unless
-// the lane is valid with positive density.
-template <int D>
-__device__ __forceinline__ void CompleteMasked(double* q, bool mask, const Gas<double>& g) {
-  if (mask && q[0] > 0.0) {
-    CompleteFromCons<D>(q, g);
+// the lane is valid with positive density.  T / A: the arithmetic of the
+// reconstruction (double IEEE, or float for the FP32-flux form).
+template <int D, class T = double, class A = IeeeArith<T>>
+__device__ __forceinline__ void CompleteMasked(T* q, bool mask, const Gas<T>& g) {
+  if (mask && q[0] > T(0)) {
+    CompleteFromCons<D, T, A>(q, g);
   } else {
 #pragma unroll
-    for (int k = 0; k < D + 4; ++k) q[k] = 0.0;
+    for (int k = 0; k < D + 4; ++k) q[k] = T(0);
   }
 }
 
 // Masked PerfectGas::Flux: zero for the zero state (density 1 stands in
 // for invalid lanes, so every product vanishes).
-template <int D, int DIR>
-__device__ __forceinline__ void FluxMasked(double* f, const double* q) {
-  if (q[0] > 0.0) {
-    PhysFlux<D, DIR>(f, q);
+template <int D, int DIR, class T = double, class A = IeeeArith<T>>
+__device__ __forceinline__ void FluxMasked(T* f, const T* q) {
+  if (q[0] > T(0)) {
+    PhysFlux<D, DIR, T, A>(f, q);
   } else {
 #pragma unroll
-    for (int k = 0; k < D + 2; ++k) f[k] = 0.0;
+    for (int k = 0; k < D + 2; ++k) f[k] = T(0);
   }
 }
 
 // Masked MUSCL-Hancock half-step states of one cell: slopes are zero unless
 // the three stencil cells are all uncovered.
-template <int D, int DIR, int LIM>
-__device__ __forceinline__ void MaskedRecon(const double* um, const double* u0, const double* up,
-                                            bool slope_mask, double lh, const Gas<double>& g,
-                                            double* recL, double* recR) {
+template <int D, int DIR, int LIM, class T = double, class A = IeeeArith<T>>
+__device__ __forceinline__ void MaskedRecon(const T* um, const T* u0, const T* up,
+                                            bool slope_mask, T lh, const Gas<T>& g,
+                                            T* recL, T* recR) {
   constexpr int NC = D + 2, NQ = D + 4;
-  double qL[NQ], qR[NQ];
+  T qL[NQ], qR[NQ];
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    const double s = slope_mask ? LimitedSlope<LIM>(um[k], u0[k], up[k]) : 0.0;
-    qL[k] = u0[k] - 0.5 * s;
-    qR[k] = u0[k] + 0.5 * s;
+    const T s = slope_mask ? LimitedSlope<LIM, T, A>(um[k], u0[k], up[k]) : T(0);
+    qL[k] = u0[k] - T(0.5) * s;
+    qR[k] = u0[k] + T(0.5) * s;
   }
-  CompleteMasked<D>(qL, true, g);
-  CompleteMasked<D>(qR, true, g);
-  double fL[NC], fR[NC];
-  FluxMasked<D, DIR>(fL, qL);
-  FluxMasked<D, DIR>(fR, qR);
+  CompleteMasked<D, T, A>(qL, true, g);
+  CompleteMasked<D, T, A>(qR, true, g);
+  T fL[NC], fR[NC];
+  FluxMasked<D, DIR, T, A>(fL, qL);
+  FluxMasked<D, DIR, T, A>(fR, qR);
 #pragma unroll
   for (int k = 0; k < NC; ++k) {
-    const double dq = lh * (fL[k] - fR[k]);
+    const T dq = lh * (fL[k] - fR[k]);
     recR[k] = qR[k] + dq;
     recL[k] = qL[k] + dq;
   }
-  CompleteMasked<D>(recR, true, g);
-  CompleteMasked<D>(recL, true, g);
+  CompleteMasked<D, T, A>(recR, true, g);
+  CompleteMasked<D, T, A>(recL, true, g);
 }
 
 struct FluxParams {
@@ -1484,7 +1489,19 @@ __device__ __forceinline__ void KbnUpdateCell(const LevelState& in, const LevelG
 // right-face flux.  Ghost columns (two per side) are filled beforehand.
 constexpr int kKbnOwned = 60;
 
-template <int LIM, bool LARR>
+// Flux arithmetic of the fused cut-cell sweeps: T = double with IEEE
+// division and square root (bit-exact), or T = float with the hardware
+// estimates (the FP32-flux form).  The masked CFL, the h-box update with its
+// divisions by the volume fraction and the accumulation stay FP64.
+template <class T> __device__ __forceinline__ Gas<T> GasAs(const Gas<double>& g) {
+  Gas<T> r;
+  r.gamma = static_cast<T>(g.gamma);
+  r.gm1inv = static_cast<T>(g.gm1inv);
+  r.gm1 = static_cast<T>(g.gm1);
+  return r;
+}
+
+template <int LIM, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelState out,
                                                           LevelGeom g, int64_t nx, int64_t ny,
                                                           double dx, const double* dt_ptr,
@@ -1492,7 +1509,9 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
                                                           int second,
                                                           unsigned long long* partials,
                                                           const int* halt) {
+  using TA = typename Policies<T, FAST>::TA;
   if (*halt) return;
+  const Gas<T> gasT = GasAs<T>(gas);
   const int lane = threadIdx.x & 63;
   const int64_t nseg = (nx + kKbnOwned - 1) / kKbnOwned;
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
@@ -1517,9 +1536,18 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   const bool mask = (alpha_m > 0.0) & (alpha > 0.0) & (alpha_p > 0.0);
   const double dt = *dt_ptr;
   const double lh = DivRn(0.5 * dt, dx);
-  double recL[6], recR[6];
-  MaskedRecon<2, 0, LIM>(qm, q, qp, mask, lh, gas, recL, recR);
-  double recRm[6];
+  T recL[6], recR[6];
+  {
+    T tm[4], t0[4], tp[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      tm[k] = static_cast<T>(qm[k]);
+      t0[k] = static_cast<T>(q[k]);
+      tp[k] = static_cast<T>(qp[k]);
+    }
+    MaskedRecon<2, 0, LIM, T, TA>(tm, t0, tp, mask, static_cast<T>(lh), gasT, recL, recR);
+  }
+  T recRm[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) recRm[k] = ShiftUp(recR[k]);
   // left face of the lane's cell: faces 0 .. nx, lanes 2 .. 62
@@ -1528,7 +1556,10 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   double F[4] = {0.0, 0.0, 0.0, 0.0};
   unsigned long long key = 0x7ff0000000000000ull;
   if (face_lane && beta > 0.0) {
-    HllemFlux<2, 0, LARR, false>(F, recRm, recL, gas);
+    T Ft[4];
+    HllemFlux<2, 0, LARR, false, T, TA>(Ft, recRm, recL, gasT);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F[k] = static_cast<double>(Ft[k]);
     key = DtKey(HllemStableDt<2, 0>(qm, q, dx, gas));
   }
   double Fr[4];
@@ -1549,7 +1580,7 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
 // cell's right half-step state and the previous face's flux.
 constexpr int kKbnSeg = 16;
 
-template <int LIM, bool LARR>
+template <int LIM, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelState out,
                                                           LevelGeom g, int64_t nx, int64_t ny,
                                                           double dx, const double* dt_ptr,
@@ -1557,7 +1588,9 @@ __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelSt
                                                           int second,
                                                           unsigned long long* partials,
                                                           const int* halt) {
+  using TA = typename Policies<T, FAST>::TA;
   if (*halt) return;
+  const Gas<T> gasT = GasAs<T>(gas);
   const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   const bool col_ok = i < nx;
   const int64_t ic = col_ok ? i : nx - 1;
@@ -1576,18 +1609,31 @@ __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelSt
   fetch(j0 - 2, qm, am);
   fetch(j0 - 1, q0, a0);
   fetch(j0, qp, ap);
-  double recRm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  T recRm[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
   double Fm[4] = {0.0, 0.0, 0.0, 0.0};
   unsigned long long key = 0x7ff0000000000000ull;
   for (int64_t t = j0 - 1; t <= j1; ++t) {
     // q0 is cell t, qm cell t - 1, qp cell t + 1
-    double recL[6], recR[6];
-    MaskedRecon<2, 1, LIM>(qm, q0, qp, am > 0.0 && a0 > 0.0 && ap > 0.0, lh, gas, recL, recR);
+    T recL[6], recR[6];
+    {
+      T tm[4], t0[4], tp[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        tm[k] = static_cast<T>(qm[k]);
+        t0[k] = static_cast<T>(q0[k]);
+        tp[k] = static_cast<T>(qp[k]);
+      }
+      MaskedRecon<2, 1, LIM, T, TA>(tm, t0, tp, am > 0.0 && a0 > 0.0 && ap > 0.0,
+                                    static_cast<T>(lh), gasT, recL, recR);
+    }
     if (t >= j0) {
       // face t between cells t - 1 and t
       double F[4] = {0.0, 0.0, 0.0, 0.0};
       if (g.beta_at(1, ic, t) > 0.0) {
-        HllemFlux<2, 1, LARR, false>(F, recRm, recL, gas);
+        T Ft[4];
+        HllemFlux<2, 1, LARR, false, T, TA>(Ft, recRm, recL, gasT);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) F[k] = static_cast<double>(Ft[k]);
         if (col_ok) key = KeyMin(key, DtKey(HllemStableDt<2, 1>(qm, q0, dx, gas)));
       }
       if (t - 1 >= j0 && col_ok) {
@@ -1919,9 +1965,14 @@ static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
   const dim3 grid = dir == 0 ? dim3(static_cast<unsigned>(nb))
                              : dim3(static_cast<unsigned>((L->nx + kBlock - 1) / kBlock),
                                     static_cast<unsigned>((L->ny + kKbnSeg - 1) / kKbnSeg));
-#define FVS_KS(K, LM, LA)                                                                 \
-  hipLaunchKernelGGL((K<LM, LA>), grid, dim3(kBlock), 0, s, in, out, L->lg, L->nx, L->ny, \
-                     dx, dtp, gas, L->fb, L->integrator, L->partials, halt)
+  const bool fp32 = h->desc.precision == FVS_FP32_FLUX;
+#define FVS_KS(K, LM, LA)                                                                     \
+  if (fp32)                                                                                   \
+    hipLaunchKernelGGL((K<LM, LA, float, true>), grid, dim3(kBlock), 0, s, in, out, L->lg,    \
+                       L->nx, L->ny, dx, dtp, gas, L->fb, L->integrator, L->partials, halt);  \
+  else                                                                                        \
+    hipLaunchKernelGGL((K<LM, LA, double, false>), grid, dim3(kBlock), 0, s, in, out, L->lg,  \
+                       L->nx, L->ny, dx, dtp, gas, L->fb, L->integrator, L->partials, halt)
 #define FVS_KS_LIM(K, LA)                                  \
   switch (h->desc.limiter) {                               \
   case kMinMod: FVS_KS(K, kMinMod, LA); break;             \

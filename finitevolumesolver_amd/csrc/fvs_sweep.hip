@@ -68,27 +68,41 @@ __device__ __forceinline__ void StOff(double* base, uint32_t byte_off, double v)
   *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
 
-// Block-wide minimum of NK keys per thread; thread 0 writes them to
-// out[d * stride] (d < NK).  Every thread of the block must call it.
-template <int NK, int NWAVES>
-__device__ __forceinline__ void BlockMinToPartials(const unsigned long long* key,
-                                                   unsigned long long* out,
-                                                   int64_t stride) {
-  __shared__ unsigned long long s_keys[NK][NWAVES];
+// Largest of two signal speeds; a NaN speed never wins (as a NaN face dt
+// never wins the minimum).
+__device__ __forceinline__ double SMax(double a, double b) { return __builtin_fmax(a, b); }
+
+__device__ __forceinline__ double WaveMaxSpeed(double s) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s = SMax(s, __shfl_xor(s, off, 64));
+  return s;
+}
+
+// Block-wide largest signal speed of NK directions (speeds are >= 0, the
+// neutral value is 0); thread 0 writes the key of the block's smallest face
+// dt, dx[d] / speed, to out[d * stride] (+inf when every speed is 0).  A
+// face's stable dt is dx / speed, and a correctly rounded quotient falls
+// monotonically with its denominator, so the smallest face dt is the
+// quotient by the largest speed, bit for bit: one division per block and
+// direction instead of one per face.  Every thread of the block must call it.
+template <int NK, int NWAVES, class A>
+__device__ __forceinline__ void BlockSpeedToPartials(const double* speed, const double* dx,
+                                                     unsigned long long* out, int64_t stride) {
+  __shared__ double s_speed[NK][NWAVES];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
 #pragma unroll
   for (int d = 0; d < NK; ++d) {
-    const unsigned long long k = WaveMinKey(key[d]);
-    if (lane == 0) s_keys[d][wave] = k;
+    const double v = WaveMaxSpeed(speed[d]);
+    if (lane == 0) s_speed[d][wave] = v;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int d = 0; d < NK; ++d) {
-      unsigned long long k = s_keys[d][0];
-      for (int w = 1; w < NWAVES; ++w) k = KMin(k, s_keys[d][w]);
-      out[d * stride] = k;
+      double v = s_speed[d][0];
+      for (int w = 1; w < NWAVES; ++w) v = SMax(v, s_speed[d][w]);
+      out[d * stride] = v > 0.0 ? DtKey(A::quot(dx[d], v)) : kInfKey;
     }
   }
 }
@@ -119,9 +133,11 @@ __device__ __forceinline__ DtCell<D> MakeDtCell(const double* u, const Gas<doubl
   return r;
 }
 
+// The face's largest signal speed max(|sL|, |sR|); its stable dt is
+// dx / speed (reduced per block by BlockSpeedToPartials).
 template <int D, int DIR, class A>
-__device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R, double dx,
-                                         double gm1) {
+__device__ __forceinline__ double FaceSpeed(const DtCell<D>& L, const DtCell<D>& R,
+                                            double gm1) {
   const double sqRhoSum = L.sq + R.sq;
   const typename A::Inv isum = A::inv(sqRhoSum);
   const double wL = A::div(L.sq, isum);
@@ -138,8 +154,67 @@ __device__ __forceinline__ double FaceDt(const DtCell<D>& L, const DtCell<D>& R,
   const double maxA = A::max(L.a, R.a);
   const double sL = A::min(L.u[DIR] - maxA, roeU[DIR] - roeA);
   const double sR = A::max(roeU[DIR] + roeA, R.u[DIR] + maxA);
-  const double maxS = A::max(rabs(sL), rabs(sR));
-  return A::quot(dx, maxS);
+  return A::max(rabs(sL), rabs(sR));
+}
+
+// Sufficient test of a sweep's CFL check (the level driver needs only its
+// outcome while it passes).  With w = sqrt(rho_L) / (sqrt(rho_L) +
+// sqrt(rho_R)) and h = a^2 / (g - 1) + |u|^2 / 2, the Roe sound speed of
+// FaceSpeed satisfies a~^2 = w aL^2 + (1 - w) aR^2 + (g - 1) / 2 w (1 - w)
+// |uR - uL|^2 <= max(aL^2, aR^2) + (g - 1) / 8 |uR - uL|^2, and |u~_n| <=
+// max(|uL_n|, |uR_n|), so the face speed is at most U + sqrt(A^2 + k |du|^2)
+// (U the larger normal speed, A^2 the larger squared sound speed, k = (g -
+// 1) / 8).  A face passes when that bound stays below dx / dt by a relative
+// margin of 2^-20, far above the rounding of either side; then its stable
+// dt, dx / speed, is at least dt and the exact check passes too.  A wave
+// evaluates the exact speeds when any of its faces does not pass the bound,
+// so a failing check reports the exact minimum dt as before.  The bound
+// uses the refined hardware reciprocal in every arithmetic mode.
+template <int D> struct BoundCell {
+  double u[D];
+  double a2;  // squared speed of sound, gamma p / rho
+};
+
+template <int D>
+__device__ __forceinline__ BoundCell<D> MakeBoundCell(const double* q, const Gas<double>& g) {
+  using FA = FastArith<double>;
+  const FA::Inv ir = FA::inv(q[0]);
+  BoundCell<D> b;
+#pragma unroll
+  for (int i = 0; i < D; ++i) b.u[i] = FA::div(q[1 + i], ir);
+  const double e_kin = FA::div(0.5 * SqNormArray<D>(q + 1), ir);
+  const double p = (q[D + 1] - e_kin) * g.gm1;
+  b.a2 = FA::div(g.gamma * p, ir);
+  return b;
+}
+
+template <int D, int DIR>
+__device__ __forceinline__ bool BoundPasses(const BoundCell<D>& L, const BoundCell<D>& R,
+                                            double s_crit, double k) {
+  const double U = __builtin_fmax(rabs(L.u[DIR]), rabs(R.u[DIR]));
+  const double T = s_crit - U;
+  double du2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const double d = R.u[i] - L.u[i];
+    du2 += d * d;
+  }
+  const double q = __builtin_fmax(L.a2, R.a2) + k * du2;
+  return T >= 0.0 && L.a2 >= 0.0 && R.a2 >= 0.0 && q <= T * T;
+}
+
+template <int D>
+__device__ __forceinline__ BoundCell<D> ShflUp(const BoundCell<D>& v) {
+  BoundCell<D> r;
+#pragma unroll
+  for (int i = 0; i < D; ++i) r.u[i] = ShiftUp(v.u[i]);
+  r.a2 = ShiftUp(v.a2);
+  return r;
+}
+
+// dx / dt shrunk by the bound's margin (+inf for dt = 0: every face passes).
+__device__ __forceinline__ double BoundSpeedLimit(double dx, double dt) {
+  return (dx / dt) * (1.0 - 0x1p-20);
 }
 
 template <int D>
@@ -238,6 +313,8 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
   const double dt_dx = DivRn(dt, a.dx);
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
+  const double s_crit = BoundSpeedLimit(a.dx, dt);
+  const double kbound = 0.125 * gd.gm1;
 
   // Slot of this lane in chunk c (fvs_xslots.hpp): the row base is lane 0's
   // row, read across the wave.
@@ -255,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
     for (int k = 0; k < NC; ++k) u[k] = LdOff(a.in[k] + s.wbase, off);
   };
 
-  unsigned long long key = kInfKey;
+  double smax = 0.0;
   // The chunk index is wave-uniform (a scalar register), so the chunk loop is
   // a uniform branch and its DPP shifts always run with the whole wave.
   int64_t c = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) +
@@ -279,17 +356,23 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
       um[k] = ShiftUp(u[k]);
       up[k] = ShiftDown(u[k]);
     }
-    // A ghost slot's stencil is three copies of the boundary cell (its own
-    // clamped value): its zero-slope reconstruction equals the ghost's
-    // complete state (up to the sign of zero components), so every lane runs
-    // the same reconstruction and no wave takes two paths.
-    if (cur.cell < 0) {
+    // A ghost slot reconstructs with a zero slope, so its half-step states
+    // equal the ghost's complete state (up to the sign of zero components)
+    // and every lane runs the same reconstruction.  The ghost's clamped value
+    // equals its inner neighbour's (slot 0 and slot 1, slot nx and nx + 1
+    // hold the same cell), so one of its one-sided differences is exactly
+    // zero, and MinMod and VanLeer return a zero slope whatever the other
+    // neighbour holds (even NaN or inf).  The central slope of NoLimiter
+    // does not vanish: there the outer neighbour is replaced by the cell.
+    if (a.muscl && a.limiter == kNoLimiter) {
+      if (cur.cell < 0) {
 #pragma unroll
-      for (int k = 0; k < NC; ++k) um[k] = u[k];
-    }
-    if (cur.cell >= nx) {
+        for (int k = 0; k < NC; ++k) um[k] = u[k];
+      }
+      if (cur.cell >= nx) {
 #pragma unroll
-      for (int k = 0; k < NC; ++k) up[k] = u[k];
+        for (int k = 0; k < NC; ++k) up[k] = u[k];
+      }
     }
 
     T recL[NQ], recR[NQ];
@@ -336,15 +419,25 @@ __global__ __launch_bounds__(kBlock) void SweepXKernel(SweepArgs a) {
       }
     }
     if (a.check) {
-      const DtCell<D> dc = MakeDtCell<D, DA>(u, gd);
-      const DtCell<D> dp = ShflUp<D>(dc);
-      if (owned && face_lane) key = KMin(key, DtKey(FaceDt<D, 0, DA>(dp, dc, a.dx, gd.gm1)));
+      // chunk loop is wave-uniform, so the vote and both paths are too
+      bool exact = true;
+      if (a.bound_check) {
+        const BoundCell<D> bc = MakeBoundCell<D>(u, gd);
+        const BoundCell<D> bp = ShflUp<D>(bc);
+        const bool pass = !(owned && face_lane) || BoundPasses<D, 0>(bp, bc, s_crit, kbound);
+        exact = !__all(pass);
+      }
+      if (exact) {
+        const DtCell<D> dc = MakeDtCell<D, DA>(u, gd);
+        const DtCell<D> dp = ShflUp<D>(dc);
+        if (owned && face_lane) smax = SMax(smax, FaceSpeed<D, 0, DA>(dp, dc, gd.gm1));
+      }
     }
     cur = nxt;
 #pragma unroll
     for (int k = 0; k < NC; ++k) u[k] = un[k];
   }
-  if (a.check) BlockMinToPartials<1, kBlock / 64>(&key, a.partials + blockIdx.x, 0);
+  if (a.check) BlockSpeedToPartials<1, kBlock / 64, DA>(&smax, &a.dx, a.partials + blockIdx.x, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,8 +469,10 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
   const double dt_dx = DivRn(dt, a.dx);
   const Gas<T> g = MakeGas<T>(a);
   const Gas<double> gd = MakeGas<double>(a);
+  const double s_crit = BoundSpeedLimit(a.dx, dt);
+  const double kbound = 0.125 * gd.gm1;
 
-  unsigned long long key = kInfKey;
+  double smax = 0.0;
   if (active) {
     double Um[NC], U0[NC], Up[NC], Unext[NC];
     auto load = [&](double* u, int64_t j) {
@@ -411,8 +506,8 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
                                 irRprev);
       }
     }
-    DtCell<D> dprev;
-    if (a.check) dprev = MakeDtCell<D, DA>(Um, gd);
+    BoundCell<D> bprev;
+    if (a.check && a.bound_check) bprev = MakeBoundCell<D>(Um, gd);
     T Fprev[NC];
     for (int64_t j = j0; j <= j1; ++j) {
 #pragma unroll
@@ -449,9 +544,19 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
         }
       }
       if (a.check && (j < j1 || j1 == nd)) {
-        const DtCell<D> d0 = MakeDtCell<D, DA>(U0, gd);
-        key = KMin(key, DtKey(FaceDt<D, DIR, DA>(dprev, d0, a.dx, gd.gm1)));
-        dprev = d0;
+        // The face index is uniform over the workgroup, so the vote covers
+        // one face per active pencil.
+        bool exact = true;
+        if (a.bound_check) {
+          const BoundCell<D> b0 = MakeBoundCell<D>(U0, gd);
+          exact = !__all(BoundPasses<D, DIR>(bprev, b0, s_crit, kbound));
+          bprev = b0;
+        }
+        if (exact) {
+          const DtCell<D> dm = MakeDtCell<D, DA>(Um, gd);
+          const DtCell<D> d0 = MakeDtCell<D, DA>(U0, gd);
+          smax = SMax(smax, FaceSpeed<D, DIR, DA>(dm, d0, gd.gm1));
+        }
       }
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
@@ -465,9 +570,8 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
     }
   }
   if (a.check) {
-    BlockMinToPartials<1, kBlock / 64>(&key,
-                                       a.partials + blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x,
-                                       0);
+    BlockSpeedToPartials<1, kBlock / 64, DA>(
+        &smax, &a.dx, a.partials + blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x, 0);
   }
 }
 
@@ -509,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
   const double dxs[2] = {dx0, dx1};
 
   __shared__ DtCell<D> s_cells[kTX];
-  unsigned long long key[3] = {kInfKey, kInfKey, kInfKey};
+  double smax[2] = {0.0, 0.0};
 
   auto load_cell = [&](int64_t off) {
     double u[NC];
@@ -523,6 +627,8 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
     for (int c = 0; c < NC; ++c) u[c] = in[c][off];
   };
 
+  // Row m0 - 1 is clamped to row 0 at the lower edge, so there dprev is the
+  // row's own transmissive ghost copy and the face pairs the cell with it.
   double ucur[NC], unext[NC];
   load(unext, m0 - 1);
   DtCell<D> dprev = MakeDtCell<D, DA>(unext, g);
@@ -535,11 +641,8 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
 #pragma unroll
     for (int c = 0; c < NC; ++c) ucur[c] = unext[c];
     // March direction y.
-    {
-      const DtCell<D>& l = (m == 0) ? dc : dprev;
-      key[M] = KMin(key[M], DtKey(FaceDt<D, M, DA>(l, dc, dxs[M], g.gm1)));
-      if (m == nm - 1) key[M] = KMin(key[M], DtKey(FaceDt<D, M, DA>(dc, dc, dxs[M], g.gm1)));
-    }
+    smax[M] = SMax(smax[M], FaceSpeed<D, M, DA>(dprev, dc, g.gm1));
+    if (m == nm - 1) smax[M] = SMax(smax[M], FaceSpeed<D, M, DA>(dc, dc, g.gm1));
     // x direction through LDS.
     s_cells[tx] = dc;
     __syncthreads();
@@ -552,24 +655,23 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
       } else {
         dl = load_cell(off - 1);
       }
-      key[0] = KMin(key[0], DtKey(FaceDt<D, 0, DA>(dl, dc, dxs[0], g.gm1)));
-      if (x == nx - 1) key[0] = KMin(key[0], DtKey(FaceDt<D, 0, DA>(dc, dc, dxs[0], g.gm1)));
+      smax[0] = SMax(smax[0], FaceSpeed<D, 0, DA>(dl, dc, g.gm1));
+      if (x == nx - 1) smax[0] = SMax(smax[0], FaceSpeed<D, 0, DA>(dc, dc, g.gm1));
     }
     __syncthreads();
     dprev = dc;
   }
-  if (!active) key[0] = key[1] = kInfKey;
+  if (!active) smax[0] = smax[1] = 0.0;
   const int64_t nblocks = static_cast<int64_t>(gridDim.x) * gridDim.y;
   const int64_t bid = blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x;
-  BlockMinToPartials<D, kBlock / 64>(key, partials + bid, nblocks);
+  BlockSpeedToPartials<D, kBlock / 64, DA>(smax, dxs, partials + bid, nblocks);
 }
 
 // 3D step-start CFL.  A workgroup is a tile of 9 x rows of 64 cells (one
 // wave each) marching along z over a segment.  Lanes 1..62 of a wave own
-// their cells and lanes 0 and 63 hold the x neighbours, so x faces pair
-// neighbouring lanes through a shuffle; row 0 is a halo row (the y - 1
-// neighbours of row 1) and y faces read the row below from LDS; z faces pair
-// the plane with the previous one kept in registers.  Each cell's
+// their cells and lanes 0 and 63 hold the x neighbours; row 0 is a halo row (the y - 1
+// neighbours of row 1) and x and y faces read their neighbours from LDS; z
+// faces pair the plane with the previous one kept in registers.  Each cell's
 // DtCell is computed 64 * 9 / (62 * 8) = 1.16 times instead of twice.
 // Tiles are grouped per XCD (workgroups are dispatched round-robin over the 8
 // XCDs): XCD k gets a contiguous range of tiles, so vertically neighbouring
@@ -613,12 +715,17 @@ __global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
   const int64_t col = xs + ys * nx;
 
   __shared__ DtCell<3> s_cells[2][kC3Rows][64];
-  unsigned long long key[3] = {kInfKey, kInfKey, kInfKey};
+  double smax[3] = {0.0, 0.0, 0.0};
+  const double dxs[3] = {dx0, dx1, dx2};
   auto load = [&](double* u, int64_t m) {
     const int64_t off = col + Clamp(m, nz) * plane;
 #pragma unroll
     for (int c = 0; c < NC; ++c) u[c] = in[c][off];
   };
+  // Indices are clamped, so at a lower patch edge the neighbour a face
+  // pairs with (lane 0 at x = -1, row 0 at y = -1, plane m0 - 1 = -1) holds
+  // the cell's own data: its DtCell equals the cell's bit for bit, which
+  // makes it the transmissive ghost copy without a select.
   double ucur[NC], un1[NC], un2[NC];
   load(un2, m0 - 1);
   DtCell<3> dprev = MakeDtCell<3, DA>(un2, g);
@@ -636,23 +743,23 @@ __global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
       un1[c] = un2[c];
     }
     s_cells[buf][row][lane] = dc;
-    const DtCell<3> dl = ShflUp<3>(dc);
     __syncthreads();
     if (own) {
-      key[0] = KMin(key[0], DtKey(FaceDt<3, 0, DA>(x == 0 ? dc : dl, dc, dx0, g.gm1)));
-      if (x == nx - 1) key[0] = KMin(key[0], DtKey(FaceDt<3, 0, DA>(dc, dc, dx0, g.gm1)));
+      const DtCell<3> dl = s_cells[buf][row][lane - 1];
+      smax[0] = SMax(smax[0], FaceSpeed<3, 0, DA>(dl, dc, g.gm1));
+      if (x == nx - 1) smax[0] = SMax(smax[0], FaceSpeed<3, 0, DA>(dc, dc, g.gm1));
       const DtCell<3> db = s_cells[buf][row - 1][lane];
-      key[1] = KMin(key[1], DtKey(FaceDt<3, 1, DA>(y == 0 ? dc : db, dc, dx1, g.gm1)));
-      if (y == ny - 1) key[1] = KMin(key[1], DtKey(FaceDt<3, 1, DA>(dc, dc, dx1, g.gm1)));
-      key[2] = KMin(key[2], DtKey(FaceDt<3, 2, DA>(m == 0 ? dc : dprev, dc, dx2, g.gm1)));
-      if (m == nz - 1) key[2] = KMin(key[2], DtKey(FaceDt<3, 2, DA>(dc, dc, dx2, g.gm1)));
+      smax[1] = SMax(smax[1], FaceSpeed<3, 1, DA>(db, dc, g.gm1));
+      if (y == ny - 1) smax[1] = SMax(smax[1], FaceSpeed<3, 1, DA>(dc, dc, g.gm1));
+      smax[2] = SMax(smax[2], FaceSpeed<3, 2, DA>(dprev, dc, g.gm1));
+      if (m == nz - 1) smax[2] = SMax(smax[2], FaceSpeed<3, 2, DA>(dc, dc, g.gm1));
     }
     dprev = dc;
     buf ^= 1;
   }
   const int64_t nblocks = static_cast<int64_t>(gridDim.x) * gridDim.y;
   const int64_t bid = blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x;
-  BlockMinToPartials<3, kC3Rows>(key, partials + bid, nblocks);
+  BlockSpeedToPartials<3, kC3Rows, DA>(smax, dxs, partials + bid, nblocks);
 }
 
 int64_t Cfl3Tiles(int64_t nx, int64_t ny) {

@@ -87,6 +87,18 @@ def resting_blob(level, p, layouts=LAYOUTS):
     return cons_from(rho, z, z, p_)
 
 
+def shock_bubble(level, p, layouts=LAYOUTS):
+    """The shock-bubble state of the batched-AMR workload on the small
+    hierarchy: a Mach 1.5 shock at x = -0.5 (on the left face of the first
+    level-1 patch, so it crosses coarse-fine boundaries in the first steps)
+    moving into rho = 1, p = 1, and a light bubble (r = 0.2, rho = 0.138)
+    at the origin, inside level 2."""
+    from finitevolumesolver_amd.initial_data import shock_bubble_state
+    x, y = centres(layouts[level][p], level)
+    rho, u, v, p_ = shock_bubble_state(x, y)
+    return cons_from(rho, u, v, p_)
+
+
 def oracle_run(o, steps, cfl, max_restarts=64, restore_cycle=True):
     """RunSimulation's loop (include/fub/RunSimulation.hpp:86-191) over the
     AMR oracle: dt = cfl * stable dt, one coarse step, each level's sweep

@@ -143,3 +143,15 @@ def test_retry_keeps_the_split_order_of_the_failed_attempt():
     assert att1[0] == 2 and att1 == att0
     assert o1.cycle == 2
     assert any(not np.array_equal(a, b) for a, b in zip(o1.U[2], o0.U[2]))
+
+
+def test_nests_follows_the_tile_rule():
+    """nests(): a level-2 patch grown by one level-1 cell must lie inside
+    the level-1 patches, clamped to the level-1 extent at the domain edge."""
+    from finitevolumesolver_amd.amr import nests
+    lay1 = A.LAYOUTS[1]
+    ext = (64, 64)
+    assert nests((36, 36, 24, 40), lay1, ext)        # inside patches 0 and 1 with margin
+    assert not nests((32, 32, 8, 8), lay1, ext)      # its margin leaves patch 0 at x = 15
+    assert nests((100, 0, 24, 16), lay1, ext)        # on the domain edge y = 0 (clamped)
+    assert not nests((96, 0, 32, 16), lay1, ext)     # margin at x = 47 is outside patch 3

@@ -54,9 +54,10 @@ def _compare(h, o, what):
 
 
 @pytest.mark.parametrize("state,graphs", [("moving", True), ("moving", False),
-                                          ("resting", True)])
+                                          ("resting", True), ("shock_bubble", True)])
 def test_amr_steps_match_oracle(cuda, state, graphs):
-    fn = A.moving_state if state == "moving" else A.resting_blob
+    fn = {"moving": A.moving_state, "resting": A.resting_blob,
+          "shock_bubble": A.shock_bubble}[state]
     _note(f"amr parity: {state} state, 3 levels, graphs {graphs}")
     fm, h, o = _pair(fn, graphs)
     _compare(h, o, "after average_down")
@@ -163,6 +164,98 @@ def test_amr_regrid_transfer(cuda):
     assert np.all(np.isfinite(new.level_cfl()))
 
 
+def _expected_transfer(old, new, level, coarse_new):
+    """Per new patch of `level`: the old same-level cell where the old
+    layout had one, else the new coarser level's cell above it."""
+    olds = old.interiors(level)
+    out = []
+    for (i0, j0, nx, ny) in new.layouts[level]:
+        exp = np.zeros((4, ny, nx))
+        for j in range(ny):
+            for i in range(nx):
+                I, J = i0 + i, j0 + j
+                src = None
+                for q, (a, b, w, hh) in enumerate(old.layouts[level]):
+                    if a <= I < a + w and b <= J < b + hh:
+                        src = olds[q][:, J - b, I - a]
+                if src is None:
+                    for q, (a, b, w, hh) in enumerate(new.layouts[level - 1]):
+                        if a <= I // 2 < a + w and b <= J // 2 < b + hh:
+                            src = coarse_new[q][:, J // 2 - b, I // 2 - a]
+                exp[:, j, i] = src
+        out.append(exp)
+    return out
+
+
+def test_amr_regrid_all_finer_levels(cuda):
+    """RegridAllFinerlevels(0) on the shock-bubble state after two steps:
+    levels 1 and 2 are rebuilt from the tags of the current levels 0 and 1,
+    level 2 nested in the new level 1; level 0 keeps its state bit for bit,
+    every new cell holds the old same-level cell or, where the old layout had
+    none, the new coarser cell above it; the new hierarchy keeps the cycle
+    and steps on."""
+    from finitevolumesolver_amd.amr import nests
+    _note("amr regrid all: shock-bubble state, two steps, regrid levels 1 and 2")
+    fm, h, o = _pair(A.shock_bubble)
+    for _ in range(2):
+        h.advance(0.8 * h.stable_dt())
+    new = h.regrid_all(0, 0.05, 0.05, tile=4)
+    _note(f"  level 1: {len(h.layouts[1])} -> {len(new.layouts[1])} patches, "
+          f"level 2: {len(h.layouts[2])} -> {len(new.layouts[2])} patches")
+    assert new.layouts[1] != h.layouts[1] or new.layouts[2] != h.layouts[2]
+    for l in (1, 2):
+        ext = (A.DOMAIN[0] * 2 ** (l - 1), A.DOMAIN[1] * 2 ** (l - 1))
+        assert all(nests(p, new.layouts[l - 1], ext, 2, 1) for p in new.layouts[l])
+    for a, b in zip(h.interiors(0), new.interiors(0)):
+        np.testing.assert_array_equal(a, b)
+    for l in (1, 2):
+        exp = _expected_transfer(h, new, l, new.interiors(l - 1))
+        for p, got in enumerate(new.interiors(l)):
+            np.testing.assert_array_equal(got, exp[p], err_msg=f"level {l} patch {p}")
+    assert new.cycle == h.cycle == 2
+    new.advance(0.8 * new.stable_dt())
+    assert np.all(np.isfinite(new.level_cfl()))
+
+
+def test_amr_run_regrids_on_a_cadence(cuda):
+    """run() with regrid_every = 2 rebuilds the finer levels before steps 2
+    and 4 of the shock-bubble run: the layouts follow the shock, every step
+    passes its CFL checks, and the state stays finite and positive."""
+    _note("amr run with regridding: shock-bubble state, 5 steps, every 2")
+    fm, h, o = _pair(A.shock_bubble)
+    before = [list(l) for l in h.layouts]
+    dts = h.run(5, cfl=0.8, regrid_every=2, tol_rho=0.05, tol_p=0.05)
+    assert len(dts) == 5 and h.cycle == 5
+    assert h.layouts != before
+    for l in range(h.nlevels):
+        for u in h.interiors(l):
+            assert np.all(np.isfinite(u)) and np.all(u[0] > 0)
+    _note(f"  dts {dts}, level sizes {[len(l) for l in h.layouts]}")
+
+
+def test_amr_checkpoint_restart_continues_bit_for_bit(cuda, tmp_path):
+    """WriteCheckpointFile after two steps of the shock-bubble run, then
+    ReadCheckpointFile into a new hierarchy: two more steps from the
+    restart equal two more steps of the original run bit for bit (dts,
+    every level, cycle and time)."""
+    from finitevolumesolver_amd import amrex_files as ck
+    _note("amr checkpoint/restart: shock-bubble state, 2 + 2 steps")
+    fm, h, o = _pair(A.shock_bubble)
+    h.run(2, cfl=0.8)
+    ck.write_checkpoint(tmp_path / "chk00002", h)
+    r = ck.read_checkpoint(tmp_path / "chk00002", fm, A.DOMAIN, A.DX0, 2)
+    assert r.layouts == h.layouts and r.cycle == h.cycle == 2 and r.time == h.time
+    for l in range(h.nlevels):
+        for a, b in zip(h.interiors(l), r.interiors(l)):
+            np.testing.assert_array_equal(a, b)
+    dts_a = h.run(2, cfl=0.8)
+    dts_b = r.run(2, cfl=0.8)
+    assert dts_a == dts_b and r.time == h.time and r.cycle == h.cycle == 4
+    for l in range(h.nlevels):
+        for a, b in zip(h.interiors(l), r.interiors(l)):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_amr_run_restarts_on_cfl_failure(cuda):
     """With cfl 1.6 every attempt of a step exceeds a level's stable dt (as
     in the reference, a cfl above 1 never passes the check): run() undoes
@@ -201,14 +294,15 @@ def test_amr_run_retry_keeps_split_order(cuda):
 @pytest.mark.parametrize("precision,tol", [("fp64_rcp", 1e-12), ("fp32", 1e-6)])
 def test_amr_tolerance_modes(cuda, precision, tol):
     """The refined-reciprocal FP64 and the FP32-flux forms of the hierarchy
-    step against the FP64 oracle over three steps (same dts): relative L1 of
+    step against the FP64 oracle over three steps (same dts) of the moving
+    blob, the resting bump and the shock-bubble state: relative L1 of
     density and energy per level within tol (1e-12 and 1e-6; momentum, whose
     sums are small, is reported);
     composite mass and energy of the resting bump conserved to tol."""
     _note(f"amr {precision}: moving state, 3 steps")
     fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock",
                         precision=precision)
-    for state in (A.moving_state, A.resting_blob):
+    for state in (A.moving_state, A.resting_blob, A.shock_bubble):
         h = AmrHierarchy(fm, A.DOMAIN, A.LAYOUTS, A.DX0, 2)
         h.set_state(state)
         ref = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock")

@@ -212,3 +212,42 @@ def test_regular_fluxes_stable_dt_and_update(case, cuda):
                                           g)
         np.testing.assert_array_equal(t_cq.cpu().numpy(), cq_ref)
         _note(f"  dir {d}: dt {dt_dev!r}, fluxes/update/reconstruction bit-exact")
+
+
+def test_channel_geometry_full_size_bitexact(cuda):
+    """The 2048 x 1024 channel (cylinder r = 0.25 and the 30-degree wedge,
+    small cells shrunk to volume fractions down to 1e-4) at its full size:
+    shielded fractions of both directions and IsCutCell over the whole grown
+    box equal the oracle's bit for bit, and the cut-cell level lists exactly
+    the oracle's interior cut cells."""
+    import torch
+    from finitevolumesolver_amd.geometry import channel_geometry
+    from finitevolumesolver_amd.initial_data import shock_channel
+    nx, ny = 2048, 1024
+    _note(f"channel geometry {nx} x {ny}")
+    geom = channel_geometry(nx, ny)
+    fm = _fm("hllem")
+    g = K.CutCellData.from_geometry2d(geom, cuda)
+    for d in range(2):
+        K.FillCutCellData(fm, g, d)
+    h = O.HostCutCellData(geom).fill()
+    for kind in h.fractions:
+        for d in range(2):
+            got = g.fractions[kind][d].data.cpu().numpy()
+            np.testing.assert_array_equal(got, h.fractions[kind][d], err_msg=f"{kind} {d}")
+    lo = geom.lo
+    hi = (lo[0] + geom.alpha.shape[1] - 1, lo[1] + geom.alpha.shape[0] - 1)
+    flags = K.IsCutCell(fm, g, lo, hi).cpu().numpy()
+    ref = np.zeros(flags.shape, np.int32)
+    s = h.c_struct()
+    O.load().oracle_kbn_is_cut_cell(2, C.byref(s), O.ptr(np.array([lo[0], lo[1], 0], np.int64)),
+                                    O.ptr(np.array([hi[0], hi[1], 0], np.int64)), O.ptr(ref))
+    np.testing.assert_array_equal(flags, ref)
+    interior = ref.reshape(geom.alpha.shape)[-lo[1]:-lo[1] + ny, -lo[0]:-lo[0] + nx]
+    cons, dx = shock_channel(nx, ny, x_shock=-0.4)
+    lvl = K.CutCellLevel(fm, (nx, ny), (dx, dx), g)
+    assert lvl.ncut == int(interior.sum())
+    _note(f"  fractions and classification bit-exact; {lvl.ncut} interior cut cells, "
+          f"min alpha {geom.alpha[geom.alpha > 0].min():.2e}")
+    del lvl, g
+    torch.cuda.empty_cache()

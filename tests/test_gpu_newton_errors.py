@@ -86,7 +86,8 @@ def test_vacuum_pair_is_reported(cuda):
     fm.newton_stats(reset=True)
     _operator_fluxes(fm, _pair_patch(rows), cuda)
     st = fm.newton_stats()
-    assert st["not_converged"] == 2 and st["max_iterations"] == 5000
+    # the patch operators count the faces that hit the cap (counter 2) only
+    assert st["not_converged"] == 2
     with pytest.raises(fvs.NewtonNotConverged):
         fm.check_newton(reset=True)
     # a level whose two halves move apart faster than their sound speeds
@@ -98,7 +99,9 @@ def test_vacuum_pair_is_reported(cuda):
     lvl.set_state(torch.from_numpy(np.ascontiguousarray(cons)).to(cuda))
     with pytest.raises(fvs.NewtonNotConverged):
         lvl.advance(2, batch=1)
-    assert fm.newton_stats(reset=True)["not_converged"] >= ny
+    # the fused level kernels also record the iteration maximum: the cap
+    st = fm.newton_stats(reset=True)
+    assert st["not_converged"] >= ny and st["max_iterations"] == 5000
 
 
 @pytest.mark.parametrize("bias", [1000.0, -1000.0])

@@ -79,3 +79,81 @@ def test_hdf5_layout_and_appended_frames(tmp_path):
     # a frame of another shape is refused
     with pytest.raises(OSError):
         output.write_hdf5(path, f1[:, :64], (-1.0, -1.0), (1.0, 1.0), 1.0, 10)
+
+
+class _FakeHierarchy:
+    """Stand-in with the attributes write_checkpoint reads (no GPU)."""
+
+    class _FM:
+        class equation:
+            gamma = 1.4
+
+    flux_method = _FM()
+    ratio = 2
+    cycle = 3
+    time = 0.1 + 0.2
+    layouts = [[(0, 0, 16, 32), (16, 0, 16, 32)], [(16, 16, 16, 24), (48, 0, 16, 16)]]
+    nlevels = 2
+
+    def interiors(self, level):
+        rng = np.random.default_rng(level)
+        out = []
+        for (_, _, nx, ny) in self.layouts[level]:
+            rho = 1.0 + rng.random((ny, nx))
+            m0, m1 = rng.random((ny, nx)) - 0.5, rng.random((ny, nx)) - 0.5
+            out.append(np.stack([rho, m0, m1, 3.0 + rng.random((ny, nx))]))
+        return out
+
+
+def test_checkpoint_layout_and_round_trip(tmp_path):
+    """The checkpoint directory of WriteCheckpointFile
+    (src/AMReX/PatchHierarchy.cpp:320-371): Header lines (title, finest
+    level, per-level cycles and time points, BoxArrays), per level a VisMF
+    header and data file whose FABs hold the complete state; reading it back
+    returns the layouts, cycles, time (bit for bit: 17 digits) and the
+    conservative components unchanged, with p and a from CompleteFromCons."""
+    from finitevolumesolver_amd import amrex_files as ck
+    from finitevolumesolver_amd.output import complete_2d
+    h = _FakeHierarchy()
+    ck.write_checkpoint(tmp_path / "chk00003", h)
+    lines = (tmp_path / "chk00003" / "Header").read_text().split("\n")
+    assert lines[:4] == ["Checkpoint File", "1", "3 6 ", "0.30000000000000004 0.30000000000000004 "]
+    assert lines[4:8] == ["(2 0", "((0,0) (15,31) (0,0))", "((16,0) (31,31) (0,0))", ")"]
+    finest, cycles, times, layouts = ck.read_header(tmp_path / "chk00003")
+    assert finest == 1 and cycles == [3, 6] and times == [h.time, h.time]
+    assert layouts == h.layouts
+    head = (tmp_path / "chk00003" / "Level_0" / "data_H").read_text().split("\n")
+    assert head[:4] == ["1", "1", "6", "0"] and head[8] == "2"
+    for l in range(2):
+        fabs = ck.read_level(tmp_path / "chk00003" / f"Level_{l}")
+        for got, u in zip(fabs, h.interiors(l)):
+            np.testing.assert_array_equal(got, complete_2d(u))
+    raw = (tmp_path / "chk00003" / "Level_0" / "data_D_00000").read_bytes()
+    assert raw.startswith(b"FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))"
+                          b"((0,0) (15,31) (0,0)) 6\n")
+
+
+def test_plotfile_layout_and_round_trip(tmp_path):
+    """WritePlotFile's HyperCLaw-V1.1 directory: variable names, time,
+    problem domain, ratios, level domains and steps in the Header, each
+    level's boxes in physical coordinates; the Cell multifabs read back to
+    the complete states."""
+    from finitevolumesolver_amd import amrex_files as ck
+    from finitevolumesolver_amd.output import complete_2d
+    h = _FakeHierarchy()
+    h.domain, h.dx = (32, 32), [2.0 / 32, 1.0 / 32]
+    ck.write_plotfile(tmp_path / "plt00003", h, prob_lo=(-1.0, -1.0))
+    head = (tmp_path / "plt00003" / "Header").read_text().split("\n")
+    assert head[:9] == ["HyperCLaw-V1.1", "6", "Density", "Momentum_0", "Momentum_1", "Energy",
+                        "Pressure", "SpeedOfSound", "2"]
+    assert head[9] == "0.30000000000000004" and head[10] == "1"
+    assert head[11:14] == ["-1 -1 ", "1 1 ", "2 "]
+    assert head[14] == "((0,0) (31,31) (0,0)) ((0,0) (63,63) (0,0)) " and head[15] == "3 6 "
+    assert "Level_1/Cell" in head
+    i = head.index("1 2 0.30000000000000004")
+    assert head[i + 1] == "6" and head[i + 2] == "-0.5 0" and head[i + 3] == "-0.5 0.25"
+    names, time, layouts, fabs = ck.read_plotfile(tmp_path / "plt00003")
+    assert names[0] == "Density" and time == h.time and layouts == h.layouts
+    for l in range(2):
+        for got, u in zip(fabs[l], h.interiors(l)):
+            np.testing.assert_array_equal(got, complete_2d(u))
