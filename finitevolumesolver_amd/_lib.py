@@ -184,6 +184,7 @@ SIGNATURES = {
     "fvs_amr_level_cfl": (_I, [_P, _P]),
     "fvs_amr_stable_dt": (_I, [_P, _PD]),
     "fvs_amr_set_graphs": (_I, [_P, _I]),
+    "fvs_amr_set_sequencing": (_I, [_P, _I]),
     "fvs_amr_cycle": (_I, [_P, C.POINTER(C.c_int64)]),
     "fvs_amr_set_cycle": (_I, [_P, C.c_int64]),
     "fvs_amr_tag_cells": (_I, [_P, _I, _D, _D, _P]),

@@ -164,6 +164,7 @@ class AmrHierarchy:
             setattr(self, k, theirs)
             setattr(new, k, mine)
         self.set_graphs(getattr(self, "_graphs", True))
+        self.set_sequencing(getattr(self, "_sequencing", "reference"))
 
     def write_hdf5(self, path, time: float, cycle: int, xlower=(0.0, 0.0), box=None) -> None:
         """Appends (or starts) the HDF5 record of the composite state on the
@@ -180,6 +181,18 @@ class AmrHierarchy:
             fab = fab[:, j0:j1 + 1, i0:i1 + 1]
         dx = self.dx[-1]
         output.write_hdf5(path, fab, xlower, (dx, dx), time, cycle)
+
+    SEQUENCINGS = {"reference": 0, "refill": 1}
+
+    def set_sequencing(self, mode: str = "reference") -> None:
+        """Ghost-layer sequencing of a level step: "reference" fills the
+        ghost layers once per level step, lets the first split sweep advance
+        the ghost rows across it and applies only the physical boundary
+        before the second direction (DimensionalSplitLevelIntegrator,
+        include/fub/solver/DimensionalSplitLevelIntegrator.hpp:169-190,
+        230-233); "refill" refills every ghost layer before each sweep."""
+        L.check(self._lib.fvs_amr_set_sequencing(self.a, self.SEQUENCINGS[mode]))
+        self._sequencing = mode
 
     def set_graphs(self, on: bool = True) -> None:
         """Replay each coarse step from a captured HIP graph (default) or

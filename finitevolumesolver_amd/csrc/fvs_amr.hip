@@ -206,6 +206,8 @@ struct fvs_amr {
     int nself = 0;
     GhostLink* coarse = nullptr;  // sources in the next coarser level (pool 0)
     int ncoarse = 0;
+    GhostLink* phys[2] = {nullptr, nullptr};  // ghost cells outside the domain
+    int nphys[2] = {0, 0};                    // across x / y: the patch's edge row
     AvgLink* avg = nullptr;  // this level onto the next coarser one
     int navg = 0;
     RegFace* reg = nullptr;  // register between this level and the coarser one
@@ -221,6 +223,7 @@ struct fvs_amr {
     int64_t max_cells = 0;
   };
   std::vector<Level> L;
+  int sequencing = FVS_AMR_SEQ_REFERENCE;  // ghost-layer sequencing of a level step
   std::vector<std::vector<int32_t>> lo;  // per level, (i0, j0) of every patch
   double* cfl = nullptr;  // one slot per sweep of a coarse step, then 2 per level
   int ncfl = 0;
@@ -365,6 +368,30 @@ int Build(fvs_amr* A, const int32_t* lo) {
           coarse.push_back(g);
         }
     }
+    // TransmissiveBoundary of direction d (src/AMReX/boundary_condition/
+    // TransmissiveBoundary.cpp:60-93): a ghost cell outside the domain
+    // across d copies the patch's own cell in the domain's edge row, at the
+    // same index along the other direction (ghost columns / rows included)
+    std::vector<GhostLink> phys[2];
+    for (int64_t p = 0; p < b->count; ++p) {
+      const PatchInfo& P = hp[p];
+      const int64_t lo_d[2] = {los[l][2 * p], los[l][2 * p + 1]};
+      const int64_t n_d[2] = {P.nx, P.ny};
+      const int64_t N_d[2] = {O.nx, O.ny};
+      for (int d = 0; d < 2; ++d)
+        for (int64_t k = -kG; k < n_d[d] + kG; ++k) {
+          const int64_t K = lo_d[d] + k;
+          if (K >= 0 && K < N_d[d]) continue;
+          const int64_t edge = (K < 0 ? 0 : N_d[d] - 1) - lo_d[d];
+          for (int64_t t = -kG; t < n_d[1 - d] + kG; ++t) {
+            GhostLink g;
+            g.dst = d == 0 ? CellOffset(P, k, t) : CellOffset(P, t, k);
+            g.src = d == 0 ? CellOffset(P, edge, t) : CellOffset(P, t, edge);
+            g.dcs = g.scs = CompStride(P);
+            phys[d].push_back(g);
+          }
+        }
+    }
     std::vector<AvgLink> avg;
     std::vector<RegFace> reg[4];
     if (l > 0) {
@@ -479,6 +506,10 @@ int Build(fvs_amr* A, const int32_t* lo) {
     LV.nreg = static_cast<int>(all.size());
     if (int rc = UploadVec(self, &LV.self)) return rc;
     if (int rc = UploadVec(coarse, &LV.coarse)) return rc;
+    for (int d = 0; d < 2; ++d) {
+      LV.nphys[d] = static_cast<int>(phys[d].size());
+      if (int rc = UploadVec(phys[d], &LV.phys[d])) return rc;
+    }
     if (int rc = UploadVec(avg, &LV.avg)) return rc;
     if (int rc = UploadVec(all, &LV.reg)) return rc;
     if (LV.nreg) FVS_HIP_CHECK(hipMalloc(&LV.regval, 4 * sizeof(double) * LV.nreg));
@@ -500,6 +531,20 @@ int FillGhosts(fvs_amr* A, int l, int which) {
   hipLaunchKernelGGL(GatherKernel, dim3(Blocks(n)), dim3(256), 0, A->s, LV.self, LV.nself,
                      LV.coarse, LV.ncoarse, b->pool[which],
                      l > 0 ? A->lv[l - 1]->pool[0] : nullptr);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+// The physical boundary of direction d on pool `which` of level l (the
+// boundary condition the reference applies before every split direction
+// but the first).
+int FillPhysical(fvs_amr* A, int l, int which, int d) {
+  fvs_amr::Level& LV = A->L[l];
+  const int n = LV.nphys[d];
+  if (!n) return FVS_OK;
+  hipLaunchKernelGGL(GatherKernel, dim3(Blocks(n)), dim3(256), 0, A->s, LV.phys[d], n,
+                     static_cast<const GhostLink*>(nullptr), 0, A->lv[l]->pool[which],
+                     static_cast<const double*>(nullptr));
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
@@ -579,11 +624,16 @@ int AdvanceLevel(fvs_amr* A, int l, int sub_first, std::vector<int>& slot) {
   // MakeSplitDirections: reversed order when (cycle % 2 + subcycle) % 2 == 1
   const bool rev = ((A->cycle % 2) + sub_first) % 2 == 1;
   const int dirs[2] = {rev ? 1 : 0, rev ? 0 : 1};
+  // Reference sequencing: the ghost layers filled above serve the whole
+  // level step; the first sweep also advances the ghost rows across its
+  // direction, and the second refills only its direction's physical
+  // boundary.  Refill sequencing: every sweep refills all ghost layers.
+  const bool ref = A->sequencing == FVS_AMR_SEQ_REFERENCE;
   int src = 0;
   for (int w = 0; w < 2; ++w) {
     const int d = dirs[w];
     if (w > 0) {
-      if (int rc = FillGhosts(A, l, src)) return rc;
+      if (int rc = ref ? FillPhysical(A, l, src, d) : FillGhosts(A, l, src)) return rc;
     }
     if (finer) {
       if (int rc = AddRegister(A, l + 1, d, 0, src)) return rc;
@@ -591,13 +641,15 @@ int AdvanceLevel(fvs_amr* A, int l, int sub_first, std::vector<int>& slot) {
     if (l > 0) {
       if (int rc = AddRegister(A, l, d, 1, src)) return rc;
     }
-    if (A->lv[l]->grid[d] > static_cast<unsigned>(A->pcap)) {
+    const bool grown = ref && w == 0;
+    if ((grown ? A->lv[l]->grid_grown[d] : A->lv[l]->grid[d]) > static_cast<unsigned>(A->pcap)) {
       SetError("batched sweep grid exceeds the per-sweep minima buffer");
       return FVS_BAD_ARGUMENT;
     }
     const int q = slot[l]++;
     if (int rc = batch::SweepPartials(A->lv[l], A->s, d, src,
-                                      A->slot_partials + static_cast<int64_t>(q) * A->pcap))
+                                      A->slot_partials + static_cast<int64_t>(q) * A->pcap,
+                                      grown))
       return rc;
     src = 1 - src;
   }
@@ -679,8 +731,8 @@ int fvs_amr_destroy(fvs_amr* A) {
   if (A->dtc) (void)hipFree(A->dtc);
   if (A->slot_partials) (void)hipFree(A->slot_partials);
   for (auto& LV : A->L) {
-    void* ps[] = {LV.self, LV.coarse, LV.avg, LV.reg, LV.regval, LV.rcells, LV.rentries,
-                  LV.tag_off};
+    void* ps[] = {LV.self, LV.coarse, LV.phys[0], LV.phys[1], LV.avg, LV.reg, LV.regval,
+                  LV.rcells, LV.rentries, LV.tag_off};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   }
@@ -863,6 +915,24 @@ int fvs_amr_set_cycle(fvs_amr* A, int64_t cycle) {
     return FVS_BAD_ARGUMENT;
   }
   A->cycle = cycle;
+  return FVS_OK;
+}
+
+int fvs_amr_set_sequencing(fvs_amr* A, int mode) {
+  if (!A || (mode != FVS_AMR_SEQ_REFERENCE && mode != FVS_AMR_SEQ_REFILL)) {
+    SetError("bad argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  if (mode != A->sequencing) {
+    // the captured steps hold the other sequence of launches
+    FVS_HIP_CHECK(hipStreamSynchronize(A->h->stream));
+    for (hipGraphExec_t& e : A->exec)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+    A->sequencing = mode;
+  }
   return FVS_OK;
 }
 

@@ -755,6 +755,8 @@ struct fvs_level {
   double* buf[3][5];
   Ctl* ctl;               // device control block
   unsigned long long* partials;  // per-workgroup minima
+  double* pm = nullptr;   // exact Godunov: the step-start CFL pass's star
+                          // pressures of the first sweep's faces
   int cur;                // buffer holding the current state
   // Kernel timing (off by default): HIP events recorded on the level's stream
   // around every launch, reduced per kernel class when read.
@@ -890,6 +892,18 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
     SetError("hipMalloc of the reduction partials failed");
     return FVS_HIP_ERROR;
   }
+  if (godunov) {
+    int64_t nf = 0;
+    for (int d = 0; d < D; ++d) nf = std::max<int64_t>(nf, L->ncell / n[d] * (n[d] + 1));
+    if (hipMalloc(&L->pm, nf * sizeof(double)) != hipSuccess) {
+      (void)hipFree(L->block);
+      (void)hipFree(L->ctl);
+      (void)hipFree(L->partials);
+      delete L;
+      SetError("hipMalloc of the star-pressure faces failed");
+      return FVS_HIP_ERROR;
+    }
+  }
   for (int b = 0; b < 3; ++b)
     for (int c = 0; c < D + 2; ++c) L->buf[b][c] = L->block + (static_cast<int64_t>(b) * (D + 2) + c) * L->ncell;
   L->cur = 0;
@@ -911,6 +925,7 @@ int fvs_level_destroy(fvs_level* L) {
   (void)hipFree(L->block);
   (void)hipFree(L->ctl);
   (void)hipFree(L->partials);
+  if (L->pm) (void)hipFree(L->pm);
   for (const auto& m : L->marks) {
     (void)hipEventDestroy(m.a);
     (void)hipEventDestroy(m.b);
@@ -1039,7 +1054,7 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
     const int64_t nb = GodunovCflBlocks(D, L->n);
     rc = Timed(L, s, FVS_TIMER_CFL, [&] {
       return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, nullptr, L->partials,
-                              &L->ctl->halt, net);
+                              &L->ctl->halt, net, L->pm, seq[0]);
     });
     for (int d = 0; d < D && rc == FVS_OK; ++d) {
       rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
@@ -1063,10 +1078,13 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
     for (int q = 0; q < D + 2; ++q) inp[q] = L->buf[src][q];
     if (godunov) {
       const int64_t nb = GodunovSweepBlocks(D, seq[w], L->n, PencilSeg(D, seq[w], L->n));
+      // The first sweep runs on the step-start state, whose star pressures
+      // in its direction the CFL pass kept: its faces are not solved again.
       rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] {
         return LaunchGodunovSweep(h->desc, s, seq[w], inp, outp, L->n, L->patch.dx[seq[w]],
-                                  &L->ctl->split_dt, 0.0, nullptr, L->partials,
-                                  &L->ctl->halt, 1, net);
+                                  &L->ctl->split_dt, 0.0, w == 0 ? L->pm : nullptr,
+                                  L->partials, &L->ctl->halt, 1, w == 0 ? nullptr : net,
+                                  w == 0 ? 1 : 0);
       });
       if (rc == FVS_OK)
         rc = LaunchGodunovFinalize(s, L->partials, nb, &L->ctl->last_check, h->newton_stats,

@@ -78,11 +78,12 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
     const int nd = DIR == 0 ? P.nx : P.ny;
     const int nt = DIR == 0 ? P.ny : P.nx;
     const int c = t.y - 1 + td;   // cell along the sweep
-    const int r = t.z + tt;       // index across
+    const int r = t.z + tt;       // index across (ghost rows when grown)
     const int64_t ldx = P.nx + 2 * kG;
     const int64_t cs = ldx * (P.ny + 2 * kG);
     const int64_t step = DIR == 0 ? 1 : ldx;
-    const bool row_ok = r < nt;
+    const bool row_ok = r >= -t.w && r < nt + t.w;
+    const bool row_inner = r >= 0 && r < nt;
     const bool recon_ok = row_ok && c >= -1 && c <= nd && c <= t.y + TD - 2;
     int64_t at = 0;
     if (row_ok) {
@@ -98,8 +99,8 @@ __global__ __launch_bounds__(TX * TY) void BatchSweepKernel(BatchArgs a) {
         u0[k] = a.in[at + k * cs];
         up[k] = a.in[at + k * cs + step];
       }
-      // CFL of face c (cells c - 1, c), FP64
-      if (c >= 0 && c <= nd && td >= 1) {
+      // CFL of face c (cells c - 1, c), FP64, interior rows
+      if (row_inner && c >= 0 && c <= nd && td >= 1) {
         double L[NQ], R[NQ];
 #pragma unroll
         for (int k = 0; k < NC; ++k) {
@@ -340,7 +341,10 @@ using namespace fvs;
 
 namespace {
 
-void BuildTiles(const std::vector<PatchInfo>& ps, int dir, Shape sh, std::vector<int4>& out) {
+// Tiles of every patch; grow > 0 extends the rows (dir 0) / columns (dir 1)
+// across the sweep to -grow .. n + grow - 1.
+void BuildTiles(const std::vector<PatchInfo>& ps, int dir, Shape sh, int grow,
+                std::vector<int4>& out) {
   out.clear();
   const int td = dir == 0 ? sh.tx : sh.ty;
   const int tt = dir == 0 ? sh.ty : sh.tx;
@@ -348,40 +352,49 @@ void BuildTiles(const std::vector<PatchInfo>& ps, int dir, Shape sh, std::vector
   for (size_t p = 0; p < ps.size(); ++p) {
     const int nd = dir == 0 ? ps[p].nx : ps[p].ny;
     const int nt = dir == 0 ? ps[p].ny : ps[p].nx;
-    for (int r = 0; r < nt; r += tt)
-      for (int s = 0; s < nd; s += own) out.push_back(make_int4(int(p), s, r, 0));
+    for (int r = -grow; r < nt + grow; r += tt)
+      for (int s = 0; s < nd; s += own) out.push_back(make_int4(int(p), s, r, grow));
   }
+}
+
+int UploadTiles(fvs_batch* b, const std::vector<int4>& tiles, int4** dst, int* n, unsigned* grid) {
+  if (*dst) {
+    FVS_HIP_CHECK(hipStreamSynchronize(b->h->stream));
+    FVS_HIP_CHECK(hipFree(*dst));
+    *dst = nullptr;
+  }
+  FVS_HIP_CHECK(hipMalloc(dst, std::max<size_t>(tiles.size(), 1) * sizeof(int4)));
+  FVS_HIP_CHECK(hipMemcpy(*dst, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+  *n = static_cast<int>(tiles.size());
+  // persistent grid: a few workgroups per CU, never more than tiles
+  const unsigned want = static_cast<unsigned>(b->cus * 8);
+  *grid = std::max(1u, std::min<unsigned>(want, static_cast<unsigned>(tiles.size())));
+  return FVS_OK;
 }
 
 int Upload(fvs_batch* b, int dir, int shape_idx) {
   const Shape sh = dir == 0 ? kShapesX[shape_idx] : kShapesY[shape_idx];
   std::vector<int4> tiles;
-  BuildTiles(b->host_patches, dir, sh, tiles);
-  if (b->tiles[dir]) {
-    FVS_HIP_CHECK(hipStreamSynchronize(b->h->stream));
-    FVS_HIP_CHECK(hipFree(b->tiles[dir]));
-    b->tiles[dir] = nullptr;
-  }
-  FVS_HIP_CHECK(hipMalloc(&b->tiles[dir], std::max<size_t>(tiles.size(), 1) * sizeof(int4)));
-  FVS_HIP_CHECK(hipMemcpy(b->tiles[dir], tiles.data(), tiles.size() * sizeof(int4),
-                          hipMemcpyHostToDevice));
-  b->ntiles[dir] = static_cast<int>(tiles.size());
+  BuildTiles(b->host_patches, dir, sh, 0, tiles);
+  if (int rc = UploadTiles(b, tiles, &b->tiles[dir], &b->ntiles[dir], &b->grid[dir])) return rc;
+  BuildTiles(b->host_patches, dir, sh, kG, tiles);
+  if (int rc = UploadTiles(b, tiles, &b->tiles_grown[dir], &b->ntiles_grown[dir],
+                           &b->grid_grown[dir]))
+    return rc;
   b->gen += 1;
   b->shape[dir] = shape_idx;
-  // persistent grid: a few workgroups per CU, never more than tiles
-  const unsigned want = static_cast<unsigned>(b->cus * 8);
-  b->grid[dir] = std::max(1u, std::min<unsigned>(want, static_cast<unsigned>(tiles.size())));
   return FVS_OK;
 }
 
 // The sweep kernel alone, its per-workgroup minima written to `partials`
 // (b->grid[dir] entries).
 int LaunchSweepOnly(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long long* partials,
-                    int precision) {
+                    int precision, bool grown = false) {
   BatchArgs a;
   a.patches = b->patches;
-  a.tiles = b->tiles[dir];
-  a.ntiles = b->ntiles[dir];
+  a.tiles = grown ? b->tiles_grown[dir] : b->tiles[dir];
+  a.ntiles = grown ? b->ntiles_grown[dir] : b->ntiles[dir];
+  const unsigned grid = grown ? b->grid_grown[dir] : b->grid[dir];
   a.in = b->pool[src];
   a.out = b->pool[1 - src];
   a.dt = b->dts;
@@ -391,9 +404,9 @@ int LaunchSweepOnly(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long
   a.limiter = b->h->desc.limiter;
   a.partials = partials;
   if (precision != FVS_FP64)
-    batch::LaunchSweepFast(s, dir, b->shape[dir], a, b->grid[dir], precision);
+    batch::LaunchSweepFast(s, dir, b->shape[dir], a, grid, precision);
   else
-    LaunchSweepShape<double, false>(s, dir, b->shape[dir], a, a.limiter, b->grid[dir]);
+    LaunchSweepShape<double, false>(s, dir, b->shape[dir], a, a.limiter, grid);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
@@ -453,8 +466,8 @@ int fvs::batch::LaunchSlotMin(hipStream_t s, unsigned long long* partials, int n
 }
 
 int fvs::batch::SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src,
-                              unsigned long long* partials) {
-  return LaunchSweepOnly(b, s, dir, src, partials, b->h->desc.precision);
+                              unsigned long long* partials, bool grown) {
+  return LaunchSweepOnly(b, s, dir, src, partials, b->h->desc.precision, grown);
 }
 
 extern "C" {
@@ -531,7 +544,8 @@ int fvs_batch_destroy(fvs_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-  void* ps[] = {b->patches, b->pool[0], b->pool[1], b->tiles[0], b->tiles[1], b->partials, b->dts};
+  void* ps[] = {b->patches, b->pool[0], b->pool[1], b->tiles[0], b->tiles[1],
+                b->tiles_grown[0], b->tiles_grown[1], b->partials, b->dts};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete b;

@@ -19,7 +19,8 @@ struct PatchInfo {
 
 struct BatchArgs {
   const PatchInfo* patches;
-  const int4* tiles;  // (patch, start along the sweep, start across, 0)
+  const int4* tiles;  // (patch, start along the sweep, start across, growth
+                      // across: rows / columns -w .. n + w - 1 are updated)
   int ntiles;
   const double* in;
   double* out;
@@ -82,6 +83,9 @@ struct fvs_batch {
   int4* tiles[2];               // tile list per direction
   int ntiles[2];
   unsigned grid[2];
+  int4* tiles_grown[2];         // the same tiles over the interior grown by
+  int ntiles_grown[2];          // kG rows / columns across the direction
+  unsigned grid_grown[2];
   unsigned long long* partials;
   int64_t partials_cap;
   double* dts;                  // device: [0] dt of the sweeps, [1..2] CFL results
@@ -98,8 +102,12 @@ namespace batch {
 // stream s; the minimum stable dt of its faces goes to *dt_out (nullable).
 int Sweep(fvs_batch* b, hipStream_t s, int dir, int src, double* dt_out);
 // The same sweep without its reduction: the per-workgroup minima (keys, at
-// most 8 per CU) go to `partials`.
-int SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long long* partials);
+// most 8 per CU) go to `partials`.  grown: the sweep also updates the kG
+// ghost rows (dir 0) / columns (dir 1) on each side across the direction,
+// reading the ghost layers along it; its CFL minimum covers the interior
+// faces only.
+int SweepPartials(fvs_batch* b, hipStream_t s, int dir, int src, unsigned long long* partials,
+                  bool grown = false);
 // Slots of `cap` partial minima: clear = true sets every key to +inf (before
 // sweeps whose grids may be smaller than cap), clear = false reduces slot q
 // into out[q].

@@ -103,6 +103,9 @@ struct GodunovArgs {
   int* halt;
   int seg;
   int update;            // 0: only fluxes' Newton + CFL (no state write)
+  int reuse;             // guess holds the star pressures the step-start CFL
+                         // pass found for these faces on the same state
+                         // (NaN where its Newton did not converge)
 };
 
 __device__ __forceinline__ Gas<double> GasOf(const GodunovArgs& a) {
@@ -172,6 +175,13 @@ __device__ __forceinline__ MiddleState SeededMiddleState(const double* L, const 
 // One face: Newton (shared by flux and CFL), flux, dt.  MIXED: the Newton
 // iteration stays FP64 (with exp/log powers) and the sample and flux are
 // float.
+//
+// With a.reuse the seed is the converged iterate of the step-start CFL pass
+// for the same face and state: Newton from it evaluates the side functions
+// at that pressure, finds the residual within the tolerance and stops at
+// once with the same star pressure and velocity, so the face is not solved
+// again (and not counted as a solved face, bit 2 of *conv).  A NaN seed (no
+// converged iterate) solves the face from the mean pressure.
 template <int D, int DIR, bool MIXED>
 __device__ __forceinline__ void GodunovFace(const double* L, const double* R, double guess,
                                             bool has_guess, const GodunovArgs& a,
@@ -182,9 +192,15 @@ __device__ __forceinline__ void GodunovFace(const double* L, const double* R, do
   np.max_it = a.max_it;
   using I = Idx<D>;
   const double mean = 0.5 * (L[I::kP] + R[I::kP]);
-  int it, fallback;
-  const MiddleState ms =
-      SeededMiddleState<D, DIR, MIXED>(L, R, g, np, has_guess, guess, mean, &it, &fallback);
+  int it, fallback = 0, reused = 0;
+  MiddleState ms;
+  if (a.reuse) {
+    reused = guess > 0.0 && guess <= 1.7976931348623157e308;
+    ms = ComputeMiddleState<D, DIR, MIXED>(L, R, g, np, reused ? guess : mean);
+    it = ms.iterations;
+  } else {
+    ms = SeededMiddleState<D, DIR, MIXED>(L, R, g, np, has_guess, guess, mean, &it, &fallback);
+  }
   if constexpr (MIXED) {
     GodunovFluxMixed<D, DIR>(F, L, R, ms, g);
   } else {
@@ -192,7 +208,7 @@ __device__ __forceinline__ void GodunovFace(const double* L, const double* R, do
   }
   *dt_face = GodunovDtFromMiddle<D, DIR>(L, R, ms.pM, g, a.dx);
   *iters = it;
-  *conv = ms.converged | (fallback << 1);
+  *conv = ms.converged | (fallback << 1) | (reused << 2);
 }
 
 // LEARNED: the network's guess is inferred inline (a.tiles set); without it
@@ -248,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void GodunovSweepX(GodunovArgs a) {
       it_sum = it;
       it_max = it;
       nfail = (conv & 1) ? 0 : 1;
-      nfaces = 1 + (static_cast<unsigned long long>(conv >> 1) << 32);
+      nfaces = ((conv >> 2) & 1) ? 0 : 1 + (static_cast<unsigned long long>((conv >> 1) & 1) << 32);
     }
   } else {
 #pragma unroll
@@ -317,7 +333,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         it_sum += it;
         it_max = (unsigned long long)it > it_max ? it : it_max;
         nfail += (conv & 1) ? 0 : 1;
-        nfaces += 1 + (static_cast<unsigned long long>(conv >> 1) << 32);
+        nfaces += ((conv >> 2) & 1) ? 0
+                                    : 1 + (static_cast<unsigned long long>((conv >> 1) & 1) << 32);
       }
       if (a.update && active && j > j0) {
         const int64_t off = base + (j - 1) * stride;
@@ -514,6 +531,8 @@ struct GodunovCflArgs {
   const float* bias;
   unsigned long long* partials;  // 5 per workgroup and direction
   int* halt;
+  double* pm_out;  // star pressures of direction pm_dir's faces (nullable;
+  int pm_dir;      // NaN where Newton did not converge), for the first sweep
 };
 
 // Newton seed of a face: the inline learned guess (lg, computed by the whole
@@ -533,6 +552,7 @@ __device__ __forceinline__ void CflFace(const double* L, const double* R, const 
       SeededMiddleState<D, DIR, MIXED>(L, R, g, np, seeded, x0, mean, &iters, &fallback);
   const unsigned long long it = static_cast<unsigned long long>(iters);
   const unsigned long long fb = static_cast<unsigned long long>(fallback);
+  if (a.pm_out && DIR == a.pm_dir) a.pm_out[fidx] = ms.converged ? ms.pM : __builtin_nan("");
   const double dt = GodunovDtFromMiddle<D, DIR>(L, R, ms.pM, g, a.dx[DIR]);
   acc[0] = KMin(acc[0], DtKey(dt));
   acc[1] += it;
@@ -730,7 +750,7 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
                        const double* const* in, double* const* out, const int64_t n[3],
                        double dx, const double* dt_ptr, double dt_val, const double* guess,
                        unsigned long long* partials, int* halt, int update,
-                       const fvs_mlp* net) {
+                       const fvs_mlp* net, int reuse) {
   GodunovArgs a;
   std::memset(&a, 0, sizeof a);
   const int D = desc.dim;
@@ -752,6 +772,7 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
   a.halt = halt;
   a.seg = PencilSeg(D, dir, a.n);
   a.update = update;
+  a.reuse = reuse && guess != nullptr;
   const bool mixed = desc.precision == FVS_FP32_FLUX;
   const bool learned = a.tiles != nullptr;
   const int64_t blocks = GodunovSweepBlocks(D, dir, a.n, a.seg);
@@ -798,7 +819,8 @@ int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int
 
 int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                      const int64_t n[3], const double dx[3], const double* const* guess,
-                     unsigned long long* partials, int* halt, const fvs_mlp* net) {
+                     unsigned long long* partials, int* halt, const fvs_mlp* net,
+                     double* pm_out, int pm_dir) {
   GodunovCflArgs a;
   std::memset(&a, 0, sizeof a);
   const int D = desc.dim;
@@ -815,6 +837,8 @@ int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* c
   if (net) MlpOperands(net, &a.tiles, &a.bias);
   a.partials = partials;
   a.halt = halt;
+  a.pm_out = pm_out;
+  a.pm_dir = pm_dir;
   const bool mixed = desc.precision == FVS_FP32_FLUX;
   const int64_t tiles = (a.n[0] + kTX - 1) / kTX;
   const int64_t gx = tiles * (D == 3 ? a.n[1] : 1);

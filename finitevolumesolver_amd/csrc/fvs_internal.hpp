@@ -121,12 +121,13 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
                        const double* const* in, double* const* out, const int64_t n[3],
                        double dx, const double* dt_ptr, double dt_val, const double* guess,
                        unsigned long long* partials, int* halt, int update,
-                       const fvs_mlp* net = nullptr);
+                       const fvs_mlp* net = nullptr, int reuse = 0);
 int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
                           double* out, unsigned long long* stats, Ctl* ctl = nullptr);
 int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                      const int64_t n[3], const double dx[3], const double* const* guess,
-                     unsigned long long* partials, int* halt, const fvs_mlp* net = nullptr);
+                     unsigned long long* partials, int* halt, const fvs_mlp* net = nullptr,
+                     double* pm_out = nullptr, int pm_dir = -1);
 int LaunchGodunovViewFlux(const fvs_method_desc& desc, hipStream_t s, const fvs_view* fluxes,
                           const fvs_view* states, double dt, double dx, int dir,
                           unsigned long long* nfail);

@@ -593,6 +593,18 @@ int fvs_amr_set_cycle(fvs_amr* amr, int64_t cycle);
  * dt is read from device memory, so one graph serves every dt; retuning a
  * level's tiles triggers a new capture.  No counterpart in the reference. */
 int fvs_amr_set_graphs(fvs_amr* amr, int on);
+/* Ghost-layer sequencing of a level step.  FVS_AMR_SEQ_REFERENCE (default):
+ * the ghost layers are filled once at the start of the level step
+ * (DimensionalSplitLevelIntegrator::PreAdvanceLevel,
+ * include/fub/solver/DimensionalSplitLevelIntegrator.hpp:169-190), the first
+ * split sweep also advances the two ghost rows across its direction (the
+ * grown scratch, include/fub/AMReX/IntegratorContext.hpp:52-53), and before
+ * the second direction only that direction's physical boundary is applied
+ * (ApplyBoundaryCondition, :230-233, TransmissiveBoundary).
+ * FVS_AMR_SEQ_REFILL: every split sweep refills all ghost layers. */
+#define FVS_AMR_SEQ_REFERENCE 0
+#define FVS_AMR_SEQ_REFILL 1
+int fvs_amr_set_sequencing(fvs_amr* amr, int mode);
 /* Refinement tags of a level's current state (pool 0, ghosts refilled
  * first): ScalarGradientDetector::TagCellsForRefinement
  * (include/fub/tagging_method/GradientDetector.hpp:150-194) with the

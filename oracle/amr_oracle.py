@@ -30,14 +30,30 @@ AMReX-side arithmetic (register sums, average_down order) is parity
 unpinned beyond this restatement; the composition is pinned by the
 conservation property the register exists for (tests/test_amr_oracle.py).
 
-Deliberate departure shared with the device library: ghost layers are
-refilled before every split sweep (the reference refills them at the start
-of a level step and relies on the sweeps updating transverse ghost rows).
+Two ghost-layer sequencings (`sequencing`):
+  * "reference" (default): DimensionalSplitLevelIntegrator fills a level's
+    ghost layers once per level step (PreAdvanceLevel,
+    include/fub/solver/DimensionalSplitLevelIntegrator.hpp:169-190) and
+    skips the boundary condition before the first split direction
+    (:230-233); the first sweep updates its grown scratch
+    (include/fub/AMReX/IntegratorContext.hpp:52-53: scratch_gcw 4,
+    flux_gcw 2; src/AMReX/TimeIntegrator.cpp:30-56 updates every cell the
+    flux box encloses), which advances the ghost rows across the sweep
+    direction; before the second direction only the physical boundary in
+    that direction is applied (ApplyBoundaryCondition -> TransmissiveBoundary,
+    src/AMReX/boundary_condition/TransmissiveBoundary.cpp:60-93: a ghost
+    cell outside the domain copies the patch's own cell on the domain's edge
+    row).  With two ghost layers the first sweep here updates the interior
+    columns of the two ghost rows on each side, the only ghost cells the
+    second sweep's interior stencils read.
+  * "refill": every split sweep refills all ghost layers (the device
+    library's earlier sequencing, kept for comparison).
 
-The per-patch sweep is injected: sweep(cons, dt, dx, d) takes a patch with
-two ghost layers (4, ny + 4, nx + 4) and returns (new interior (4, ny, nx),
-face fluxes along d of the interior rows/columns (4, ny, nx + 1) or
-(4, ny + 1, nx), minimum stable dt of those faces).
+The per-patch sweep is injected: sweep(cons, dt, dx, d, grow=0) takes a
+patch with two ghost layers (4, ny + 4, nx + 4) and returns (new values of
+the interior grown by `grow` cells across d: (4, ny + 2 grow, nx) for d = 0,
+(4, ny, nx + 2 grow) for d = 1; face fluxes along d of the interior rows /
+columns (4, ny, nx + 1) or (4, ny + 1, nx); minimum stable dt of those faces).
 """
 from __future__ import annotations
 
@@ -47,7 +63,10 @@ G = 2  # ghost layers
 
 
 class AmrOracle:
-    def __init__(self, domain, layouts, dx0, ratio, sweep):
+    def __init__(self, domain, layouts, dx0, ratio, sweep, sequencing="reference"):
+        if sequencing not in ("reference", "refill"):
+            raise ValueError(f"unknown sequencing {sequencing!r}")
+        self.sequencing = sequencing
         self.domain = tuple(int(v) for v in domain)
         self.layouts = [[tuple(int(v) for v in p) for p in lay] for lay in layouts]
         self.r = int(ratio)
@@ -92,6 +111,24 @@ class AmrOracle:
                         u[:, j + G, i + G] = self.U[l - 1][q][:, J // self.r - b + G,
                                                               I // self.r - a + G]
 
+    def fill_physical(self, l, d):
+        """TransmissiveBoundary in direction d: every ghost cell outside the
+        level's domain across d takes the value of the patch's own cell in
+        the domain's edge row (same index along the other direction)."""
+        Nd = self.extent(l)[d]
+        for p, (i0, j0, nx, ny) in enumerate(self.layouts[l]):
+            u = self.U[l][p]
+            lo, n = (i0, nx) if d == 0 else (j0, ny)
+            for k in range(-G, n + G):
+                K = lo + k
+                if 0 <= K < Nd:
+                    continue
+                edge = (0 if K < 0 else Nd - 1) - lo
+                if d == 0:
+                    u[:, :, k + G] = u[:, :, edge + G]
+                else:
+                    u[:, k + G, :] = u[:, edge + G, :]
+
     # register -----------------------------------------------------------------
     def register_faces(self, f):
         """Coarse-fine faces of level f: (p, dir, side, ct, coarse patch q,
@@ -133,15 +170,20 @@ class AmrOracle:
             for s in range(self.r):
                 self.advance_level(l + 1, dt / self.r, s)
         rev = ((self.cycle % 2) + sub) % 2 == 1
+        ref = self.sequencing == "reference"
         for w, d in enumerate((1, 0) if rev else (0, 1)):
             if w > 0:
-                self.fill_ghosts(l)
+                if ref:
+                    self.fill_physical(l, d)
+                else:
+                    self.fill_ghosts(l)
+            grow = G if (ref and w == 0) else 0
             fluxes = []
             for p, (i0, j0, nx, ny) in enumerate(self.layouts[l]):
-                new, flux, sdt = self.sweep(self.U[l][p], dt, self.dx[l], d)
+                new, flux, sdt = self.sweep(self.U[l][p], dt, self.dx[l], d, grow)
                 fluxes.append(flux)
                 self.level_cfl[l] = min(self.level_cfl[l], sdt)
-                self._new[p] = new
+                self._new[p] = (new, grow, d)
             if finer:  # CrseInit: -(dx dt) F of the coarse face
                 faces, reg = self._reg[l + 1]
                 w_c = self.dx[l] * dt
@@ -166,7 +208,11 @@ class AmrOracle:
                         f = fl[:, t, fi] if d == 0 else fl[:, fi, t]
                         reg[e] = reg[e] + 1.0 * (w_f * f)
             for p, (i0, j0, nx, ny) in enumerate(self.layouts[l]):
-                self.U[l][p][:, G:-G, G:-G] = self._new.pop(p)
+                new, grow, dd = self._new.pop(p)
+                if dd == 0:
+                    self.U[l][p][:, G - grow:G + ny + grow, G:-G] = new
+                else:
+                    self.U[l][p][:, G:-G, G - grow:G + nx + grow] = new
         if finer:
             self.reflux(l + 1)
             self.average_down(l + 1)

@@ -23,23 +23,29 @@ LAYOUTS = [
 
 
 def oracle_sweep(desc):
-    """sweep(cons, dt, dx, d) of one 2D patch with two ghost layers: the
-    oracle's FluxMethod + UpdateConservatively over the interior rows
-    (src/HyperbolicPatchIntegrator.cpp:35-154) and the minimum HLLEM stable
-    dt of the faces 0..n of those rows."""
+    """sweep(cons, dt, dx, d, grow=0) of one 2D patch with two ghost layers:
+    the oracle's FluxMethod + UpdateConservatively over the interior rows
+    grown by `grow` ghost rows on each side across d
+    (src/HyperbolicPatchIntegrator.cpp:35-154), the face fluxes of the
+    interior rows and the minimum HLLEM stable dt of their faces 0..n."""
     lib = O.load()
 
-    def sweep(cons, dt, dx, d):
+    def sweep(cons, dt, dx, d, grow=0):
         ny, nx = cons.shape[1] - 4, cons.shape[2] - 4
         comp = O.complete_from_cons(np.ascontiguousarray(cons).reshape(4, -1), 2)
         comp = comp.reshape(6, 1, ny + 4, nx + 4)
         sv = O.host_view(np.ascontiguousarray(comp), (-2, -2, 0))
-        f = np.zeros((4, 1, ny + (d == 1), nx + (d == 0)))
-        lib.oracle_numeric_fluxes(C.byref(desc), C.byref(O.host_view(f)), C.byref(sv), dt, dx, d)
-        prev = np.ascontiguousarray(cons[:, 2:-2, 2:-2].reshape(4, 1, ny, nx))
+        gy, gx = (grow, 0) if d == 0 else (0, grow)
+        fg = np.zeros((4, 1, ny + 2 * gy + (d == 1), nx + 2 * gx + (d == 0)))
+        lib.oracle_numeric_fluxes(C.byref(desc), C.byref(O.host_view(fg, (-gx, -gy, 0))),
+                                  C.byref(sv), dt, dx, d)
+        prev = np.ascontiguousarray(
+            cons[:, 2 - gy:2 + ny + gy, 2 - gx:2 + nx + gx].reshape(4, 1, ny + 2 * gy, nx + 2 * gx))
         nxt = prev.copy()
-        lib.oracle_update_conservatively(C.byref(O.host_view(nxt)), C.byref(O.host_view(prev)),
-                                         C.byref(O.host_view(f)), dt, dx, d)
+        lib.oracle_update_conservatively(C.byref(O.host_view(nxt, (-gx, -gy, 0))),
+                                         C.byref(O.host_view(prev, (-gx, -gy, 0))),
+                                         C.byref(O.host_view(fg, (-gx, -gy, 0))), dt, dx, d)
+        f = fg[:, :, gy:gy + ny + (d == 1), gx:gx + nx + (d == 0)]
         c = comp[:, 0, 2:-2, :] if d == 0 else comp[:, 0, :, 2:-2]
         if d == 0:
             L, R = c[:, :, 1:nx + 2], c[:, :, 2:nx + 3]

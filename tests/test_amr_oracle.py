@@ -14,6 +14,7 @@ import sys
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
 from amr_oracle import AmrOracle  # noqa: E402
@@ -22,9 +23,9 @@ import amr_cases as A  # noqa: E402
 from finitevolumesolver_amd.patch import PerfectGas, make_desc  # noqa: E402
 
 
-def _oracle(state):
+def _oracle(state, sequencing="reference"):
     desc = make_desc(PerfectGas(dim=2))
-    o = AmrOracle(A.DOMAIN, A.LAYOUTS, A.DX0, 2, A.oracle_sweep(desc))
+    o = AmrOracle(A.DOMAIN, A.LAYOUTS, A.DX0, 2, A.oracle_sweep(desc), sequencing)
     for l, lay in enumerate(A.LAYOUTS):
         for p in range(len(lay)):
             o.U[l][p][:, 2:-2, 2:-2] = state(l, p)
@@ -33,8 +34,9 @@ def _oracle(state):
     return o
 
 
-def test_conservation_over_subcycled_steps():
-    o = _oracle(A.resting_blob)
+@pytest.mark.parametrize("sequencing", ["reference", "refill"])
+def test_conservation_over_subcycled_steps(sequencing):
+    o = _oracle(A.resting_blob, sequencing)
     before = o.total()
     for _ in range(3):
         o.advance(0.8 * o.stable_dt())
@@ -155,3 +157,53 @@ def test_nests_follows_the_tile_rule():
     assert not nests((32, 32, 8, 8), lay1, ext)      # its margin leaves patch 0 at x = 15
     assert nests((100, 0, 24, 16), lay1, ext)        # on the domain edge y = 0 (clamped)
     assert not nests((96, 0, 32, 16), lay1, ext)     # margin at x = 47 is outside patch 3
+
+
+def _single_level(layout, sequencing, state, steps=4):
+    desc = make_desc(PerfectGas(dim=2))
+    o = AmrOracle(A.DOMAIN, [layout], A.DX0, 2, A.oracle_sweep(desc), sequencing)
+    for p, (i0, j0, nx, ny) in enumerate(layout):
+        o.U[0][p][:, 2:-2, 2:-2] = state(0, p, [layout])
+    for _ in range(steps):
+        o.advance(0.8 * o.stable_dt())
+    full = np.zeros((4, A.DOMAIN[1], A.DOMAIN[0]))
+    for p, (i0, j0, nx, ny) in enumerate(layout):
+        full[:, j0:j0 + ny, i0:i0 + nx] = o.U[0][p][:, 2:-2, 2:-2]
+    return full
+
+
+def test_split_level_equals_one_patch_in_both_sequencings():
+    """A level of two or four patches gives the single-patch level's state
+    bit for bit over four steps (both split orders): in the reference
+    sequencing the ghost rows the first sweep advances are exactly the
+    neighbour's rows it updates, and the physical boundary refill copies
+    the advanced edge row."""
+    one = [(0, 0, 32, 32)]
+    ref = _single_level(one, "refill", A.moving_state)
+    for layout in ([(0, 0, 16, 32), (16, 0, 16, 32)], [(0, 0, 32, 12), (0, 12, 32, 20)],
+                   [(0, 0, 8, 8), (8, 0, 24, 8), (0, 8, 8, 24), (8, 8, 24, 24)]):
+        for seq in ("reference", "refill"):
+            np.testing.assert_array_equal(_single_level(layout, seq, A.moving_state), ref,
+                                          err_msg=f"{layout} {seq}")
+    np.testing.assert_array_equal(_single_level(one, "reference", A.moving_state), ref)
+
+
+def test_sequencings_differ_only_near_coarse_fine_boundaries():
+    """With coarse-fine interfaces the two sequencings part: the reference
+    one advances the piecewise-constant coarse ghost rows by the first
+    sweep before the second sweep reads them.  Level 0 away from every
+    level-1 patch (more than the stencil reach of a coarse step) agrees."""
+    a = _oracle(A.moving_state, "reference")
+    b = _oracle(A.moving_state, "refill")
+    dt = 0.8 * a.stable_dt()
+    a.advance(dt)
+    b.advance(dt)
+    assert any(not np.array_equal(x, y) for x, y in zip(a.U[2], b.U[2]))
+    # level-0 cells at least 4 coarse cells from any level-1 patch
+    cover = np.zeros(A.DOMAIN[::-1], dtype=bool)
+    for (i0, j0, nx, ny) in A.LAYOUTS[1]:
+        cover[max(j0 // 2 - 4, 0):(j0 + ny) // 2 + 4, max(i0 // 2 - 4, 0):(i0 + nx) // 2 + 4] = True
+    for p, (i0, j0, nx, ny) in enumerate(A.LAYOUTS[0]):
+        far = ~cover[j0:j0 + ny, i0:i0 + nx]
+        np.testing.assert_array_equal(a.U[0][p][:, 2:-2, 2:-2][:, far],
+                                      b.U[0][p][:, 2:-2, 2:-2][:, far])

@@ -30,12 +30,13 @@ def _note(msg):
     print(msg, flush=True)
 
 
-def _pair(state, graphs=True):
+def _pair(state, graphs=True, sequencing="reference"):
     fm = fvs.FluxMethod(fvs.PerfectGas(dim=2), base="HLLEM", reconstruction="MusclHancock")
     h = AmrHierarchy(fm, A.DOMAIN, A.LAYOUTS, A.DX0, 2)
     h.set_graphs(graphs)
+    h.set_sequencing(sequencing)
     h.set_state(state)
-    o = AmrOracle(A.DOMAIN, A.LAYOUTS, A.DX0, 2, A.oracle_sweep(fm._h.desc))
+    o = AmrOracle(A.DOMAIN, A.LAYOUTS, A.DX0, 2, A.oracle_sweep(fm._h.desc), sequencing)
     for l, lay in enumerate(A.LAYOUTS):
         for p in range(len(lay)):
             o.U[l][p][:, 2:-2, 2:-2] = state(l, p)
@@ -53,13 +54,15 @@ def _compare(h, o, what):
                                           err_msg=f"{what}: level {l} patch {p}")
 
 
-@pytest.mark.parametrize("state,graphs", [("moving", True), ("moving", False),
-                                          ("resting", True), ("shock_bubble", True)])
-def test_amr_steps_match_oracle(cuda, state, graphs):
+@pytest.mark.parametrize("state,graphs,sequencing", [
+    ("moving", True, "reference"), ("moving", False, "reference"), ("resting", True, "reference"),
+    ("shock_bubble", True, "reference"), ("moving", True, "refill"),
+    ("shock_bubble", False, "refill")])
+def test_amr_steps_match_oracle(cuda, state, graphs, sequencing):
     fn = {"moving": A.moving_state, "resting": A.resting_blob,
           "shock_bubble": A.shock_bubble}[state]
-    _note(f"amr parity: {state} state, 3 levels, graphs {graphs}")
-    fm, h, o = _pair(fn, graphs)
+    _note(f"amr parity: {state} state, 3 levels, graphs {graphs}, {sequencing} sequencing")
+    fm, h, o = _pair(fn, graphs, sequencing)
     _compare(h, o, "after average_down")
     for step in range(4):
         if step == 2:
