@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -289,6 +290,48 @@ int64_t Count(const DevView& v) { return v.ext[0] * v.ext[1] * v.ext[2]; }
 
 using namespace fvs;
 
+// Partial minima of fvs_patch_stable_dt_noalloc: a fixed device array that
+// every call on a device shares (2 MiB per device, held until the process
+// ends).  Every workgroup of the CFL pass writes its keys before the
+// finalize kernel reads them, so a call never reads an earlier call's
+// contents and nothing is cleared.  Host code never passes the array's name
+// to a kernel: it uses the device address hipGetSymbolAddress returns for
+// the device, looked up once per device when the first handle on it is
+// created.
+static constexpr int64_t kNoAllocKeys = 262144;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocKeys];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device.
+int NoAllocScratchInit(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    SetError("device %d outside the %d devices the scratch table holds", device, kMaxDevices);
+    return FVS_BAD_ARGUMENT;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return FVS_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    SetError("hipGetSymbolAddress of the scratch array failed on device %d: %s", device,
+             hipGetErrorString(e));
+    return FVS_HIP_ERROR;
+  }
+  g_scratch_ptr[device] = static_cast<unsigned long long*>(ptr);
+  return FVS_OK;
+}
+
+unsigned long long* NoAllocScratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+} // namespace
+
 static int EnsurePartials(fvs_handle* h, int64_t n) {
   if (n <= h->partials_cap) return FVS_OK;
   if (h->partials) {
@@ -349,6 +392,11 @@ int fvs_create(const fvs_method_desc* desc, void* hip_stream, fvs_handle** out) 
     return FVS_HIP_ERROR;
   }
   if (int rc = LaunchResetKeys(h->stream, h->scratch_keys, 8, h->scratch_counter, 4)) {
+    (void)hipFree(mem);
+    delete h;
+    return rc;
+  }
+  if (int rc = NoAllocScratchInit(h->device)) {
     (void)hipFree(mem);
     delete h;
     return rc;
@@ -622,8 +670,12 @@ int fvs_split_sweep(fvs_handle* h, const double* const* cons_in, double* const* 
   return FVS_OK;
 }
 
-int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t n[3],
-                        const double dx[3], double* dt_dir_device) {
+// Step-start CFL of a ghost-free patch with the partial minima in `scratch`
+// (capacity `cap` keys) when it is given, else in the handle's buffer, grown
+// as needed.
+static int PatchStableDt(fvs_handle* h, const double* const* cons, const int64_t n[3],
+                         const double dx[3], double* dt_dir_device,
+                         unsigned long long* scratch_keys, int64_t cap) {
   if (!h || !cons || !n || !dx || !dt_dir_device) {
     SetError("null argument");
     return FVS_BAD_ARGUMENT;
@@ -642,26 +694,56 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t 
       return FVS_BAD_ARGUMENT;
     }
   }
-  if (h->desc.base == FVS_BASE_GODUNOV_EXACT) {
-    const int64_t nb = GodunovCflBlocks(D, nn);
-    if (int rc = EnsurePartials(h, 5 * D * nb)) return rc;
-    if (int rc = LaunchGodunovCfl(h->desc, h->stream, cons, nn, dx, nullptr, h->partials, nullptr))
+  const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
+  if (!godunov && (h->desc.base == FVS_BASE_HLL_EINFELDT ||
+                   h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM)) {
+    SetError("fvs_patch_stable_dt has no HLL or MusclHancockPrim kernel (use fvs_stable_dt on views)");
+    return FVS_BAD_ARGUMENT;
+  }
+  const int64_t nb = godunov ? GodunovCflBlocks(D, nn) : CflBlocks(D, nn);
+  const int64_t keys = (godunov ? 5 : 1) * D * nb;
+  unsigned long long* partials = scratch_keys;
+  if (partials) {
+    if (keys > cap) {
+      SetError("patch needs %lld reduction keys, the allocation-free entry point holds %lld",
+               static_cast<long long>(keys), static_cast<long long>(cap));
+      return FVS_BAD_ARGUMENT;
+    }
+  } else {
+    if (int rc = EnsurePartials(h, keys)) return rc;
+    partials = h->partials;
+  }
+  if (godunov) {
+    if (int rc = LaunchGodunovCfl(h->desc, h->stream, cons, nn, dx, nullptr, partials, nullptr))
       return rc;
     for (int d = 0; d < D; ++d) {
-      if (int rc = LaunchGodunovFinalize(h->stream, h->partials + 5 * d * nb, nb,
+      if (int rc = LaunchGodunovFinalize(h->stream, partials + 5 * d * nb, nb,
                                          dt_dir_device + d, h->newton_stats))
         return rc;
     }
     return FVS_OK;
   }
-  if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
-      h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM) {
-    SetError("fvs_patch_stable_dt has no HLL or MusclHancockPrim kernel (use fvs_stable_dt on views)");
+  return LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, partials, dt_dir_device, nullptr,
+                        nullptr, 0, 0);
+}
+
+int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons, const int64_t n[3],
+                        const double dx[3], double* dt_dir_device) {
+  return PatchStableDt(h, cons, n, dx, dt_dir_device, nullptr, 0);
+}
+
+int fvs_patch_stable_dt_noalloc(fvs_handle* h, const double* const* cons, const int64_t n[3],
+                                const double dx[3], double* dt_dir_device) {
+  if (!h) {
+    SetError("null handle");
     return FVS_BAD_ARGUMENT;
   }
-  if (int rc = EnsurePartials(h, D * CflBlocks(D, nn))) return rc;
-  return LaunchPatchCfl(h->desc, h->stream, cons, nn, dx, h->partials, dt_dir_device, nullptr,
-                        nullptr, 0, 0);
+  unsigned long long* ptr = NoAllocScratch(h->device);
+  if (!ptr) {
+    SetError("no scratch address for device %d (handle not created on it)", h->device);
+    return FVS_BAD_ARGUMENT;
+  }
+  return PatchStableDt(h, cons, n, dx, dt_dir_device, ptr, kNoAllocKeys);
 }
 
 int fvs_godunov_sweep(fvs_handle* h, const double* const* cons_in, double* const* cons_out,

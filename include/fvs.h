@@ -313,6 +313,18 @@ int fvs_patch_stable_dt(fvs_handle* h, const double* const* cons,
                         const int64_t n[3], const double dx[3],
                         double* dt_dir_device);
 
+/* fvs_patch_stable_dt without any allocation: the same kernels and results,
+ * with the per-workgroup partial minima in a fixed 2 MiB device array of
+ * the library (262,144 keys; a patch needing more returns
+ * FVS_BAD_ARGUMENT).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  Every call on one device shares that array, so calls
+ * that overlap on one device (different streams or threads) race; calls on
+ * one stream are ordered and safe. */
+int fvs_patch_stable_dt_noalloc(fvs_handle* h, const double* const* cons,
+                                const int64_t n[3], const double dx[3],
+                                double* dt_dir_device);
+
 /* ---- exact Godunov -------------------------------------------------------- */
 
 /* First-order exact-Godunov split sweep (FluxMethod<Godunov<PerfectGas<D>,
