@@ -90,6 +90,7 @@ class PatchDesc(C.Structure):
         ("dx", C.c_double * 3),
         ("splitting", C.c_int),
         ("cfl", C.c_double),
+        ("smallest_dt", C.c_double),
     ]
 
 

@@ -226,17 +226,23 @@ class AmrHierarchy:
         return out
 
     def run(self, steps: int, cfl: float = 0.8, max_restarts: int = 64,
-            regrid_every: int = 0, tol_rho: float = 0.1, tol_p: float = 0.1) -> list:
+            regrid_every: int = 0, tol_rho: float = 0.1, tol_p: float = 0.1,
+            final_time: float = float("inf"), output_times=(), output=None,
+            smallest_time_step_size: float = 1e-12) -> list:
         """RunSimulation-style loop (include/fub/RunSimulation.hpp:86-191):
-        dt = cfl * stable dt, one hierarchy step, then each level's sweeps'
-        stable dt checked against the level's own time step.  A failed check
-        is the reference's TimeStepTooLarge: the step is undone (the state
-        before it is kept on the device, like the reference's data /
-        scratch pair) and retried with cfl times the stable dt the sweeps
-        found; more than max_restarts failures in a row raise
-        L.TimeStepTooLarge.  Undoing a step restores the cycle count too
-        (the reference restores its whole solver, RunSimulation.hpp:151),
-        so the retry runs in the split order of the failed attempt.
+        at most `steps` coarse steps while t + smallest_time_step_size <
+        final_time; dt = min(final_time - t, next output time - t, cfl *
+        stable dt) (limited_dt, :125-131), one hierarchy step, then each
+        level's sweeps' stable dt checked against the level's own time step.
+        A failed check is the reference's TimeStepTooLarge: the step is
+        undone (the state before it is kept on the device, like the
+        reference's data / scratch pair) and retried with cfl times the
+        stable dt the sweeps found (capped the same way); more than
+        max_restarts failures in a row raise L.TimeStepTooLarge.  Undoing a
+        step restores the cycle count and time too (the reference restores
+        its whole solver, RunSimulation.hpp:151), so the retry runs in the
+        split order of the failed attempt.  output(hierarchy) is called when
+        the run reaches each of output_times.
         With regrid_every = k > 0, the levels finer than 0 are rebuilt from
         gradient tags (regrid_all(0, tol_rho, tol_p)) before every k-th
         coarse step after the first, as the reference's PreAdvanceLevel
@@ -244,11 +250,22 @@ class AmrHierarchy:
         (src/AMReX/IntegratorContext.cpp:600-626; its regrid_frequency
         option, include/fub/AMReX/IntegratorContext.hpp:54).
         Returns the accepted dts."""
+        eps = smallest_time_step_size
+        outs = sorted(float(x) for x in output_times if x > self.time + eps)
         dts = []
+
+        def limited(dt):
+            cap = final_time - self.time
+            if outs:
+                cap = min(cap, outs[0] - self.time)
+            return min(cap, dt)
+
         for step in range(steps):
+            if not self.time + eps < final_time:
+                break
             if regrid_every > 0 and step > 0 and step % regrid_every == 0:
                 self._adopt(self.regrid_all(0, tol_rho, tol_p))
-            dt = cfl * self.stable_dt()
+            dt = limited(cfl * self.stable_dt())
             saved = [b.download(0) for b in self.levels]
             cycle, time = self.cycle, self.time
             for attempt in range(max_restarts + 1):
@@ -260,11 +277,15 @@ class AmrHierarchy:
                 for b, pool in zip(self.levels, saved):
                     b.upload(pool, 0)
                 self.cycle, self.time = cycle, time
-                dt = cfl * limit
+                dt = limited(cfl * limit)
             else:
                 raise L.TimeStepTooLarge(L.FVS_TIME_STEP_TOO_LARGE,
                                          f"{max_restarts} restarts of one coarse step")
             dts.append(dt)
+            while outs and self.time + eps >= outs[0]:
+                if output is not None:
+                    output(self)
+                outs.pop(0)
         return dts
 
 def nests(patch, coarse_layout, extent, ratio: int = 2, margin: int = 1) -> bool:

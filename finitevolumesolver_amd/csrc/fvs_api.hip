@@ -1257,6 +1257,7 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
   double* dlog = nullptr;
   if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
   c.final_time = final_time;
+  c.eps = SmallestDt(L->patch);
   c.cfl = L->patch.cfl;
   c.halt = 0;
   c.dt_log = dlog;

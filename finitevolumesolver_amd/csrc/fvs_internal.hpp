@@ -70,10 +70,17 @@ struct SweepArgs {
   int* halt;              // nonzero: skip all work (nullable)
 };
 
+// RunOptions::smallest_time_step_size of a patch descriptor (1e-12, the
+// reference's default, when the field is not positive).
+inline double SmallestDt(const fvs_patch_desc& p) {
+  return p.smallest_dt > 0.0 ? p.smallest_dt : 1e-12;
+}
+
 // Device control block of the single-patch level driver.
 struct Ctl {
   double t;
   double final_time;
+  double eps;         // smallest time step size: stop once t + eps >= final_time
   double cfl;
   double dt;          // limited dt of the current step
   double split_dt;    // dt or dt / 2

@@ -2014,6 +2014,7 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
   double* dlog = nullptr;
   if (dt_log && max_steps > 0) FVS_HIP_CHECK(hipMalloc(&dlog, max_steps * sizeof(double)));
   c.final_time = final_time;
+  c.eps = SmallestDt(L->patch);
   c.cfl = L->patch.cfl;
   c.halt = 0;
   c.dt_log = dlog;

@@ -825,7 +825,7 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
     ctl->dt_dir[d] = v[d];
     stable = (v[d] < stable) ? v[d] : stable;
   }
-  if (!(ctl->t < ctl->final_time)) {
+  if (!(ctl->t + ctl->eps < ctl->final_time)) {
     ctl->halt = 2;
     return;
   }
@@ -851,7 +851,7 @@ __global__ void StepDtKernel(Ctl* ctl, int dim, int first_dir, int strang) {
   if (ctl->halt) return;
   double stable = 1.7976931348623157e308;
   for (int d = 0; d < dim; ++d) stable = (ctl->dt_dir[d] < stable) ? ctl->dt_dir[d] : stable;
-  if (!(ctl->t < ctl->final_time)) {
+  if (!(ctl->t + ctl->eps < ctl->final_time)) {
     ctl->halt = 2;
     return;
   }

@@ -306,7 +306,8 @@ class SinglePatchLevel:
     """Single-patch level integrator: DimensionalSplitLevelIntegrator with a
     NoSubcycleSolver and RunSimulation's dt logic, driven on the device."""
 
-    def __init__(self, flux_method: FluxMethod, n, dx, splitting="Godunov", cfl=0.8):
+    def __init__(self, flux_method: FluxMethod, n, dx, splitting="Godunov", cfl=0.8,
+                 smallest_time_step_size: float = 1e-12):
         self.fm = flux_method
         D = flux_method.equation.dim
         self.D = D
@@ -315,6 +316,8 @@ class SinglePatchLevel:
         pd.dx = (C.c_double * 3)(*(list(dx) + [1.0] * (3 - len(dx))))
         pd.splitting = L.SPLIT_STRANG if splitting == "Strang" else L.SPLIT_GODUNOV
         pd.cfl = cfl
+        pd.smallest_dt = smallest_time_step_size
+        self.eps = smallest_time_step_size
         self.n = tuple(n)
         h = C.c_void_p()
         L.check(L.load().fvs_level_create(flux_method.handle, C.byref(pd), C.byref(h)))
@@ -355,6 +358,36 @@ class SinglePatchLevel:
         if return_dts:
             return done.value, fails.value, [dts[i] for i in range(done.value)]
         return done.value, fails.value
+
+    def run(self, final_time: float, output_times=(), output=None, max_cycles: int = -1,
+            batch: int = 16) -> list:
+        """RunSimulation (include/fub/RunSimulation.hpp:86-191): steps until
+        t + smallest_time_step_size reaches final_time (or max_cycles steps
+        are taken), each step's dt capped by the final time and by the next
+        output time (limited_dt, :125-131), so the run lands exactly on every
+        output time in output_times; output(level, t, cycle) is called there
+        (the reference's output functor).  CFL failures restart the step as
+        advance() does.  Returns the accepted dts."""
+        dts = []
+        t, cycle = self.time()
+        outs = sorted(float(x) for x in output_times if x > t + self.eps)
+        while t + self.eps < final_time and (max_cycles < 0 or cycle < max_cycles):
+            stop = min(outs[0], final_time) if outs else final_time
+            # the device driver ends a call when t + smallest dt reaches `stop`
+            while True:
+                k = 256 if max_cycles < 0 else min(256, max_cycles - cycle)
+                done, _, got = self.advance(k, final_time=stop, batch=batch, return_dts=True)
+                dts += got
+                t, cycle = self.time()
+                if done < k or k == 0 or (max_cycles >= 0 and cycle >= max_cycles):
+                    break
+            while outs and t + self.eps >= outs[0]:
+                if output is not None:
+                    output(self, t, cycle)
+                outs.pop(0)
+            if done == 0 and t + self.eps < stop:
+                break
+        return dts
 
     def time(self):
         t = C.c_double()

@@ -240,6 +240,10 @@ typedef struct fvs_patch_desc {
   double dx[3];
   int splitting; /* enum fvs_splitting */
   double cfl;    /* RunOptions::cfl, 0.8 in the reference */
+  double smallest_dt; /* RunOptions::smallest_time_step_size
+                         (include/fub/RunSimulation.hpp:61-63, 97, 109): a run
+                         stops once t + smallest_dt >= final_time; 0 (or
+                         less) selects the reference's default 1e-12 */
 } fvs_patch_desc;
 
 typedef struct fvs_level fvs_level;

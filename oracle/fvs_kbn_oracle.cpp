@@ -793,7 +793,9 @@ int KbnLevelRun(const fvs_method_desc* d, const fvs_patch_desc* p, const fvs_cut
   double failure_dt = 0.0;
   std::vector<double> backup;
   int rc = FVS_OK;
-  while (t < final_time && cycle < max_steps) {
+  // RunOptions::smallest_time_step_size (RunSimulation.hpp:61-63, 97, 109)
+  const double eps = p->smallest_dt > 0.0 ? p->smallest_dt : 1e-12;
+  while (t + eps < final_time && cycle < max_steps) {
     backup = L.scratch;
     double stable = std::numeric_limits<double>::max();
     if (have_failure) {

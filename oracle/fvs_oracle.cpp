@@ -268,7 +268,7 @@ template <int D> struct Level {
 
 template <int D>
 int LevelRun(const fvs_method_desc* d, const fvs_patch_desc* p, double* cons,
-             double t0, int max_steps, double final_time, double* dt_log,
+             double t0, int64_t cycle0, int max_steps, double final_time, double* dt_log,
              int* steps_done, int* failures) {
   Level<D> L(d, p);
   const int64_t ncell = L.n[0] * L.n[1] * (D == 3 ? L.n[2] : 1);
@@ -290,12 +290,13 @@ int LevelRun(const fvs_method_desc* d, const fvs_patch_desc* p, double* cons,
     CompleteView<D>(L.m.gas, &iv, &uv);
   }
   double t = t0;
-  int64_t cycle = 0;
+  int64_t cycle = 0;  // steps taken here; the split order follows cycle0 + cycle
   int nfail = 0;
   int consecutive_fail = 0;
   bool have_failure = false;
   double failure_dt = 0.0;
-  const double eps = 0.0; // RunOptions::smallest_time_step_size default
+  // RunOptions::smallest_time_step_size (RunSimulation.hpp:61-63, 97, 109)
+  const double eps = p->smallest_dt > 0.0 ? p->smallest_dt : 1e-12;
   std::vector<double> backup;
   int rc = FVS_OK;
   while (t + eps < final_time && cycle < max_steps) {
@@ -303,7 +304,7 @@ int LevelRun(const fvs_method_desc* d, const fvs_patch_desc* p, double* cons,
     const double stable_dt = have_failure ? failure_dt : L.ComputeStableDt();
     const double limited_dt = std::min(final_time - t, p->cfl * stable_dt);
     // MakeSplitDirections (SplittingMethod.hpp:115-126) with subcycle {0, 1}.
-    const int is_odd = static_cast<int>((cycle % 2 + 0) % 2);
+    const int is_odd = static_cast<int>(((cycle0 + cycle) % 2 + 0) % 2);
     int dirs[3];
     for (int i = 0; i < D; ++i) dirs[i] = is_odd ? (D - 1 - i) : i;
     std::vector<std::pair<int, double>> seq;
@@ -642,15 +643,22 @@ int oracle_transmissive_fill(const fvs_view* states, const int64_t dlo[3],
 
 // ---- single-patch level integrator ------------------------------------------
 
+int oracle_level_run_from(const fvs_method_desc* desc, const fvs_patch_desc* patch,
+                          double* cons, double t0, int64_t cycle0, int max_steps,
+                          double final_time, double* dt_log, int* steps_done, int* failures) {
+  if (CheckDesc(desc) || !patch) return FVS_BAD_ARGUMENT;
+  return desc->dim == 2
+             ? LevelRun<2>(desc, patch, cons, t0, cycle0, max_steps, final_time, dt_log,
+                           steps_done, failures)
+             : LevelRun<3>(desc, patch, cons, t0, cycle0, max_steps, final_time, dt_log,
+                           steps_done, failures);
+}
+
 int oracle_level_run(const fvs_method_desc* desc, const fvs_patch_desc* patch,
                      double* cons, double t0, int max_steps, double final_time,
                      double* dt_log, int* steps_done, int* failures) {
-  if (CheckDesc(desc) || !patch) return FVS_BAD_ARGUMENT;
-  return desc->dim == 2
-             ? LevelRun<2>(desc, patch, cons, t0, max_steps, final_time, dt_log,
-                           steps_done, failures)
-             : LevelRun<3>(desc, patch, cons, t0, max_steps, final_time, dt_log,
-                           steps_done, failures);
+  return oracle_level_run_from(desc, patch, cons, t0, 0, max_steps, final_time, dt_log,
+                               steps_done, failures);
 }
 
 } // extern "C"

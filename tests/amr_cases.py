@@ -105,7 +105,8 @@ def shock_bubble(level, p, layouts=LAYOUTS):
     return cons_from(rho, u, v, p_)
 
 
-def oracle_run(o, steps, cfl, max_restarts=64, restore_cycle=True):
+def oracle_run(o, steps, cfl, max_restarts=64, restore_cycle=True, final_time=float("inf"),
+               output_times=(), eps=1e-12):
     """RunSimulation's loop (include/fub/RunSimulation.hpp:86-191) over the
     AMR oracle: dt = cfl * stable dt, one coarse step, each level's sweep
     minima checked against the level's own time step; a failed attempt is
@@ -114,8 +115,19 @@ def oracle_run(o, steps, cfl, max_restarts=64, restore_cycle=True):
     the sweeps found.  Returns (accepted dts, attempts per step)."""
     import copy
     dts, attempts = [], []
+    t = 0.0
+    outs = sorted(output_times)
+
+    def limited(dt):
+        cap = final_time - t
+        if outs:
+            cap = min(cap, outs[0] - t)
+        return min(cap, dt)
+
     for _ in range(steps):
-        dt = cfl * o.stable_dt()
+        if not t + eps < final_time:
+            break
+        dt = limited(cfl * o.stable_dt())
         saved, cycle = copy.deepcopy(o.U), o.cycle
         for attempt in range(max_restarts + 1):
             o.advance(dt)
@@ -126,9 +138,12 @@ def oracle_run(o, steps, cfl, max_restarts=64, restore_cycle=True):
             o.U = copy.deepcopy(saved)
             if restore_cycle:
                 o.cycle = cycle
-            dt = cfl * limit
+            dt = limited(cfl * limit)
         else:
             raise RuntimeError("too many restarts")
         dts.append(dt)
         attempts.append(attempt + 1)
+        t += dt
+        while outs and t + eps >= outs[0]:
+            outs.pop(0)
     return dts, attempts

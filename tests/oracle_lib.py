@@ -49,6 +49,8 @@ def load():
                                              C.POINTER(C.c_int64), I]),
             "oracle_level_run": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.PatchDesc), P, D, I,
                                      D, _PD, _PI, _PI]),
+            "oracle_level_run_from": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.PatchDesc), P, D,
+                                          C.c_int64, I, D, _PD, _PI, _PI]),
             "oracle_kbn_fill_cutcell_data": (I, [P, I, P, P]),
             "oracle_kbn_is_cut_cell": (I, [I, P, P, P, P]),
             "oracle_kbn_reference_states": (I, [C.POINTER(L.MethodDesc), C.POINTER(L.View),
@@ -204,8 +206,10 @@ def exact_signals(left, right, dim, dir, gamma=1.4):
 
 
 def level_run(desc: L.MethodDesc, cons: np.ndarray, dx, n_steps, splitting=L.SPLIT_GODUNOV,
-              cfl=0.8, final_time=1e300, t0=0.0):
-    """Runs the oracle's single-patch level driver; returns (cons, dts, failures)."""
+              cfl=0.8, final_time=1e300, t0=0.0, cycle0=0):
+    """Runs the oracle's single-patch level driver from time t0 and cycle
+    count cycle0 (which sets the first step's split order); returns (cons,
+    dts, failures)."""
     D = desc.dim
     cons = np.ascontiguousarray(cons.copy())
     pd = L.PatchDesc()
@@ -217,8 +221,9 @@ def level_run(desc: L.MethodDesc, cons: np.ndarray, dx, n_steps, splitting=L.SPL
     dts = np.zeros(max(n_steps, 1))
     done = C.c_int()
     fails = C.c_int()
-    rc = load().oracle_level_run(C.byref(desc), C.byref(pd), ptr(cons), t0, n_steps, final_time,
-                                 dts.ctypes.data_as(_PD), C.byref(done), C.byref(fails))
+    rc = load().oracle_level_run_from(C.byref(desc), C.byref(pd), ptr(cons), t0, cycle0, n_steps,
+                                      final_time, dts.ctypes.data_as(_PD), C.byref(done),
+                                      C.byref(fails))
     assert rc == 0, rc
     return cons, dts[:done.value], fails.value
 

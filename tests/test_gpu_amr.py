@@ -259,6 +259,27 @@ def test_amr_checkpoint_restart_continues_bit_for_bit(cuda, tmp_path):
             np.testing.assert_array_equal(a, b)
 
 
+def test_amr_run_lands_on_output_times_and_final_time(cuda):
+    """run() caps each coarse step at the next output time and the final
+    time (RunSimulation's limited_dt) and calls the output there; dts and
+    every level equal the oracle's run loop with the same caps bit for bit,
+    and the run ends at the final time."""
+    _note("amr run with output times: moving state")
+    fm, h, o = _pair(A.moving_state)
+    dt0 = 0.8 * h.stable_dt()
+    outs = [1.5 * dt0, 2.2 * dt0]
+    final = 3.7 * dt0
+    seen = []
+    dts = h.run(20, cfl=0.8, final_time=final, output_times=outs,
+                output=lambda hh: seen.append(hh.time))
+    ref_dts, _ = A.oracle_run(o, 20, 0.8, final_time=final, output_times=outs)
+    _note(f"  dts {dts}, outputs at {seen}")
+    assert dts == ref_dts
+    assert len(seen) == 2 and all(abs(a - b) <= 1e-15 for a, b in zip(seen, outs))
+    assert abs(h.time - final) <= 1e-15
+    _compare(h, o, "after the capped run")
+
+
 def test_amr_run_restarts_on_cfl_failure(cuda):
     """With cfl 1.6 every attempt of a step exceeds a level's stable dt (as
     in the reference, a cfl above 1 never passes the check): run() undoes

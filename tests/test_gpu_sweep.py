@@ -227,6 +227,37 @@ def test_level_final_time_and_cfl_failure(cuda):
             assert fails > 0
 
 
+def test_level_run_lands_on_output_times(cuda):
+    """RunSimulation's loop (include/fub/RunSimulation.hpp:86-191): each step
+    capped by the next output time and the final time, the output called at
+    exactly those times, the run ending once t + 1e-12 reaches the final
+    time; state and dt sequence equal the oracle run piecewise between the
+    output times (each piece resumed at the previous piece's time and cycle
+    count) bit for bit."""
+    torch = _torch()
+    from finitevolumesolver_amd.initial_data import circular_riemann_2d
+    cons, dx = circular_riemann_2d(32)
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=2))
+    lvl = fvs.SinglePatchLevel(fm, (32, 32), dx, splitting="Godunov", cfl=0.8)
+    lvl.set_state(torch.from_numpy(cons).to(cuda))
+    seen = []
+    dts = lvl.run(0.12, output_times=[0.03, 0.07, 0.1],
+                  output=lambda level, t, cycle: seen.append((t, cycle)))
+    got = lvl.get_state().cpu().numpy()
+    ref, ref_dts, cyc, t = cons, [], 0, 0.0
+    for stop in (0.03, 0.07, 0.1, 0.12):
+        ref, d, _ = O.level_run(fm._h.desc, ref, dx, 1000, L.SPLIT_GODUNOV, final_time=stop,
+                                t0=t, cycle0=cyc)
+        ref_dts += list(d)
+        cyc += len(d)
+        t = stop
+    _note(f"run: {len(dts)} steps, outputs at {seen}")
+    assert [round(x[0], 15) for x in seen] == [0.03, 0.07, 0.1]
+    assert dts == ref_dts and lvl.time()[1] == cyc
+    assert np.array_equal(got, ref)
+    assert abs(lvl.time()[0] - 0.12) < 1e-15
+
+
 @pytest.mark.parametrize("case", ["2d_strang", "3d_godunov", "godunov_base"])
 def test_level_graph_replay_matches_eager(cuda, case):
     """The level's captured step graphs, replayed over several batches with

@@ -368,3 +368,26 @@ def test_level_cfl_restart():
     desc = make_desc(PerfectGas(2))
     _, dts, fails = O.level_run(desc, cons, dx, 12, L.SPLIT_STRANG, cfl=1.5)
     assert fails >= 1 and len(dts) == 12
+
+
+def test_level_run_in_pieces_and_smallest_time_step():
+    """RunSimulation's loop restarted from (t, cycle) continues the same run:
+    two halves with the second starting at the first's cycle count equal one
+    run bit for bit (the split order follows the cycle count).  The run stops
+    once t + smallest_time_step_size (1e-12 by default,
+    include/fub/RunSimulation.hpp:61-63, 109) reaches final_time: a final
+    time closer than that takes no step."""
+    from finitevolumesolver_amd.initial_data import circular_riemann_2d
+    cons, dx = circular_riemann_2d(24)
+    desc = make_desc(PerfectGas(2))
+    for split in (L.SPLIT_GODUNOV, L.SPLIT_STRANG):
+        full, dts, _ = O.level_run(desc, cons, dx, 6, split)
+        a, d1, _ = O.level_run(desc, cons, dx, 3, split)
+        b, d2, _ = O.level_run(desc, a, dx, 3, split, t0=float(sum(d1)), cycle0=3)
+        assert np.array_equal(b, full) and list(d1) + list(d2) == list(dts)
+        c, d3, _ = O.level_run(desc, a, dx, 3, split, t0=float(sum(d1)), cycle0=0)
+        assert not np.array_equal(c, full)
+    _, none, _ = O.level_run(desc, cons, dx, 4, final_time=0.5e-12)
+    assert len(none) == 0
+    _, one, _ = O.level_run(desc, cons, dx, 4, final_time=2e-12)
+    assert len(one) == 1 and one[0] == 2e-12
