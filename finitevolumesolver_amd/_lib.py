@@ -117,6 +117,8 @@ SIGNATURES = {
     "fvs_level_get_state": (_I, [_P, C.POINTER(_P)]),
     "fvs_level_advance": (_I, [_P, _I, _D, _I, _PD, C.POINTER(_I), C.POINTER(_I)]),
     "fvs_level_time": (_I, [_P, _PD, _PI64]),
+    "fvs_level_hint_max_speed": (_I, [_P, _PD]),
+    "fvs_level_cfl_redos": (_I, [_P, _PI64]),
     "fvs_level_set_timing": (_I, [_P, _I]),
     "fvs_level_set_graphs": (_I, [_P, _I]),
     "fvs_level_kernel_times": (_I, [_P, _PD, _PI64]),

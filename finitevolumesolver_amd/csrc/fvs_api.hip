@@ -837,6 +837,7 @@ struct fvs_level {
   double* buf[3][5];
   Ctl* ctl;               // device control block
   unsigned long long* partials;  // per-workgroup minima
+  int64_t cfl_redos = 0;  // steps repeated because a speed floor was wrong
   double* pm = nullptr;   // exact Godunov: the step-start CFL pass's star
                           // pressures of the first sweep's faces
   int cur;                // buffer holding the current state
@@ -1109,6 +1110,37 @@ int fvs_level_time(fvs_level* L, double* t, int64_t* cycles) {
   return FVS_OK;
 }
 
+int fvs_level_hint_max_speed(fvs_level* L, const double speed[3]) {
+  if (!L || !speed) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (!(speed[d] >= 0.0) || !(speed[d] < 1.7976931348623157e308)) {
+      SetError("speed hint %d must be finite and non-negative", d);
+      return FVS_BAD_ARGUMENT;
+    }
+  }
+  // only the 3D HLLEM pass has a floor
+  if (L->h->desc.dim != 3 || L->h->desc.base == FVS_BASE_GODUNOV_EXACT) return FVS_OK;
+  Ctl c;
+  FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, L->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
+  for (int d = 0; d < 3; ++d) c.cfl_floor[d] = speed[d];
+  FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, L->h->stream));
+  FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
+  return FVS_OK;
+}
+
+int fvs_level_cfl_redos(fvs_level* L, int64_t* redos) {
+  if (!L || !redos) {
+    SetError("null argument");
+    return FVS_BAD_ARGUMENT;
+  }
+  *redos = L->cfl_redos;
+  return FVS_OK;
+}
+
 // One time step of the level (CFL pass, the split sweeps with their CFL
 // checks, step end) launched on the handle's stream from buffer `cur`;
 // *next receives the buffer holding the result.  The launch sequence
@@ -1318,6 +1350,14 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
     L->cur = start_buf[completed];
     cycle = c.cycle;
     if (c.halt == 2) break;
+    if (c.halt == 3) {
+      // The CFL pass's speed floor (already cleared) was not reached: the
+      // step runs again with every face evaluated.  Not a CFL failure.
+      ++L->cfl_redos;
+      c.halt = 0;
+      FVS_HIP_CHECK(hipMemcpyAsync(L->ctl, &c, sizeof c, hipMemcpyHostToDevice, s));
+      continue;
+    }
     // TimeStepTooLarge: restart the step with cfl * failure_dt
     // (RunSimulation.hpp:141-151).
     ++nfail;

@@ -76,6 +76,14 @@ inline double SmallestDt(const fvs_patch_desc& p) {
   return p.smallest_dt > 0.0 ? p.smallest_dt : 1e-12;
 }
 
+// Relative margin of the CFL pass's speed floor below the previous pass's
+// largest speed: four times the largest relative drop of that speed over
+// the last step, at least half the previous margin and kFloorMarginMin, at
+// most kFloorMarginMax; a floor found unreached quadruples it.
+constexpr double kFloorMarginMin = 0x1p-10;
+constexpr double kFloorMarginFirst = 0x1p-6;
+constexpr double kFloorMarginMax = 0.25;
+
 // Device control block of the single-patch level driver.
 struct Ctl {
   double t;
@@ -86,10 +94,21 @@ struct Ctl {
   double split_dt;    // dt or dt / 2
   double failure_dt;  // estimate after a TimeStepTooLarge
   int have_failure;   // use failure_dt for the next step
-  int halt;           // 0 running, 1 TimeStepTooLarge, 2 final time reached
+  int halt;           // 0 running, 1 TimeStepTooLarge, 2 final time reached,
+                      // 3 the speed floor of the step-start CFL pass was wrong
+                      // (the host repeats the step without it)
   int64_t cycle;
   double dt_dir[3];   // step-start stable dt per direction (decoded)
   double last_check;
+  // 3D step-start CFL pass: a speed floor per direction (0: none), the
+  // previous pass's largest signal speed less floor_margin.  The pass
+  // evaluates the exact speed only of faces whose cheap upper bound reaches
+  // it; floor_hit[d] records that some evaluated face did reach it, which
+  // proves that the skipped faces are below the maximum.
+  double cfl_floor[3];
+  int floor_hit[3];
+  double floor_margin;    // relative margin of the last floor (0: none yet)
+  double floor_speed[3];  // largest speeds of the last pass (0: none yet)
   double* dt_log;       // optional log of the steps' dt (set per advance call)
   int64_t dt_log_base;  // cycle of dt_log[0]
   int64_t dt_log_len;
@@ -109,13 +128,15 @@ int LaunchSweepFast(int dim, int precision, bool larr, hipStream_t stream, int d
                     const SweepArgs& a);
 int LaunchPatchCflFast(int dim, unsigned gx, unsigned gy, hipStream_t stream,
                        const double* const* cons, const int64_t n[3], const double dx[3],
-                       double gamma, double gm1inv, unsigned long long* partials, int* halt);
+                       double gamma, double gm1inv, unsigned long long* partials, int* halt,
+                       Ctl* floor_ctl);
 // Reduces a sweep's partial minima: writes the value to out (nullable) and,
 // with a control block, applies the split step's CFL check.
 int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials,
                         int64_t nblocks, double* out, Ctl* ctl);
 // Step-start CFL pass: per-direction partial minima, then a finalize kernel
 // writing dt_dir (out, nullable) or, with a control block, the step's dt.
+// In 3D with a control block the pass skips the faces below ctl->cfl_floor.
 int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t stream,
                    const double* const* cons, const int64_t n[3],
                    const double dx[3], unsigned long long* partials,

@@ -85,9 +85,13 @@ __device__ __forceinline__ double WaveMaxSpeed(double s) {
 // monotonically with its denominator, so the smallest face dt is the
 // quotient by the largest speed, bit for bit: one division per block and
 // direction instead of one per face.  Every thread of the block must call it.
+// With a speed floor (floor[d] > 0), a block whose largest speed reaches it
+// sets hit[d].
 template <int NK, int NWAVES, class A>
 __device__ __forceinline__ void BlockSpeedToPartials(const double* speed, const double* dx,
-                                                     unsigned long long* out, int64_t stride) {
+                                                     unsigned long long* out, int64_t stride,
+                                                     const double* floor = nullptr,
+                                                     int* hit = nullptr) {
   __shared__ double s_speed[NK][NWAVES];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -103,6 +107,7 @@ __device__ __forceinline__ void BlockSpeedToPartials(const double* speed, const 
       double v = s_speed[d][0];
       for (int w = 1; w < NWAVES; ++w) v = SMax(v, s_speed[d][w]);
       out[d * stride] = v > 0.0 ? DtKey(A::quot(dx[d], v)) : kInfKey;
+      if (hit && floor[d] > 0.0 && v >= floor[d]) hit[d] = 1;
     }
   }
 }
@@ -667,27 +672,40 @@ __global__ __launch_bounds__(kBlock) void PatchCfl2Kernel(
   BlockSpeedToPartials<D, kBlock / 64, DA>(smax, dxs, partials + bid, nblocks);
 }
 
-// 3D step-start CFL.  A workgroup is a tile of 9 x rows of 64 cells (one
+// 3D step-start CFL.  A workgroup is a tile of 8 x rows of 64 cells (one
 // wave each) marching along z over a segment.  Lanes 1..62 of a wave own
-// their cells and lanes 0 and 63 hold the x neighbours; row 0 is a halo row (the y - 1
-// neighbours of row 1) and x and y faces read their neighbours from LDS; z
-// faces pair the plane with the previous one kept in registers.  Each cell's
-// DtCell is computed 64 * 9 / (62 * 8) = 1.16 times instead of twice.
-// Tiles are grouped per XCD (workgroups are dispatched round-robin over the 8
-// XCDs): XCD k gets a contiguous range of tiles, so vertically neighbouring
-// tiles, whose halo rows overlap, share one L2.
-constexpr int kC3Rows = 9;
+// their cells and lane 0 holds the x - 1 neighbours; row 0 is a halo row (the
+// y - 1 neighbours of row 1).  Each cell yields a BoundCell (u and a^2, no
+// square root) once per visit, 64 * 8 / (62 * 7) = 1.18 times; x and y faces
+// read the neighbour's from LDS, z faces pair the plane with the previous one
+// kept in registers.  Two workgroups of eight waves fill a CU at four waves
+// per SIMD.  Tiles are grouped per XCD (workgroups are dispatched round-robin
+// over the 8 XCDs): XCD k gets a contiguous range of tiles, so vertically
+// neighbouring tiles, whose halo rows overlap, share one L2.
+constexpr int kC3Rows = 8;
 constexpr int kC3OwnX = 62;
 constexpr int kC3OwnY = kC3Rows - 1;
 constexpr int kXcds = 8;
 
+//
+// A face takes the exact FaceSpeed only when its upper bound U + sqrt(A^2 +
+// k |du|^2) (see BoundPasses) reaches the speed floor shrunk by 2^-20 (the
+// level driver's ctl->cfl_floor; without a control block, or with a zero
+// floor, every face with a nonzero bound is evaluated).  A skipped face is
+// then slower than the floor, and the floor is at most the largest evaluated
+// speed whenever floor_hit is set, so the maximum, and every dt derived from
+// it, is that of evaluating every face, bit for bit: the evaluated faces run
+// the same instructions with or without a floor.  FinalizeCflKernel rejects
+// the step (halt 3) when no evaluated face reached the floor.  Exact speeds
+// take the x neighbour from the wave (DPP) and the y and z neighbours from
+// memory (cached reloads of the row below and the plane before).
 template <bool FAST>
 __global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
     const double* __restrict__ u0, const double* __restrict__ u1,
     const double* __restrict__ u2, const double* __restrict__ u3,
     const double* __restrict__ u4, int64_t nx, int64_t ny, int64_t nz, double dx0,
     double dx1, double dx2, double gamma, double gm1inv, int64_t ntiles,
-    unsigned long long* partials, int* halt) {
+    unsigned long long* partials, int* halt, Ctl* ctl) {
   using DA = typename Policies<double, FAST>::DA;
   constexpr int NC = 5;
   if (halt && *halt) return;
@@ -696,6 +714,15 @@ __global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
   g.gamma = gamma;
   g.gm1inv = gm1inv;
   g.gm1 = gamma - 1.0;
+  const double kbound = 0.125 * g.gm1;
+  double floor[3] = {0.0, 0.0, 0.0};
+  if (ctl) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) floor[d] = ctl->cfl_floor[d];
+  }
+  double s_crit[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) s_crit[d] = floor[d] * (1.0 - 0x1p-20);
   const int lane = threadIdx.x & 63;
   const int row = threadIdx.x >> 6;
   const int64_t per_xcd = (ntiles + kXcds - 1) / kXcds;
@@ -713,53 +740,78 @@ __global__ __launch_bounds__(64 * kC3Rows) void PatchCfl3Kernel(
   const int64_t m1 = (m0 + kCflSeg < nz) ? m0 + kCflSeg : nz;
   const int64_t plane = nx * ny;
   const int64_t col = xs + ys * nx;
+  // the y - 1 neighbour's column (clamped: at y = 0 the cell itself)
+  const int64_t col_below = xs + Clamp(ys - 1, ny) * nx;
 
-  __shared__ DtCell<3> s_cells[2][kC3Rows][64];
+  __shared__ BoundCell<3> s_b[2][kC3Rows][64];
   double smax[3] = {0.0, 0.0, 0.0};
   const double dxs[3] = {dx0, dx1, dx2};
-  auto load = [&](double* u, int64_t m) {
-    const int64_t off = col + Clamp(m, nz) * plane;
+  auto load = [&](double* u, int64_t c, int64_t m) {
+    const int64_t off = c + Clamp(m, nz) * plane;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) u[c] = in[c][off];
+    for (int k = 0; k < NC; ++k) u[k] = in[k][off];
   };
-  // Indices are clamped, so at a lower patch edge the neighbour a face
-  // pairs with (lane 0 at x = -1, row 0 at y = -1, plane m0 - 1 = -1) holds
-  // the cell's own data: its DtCell equals the cell's bit for bit, which
-  // makes it the transmissive ghost copy without a select.
   double ucur[NC], un1[NC], un2[NC];
-  load(un2, m0 - 1);
-  DtCell<3> dprev = MakeDtCell<3, DA>(un2, g);
-  load(ucur, m0);
-  load(un1, m0 + 1);
+  load(ucur, col, m0 - 1);
+  BoundCell<3> bprev = MakeBoundCell<3>(ucur, g);
+  load(ucur, col, m0);
+  load(un1, col, m0 + 1);
   int buf = 0;
   for (int64_t m = m0; m < m1; ++m) {
-    // The loads of the next two planes are in flight while this plane is
-    // reduced (indices are clamped, so reading past the segment is harmless).
-    load(un2, m + 2);
-    const DtCell<3> dc = MakeDtCell<3, DA>(ucur, g);
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      ucur[c] = un1[c];
-      un1[c] = un2[c];
-    }
-    s_cells[buf][row][lane] = dc;
+    load(un2, col, m + 2);
+    const BoundCell<3> bc = MakeBoundCell<3>(ucur, g);
+    s_b[buf][row][lane] = bc;
     __syncthreads();
+    bool px = true, py = true, pz = true;
     if (own) {
-      const DtCell<3> dl = s_cells[buf][row][lane - 1];
-      smax[0] = SMax(smax[0], FaceSpeed<3, 0, DA>(dl, dc, g.gm1));
-      if (x == nx - 1) smax[0] = SMax(smax[0], FaceSpeed<3, 0, DA>(dc, dc, g.gm1));
-      const DtCell<3> db = s_cells[buf][row - 1][lane];
-      smax[1] = SMax(smax[1], FaceSpeed<3, 1, DA>(db, dc, g.gm1));
-      if (y == ny - 1) smax[1] = SMax(smax[1], FaceSpeed<3, 1, DA>(dc, dc, g.gm1));
-      smax[2] = SMax(smax[2], FaceSpeed<3, 2, DA>(dprev, dc, g.gm1));
-      if (m == nz - 1) smax[2] = SMax(smax[2], FaceSpeed<3, 2, DA>(dc, dc, g.gm1));
+      const BoundCell<3> bl = s_b[buf][row][lane - 1];
+      px = BoundPasses<3, 0>(bl, bc, s_crit[0], kbound) &&
+           (x != nx - 1 || BoundPasses<3, 0>(bc, bc, s_crit[0], kbound));
+      const BoundCell<3> bb = s_b[buf][row - 1][lane];
+      py = BoundPasses<3, 1>(bb, bc, s_crit[1], kbound) &&
+           (y != ny - 1 || BoundPasses<3, 1>(bc, bc, s_crit[1], kbound));
+      pz = BoundPasses<3, 2>(bprev, bc, s_crit[2], kbound) &&
+           (m != nz - 1 || BoundPasses<3, 2>(bc, bc, s_crit[2], kbound));
     }
-    dprev = dc;
+    const bool all_x = __all(px), all_y = __all(py), all_z = __all(pz);
+    if (!(all_x && all_y && all_z)) {
+      // wave-uniform: every lane computes its DtCell, so the x neighbour's
+      // is one lane away
+      const DtCell<3> dc = MakeDtCell<3, DA>(ucur, g);
+      if (!all_x) {
+        const DtCell<3> dl = ShflUp<3>(dc);
+        if (!px) {
+          smax[0] = SMax(smax[0], FaceSpeed<3, 0, DA>(dl, dc, g.gm1));
+          if (x == nx - 1) smax[0] = SMax(smax[0], FaceSpeed<3, 0, DA>(dc, dc, g.gm1));
+        }
+      }
+      if (!py) {
+        double ub[NC];
+        load(ub, col_below, m);
+        const DtCell<3> db = MakeDtCell<3, DA>(ub, g);
+        smax[1] = SMax(smax[1], FaceSpeed<3, 1, DA>(db, dc, g.gm1));
+        if (y == ny - 1) smax[1] = SMax(smax[1], FaceSpeed<3, 1, DA>(dc, dc, g.gm1));
+      }
+      if (!pz) {
+        double up[NC];
+        load(up, col, m - 1);
+        const DtCell<3> dp = MakeDtCell<3, DA>(up, g);
+        smax[2] = SMax(smax[2], FaceSpeed<3, 2, DA>(dp, dc, g.gm1));
+        if (m == nz - 1) smax[2] = SMax(smax[2], FaceSpeed<3, 2, DA>(dc, dc, g.gm1));
+      }
+    }
+    bprev = bc;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      ucur[k] = un1[k];
+      un1[k] = un2[k];
+    }
     buf ^= 1;
   }
   const int64_t nblocks = static_cast<int64_t>(gridDim.x) * gridDim.y;
   const int64_t bid = blockIdx.x + static_cast<int64_t>(blockIdx.y) * gridDim.x;
-  BlockSpeedToPartials<3, kC3Rows, DA>(smax, dxs, partials + bid, nblocks);
+  BlockSpeedToPartials<3, kC3Rows, DA>(smax, dxs, partials + bid, nblocks, floor,
+                                       ctl ? ctl->floor_hit : nullptr);
 }
 
 int64_t Cfl3Tiles(int64_t nx, int64_t ny) {
@@ -807,9 +859,12 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCheckKernel(
   }
 }
 
+// With use_floor (the level's 3D pass, fdx the cell widths), the floor of
+// the pass is validated first and the next step's floor set from this pass's
+// largest speeds.
 __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
     const unsigned long long* partials, int64_t n, int dim, double* out, Ctl* ctl,
-    int first_dir, int strang) {
+    int first_dir, int strang, double fdx0, double fdx1, double fdx2, int use_floor) {
   if (ctl && ctl->halt) return;
   double v[3];
   for (int d = 0; d < dim; ++d) v[d] = KeyToDt(ReduceKeys(partials + d * n, n));
@@ -818,6 +873,40 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
     for (int d = 0; d < dim; ++d) out[d] = v[d];
   }
   if (!ctl) return;
+  if (use_floor) {
+    // A floor that no evaluated face reached may have hidden the fastest
+    // face: the step is repeated without a floor, and the margin quadruples.
+    bool floor_ok = true;
+    for (int d = 0; d < 3; ++d) {
+      if (ctl->cfl_floor[d] > 0.0 && !ctl->floor_hit[d]) floor_ok = false;
+      ctl->floor_hit[d] = 0;
+    }
+    const double old_margin = ctl->floor_margin > 0.0 ? ctl->floor_margin : kFloorMarginFirst;
+    if (!floor_ok) {
+      for (int d = 0; d < 3; ++d) ctl->cfl_floor[d] = 0.0;
+      ctl->floor_margin = fmin(4.0 * old_margin, kFloorMarginMax);
+      ctl->halt = 3;
+      return;
+    }
+    // The next step's floor: this state's largest speeds dx / dt less a
+    // margin that follows how fast they fall.
+    const double fdx[3] = {fdx0, fdx1, fdx2};
+    double speed[3], drop = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      speed[d] = fdx[d] / v[d];
+      if (ctl->floor_speed[d] > 0.0 && speed[d] > 0.0)
+        drop = fmax(drop, (ctl->floor_speed[d] - speed[d]) / speed[d]);
+    }
+    const double margin =
+        fmin(kFloorMarginMax, fmax(fmax(kFloorMarginMin, 4.0 * drop), 0.5 * old_margin));
+    ctl->floor_margin = margin;
+    for (int d = 0; d < 3; ++d) {
+      const double f = speed[d] * (1.0 - margin);
+      const bool finite = f > 0.0 && f < 1.7976931348623157e308;
+      ctl->cfl_floor[d] = finite ? f : 0.0;
+      ctl->floor_speed[d] = finite ? speed[d] : 0.0;
+    }
+  }
   // DimensionalSplitLevelIntegrator::ComputeStableDt (:204-219) starts from
   // DBL_MAX; RunSimulation's limited_dt (RunSimulation.hpp:121-131).
   double stable = 1.7976931348623157e308;
@@ -999,7 +1088,8 @@ int LaunchSweepFast(int dim, int precision, bool larr, hipStream_t s, int dir,
 
 int LaunchPatchCflFast(int dim, unsigned gx, unsigned gy, hipStream_t s,
                        const double* const* cons, const int64_t n[3], const double dx[3],
-                       double gamma, double gm1inv, unsigned long long* partials, int* halt) {
+                       double gamma, double gm1inv, unsigned long long* partials, int* halt,
+                       Ctl* floor_ctl) {
   const dim3 grid(gx, gy);
   if (dim == 2) {
     hipLaunchKernelGGL(PatchCfl2Kernel<true>, grid, dim3(kBlock), 0, s, cons[0], cons[1],
@@ -1008,7 +1098,7 @@ int LaunchPatchCflFast(int dim, unsigned gx, unsigned gy, hipStream_t s,
   } else {
     hipLaunchKernelGGL(PatchCfl3Kernel<true>, grid, dim3(64 * kC3Rows), 0, s, cons[0], cons[1],
                        cons[2], cons[3], cons[4], n[0], n[1], n[2], dx[0], dx[1], dx[2], gamma,
-                       gm1inv, Cfl3Tiles(n[0], n[1]), partials, halt);
+                       gm1inv, Cfl3Tiles(n[0], n[1]), partials, halt, floor_ctl);
   }
   FVS_LAUNCH_CHECK();
   return FVS_OK;
@@ -1077,7 +1167,7 @@ int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials, int64
 int LaunchFinalizeCfl(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
                       int dim, double* out, Ctl* ctl, int first_dir, int strang) {
   hipLaunchKernelGGL(FinalizeCflKernel, dim3(1), dim3(kRedBlock), 0, s, partials, nblocks, dim,
-                     out, ctl, first_dir, strang);
+                     out, ctl, first_dir, strang, 1.0, 1.0, 1.0, 0);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
@@ -1092,9 +1182,11 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
   if (gx <= 0 || gx > 0x7fffffff || gy <= 0 || gy > 65535) return FVS_BAD_ARGUMENT;
   const double gm1inv = 1.0 / (desc.gamma - 1.0);
   const dim3 grid(static_cast<unsigned>(gx), static_cast<unsigned>(gy));
+  // The level driver's 3D pass (a control block) uses the speed floor.
+  Ctl* floor_ctl = (D == 3 && ctl) ? ctl : nullptr;
   if (desc.precision != FVS_FP64) {
     if (int rc = LaunchPatchCflFast(D, grid.x, grid.y, s, cons, n, dx, desc.gamma, gm1inv,
-                                    partials, halt))
+                                    partials, halt, floor_ctl))
       return rc;
   } else if (D == 2) {
     hipLaunchKernelGGL(PatchCfl2Kernel<false>, grid, dim3(kBlock), 0, s, cons[0], cons[1],
@@ -1103,11 +1195,12 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
   } else {
     hipLaunchKernelGGL(PatchCfl3Kernel<false>, grid, dim3(64 * kC3Rows), 0, s, cons[0],
                        cons[1], cons[2], cons[3], cons[4], n[0], n[1], n[2], dx[0], dx[1],
-                       dx[2], desc.gamma, gm1inv, Cfl3Tiles(n[0], n[1]), partials, halt);
+                       dx[2], desc.gamma, gm1inv, Cfl3Tiles(n[0], n[1]), partials, halt,
+                       floor_ctl);
   }
   FVS_LAUNCH_CHECK();
   hipLaunchKernelGGL(FinalizeCflKernel, dim3(1), dim3(kRedBlock), 0, s, partials, gx * gy, D,
-                     out, ctl, first_dir, strang);
+                     out, ctl, first_dir, strang, dx[0], dx[1], dx[2], floor_ctl ? 1 : 0);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }

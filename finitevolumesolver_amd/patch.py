@@ -395,6 +395,21 @@ class SinglePatchLevel:
         L.check(self._lib.fvs_level_time(self.h, C.byref(t), C.byref(cyc)))
         return t.value, cyc.value
 
+    def hint_max_speed(self, speed) -> None:
+        """Speed floor of the next 3D step-start CFL pass (per direction; by
+        default the previous pass's largest speed less a margin that follows
+        how fast that speed falls).  A floor above
+        the true largest signal speed costs one repeated step and never
+        changes a result."""
+        arr = (C.c_double * 3)(*(list(speed) + [0.0] * (3 - len(speed))))
+        L.check(self._lib.fvs_level_hint_max_speed(self.h, arr))
+
+    def cfl_redos(self) -> int:
+        """Steps repeated because the CFL pass's speed floor was not reached."""
+        n = C.c_int64()
+        L.check(self._lib.fvs_level_cfl_redos(self.h, C.byref(n)))
+        return n.value
+
     def set_timing(self, on: bool = True) -> None:
         """Brackets every kernel launch of advance() with HIP events."""
         L.check(self._lib.fvs_level_set_timing(self.h, 1 if on else 0))
