@@ -963,7 +963,12 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
     return FVS_HIP_ERROR;
   }
   int64_t np = D * CflBlocks(D, n);
-  for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, SweepBlocks(D, d, n, PencilSeg(D, d, n)));
+  int64_t sweep_blocks = 0;
+  for (int d = 0; d < D; ++d)
+    sweep_blocks = std::max<int64_t>(sweep_blocks, SweepBlocks(D, d, n, PencilSeg(D, d, n)));
+  // the checked sweeps of a step keep their partial minima side by side
+  // (at most 2D - 1 of them, Strang)
+  np = std::max<int64_t>(np, (2 * D - 1) * sweep_blocks);
   if (godunov) {
     np = std::max<int64_t>(np, 5 * D * GodunovCflBlocks(D, n));
     for (int d = 0; d < D; ++d) np = std::max<int64_t>(np, 5 * GodunovSweepBlocks(D, d, n, PencilSeg(D, d, n)));
@@ -1184,6 +1189,10 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
   if (rc) return rc;
   const int b0 = cur, b1 = (cur + 1) % 3, b2 = (cur + 2) % 3;
   int src = b0, dst = b1;
+  CheckList checks{};
+  checks.stride = 0;
+  for (int d = 0; d < D; ++d)
+    checks.stride = std::max<int64_t>(checks.stride, SweepBlocks(D, d, L->n, PencilSeg(D, d, L->n)));
   for (int w = 0; w < ns; ++w) {
     SweepArgs a;
     double* outp[5];
@@ -1215,20 +1224,20 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
     a.halt = &L->ctl->halt;
     a.check = (w > 0);
     a.bound_check = 1;
-    a.partials = L->partials;
+    if (a.check) {
+      if (checks.count >= kMaxChecks) return FVS_BAD_ARGUMENT;
+      a.partials = L->partials + checks.count * checks.stride;
+      checks.n[checks.count++] = SweepBlocks(D, seq[w], L->n, a.seg);
+    }
     rc = Timed(L, s, FVS_TIMER_SWEEP_X + seq[w], [&] { return LaunchSweep(h->desc, s, seq[w], a); });
     if (rc) break;
-    if (a.check) {
-      rc = LaunchFinalizeCheck(s, L->partials, SweepBlocks(D, seq[w], L->n, a.seg), nullptr,
-                               L->ctl);
-      if (rc) break;
-    }
     src = dst;
     dst = (dst == b1) ? b2 : b1;
   }
   if (rc) return rc;
   *next = src;
-  return LaunchStepEnd(s, L->ctl);
+  if (godunov) return LaunchStepEnd(s, L->ctl);
+  return LaunchFinalizeStep(s, L->partials, checks, L->ctl);
 }
 
 // The step as a replayed HIP graph, captured on first use per key on the

@@ -134,6 +134,17 @@ int LaunchPatchCflFast(int dim, unsigned gx, unsigned gy, hipStream_t stream,
 // with a control block, applies the split step's CFL check.
 int LaunchFinalizeCheck(hipStream_t s, const unsigned long long* partials,
                         int64_t nblocks, double* out, Ctl* ctl);
+// The CFL checks of a level step's sweeps (partial minima of check r at
+// partials + r * stride, n[r] of them), reduced in order after the last
+// sweep, then the step end.
+constexpr int kMaxChecks = 5;
+struct CheckList {
+  int64_t n[kMaxChecks];
+  int64_t stride;
+  int count;
+};
+int LaunchFinalizeStep(hipStream_t s, const unsigned long long* partials, const CheckList& cl,
+                       Ctl* ctl);
 // Step-start CFL pass: per-direction partial minima, then a finalize kernel
 // writing dt_dir (out, nullable) or, with a control block, the step's dt.
 // In 3D with a control block the pass skips the faces below ctl->cfl_floor.

@@ -969,13 +969,41 @@ __global__ void CheckValueKernel(Ctl* ctl, const double* value) {
   }
 }
 
-__global__ void StepEndKernel(Ctl* ctl) {
-  if (ctl->halt) return;
+__device__ __forceinline__ void EndStep(Ctl* ctl) {
   const int64_t slot = ctl->cycle - ctl->dt_log_base;
   if (ctl->dt_log && slot >= 0 && slot < ctl->dt_log_len) ctl->dt_log[slot] = ctl->dt;
   ctl->t = ctl->t + ctl->dt;
   ctl->cycle += 1;
   ctl->have_failure = 0;
+}
+
+__global__ void StepEndKernel(Ctl* ctl) {
+  if (ctl->halt) return;
+  EndStep(ctl);
+}
+
+// The split sweeps' CFL checks of one level step, reduced after the last
+// sweep in sweep order (the first failing check gives failure_dt, as if the
+// step had stopped there: the sweeps after it only wrote buffers the restart
+// discards), then the step end.  One launch instead of a finalize per sweep
+// and a step-end kernel.
+__global__ __launch_bounds__(kRedBlock) void FinalizeStepKernel(
+    const unsigned long long* partials, CheckList cl, Ctl* ctl) {
+  if (ctl->halt) return;
+  for (int r = 0; r < cl.count; ++r) {
+    const double v = KeyToDt(ReduceKeys(partials + r * cl.stride, cl.n[r]));
+    const bool fail = v < ctl->split_dt;
+    if (threadIdx.x == 0) {
+      ctl->last_check = v;
+      if (fail) {
+        // TimeStepTooLarge{level_dt} (DimensionalSplitLevelIntegrator.hpp:243-245)
+        ctl->halt = 1;
+        ctl->failure_dt = v;
+      }
+    }
+    if (fail) return;
+  }
+  if (threadIdx.x == 0) EndStep(ctl);
 }
 
 __global__ void KeysToDtKernel(unsigned long long* keys, int n, double* out) {
@@ -1207,6 +1235,14 @@ int LaunchPatchCfl(const fvs_method_desc& desc, hipStream_t s, const double* con
 
 int LaunchStepEnd(hipStream_t s, Ctl* ctl) {
   hipLaunchKernelGGL(StepEndKernel, dim3(1), dim3(1), 0, s, ctl);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
+int LaunchFinalizeStep(hipStream_t s, const unsigned long long* partials, const CheckList& cl,
+                       Ctl* ctl) {
+  if (cl.count < 0 || cl.count > kMaxChecks) return FVS_BAD_ARGUMENT;
+  hipLaunchKernelGGL(FinalizeStepKernel, dim3(1), dim3(kRedBlock), 0, s, partials, cl, ctl);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }
