@@ -41,15 +41,19 @@ def log(msg: str) -> None:
 def measured_traffic(precision: str, kernel: str):
     """HBM bytes per launch of the dominant sweep kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter scaled by what it
-    reads for a 1 GiB device copy), or None when the summary is absent."""
+    reads for the streaming probe's known traffic), or None when the summary
+    is absent."""
     path = ROOT / "profiles" / "sweep3d_256_hbm_traffic.txt"
     if not path.exists():
         return None
     names = {"X": "SweepXKernel", "Y": "SweepPencilKernel<3, 1", "Z": "SweepPencilKernel<3, 2"}
-    mode = {"fp64": "double, false", "fp64_rcp": "double, true", "fp32": "float, true"}[precision]
+    section = None
     for line in path.read_text().splitlines():
-        if line.startswith(names[kernel]) and mode in line and "bytes/launch" in line:
-            return int(line.split("bytes/launch")[1].split()[0])
+        if line.startswith("-- "):
+            section = line[3:].strip()
+        elif section == precision and line.startswith(names[kernel]):
+            # kernel, grid, launches, read MiB, write MiB, bytes/launch, vs alg
+            return int(line.split(">")[-1].split()[4])
     return None
 
 
@@ -69,6 +73,31 @@ def measured_copy_bandwidth(torch) -> float:
         e1.synchronize()
         best = max(best, 2 * a.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
     del a, b
+    torch.cuda.empty_cache()
+    return best
+
+
+def measured_stream_bandwidth(torch, fvs, fm, state) -> float:
+    """Achievable HBM bandwidth of a sweep's traffic: the library's streaming
+    probe (fvs_stream_copy) copies the five 256^3 state arrays, every element
+    read and written once (bytes read + written per second, best of five,
+    HIP events on the handle's stream, the current torch stream)."""
+    import ctypes as C
+    from finitevolumesolver_amd import _lib as L
+    out = torch.empty_like(state)
+    nc, n = state.shape[0], state[0].numel()
+    src = L.ptr_array([state[c].data_ptr() for c in range(nc)])
+    dst = L.ptr_array([out[c].data_ptr() for c in range(nc)])
+    best = 0.0
+    for _ in range(5):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        L.check(L.load().fvs_stream_copy(fm.handle, src, dst, nc, C.c_int64(n)))
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * nc * n * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del out
     torch.cuda.empty_cache()
     return best
 
@@ -251,6 +280,10 @@ def kernels_only(args, torch, fvs) -> int:
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
+    del a, b
+    torch.cuda.empty_cache()
+    log("[kernels] stream probe: five state arrays copied (fvs_stream_copy), five times")
+    measured_stream_bandwidth(torch, fvs, fvs.FluxMethod(fvs.PerfectGas(3)), state)
     log("[kernels] done")
     return 0
 
@@ -598,6 +631,7 @@ def main() -> int:
     log("[bench] per-launch ms: " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
     traffic = measured_traffic(args.precision, dominant)
     hbm_copy = measured_copy_bandwidth(torch)
+    hbm_stream = measured_stream_bandwidth(torch, fvs, fm, t_cons)
 
     result = {
         "metric": "Gcell-updates/s per split sweep, % HBM roofline, L1 err vs FP64 CPU oracle",
@@ -628,6 +662,10 @@ def main() -> int:
             "per_launch_ms": {k: round(v, 4) for k, v in per.items()},
             "hbm_copy_GBs": round(hbm_copy, 1),
             "frac_of_copy": round(achieved / hbm_copy, 4),
+            "hbm_stream_GBs": round(hbm_stream, 1),
+            "frac_of_stream": round(achieved / hbm_stream, 4),
+            "stream_probe": "fvs_stream_copy of the five state arrays (the sweep's 80 B/cell "
+                            "without its arithmetic)",
         },
     }
     try:

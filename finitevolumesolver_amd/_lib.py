@@ -125,6 +125,7 @@ SIGNATURES = {
     "fvs_split_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P]),
     "fvs_patch_stable_dt": (_I, [_P, C.POINTER(_P), _PI64, _PD, _P]),
     "fvs_patch_stable_dt_noalloc": (_I, [_P, C.POINTER(_P), _PI64, _PD, _P]),
+    "fvs_stream_copy": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _I, C.c_int64]),
     "fvs_godunov_sweep": (_I, [_P, C.POINTER(_P), C.POINTER(_P), _PI64, _D, _D, _I, _P, _P]),
     "fvs_newton_stats": (_I, [_P, C.POINTER(C.c_uint64), _I]),
     "fvs_exact_middle_state": (_I, [_P, _I, _P, _P, C.c_int64, _P, _P, _P, _P, _P]),

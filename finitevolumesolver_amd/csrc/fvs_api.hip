@@ -746,6 +746,68 @@ int fvs_patch_stable_dt_noalloc(fvs_handle* h, const double* const* cons, const 
   return PatchStableDt(h, cons, n, dx, dt_dir_device, ptr, kNoAllocKeys);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Streams ncomp arrays of n doubles through the chip, 16 bytes per lane and
+// array per access, grid-stride: the traffic of a fused sweep (every
+// component read and written once) without its arithmetic.
+struct StreamArgs {
+  const double* src[5];
+  double* dst[5];
+  int ncomp;
+  int64_t n2;  // double2 elements per array
+};
+
+__global__ __launch_bounds__(256) void StreamCopyKernel(StreamArgs a) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n2;
+       i += stride) {
+    double2 v[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      if (c < a.ncomp) v[c] = reinterpret_cast<const double2*>(a.src[c])[i];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      if (c < a.ncomp) reinterpret_cast<double2*>(a.dst[c])[i] = v[c];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvs_stream_copy(fvs_handle* h, const double* const* src, double* const* dst, int ncomp,
+                    int64_t n) {
+  if (!h || !src || !dst || ncomp < 1 || ncomp > 5 || n < 0 || n % 2 != 0) {
+    SetError("bad argument (1..5 arrays of an even number of doubles)");
+    return FVS_BAD_ARGUMENT;
+  }
+  StreamArgs a{};
+  for (int c = 0; c < ncomp; ++c) {
+    if (!src[c] || !dst[c] || (reinterpret_cast<uintptr_t>(src[c]) | reinterpret_cast<uintptr_t>(dst[c])) % 16) {
+      SetError("array %d is null or not 16-byte aligned", c);
+      return FVS_BAD_ARGUMENT;
+    }
+    a.src[c] = src[c];
+    a.dst[c] = dst[c];
+  }
+  a.ncomp = ncomp;
+  a.n2 = n / 2;
+  if (a.n2 == 0) return FVS_OK;
+  // eight waves per SIMD
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  const int64_t blocks = std::min<int64_t>((a.n2 + 255) / 256, static_cast<int64_t>(cus) * 8);
+  hipLaunchKernelGGL(StreamCopyKernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                     h->stream, a);
+  FVS_LAUNCH_CHECK();
+  return FVS_OK;
+}
+
 int fvs_godunov_sweep(fvs_handle* h, const double* const* cons_in, double* const* cons_out,
                       const int64_t n[3], double dt, double dx, int dir,
                       const double* guess_device, double* stable_dt_device) {

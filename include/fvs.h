@@ -342,6 +342,14 @@ int fvs_patch_stable_dt_noalloc(fvs_handle* h, const double* const* cons,
                                 const int64_t n[3], const double dx[3],
                                 double* dt_dir_device);
 
+/* HBM streaming probe (no counterpart in the reference; a measurement aid
+ * for callers that want the achievable bandwidth of a sweep's traffic):
+ * copies ncomp (1..5) device arrays of n doubles (n even, 16-byte aligned),
+ * src[c] to dst[c], on the handle's stream; every element is read and written
+ * once, 16 bytes per lane and array per access. */
+int fvs_stream_copy(fvs_handle* h, const double* const* src, double* const* dst,
+                    int ncomp, int64_t n);
+
 /* ---- exact Godunov -------------------------------------------------------- */
 
 /* First-order exact-Godunov split sweep (FluxMethod<Godunov<PerfectGas<D>,

@@ -292,3 +292,21 @@ def test_level_graph_replay_matches_eager(cuda, case):
         assert outs[0][1] == outs[1][1], (case, batch)
         assert np.array_equal(outs[0][0], outs[1][0]), (case, batch)
     print(f"{case}: 4 batches of 5 graph-replayed steps equal the eager ones", flush=True)
+
+
+def test_stream_copy_probe(cuda):
+    """fvs_stream_copy (the bench's bandwidth probe) copies every element of
+    each array once; odd lengths and misaligned arrays are rejected."""
+    torch = _torch()
+    import ctypes as C
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=3))
+    src = torch.rand((5, 1 << 20), dtype=torch.float64, device=cuda)
+    dst = torch.zeros_like(src)
+    sp = L.ptr_array([src[c].data_ptr() for c in range(5)])
+    dp = L.ptr_array([dst[c].data_ptr() for c in range(5)])
+    L.check(L.load().fvs_stream_copy(fm.handle, sp, dp, 5, C.c_int64(1 << 20)))
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    assert L.load().fvs_stream_copy(fm.handle, sp, dp, 5, C.c_int64(3)) == L.FVS_BAD_ARGUMENT
+    odd = L.ptr_array([src[c].data_ptr() + 8 for c in range(5)])
+    assert L.load().fvs_stream_copy(fm.handle, odd, dp, 5, C.c_int64(16)) == L.FVS_BAD_ARGUMENT
