@@ -67,21 +67,27 @@ struct MiddleState {
   int converged;
 };
 
-// Toro's pressure function of one side and its derivative.
+// Toro's pressure function of one side and its derivative, both at x (the
+// Newton iteration needs the derivative at the point whose residual it just
+// tested).  Shock branch: the square root sqrt((x + B) / A) is shared, as in
+// the reference's two calls (same arguments, same value).  Rarefaction
+// branch: the reference evaluates (x/p)^((gamma+1)/(-2 gamma)) with a second
+// pow; here it is (x/p)^((gamma-1)/(2 gamma)) / (x/p), the same number to
+// rounding (one pow instead of two per side and iteration; at x = 0 the
+// reference's pow form is kept).
 template <bool FAST = false> struct SideFn {
   double rho, p, a, A, B;
-  __device__ __forceinline__ double f(double x, double gm1, double exponent) const {
+  __device__ __forceinline__ double eval(double x, double gm1, double exponent,
+                                         double d_exponent, double* df) const {
     if (x > p) {
-      return (x - p) * rsqrt_ieee(A / (x + B));
+      const double r = rsqrt_ieee(A / (x + B));
+      *df = r * (1.0 - 0.5 * (x - p) / (B + x));
+      return (x - p) * r;
     } else {
-      return 2.0 * a / gm1 * (PowN<FAST>(x / p, exponent) - 1.0);
-    }
-  }
-  __device__ __forceinline__ double df(double x, double d_exponent) const {
-    if (x > p) {
-      return rsqrt_ieee(A / (B + x)) * (1.0 - 0.5 * (x - p) / (B + x));
-    } else {
-      return PowN<FAST>(x / p, d_exponent) / (rho * a);
+      const double t = x / p;
+      const double s = PowN<FAST>(t, exponent);
+      *df = (t > 0.0 ? s / t : PowN<FAST>(t, d_exponent)) / (rho * a);
+      return 2.0 * a / gm1 * (s - 1.0);
     }
   }
 };
@@ -117,18 +123,19 @@ __device__ __forceinline__ MiddleState ComputeMiddleState(const double* L, const
   const double du = uR - uL;
   // NewtonIteration (NewtonIteration.hpp:28-52).
   double x = x0;
-  double fR = right.f(x, gm1, exponent);
-  double fL = left.f(x, gm1, exponent);
+  double dR, dL;
+  double fR = right.eval(x, gm1, exponent, d_exponent, &dR);
+  double fL = left.eval(x, gm1, exponent, d_exponent, &dL);
   double fx = fR + fL + du;
   int counter = 0;
   double err = rabs(fx);
   while (err > np.tol && counter < np.max_it) {
-    const double dfx = right.df(x, d_exponent) + left.df(x, d_exponent);
+    const double dfx = dR + dL;
     const double offset = fx / dfx;
     const double next = x - offset;
     x = ((next - next) != (next - next) || next < 0.0) ? 0.5 * x : next;
-    fR = right.f(x, gm1, exponent);
-    fL = left.f(x, gm1, exponent);
+    fR = right.eval(x, gm1, exponent, d_exponent, &dR);
+    fL = left.eval(x, gm1, exponent, d_exponent, &dL);
     fx = fR + fL + du;
     err = rabs(fx);
     counter += 1;
