@@ -1,6 +1,6 @@
 // Learned initial guess for the exact Riemann solver's Newton iteration.
 //
-// A 6 -> 64 -> 64 -> 1 tanh MLP maps a state pair to the log of the star
+// A 6 -> 64 -> 64 -> 1 ReLU MLP maps a state pair to the log of the star
 // pressure relative to sqrt(pL pR):
 //   features  x = (ln rhoL s, uL / 5, ln pL s, ln rhoR s, uR / 5, ln pR s),
 //             s = 1 / ln(1000)
@@ -225,14 +225,14 @@ __global__ __launch_bounds__(kBlock) void MlpForwardKernel(const float* __restri
   for (int o = 0; o < kH; ++o) {
     float s = sp[kOffB1 + o];
     for (int k = 0; k < kIn; ++k) s += sp[kOffW1 + o * kInPad + k] * xi[k];
-    a1[o] = tanhf(s);
+    a1[o] = fmaxf(s, 0.0f);
     if (h1) h1[o * n + i] = a1[o];
   }
   float out = sp[kOffB3];
   for (int o = 0; o < kH; ++o) {
     float s = sp[kOffB2 + o];
     for (int k = 0; k < kH; ++k) s += sp[kOffW2 + o * kH + k] * a1[k];
-    const float a2 = tanhf(s);
+    const float a2 = fmaxf(s, 0.0f);
     if (h2) h2[o * n + i] = a2;
     out += sp[kOffW3 + o] * a2;
   }
@@ -241,8 +241,8 @@ __global__ __launch_bounds__(kBlock) void MlpForwardKernel(const float* __restri
 
 // Backward part 1 (per sample): loss e^2 (or the Huber form 2 h |e| - h^2
 // beyond |e| > h when huber h > 0) over valid samples, e = y - t;
-// dy = 2 clamp(e, -h, h) / N (mask 0 otherwise), dh2 = dy W3 (1 - h2^2),
-// dh1 = W2^T dh2 (1 - h1^2).
+// dy = 2 clamp(e, -h, h) / N (mask 0 otherwise), dh2 = dy W3 [h2 > 0],
+// dh1 = W2^T dh2 [h1 > 0] (ReLU derivative).
 // dh2 / dh1 overwrite h2 / h1's storage in separate arrays.
 __global__ __launch_bounds__(kBlock) void MlpBackwardSamples(
     const float* __restrict__ p, const float* __restrict__ y, const float* __restrict__ t,
@@ -268,14 +268,14 @@ __global__ __launch_bounds__(kBlock) void MlpBackwardSamples(
     float g2[kH];
     for (int o = 0; o < kH; ++o) {
       const float a = h2[o * n + i];
-      g2[o] = d * sp[kOffW3 + o] * (1.0f - a * a);
+      g2[o] = a > 0.0f ? d * sp[kOffW3 + o] : 0.0f;
       dh2[o * n + i] = g2[o];
     }
     for (int k = 0; k < kH; ++k) {
       float s = 0.0f;
       for (int o = 0; o < kH; ++o) s += sp[kOffW2 + o * kH + k] * g2[o];
       const float a = h1[k * n + i];
-      dh1[k * n + i] = s * (1.0f - a * a);
+      dh1[k * n + i] = a > 0.0f ? s : 0.0f;
     }
   }
   for (int off = 32; off > 0; off >>= 1) li += __shfl_xor(li, off, 64);
@@ -512,13 +512,14 @@ int fvs_mlp_create(void* hip_stream, uint64_t seed, fvs_mlp** out) {
     delete m;
     return FVS_HIP_ERROR;
   }
-  // Glorot-uniform initialisation from SplitMix64, zero biases and padding.
+  // He-uniform initialisation of the ReLU layers (Glorot for the linear
+  // output) from SplitMix64, zero biases and padding.
   std::vector<float> p(kNParams, 0.0f);
   uint64_t st = seed;
   auto uni = [&](float a) {
     return static_cast<float>((HostSplitMix(st) >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0) * a;
   };
-  const float a1 = std::sqrt(6.0f / (kIn + kH)), a2 = std::sqrt(6.0f / (kH + kH)),
+  const float a1 = std::sqrt(6.0f / kIn), a2 = std::sqrt(6.0f / kH),
               a3 = std::sqrt(6.0f / (kH + 1));
   for (int o = 0; o < kH; ++o)
     for (int k = 0; k < kIn; ++k) p[kOffW1 + o * kInPad + k] = uni(a1);

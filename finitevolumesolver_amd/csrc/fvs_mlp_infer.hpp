@@ -1,10 +1,12 @@
 // Matrix-core inference of the learned Newton guess, shared by the network's
 // own kernels and the exact-Godunov kernels that call it inline.
 //
-// A 6 -> 64 -> 64 -> 1 tanh MLP maps a state pair to the log of the star
+// A 6 -> 64 -> 64 -> 1 ReLU MLP maps a state pair to the log of the star
 // pressure relative to sqrt(pL pR).  BF16 operands, FP32 accumulation,
 // v_mfma_f32_16x16x16_bf16; activations are kept transposed (hidden units x
 // faces) so that a layer's accumulator tile is the next layer's B operand.
+// The biases seed the accumulators, so a hidden activation costs one max
+// and half a packed bf16 conversion per lane.
 #pragma once
 
 #include "fvs_physics.hpp"
@@ -45,14 +47,9 @@ __host__ __device__ __forceinline__ short ToBf16(float f) {
   return static_cast<short>(u >> 16);
 }
 
-// Hidden activation of the inference path: tanh x = 1 - 2 / (exp(2x) + 1)
-// from the hardware exp2 and reciprocal estimates (saturates to +-1; about
-// 1e-7 absolute error, far below the bf16 spacing of the activations it
-// feeds), and the hardware round-to-nearest-even bf16 conversion.
-__device__ __forceinline__ float FastTanh(float x) {
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // exp(2x)
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
-}
+// Hidden activation: max(x, 0), then the hardware round-to-nearest-even
+// bf16 conversion.
+__device__ __forceinline__ float Relu(float x) { return __builtin_fmaxf(x, 0.0f); }
 __device__ __forceinline__ short Bf16Dev(float f) {
   return __builtin_bit_cast(short, static_cast<__bf16>(f));
 }
@@ -112,17 +109,19 @@ __device__ __forceinline__ float MlpWave(const float* xf, const short* __restric
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const bf16x4_t aw = *reinterpret_cast<const bf16x4_t*>(tiles + (m * 64 + lane) * 4);
-      floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+      const float* b1 = bias + m * 16 + kq * 4;
+      floatx4_t acc = {b1[0], b1[1], b1[2], b1[3]};
       acc = Mfma16(aw, bx, acc);
       bf16x4_t hb;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) hb[e] = Bf16Dev(FastTanh(acc[e] + bias[m * 16 + kq * 4 + e]));
+      for (int e = 0; e < 4; ++e) hb[e] = Bf16Dev(Relu(acc[e]));
       h1[m] = hb;
     }
     bf16x4_t h2[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+      const float* b2 = bias + kH + m * 16 + kq * 4;
+      floatx4_t acc = {b2[0], b2[1], b2[2], b2[3]};
 #pragma unroll
       for (int kb = 0; kb < 4; ++kb) {
         const bf16x4_t aw =
@@ -131,7 +130,7 @@ __device__ __forceinline__ float MlpWave(const float* xf, const short* __restric
       }
       bf16x4_t hb;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) hb[e] = Bf16Dev(FastTanh(acc[e] + bias[kH + m * 16 + kq * 4 + e]));
+      for (int e = 0; e < 4; ++e) hb[e] = Bf16Dev(Relu(acc[e]));
       h2[m] = hb;
     }
     floatx4_t acc = {0.f, 0.f, 0.f, 0.f};

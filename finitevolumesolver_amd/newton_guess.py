@@ -4,7 +4,7 @@ The reference seeds Newton with the mean pressure 0.5 (pL + pR)
 (include/fub/equations/perfect_gas/ExactRiemannSolver.hpp:91,
 src/equations/perfect_gas/ExactRiemannSolver.cpp); the iteration itself
 (include/fub/NewtonIteration.hpp:28-52) is unchanged here.  A 6 -> 64 -> 64 -> 1
-tanh network predicts r = ln p* - (ln pL + ln pR) / 2 from
+ReLU network predicts r = ln p* - (ln pL + ln pR) / 2 from
 (ln rhoL / ln 1000, uL / 5, ln pL / ln 1000, ln rhoR / ln 1000, uR / 5,
 ln pR / ln 1000); the guess is sqrt(pL pR) exp(r).  Data generation (labels
 from the FP64 Newton solve), FP32 training (forward, backward, Adam) and BF16

@@ -41,8 +41,8 @@ def forward(p, x):
     """x: (6, n) -> (y (n), h1 (64, n), h2 (64, n)) in FP64."""
     W1, b1, W2, b2, W3, b3 = unpack(p)
     x = np.asarray(x, dtype=np.float64)
-    h1 = np.tanh(W1 @ x + b1[:, None])
-    h2 = np.tanh(W2 @ h1 + b2[:, None])
+    h1 = np.maximum(W1 @ x + b1[:, None], 0.0)
+    h2 = np.maximum(W2 @ h1 + b2[:, None], 0.0)
     y = W3 @ h2 + b3
     return y, h1, h2
 
@@ -77,8 +77,8 @@ def loss_and_grad(p, x, t, flag, huber=0.0):
     e = (y - np.asarray(t, dtype=np.float64)) * ok
     ec = np.clip(e, -huber, huber) if huber > 0 else e
     dy = 2.0 * ec / cnt
-    dz2 = (W3[:, None] * dy[None, :]) * (1.0 - h2 * h2)
-    dz1 = (W2.T @ dz2) * (1.0 - h1 * h1)
+    dz2 = (W3[:, None] * dy[None, :]) * (h2 > 0.0)
+    dz1 = (W2.T @ dz2) * (h1 > 0.0)
     g = np.zeros(NUM_PARAMS)
     gW1 = np.zeros((N_HIDDEN, 8))
     gW1[:, :N_IN] = dz1 @ x.T

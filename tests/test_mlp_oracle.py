@@ -47,7 +47,8 @@ def test_huber_gradient_matches_central_differences():
     rng = np.random.default_rng(7)
     p = _params(seed=8)
     x = rng.normal(0.0, 1.0, (6, 20))
-    t = rng.normal(0.0, 2.0, 20)  # errors on both sides of the threshold
+    # targets around the outputs: errors on both sides of the threshold
+    t = M.forward(p, x)[0] + rng.normal(0.0, 0.4, 20)
     flag = np.zeros(20, dtype=np.int32)
     h = 0.25
     l0, g = M.loss_and_grad(p, x, t, flag, huber=h)
@@ -77,7 +78,8 @@ def test_rejected_samples_do_not_change_loss():
 
 
 def test_forward_one_unit_network():
-    """With a single active path the network is d tanh(c tanh(a x0 + b)) + e."""
+    """With a single active path the network is d relu(c relu(a x0 + b)) + e
+    (one sample on each side of the first unit's kink)."""
     p = np.zeros(M.NUM_PARAMS)
     a, b, c, d, e = 0.7, -0.2, 1.3, -2.0, 0.25
     p[M.OFF_W1 + 0 * 8 + 0] = a
@@ -87,7 +89,9 @@ def test_forward_one_unit_network():
     p[M.OFF_B3] = e
     x = np.array([[0.5, -1.0], [9, 9], [9, 9], [9, 9], [9, 9], [9, 9]], dtype=np.float64)
     y, h1, h2 = M.forward(p, x)
-    np.testing.assert_allclose(y, d * np.tanh(c * np.tanh(a * x[0] + b)) + e, rtol=1e-15)
+    relu = lambda v: np.maximum(v, 0.0)
+    np.testing.assert_allclose(y, d * relu(c * relu(a * x[0] + b)) + e, rtol=1e-15)
+    assert y[1] == e and y[0] != e
     assert np.all(h2[:5] == 0.0) and np.all(h2[6:] == 0.0)
 
 
