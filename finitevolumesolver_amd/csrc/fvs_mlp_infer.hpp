@@ -162,9 +162,16 @@ __device__ __forceinline__ bool WeakPair(double rhoL, double uL, double pL, doub
 }
 
 
+// Waves run the network only when at least this many of their 64 faces are
+// non-weak.  A wave's Newton loop runs until its slowest face converges, so
+// a few strong faces among weak ones gain little from the guess while the
+// whole wave pays for the inference.
+constexpr int kWaveStrongLanes = 48;
+
 // The learned initial guess of one face per lane; every lane of the wave
 // must call it (the matrix-core step is wave-wide).  Weak pairs, and every
-// lane of a wave whose faces are all weak, keep the mean pressure.
+// lane of a wave with fewer than kWaveStrongLanes non-weak faces, keep the
+// mean pressure.
 template <int D, int DIR>
 __device__ __forceinline__ double LearnedGuess(const double* L, const double* R,
                                                const short* __restrict__ tiles,
@@ -174,7 +181,7 @@ __device__ __forceinline__ double LearnedGuess(const double* L, const double* R,
   const bool weak = WeakPair(rhoL, L[1 + DIR] / rhoL, pL, L[D + 3], rhoR, R[1 + DIR] / rhoR, pR,
                              R[D + 3]);
   double p0 = 0.5 * (pL + pR);
-  if (__ballot(!weak) != 0) {
+  if (__builtin_popcountll(__ballot(!weak)) >= kWaveStrongLanes) {
     float x[kIn];
     Features<D>(L, R, DIR, x);
     // Opaque copies of the operand pointers: the compiler cannot hoist the

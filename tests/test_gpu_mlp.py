@@ -280,8 +280,9 @@ def test_level_with_learned_guess_smooth(trained, cuda):
 
 def test_level_with_learned_guess_strong_waves(trained, cuda):
     """3D first-order exact Godunov on blocky data with strong jumps, 24^3,
-    4 steps: the learned guess saves Newton iterations; L1 density error
-    <= 1e-6 against the FP64 oracle."""
+    4 steps: the learned guess saves Newton iterations (only waves with at
+    least 48 non-weak faces run the network, so about a tenth here); L1
+    density error <= 1e-6 against the FP64 oracle."""
     n = 24
     cons = _blocky_field(n, seed=5)
     dx = (1.0 / n, 1.0 / n, 1.0 / n)
@@ -294,7 +295,7 @@ def test_level_with_learned_guess_strong_waves(trained, cuda):
           f"{st1['faces']} faces, fallbacks {st1['fallbacks']}")
     assert l1 <= 1e-6
     assert st1["not_converged"] == 0
-    assert st1["iterations"] < 0.85 * st0["iterations"]
+    assert st1["iterations"] < 0.93 * st0["iterations"]
 
 
 @pytest.mark.parametrize("field", ["smooth", "strong"])
