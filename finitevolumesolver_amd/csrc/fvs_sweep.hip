@@ -842,6 +842,41 @@ __device__ unsigned long long ReduceKeys(const unsigned long long* p, int64_t n)
   return r;
 }
 
+// Minima of up to N key arrays (array r: p[r * stride .. + n[r]]) in one
+// pass, every thread's loads of all arrays in flight together; out[r] for
+// r < count.  Every thread of the block must call it.
+template <int N>
+__device__ void ReduceKeysMulti(const unsigned long long* p, int64_t stride, const int64_t* n,
+                                int count, unsigned long long* out) {
+  __shared__ unsigned long long s[N][kRedBlock / 64];
+  unsigned long long k[N];
+  int64_t nmax = 0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    k[r] = kInfKey;
+    if (r < count && n[r] > nmax) nmax = n[r];
+  }
+#pragma unroll 4
+  for (int64_t i = threadIdx.x; i < nmax; i += kRedBlock) {
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+      if (r < count && i < n[r]) k[r] = KMin(k[r], p[r * stride + i]);
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    const unsigned long long w = WaveMinKey(k[r]);
+    if ((threadIdx.x & 63) == 0) s[r][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    unsigned long long m = s[r][0];
+    for (int w = 1; w < kRedBlock / 64; ++w) m = KMin(m, s[r][w]);
+    out[r] = m;
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kRedBlock) void FinalizeCheckKernel(
     const unsigned long long* partials, int64_t n, double* out, Ctl* ctl) {
   if (ctl && ctl->halt) return;
@@ -867,7 +902,12 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
     int first_dir, int strang, double fdx0, double fdx1, double fdx2, int use_floor) {
   if (ctl && ctl->halt) return;
   double v[3];
-  for (int d = 0; d < dim; ++d) v[d] = KeyToDt(ReduceKeys(partials + d * n, n));
+  {
+    const int64_t ns[3] = {n, n, n};
+    unsigned long long keys[3];
+    ReduceKeysMulti<3>(partials, n, ns, dim, keys);
+    for (int d = 0; d < dim; ++d) v[d] = KeyToDt(keys[d]);
+  }
   if (threadIdx.x != 0) return;
   if (out) {
     for (int d = 0; d < dim; ++d) out[d] = v[d];
@@ -990,8 +1030,10 @@ __global__ void StepEndKernel(Ctl* ctl) {
 __global__ __launch_bounds__(kRedBlock) void FinalizeStepKernel(
     const unsigned long long* partials, CheckList cl, Ctl* ctl) {
   if (ctl->halt) return;
+  unsigned long long keys[kMaxChecks];
+  ReduceKeysMulti<kMaxChecks>(partials, cl.stride, cl.n, cl.count, keys);
   for (int r = 0; r < cl.count; ++r) {
-    const double v = KeyToDt(ReduceKeys(partials + r * cl.stride, cl.n[r]));
+    const double v = KeyToDt(keys[r]);
     const bool fail = v < ctl->split_dt;
     if (threadIdx.x == 0) {
       ctl->last_check = v;
