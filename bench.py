@@ -118,8 +118,9 @@ def bench_c1(torch, fvs) -> dict:
     for graphs in (False, True):
         lvl = fvs.SinglePatchLevel(fm, (256, 256), dx, splitting="Strang", cfl=0.8)
         lvl.set_graphs(graphs)
-        # untimed run: captures the six distinct step graphs (start buffer x
-        # cycle parity); set_state then restarts the run at t = 0
+        # untimed run: the first advance captures the level's step graphs
+        # (single steps and 8-step chains per start buffer and cycle parity);
+        # set_state then restarts the run at t = 0
         lvl.set_state(torch.from_numpy(cons).cuda())
         lvl.advance(6, batch=6)
         lvl.set_state(torch.from_numpy(cons).cuda())

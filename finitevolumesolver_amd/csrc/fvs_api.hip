@@ -917,13 +917,21 @@ struct fvs_level {
   // Captured steps (HIP graphs), one per (start buffer, cycle parity, network).
   int use_graphs = 1;
   hipStream_t cap = nullptr;  // capture stream (default priority)
+  // A graph holds nsteps consecutive steps (1, or kChainSteps for runs of
+  // steps: one graph launch instead of one per step); curs[i] is the buffer
+  // step i starts from, curs[nsteps] the buffer holding the result.
   struct StepGraph {
-    int cur, parity, next;
+    int cur, parity, nsteps;
+    std::vector<int> curs;
     const fvs_mlp* net;
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs;
 };
+
+// Steps per chained graph (the step sequence depends only on the start
+// buffer and the cycle parity, so a chain is captured once per pair).
+constexpr int kChainSteps = 8;
 
 namespace {
 
@@ -1305,13 +1313,13 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
 // The step as a replayed HIP graph, captured on first use per key on the
 // level's own capture stream (the handle's stream may be the null stream,
 // which cannot capture) and launched on the handle's stream.
-static int LaunchLevelStepGraph(fvs_level* L, int cur, int64_t cyc, const fvs_mlp* net, int* next) {
-  hipStream_t s = L->h->stream;
-  const int parity = static_cast<int>(cyc % 2);
+// Captures (without running) the graph of nsteps steps from buffer cur at a
+// cycle of the given parity, unless it exists; returns its entry.
+static int CaptureLevelSteps(fvs_level* L, int cur, int parity, const fvs_mlp* net, int nsteps,
+                             const fvs_level::StepGraph** out) {
   for (const auto& g : L->graphs) {
-    if (g.cur == cur && g.parity == parity && g.net == net) {
-      *next = g.next;
-      FVS_HIP_CHECK(hipGraphLaunch(g.exec, s));
+    if (g.cur == cur && g.parity == parity && g.net == net && g.nsteps == nsteps) {
+      *out = &g;
       return FVS_OK;
     }
   }
@@ -1319,9 +1327,16 @@ static int LaunchLevelStepGraph(fvs_level* L, int cur, int64_t cyc, const fvs_ml
   g.cur = cur;
   g.parity = parity;
   g.net = net;
+  g.nsteps = nsteps;
+  g.curs.assign(1, cur);
   if (!L->cap) FVS_HIP_CHECK(hipStreamCreateWithFlags(&L->cap, hipStreamNonBlocking));
   FVS_HIP_CHECK(hipStreamBeginCapture(L->cap, hipStreamCaptureModeThreadLocal));
-  const int rc = LaunchLevelStep(L, L->cap, cur, cyc, net, &g.next);
+  int rc = FVS_OK;
+  for (int i = 0; i < nsteps && rc == FVS_OK; ++i) {
+    int next = g.curs.back();
+    rc = LaunchLevelStep(L, L->cap, g.curs.back(), parity + i, net, &next);
+    g.curs.push_back(next);
+  }
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(L->cap, &graph);
   if (rc != FVS_OK || ec != hipSuccess) {
@@ -1336,9 +1351,31 @@ static int LaunchLevelStepGraph(fvs_level* L, int cur, int64_t cyc, const fvs_ml
     SetError("step graph instantiation failed: %s", hipGetErrorString(ei));
     return FVS_HIP_ERROR;
   }
-  L->graphs.push_back(g);
-  *next = g.next;
-  FVS_HIP_CHECK(hipGraphLaunch(g.exec, s));
+  L->graphs.push_back(std::move(g));
+  *out = &L->graphs.back();
+  return FVS_OK;
+}
+
+// Every single-step and chained graph of the level for this network (six
+// start states: three buffers, two cycle parities), captured before the
+// first replay so that no capture falls inside a run of steps.
+static int EnsureLevelGraphs(fvs_level* L, const fvs_mlp* net) {
+  const fvs_level::StepGraph* g = nullptr;
+  for (int cur = 0; cur < 3; ++cur)
+    for (int parity = 0; parity < 2; ++parity)
+      for (int nsteps : {1, kChainSteps})
+        if (int rc = CaptureLevelSteps(L, cur, parity, net, nsteps, &g)) return rc;
+  return FVS_OK;
+}
+
+// Launches nsteps (1 or kChainSteps) steps from buffer cur as one replayed
+// graph; curs receives the start buffer of each step and, last, the result's.
+static int LaunchLevelStepGraph(fvs_level* L, int cur, int64_t cyc, const fvs_mlp* net,
+                                int nsteps, std::vector<int>* curs) {
+  const fvs_level::StepGraph* g = nullptr;
+  if (int rc = CaptureLevelSteps(L, cur, static_cast<int>(cyc % 2), net, nsteps, &g)) return rc;
+  *curs = g->curs;
+  FVS_HIP_CHECK(hipGraphLaunch(g->exec, L->h->stream));
   return FVS_OK;
 }
 
@@ -1382,18 +1419,34 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
                                  hipMemcpyDeviceToHost, s));
     FVS_HIP_CHECK(hipStreamSynchronize(s));
   }
+  if (!L->timing && L->use_graphs && max_steps > 0) {
+    if (int rc2 = EnsureLevelGraphs(L, net)) {
+      if (dlog) (void)hipFree(dlog);
+      return rc2;
+    }
+  }
   while (cycle - cycle0 < max_steps) {
     const int k = static_cast<int>(std::min<int64_t>(batch, max_steps - (cycle - cycle0)));
     start_buf.assign(k + 1, 0);
     int cur = L->cur;
-    for (int sidx = 0; sidx < k && rc == FVS_OK; ++sidx) {
-      start_buf[sidx] = cur;
+    std::vector<int> curs;
+    for (int sidx = 0; sidx < k && rc == FVS_OK;) {
       const int64_t cyc = cycle + sidx;
-      int next = cur;
-      rc = (L->timing || !L->use_graphs) ? LaunchLevelStep(L, s, cur, cyc, net, &next)
-                                         : LaunchLevelStepGraph(L, cur, cyc, net, &next);
+      if (L->timing || !L->use_graphs) {
+        start_buf[sidx] = cur;
+        int next = cur;
+        rc = LaunchLevelStep(L, s, cur, cyc, net, &next);
+        if (rc) break;
+        cur = next;
+        ++sidx;
+        continue;
+      }
+      const int nsteps = (k - sidx >= kChainSteps) ? kChainSteps : 1;
+      rc = LaunchLevelStepGraph(L, cur, cyc, net, nsteps, &curs);
       if (rc) break;
-      cur = next;
+      for (int i = 0; i < nsteps; ++i) start_buf[sidx + i] = curs[i];
+      cur = curs[nsteps];
+      sidx += nsteps;
     }
     if (rc) break;
     FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, s));

@@ -118,7 +118,7 @@ struct Ctl {
 int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg);
 int64_t CflBlocks(int dim, const int64_t n[3]);
 // Cells per thread of a Y/Z pencil sweep: 32, fewer when the patch is too
-// small to give the device about 1,024 workgroups (down to 4).
+// small to give the device about 1,024 workgroups (down to 1).
 int PencilSeg(int dim, int dir, const int64_t n[3]);
 int LaunchSweep(const fvs_method_desc& desc, hipStream_t stream, int dir,
                 const SweepArgs& a);

@@ -1192,11 +1192,11 @@ int64_t SweepBlocks(int dim, int dir, const int64_t n[3], int seg) {
 }
 
 // Row segments of the 2D CFL pass: 32 rows, fewer when the patch is too
-// small to give the device about 1,024 workgroups (down to 4 rows).
+// small to give the device about 1,024 workgroups (down to 1 row).
 int64_t Cfl2Segments(int64_t nx, int64_t ny) {
   const int64_t tiles = (nx + kTX - 1) / kTX;
   int64_t seg = (ny * tiles + 1023) / 1024;
-  seg = seg < 4 ? 4 : (seg > kCflSeg ? kCflSeg : seg);
+  seg = seg < 1 ? 1 : (seg > kCflSeg ? kCflSeg : seg);
   return (ny + seg - 1) / seg;
 }
 
@@ -1206,7 +1206,7 @@ int PencilSeg(int dim, int dir, const int64_t n[3]) {
   const int64_t bx = (npencil + kBlock - 1) / kBlock;
   const int64_t nseg = (1024 + bx - 1) / bx;
   int64_t seg = (n[dir] + nseg - 1) / nseg;
-  seg = seg < 4 ? 4 : (seg > 32 ? 32 : seg);
+  seg = seg < 1 ? 1 : (seg > 32 ? 32 : seg);
   return static_cast<int>(seg);
 }
 

@@ -307,9 +307,11 @@ enum fvs_level_timer {
 int fvs_level_set_timing(fvs_level* level, int on);
 int fvs_level_kernel_times(fvs_level* level, double* ms, int64_t* count);
 
-/* Capture each distinct time step of the level (per start buffer, cycle
- * parity and attached network) once into a HIP graph and replay it (on by
- * default; the timing mode launches eagerly).  No counterpart in the
+/* Replay the level's time steps from HIP graphs (on by default; the timing
+ * mode launches eagerly): on the first advance with graphs on, every
+ * distinct step (per start buffer, cycle parity and attached network) is
+ * captured once as a single-step graph and as an 8-step chain, and runs of
+ * eight steps are replayed as one graph launch.  No counterpart in the
  * reference, whose steps are host loops; results are identical either way. */
 int fvs_level_set_graphs(fvs_level* level, int on);
 
