@@ -1196,8 +1196,8 @@ int fvs_level_hint_max_speed(fvs_level* L, const double speed[3]) {
       return FVS_BAD_ARGUMENT;
     }
   }
-  // only the 3D HLLEM pass has a floor
-  if (L->h->desc.dim != 3 || L->h->desc.base == FVS_BASE_GODUNOV_EXACT) return FVS_OK;
+  // the exact-Godunov passes and the 3D HLLEM pass have a floor
+  if (L->h->desc.dim != 3 && L->h->desc.base != FVS_BASE_GODUNOV_EXACT) return FVS_OK;
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, L->h->stream));
   FVS_HIP_CHECK(hipStreamSynchronize(L->h->stream));
@@ -1243,13 +1243,14 @@ static int LaunchLevelStep(fvs_level* L, hipStream_t s, int cur, int64_t cyc,
     const int64_t nb = GodunovCflBlocks(D, L->n);
     rc = Timed(L, s, FVS_TIMER_CFL, [&] {
       return LaunchGodunovCfl(h->desc, s, cin, L->n, L->patch.dx, nullptr, L->partials,
-                              &L->ctl->halt, net, L->pm, seq[0]);
+                              &L->ctl->halt, net, L->pm, seq[0], L->ctl);
     });
     for (int d = 0; d < D && rc == FVS_OK; ++d) {
       rc = LaunchGodunovFinalize(s, L->partials + 5 * d * nb, nb, &L->ctl->dt_dir[d],
                                  h->newton_stats, L->ctl);
     }
-    if (rc == FVS_OK) rc = LaunchStepDt(s, L->ctl, D, seq[0], strang);
+    const int use_floor = h->desc.max_newton_iterations >= kGodunovFloorCap;
+    if (rc == FVS_OK) rc = LaunchStepDt(s, L->ctl, D, seq[0], strang, L->patch.dx, use_floor);
   } else {
     rc = Timed(L, s, FVS_TIMER_CFL, [&] {
       return LaunchPatchCfl(h->desc, s, cin, L->n, L->patch.dx, L->partials, nullptr, L->ctl,

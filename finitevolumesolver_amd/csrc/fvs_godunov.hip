@@ -533,14 +533,74 @@ struct GodunovCflArgs {
   int* halt;
   double* pm_out;  // star pressures of direction pm_dir's faces (nullable;
   int pm_dir;      // NaN where Newton did not converge), for the first sweep
+  // Speed floor (nullable: none; the level driver's ctl->cfl_floor): faces of
+  // directions other than pm_dir whose certified speed bound stays below it
+  // skip the Newton solve; blocks whose evaluated faces reach it set hit[d].
+  const double* floor;
+  int* hit;
 };
+
+// kGodunovFloorCap: Newton iterations within which Newton from the mean
+// pressure converges on every face that GodunovSpeedBound certifies (at most
+// 6 were observed over millions of sampled pairs of that class); with a
+// smaller iteration cap the pass solves every face, so the cap is reported
+// where the reference's would be.
+
+// Certified upper bound of the exact solver's signal speeds max(|s0|, |s1|)
+// (GodunovDtFromMiddle) of a face, without Newton, for the faces of the
+// class on which Newton from the mean pressure converges quickly: positive
+// densities and pressures, pressure ratio at most 4, sound speeds at most
+// 1e6 and |uR - uL| at most the smaller sound speed (no vacuum).  Toro's f(p)
+// = fL(p) + fR(p) + (uR - uL) increases, so any P with f(P) >= 0 bounds the
+// star pressure p*: P = max(pL, pR) when uR >= uL (fK(max p) >= 0 on both
+// sides), else P = max(pL, pR) + 2 (uL - uR) / S with S the two shock
+// slopes sqrt(AK / (max p + BK)), checked on the shock branches.  Newton
+// stops at |f(pM)| <= tol, so pM <= P + tol / Dlo with Dlo the smaller of the
+// two branches' slopes on [0, 2P] summed over the sides (valid when that stays
+// below 2P).  The signal speeds grow with the star pressure, so |uK| + aK
+// w(P_eff / pK), w(r) = sqrt(1 + (g + 1) / (2 g) (r - 1)) for r > 1 and 1
+// otherwise, bounds them.  Returns false (no bound) outside the class.
+template <int D, int DIR>
+__device__ __forceinline__ bool GodunovSpeedBound(const double* L, const double* R,
+                                                  const Gas<double>& eq, double tol,
+                                                  double* bound) {
+  using I = Idx<D>;
+  const double rhoL = L[I::kRho], rhoR = R[I::kRho];
+  const double pL = L[I::kP], pR = R[I::kP], aL = L[I::kA], aR = R[I::kA];
+  const double uL = L[I::kMom + DIR] / rhoL, uR = R[I::kMom + DIR] / rhoR;
+  const double pmin = fmin(pL, pR), pmax = fmax(pL, pR);
+  const double amin = fmin(aL, aR), amax = fmax(aL, aR);
+  const double du = uR - uL;
+  if (!(rhoL > 0.0 && rhoR > 0.0 && pmin > 0.0 && pmax <= 4.0 * pmin && amax <= 1e6 &&
+        fabs(du) <= amin))
+    return false;  // NaN fails too
+  const double g = eq.gamma, gp1 = g + 1.0, gm1 = g - 1.0;
+  const double AL = 2.0 / (gp1 * rhoL), AR = 2.0 / (gp1 * rhoR);
+  const double BL = gm1 / gp1 * pL, BR = gm1 / gp1 * pR;
+  double P = pmax;
+  if (du < 0.0) {
+    const double S = sqrt(AL / (pmax + BL)) + sqrt(AR / (pmax + BR));
+    P = pmax + 2.0 * (-du) / S;
+    const double fP = (P - pL) * sqrt(AL / (P + BL)) + (P - pR) * sqrt(AR / (P + BR));
+    if (!(fP >= -du * (1.0 + 0x1p-30))) return false;
+  }
+  const double Dlo = fmin(1.0 / (rhoL * aL), 0.5 * sqrt(AL / (2.0 * P + BL))) +
+                     fmin(1.0 / (rhoR * aR), 0.5 * sqrt(AR / (2.0 * P + BR)));
+  const double Peff = P + tol / Dlo;
+  if (!(Peff <= 2.0 * P)) return false;
+  const double c = gp1 / (2.0 * g);
+  const double wL = Peff > pL ? sqrt(1.0 + c * (Peff / pL - 1.0)) : 1.0;
+  const double wR = Peff > pR ? sqrt(1.0 + c * (Peff / pR - 1.0)) : 1.0;
+  *bound = fmax(fabs(uL) + aL * wL, fabs(uR) + aR * wR);
+  return true;
+}
 
 // Newton seed of a face: the inline learned guess (lg, computed by the whole
 // wave), else the face's guess array entry, else the mean pressure.
 template <int D, int DIR, bool MIXED, bool LEARNED>
-__device__ __forceinline__ void CflFace(const double* L, const double* R, const GodunovCflArgs& a,
-                                        const Gas<double>& g, int64_t fidx, double lg,
-                                        unsigned long long* acc) {
+__device__ __forceinline__ double CflFace(const double* L, const double* R,
+                                          const GodunovCflArgs& a, const Gas<double>& g,
+                                          int64_t fidx, double lg, unsigned long long* acc) {
   NewtonParams np;
   np.tol = a.tol;
   np.max_it = a.max_it;
@@ -559,6 +619,7 @@ __device__ __forceinline__ void CflFace(const double* L, const double* R, const 
   acc[2] = it > acc[2] ? it : acc[2];
   acc[3] += ms.converged ? 0 : 1;
   acc[4] += 1 + (fb << 32);
+  return dt;
 }
 
 template <int D, bool MIXED, bool LEARNED>
@@ -583,6 +644,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const int64_t m0 = static_cast<int64_t>(blockIdx.y) * kCflSeg;
   const int64_t m1 = (m0 + kCflSeg < nm) ? m0 + kCflSeg : nm;
   const int64_t rowoff = (D == 3) ? y * nx : 0;
+  // Speed floor of the directions other than pm_dir (0: every face solved).
+  double F[3] = {0.0, 0.0, 0.0}, dtF[3] = {0.0, 0.0, 0.0};
+  if (a.floor) {
+    for (int d = 0; d < D; ++d) {
+      if (d != a.pm_dir && a.floor[d] > 0.0) {
+        F[d] = a.floor[d];
+        dtF[d] = a.dx[d] / F[d];
+      }
+    }
+  }
+  bool hit[3] = {false, false, false};
+  // A face skips its Newton solve when its certified speed bound stays below
+  // the floor by a relative 2^-30 (far above the rounding of either side).
+  auto skipped = [&](auto dir_tag, const double* l, const double* r) -> bool {
+    constexpr int DD = decltype(dir_tag)::value;
+    if (!(F[DD] > 0.0)) return false;
+    double sb;
+    return GodunovSpeedBound<D, DD>(l, r, g, a.tol, &sb) && sb * (1.0 + 0x1p-30) < F[DD];
+  };
   __shared__ double s_q[NQ][kTX];
   unsigned long long acc[3][5];
   for (int d = 0; d < 3; ++d) {
@@ -621,14 +701,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     // March direction.
     {
       const double* l = (m == 0) ? qc : qprev;
-      const double lg = learned(XM{}, l, qc);
-      if (active) {
-        CflFace<D, M, MIXED, LEARNED>(l, qc, a, g, (M == 1) ? fidx(1, xs, m, 0) : fidx(2, xs, cy, m), lg,
-                             acc[M]);
-        if (m == nm - 1)
-          CflFace<D, M, MIXED, LEARNED>(qc, qc, a, g,
-                               (M == 1) ? fidx(1, xs, m + 1, 0) : fidx(2, xs, cy, m + 1),
-                               0.5 * (qc[D + 2] + qc[D + 2]), acc[M]);
+      const bool skip = active && skipped(XM{}, l, qc);
+      const double lg = __all(skip || !active) ? 0.0 : learned(XM{}, l, qc);
+      if (active && !skip) {
+        const double dt = CflFace<D, M, MIXED, LEARNED>(
+            l, qc, a, g, (M == 1) ? fidx(1, xs, m, 0) : fidx(2, xs, cy, m), lg, acc[M]);
+        hit[M] = hit[M] || dt <= dtF[M];
+      }
+      if (active && m == nm - 1 && !skipped(XM{}, qc, qc)) {
+        const double dt = CflFace<D, M, MIXED, LEARNED>(
+            qc, qc, a, g, (M == 1) ? fidx(1, xs, m + 1, 0) : fidx(2, xs, cy, m + 1),
+            0.5 * (qc[D + 2] + qc[D + 2]), acc[M]);
+        hit[M] = hit[M] || dt <= dtF[M];
       }
     }
     if constexpr (D == 3) {
@@ -638,12 +722,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       } else {
         load(off - nx, qy);
       }
-      const double lg = learned(X1{}, qy, qc);
-      if (active) {
-        CflFace<D, 1, MIXED, LEARNED>(qy, qc, a, g, fidx(1, xs, y, cz), lg, acc[1]);
-        if (y == ny - 1)
-          CflFace<D, 1, MIXED, LEARNED>(qc, qc, a, g, fidx(1, xs, y + 1, cz), 0.5 * (qc[D + 2] + qc[D + 2]),
-                               acc[1]);
+      const bool skip = active && skipped(X1{}, qy, qc);
+      const double lg = __all(skip || !active) ? 0.0 : learned(X1{}, qy, qc);
+      if (active && !skip) {
+        const double dt = CflFace<D, 1, MIXED, LEARNED>(qy, qc, a, g, fidx(1, xs, y, cz), lg, acc[1]);
+        hit[1] = hit[1] || dt <= dtF[1];
+      }
+      if (active && y == ny - 1 && !skipped(X1{}, qc, qc)) {
+        const double dt = CflFace<D, 1, MIXED, LEARNED>(qc, qc, a, g, fidx(1, xs, y + 1, cz),
+                                                        0.5 * (qc[D + 2] + qc[D + 2]), acc[1]);
+        hit[1] = hit[1] || dt <= dtF[1];
       }
     }
 #pragma unroll
@@ -658,12 +746,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       } else {
         load(off - 1, ql);
       }
-      const double lg = learned(X0{}, ql, qc);
-      if (active) {
-        CflFace<D, 0, MIXED, LEARNED>(ql, qc, a, g, fidx(0, xs, cy, cz), lg, acc[0]);
-        if (x == nx - 1)
-          CflFace<D, 0, MIXED, LEARNED>(qc, qc, a, g, fidx(0, xs + 1, cy, cz),
-                               0.5 * (qc[D + 2] + qc[D + 2]), acc[0]);
+      const bool skip = active && skipped(X0{}, ql, qc);
+      const double lg = __all(skip || !active) ? 0.0 : learned(X0{}, ql, qc);
+      if (active && !skip) {
+        const double dt = CflFace<D, 0, MIXED, LEARNED>(ql, qc, a, g, fidx(0, xs, cy, cz), lg, acc[0]);
+        hit[0] = hit[0] || dt <= dtF[0];
+      }
+      if (active && x == nx - 1 && !skipped(X0{}, qc, qc)) {
+        const double dt = CflFace<D, 0, MIXED, LEARNED>(qc, qc, a, g, fidx(0, xs + 1, cy, cz),
+                                                        0.5 * (qc[D + 2] + qc[D + 2]), acc[0]);
+        hit[0] = hit[0] || dt <= dtF[0];
       }
     }
     __syncthreads();
@@ -675,6 +767,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   for (int d = 0; d < D; ++d)
     BlockReduceGodunov(acc[d][0], acc[d][1], acc[d][2], acc[d][3], acc[d][4],
                        a.partials + 5 * (d * nblocks + bid));
+  for (int d = 0; d < D; ++d) {
+    if (__syncthreads_or(hit[d] ? 1 : 0) && threadIdx.x == 0 && a.hit) a.hit[d] = 1;
+  }
 }
 
 // ComputeMiddleState on arrays of complete state pairs (SoA (D+4) x n) with
@@ -820,7 +915,7 @@ int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int
 int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                      const int64_t n[3], const double dx[3], const double* const* guess,
                      unsigned long long* partials, int* halt, const fvs_mlp* net,
-                     double* pm_out, int pm_dir) {
+                     double* pm_out, int pm_dir, Ctl* floor_ctl) {
   GodunovCflArgs a;
   std::memset(&a, 0, sizeof a);
   const int D = desc.dim;
@@ -839,6 +934,10 @@ int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* c
   a.halt = halt;
   a.pm_out = pm_out;
   a.pm_dir = pm_dir;
+  if (floor_ctl && desc.max_newton_iterations >= kGodunovFloorCap) {
+    a.floor = floor_ctl->cfl_floor;
+    a.hit = floor_ctl->floor_hit;
+  }
   const bool mixed = desc.precision == FVS_FP32_FLUX;
   const int64_t tiles = (a.n[0] + kTX - 1) / kTX;
   const int64_t gx = tiles * (D == 3 ? a.n[1] : 1);

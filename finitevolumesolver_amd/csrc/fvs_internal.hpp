@@ -163,10 +163,17 @@ int LaunchGodunovSweep(const fvs_method_desc& desc, hipStream_t s, int dir,
                        const fvs_mlp* net = nullptr, int reuse = 0);
 int LaunchGodunovFinalize(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
                           double* out, unsigned long long* stats, Ctl* ctl = nullptr);
+// Newton iteration cap from which the exact-Godunov CFL pass uses the speed
+// floor (below it every face is solved, so a small cap is reported as the
+// reference's would be).
+constexpr int kGodunovFloorCap = 16;
+// With floor_ctl (the level driver) and an iteration cap of at least 16, the
+// faces of directions other than pm_dir that a certified speed bound keeps
+// below ctl->cfl_floor skip their Newton solve.
 int LaunchGodunovCfl(const fvs_method_desc& desc, hipStream_t s, const double* const* cons,
                      const int64_t n[3], const double dx[3], const double* const* guess,
                      unsigned long long* partials, int* halt, const fvs_mlp* net = nullptr,
-                     double* pm_out = nullptr, int pm_dir = -1);
+                     double* pm_out = nullptr, int pm_dir = -1, Ctl* floor_ctl = nullptr);
 int LaunchGodunovViewFlux(const fvs_method_desc& desc, hipStream_t s, const fvs_view* fluxes,
                           const fvs_view* states, double dt, double dx, int dir,
                           unsigned long long* nfail);
@@ -188,7 +195,10 @@ int LaunchMiddleState(int dim, hipStream_t s, const double* L, const double* R, 
 int LaunchFinalizeCfl(hipStream_t s, const unsigned long long* partials, int64_t nblocks,
                       int dim, double* out, Ctl* ctl, int first_dir, int strang);
 int LaunchStepEnd(hipStream_t s, Ctl* ctl);
-int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang);
+// Step dt from ctl->dt_dir; with use_floor (the exact-Godunov CFL pass ran
+// with a speed floor) the floor is validated and renewed (dx: cell widths).
+int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang,
+                 const double* dx = nullptr, int use_floor = 0);
 int LaunchCheckValue(hipStream_t s, Ctl* ctl, const double* value);
 int LaunchKeysToDt(hipStream_t s, unsigned long long* keys, int n, double* out);
 int LaunchResetKeys(hipStream_t s, unsigned long long* keys, int n,

@@ -894,6 +894,44 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCheckKernel(
   }
 }
 
+// The CFL pass's speed floor after a pass: a floor that no evaluated face
+// reached may have hidden the fastest face, so the step is repeated without
+// a floor (halt 3) and the margin quadruples; otherwise the next floor is
+// this pass's largest speed per direction (dx / dt) less a margin that
+// follows how fast that speed falls.  Directions equal to solved_dir had
+// every face evaluated.  Returns false after setting halt 3.
+__device__ bool RenewSpeedFloor(Ctl* ctl, const double* v, const double* fdx, int dim,
+                                int solved_dir) {
+  bool floor_ok = true;
+  for (int d = 0; d < 3; ++d) {
+    if (d != solved_dir && ctl->cfl_floor[d] > 0.0 && !ctl->floor_hit[d]) floor_ok = false;
+    ctl->floor_hit[d] = 0;
+  }
+  const double old_margin = ctl->floor_margin > 0.0 ? ctl->floor_margin : kFloorMarginFirst;
+  if (!floor_ok) {
+    for (int d = 0; d < 3; ++d) ctl->cfl_floor[d] = 0.0;
+    ctl->floor_margin = fmin(4.0 * old_margin, kFloorMarginMax);
+    ctl->halt = 3;
+    return false;
+  }
+  double speed[3] = {0.0, 0.0, 0.0}, drop = 0.0;
+  for (int d = 0; d < dim; ++d) {
+    speed[d] = fdx[d] / v[d];
+    if (ctl->floor_speed[d] > 0.0 && speed[d] > 0.0)
+      drop = fmax(drop, (ctl->floor_speed[d] - speed[d]) / speed[d]);
+  }
+  const double margin =
+      fmin(kFloorMarginMax, fmax(fmax(kFloorMarginMin, 4.0 * drop), 0.5 * old_margin));
+  ctl->floor_margin = margin;
+  for (int d = 0; d < 3; ++d) {
+    const double f = speed[d] * (1.0 - margin);
+    const bool finite = d < dim && f > 0.0 && f < 1.7976931348623157e308;
+    ctl->cfl_floor[d] = finite ? f : 0.0;
+    ctl->floor_speed[d] = finite ? speed[d] : 0.0;
+  }
+  return true;
+}
+
 // With use_floor (the level's 3D pass, fdx the cell widths), the floor of
 // the pass is validated first and the next step's floor set from this pass's
 // largest speeds.
@@ -914,38 +952,8 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
   }
   if (!ctl) return;
   if (use_floor) {
-    // A floor that no evaluated face reached may have hidden the fastest
-    // face: the step is repeated without a floor, and the margin quadruples.
-    bool floor_ok = true;
-    for (int d = 0; d < 3; ++d) {
-      if (ctl->cfl_floor[d] > 0.0 && !ctl->floor_hit[d]) floor_ok = false;
-      ctl->floor_hit[d] = 0;
-    }
-    const double old_margin = ctl->floor_margin > 0.0 ? ctl->floor_margin : kFloorMarginFirst;
-    if (!floor_ok) {
-      for (int d = 0; d < 3; ++d) ctl->cfl_floor[d] = 0.0;
-      ctl->floor_margin = fmin(4.0 * old_margin, kFloorMarginMax);
-      ctl->halt = 3;
-      return;
-    }
-    // The next step's floor: this state's largest speeds dx / dt less a
-    // margin that follows how fast they fall.
     const double fdx[3] = {fdx0, fdx1, fdx2};
-    double speed[3], drop = 0.0;
-    for (int d = 0; d < 3; ++d) {
-      speed[d] = fdx[d] / v[d];
-      if (ctl->floor_speed[d] > 0.0 && speed[d] > 0.0)
-        drop = fmax(drop, (ctl->floor_speed[d] - speed[d]) / speed[d]);
-    }
-    const double margin =
-        fmin(kFloorMarginMax, fmax(fmax(kFloorMarginMin, 4.0 * drop), 0.5 * old_margin));
-    ctl->floor_margin = margin;
-    for (int d = 0; d < 3; ++d) {
-      const double f = speed[d] * (1.0 - margin);
-      const bool finite = f > 0.0 && f < 1.7976931348623157e308;
-      ctl->cfl_floor[d] = finite ? f : 0.0;
-      ctl->floor_speed[d] = finite ? speed[d] : 0.0;
-    }
+    if (!RenewSpeedFloor(ctl, v, fdx, dim, -1)) return;
   }
   // DimensionalSplitLevelIntegrator::ComputeStableDt (:204-219) starts from
   // DBL_MAX; RunSimulation's limited_dt (RunSimulation.hpp:121-131).
@@ -976,8 +984,13 @@ __global__ __launch_bounds__(kRedBlock) void FinalizeCflKernel(
 
 // Step dt from per-direction stable dts already in ctl->dt_dir (used by
 // methods whose CFL pass reduces into ctl->dt_dir itself).
-__global__ void StepDtKernel(Ctl* ctl, int dim, int first_dir, int strang) {
+__global__ void StepDtKernel(Ctl* ctl, int dim, int first_dir, int strang, double fdx0,
+                             double fdx1, double fdx2, int use_floor) {
   if (ctl->halt) return;
+  if (use_floor) {
+    const double fdx[3] = {fdx0, fdx1, fdx2};
+    if (!RenewSpeedFloor(ctl, ctl->dt_dir, fdx, dim, first_dir)) return;
+  }
   double stable = 1.7976931348623157e308;
   for (int d = 0; d < dim; ++d) stable = (ctl->dt_dir[d] < stable) ? ctl->dt_dir[d] : stable;
   if (!(ctl->t + ctl->eps < ctl->final_time)) {
@@ -1289,8 +1302,11 @@ int LaunchFinalizeStep(hipStream_t s, const unsigned long long* partials, const 
   return FVS_OK;
 }
 
-int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang) {
-  hipLaunchKernelGGL(StepDtKernel, dim3(1), dim3(1), 0, s, ctl, dim, first_dir, strang);
+int LaunchStepDt(hipStream_t s, Ctl* ctl, int dim, int first_dir, int strang, const double* dx,
+                 int use_floor) {
+  const double f0 = dx ? dx[0] : 1.0, f1 = dx ? dx[1] : 1.0, f2 = dx ? dx[2] : 1.0;
+  hipLaunchKernelGGL(StepDtKernel, dim3(1), dim3(1), 0, s, ctl, dim, first_dir, strang, f0, f1,
+                     f2, (dx && use_floor) ? 1 : 0);
   FVS_LAUNCH_CHECK();
   return FVS_OK;
 }

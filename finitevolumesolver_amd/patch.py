@@ -396,9 +396,10 @@ class SinglePatchLevel:
         return t.value, cyc.value
 
     def hint_max_speed(self, speed) -> None:
-        """Speed floor of the next 3D step-start CFL pass (per direction; by
-        default the previous pass's largest speed less a margin that follows
-        how fast that speed falls).  A floor above
+        """Speed floor of the next step-start CFL pass of a 3D HLLEM or an
+        exact-Godunov level (per direction; by default the previous pass's
+        largest speed less a margin that follows how fast that speed falls).
+        A floor above
         the true largest signal speed costs one repeated step and never
         changes a result."""
         arr = (C.c_double * 3)(*(list(speed) + [0.0] * (3 - len(speed))))

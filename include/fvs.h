@@ -277,16 +277,19 @@ int fvs_level_advance(fvs_level* level, int max_steps, double final_time,
 /* Time point and cycle count of the current state. */
 int fvs_level_time(fvs_level* level, double* t, int64_t* cycles);
 
-/* Speed floor of the 3D step-start CFL pass (no counterpart in the
- * reference, whose ComputeStableDt, HllemMethod.hpp:1152-1216, evaluates
- * every face).  The CFL pass of a 3D HLLEM level evaluates the exact speed
- * only of faces whose cheap upper bound reaches a floor, the previous pass's
- * largest signal speed less a margin (four times its last relative drop,
- * between 2^-10 and 1/4), and confirms that some evaluated face did reach
- * it.  When none did, the step is repeated with every face evaluated (and
- * the margin quadruples), so the floor never changes a result.  fvs_level_hint_max_speed sets the floor for the current
- * state (speed[d] >= 0; 0: none; ignored by other levels);
- * fvs_level_cfl_redos reports how many steps were repeated. */
+/* Speed floor of the step-start CFL pass (no counterpart in the reference,
+ * whose ComputeStableDt, HllemMethod.hpp:1152-1216 and
+ * GodunovMethod.hpp:163-171, evaluates every face).  The CFL pass of a 3D
+ * HLLEM level, and of an exact-Godunov level (2D or 3D, with a Newton
+ * iteration cap of at least 16), evaluates the exact speed only of faces
+ * whose certified upper bound reaches a floor, the previous pass's largest
+ * signal speed less a margin (four times its last relative drop, between
+ * 2^-10 and 1/4), and confirms that some evaluated face did reach it.  When
+ * none did, the step is repeated with every face evaluated (and the margin
+ * quadruples), so the floor never changes a result.
+ * fvs_level_hint_max_speed sets the floor for the current state
+ * (speed[d] >= 0; 0: none; ignored by other levels); fvs_level_cfl_redos
+ * reports how many steps were repeated. */
 int fvs_level_hint_max_speed(fvs_level* level, const double speed[3]);
 int fvs_level_cfl_redos(fvs_level* level, int64_t* redos);
 

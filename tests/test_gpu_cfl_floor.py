@@ -88,3 +88,68 @@ def test_wrong_speed_hint_repeats_the_step(cuda, factor):
         assert runs[1][2] >= 1
     _note(f"hint x{factor}: {runs[1][2]} repeated step(s) (unhinted {runs[0][2]}), "
           "results unchanged")
+
+
+@pytest.mark.parametrize("prec,kind,dim", [
+    ("fp64", "fourier", 3),
+    ("fp64", "rough", 3),
+    ("fp32", "rough", 3),
+    ("fp64", "rough", 2),
+])
+def test_godunov_level_dt_equals_full_cfl_pass(cuda, prec, kind, dim):
+    """First-order exact Godunov: the level's CFL pass solves Newton only on
+    faces whose certified speed bound reaches the floor (and on every face
+    of the first sweep's direction, whose star pressures the sweep reuses);
+    every step's dt still equals cfl times the smallest stable dt of the pass
+    that solves every face (fvs_patch_stable_dt) bit for bit, and fewer
+    faces are solved than the three directions hold."""
+    import torch
+    _note(f"godunov speed floor: {prec} {kind} {dim}D")
+    if dim == 3:
+        cons, dx = _field(kind, 20)
+        shape = (cons.shape[3], cons.shape[2], cons.shape[1])
+    else:
+        cons = rough_field((1, 40, 48), seed=9)
+        dx = (1.0 / 48, 1.0 / 40)
+        shape = (48, 40)
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=dim), base="Godunov", reconstruction="None",
+                        precision=prec)
+    lvl = fvs.SinglePatchLevel(fm, shape, dx, splitting="Godunov", cfl=0.8)
+    lvl.set_state(torch.from_numpy(cons).to(cuda))
+    solved = []
+    for step in range(8):
+        sdt = patch_stable_dt(fm, lvl.get_state(), dx).cpu().numpy()
+        fm.newton_stats(reset=True)
+        done, fails, dts = lvl.advance(1, batch=1, return_dts=True)
+        assert done == 1 and fails == 0
+        assert dts[0] == 0.8 * float(np.min(sdt)), (step, dts[0], sdt)
+        solved.append(fm.newton_stats()["faces"])
+    cells = int(np.prod(shape))
+    _note(f"  faces solved per step {solved} ({cells} cells, {lvl.cfl_redos()} repeated)")
+    # after the first step the floor skips faces: fewer than the 2D + 1 face
+    # sets a step solves without it (CFL pass D sets, two sweeps)
+    assert min(solved[1:]) < dim * cells + (dim - 1) * cells
+
+
+def test_godunov_wrong_speed_hint_repeats_the_step(cuda):
+    """A floor ten times the largest speed on the exact-Godunov level: the
+    pass reaches none, the step is repeated and the run equals the unhinted
+    one bit for bit."""
+    import torch
+    cons, dx = _field("rough", 16)
+    shape = (cons.shape[3], cons.shape[2], cons.shape[1])
+    runs = []
+    for hint in (False, True):
+        fm = fvs.FluxMethod(fvs.PerfectGas(dim=3), base="Godunov", reconstruction="None")
+        lvl = fvs.SinglePatchLevel(fm, shape, dx, splitting="Godunov", cfl=0.8)
+        lvl.set_state(torch.from_numpy(cons).to(cuda))
+        if hint:
+            sdt = patch_stable_dt(fm, lvl.get_state(), dx).cpu().numpy()
+            lvl.hint_max_speed([10.0 * dx[d] / sdt[d] for d in range(3)])
+        done, fails, dts = lvl.advance(5, batch=5, return_dts=True)
+        assert done == 5 and fails == 0
+        runs.append((lvl.get_state().cpu().numpy(), dts, lvl.cfl_redos()))
+    assert runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[1][2] >= 1
+    _note(f"godunov hint x10: {runs[1][2]} repeated step(s), results unchanged")
