@@ -752,7 +752,10 @@ namespace {
 
 // Streams ncomp arrays of n doubles through the chip, 16 bytes per lane and
 // array per access, grid-stride: the traffic of a fused sweep (every
-// component read and written once) without its arithmetic.
+// component read and written once) without its arithmetic.  Loads and stores
+// carry the nontemporal hint, the fastest of the copy forms measured on
+// MI355X (plain accesses and deeper unrolling move about 5% less).
+typedef double StreamD2 __attribute__((ext_vector_type(2)));
 struct StreamArgs {
   const double* src[5];
   double* dst[5];
@@ -764,13 +767,15 @@ __global__ __launch_bounds__(256) void StreamCopyKernel(StreamArgs a) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < a.n2;
        i += stride) {
-    double2 v[5];
+    StreamD2 v[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c)
-      if (c < a.ncomp) v[c] = reinterpret_cast<const double2*>(a.src[c])[i];
+      if (c < a.ncomp)
+        v[c] = __builtin_nontemporal_load(reinterpret_cast<const StreamD2*>(a.src[c]) + i);
 #pragma unroll
     for (int c = 0; c < 5; ++c)
-      if (c < a.ncomp) reinterpret_cast<double2*>(a.dst[c])[i] = v[c];
+      if (c < a.ncomp)
+        __builtin_nontemporal_store(v[c], reinterpret_cast<StreamD2*>(a.dst[c]) + i);
   }
 }
 

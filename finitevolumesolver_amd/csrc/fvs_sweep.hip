@@ -67,6 +67,14 @@ __device__ __forceinline__ double LdOff(const double* base, uint32_t byte_off) {
 __device__ __forceinline__ void StOff(double* base, uint32_t byte_off, double v) {
   *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
 }
+// Streaming store (nontemporal hint) for the pencil sweeps, whose waves write
+// whole 512-byte rows: the lines go to memory without being kept in the
+// caches (about 4% faster pencils, and the next pass's loads find the caches
+// free).  The X sweep keeps StOff: its 60-cell chunks end inside cache lines,
+// and streaming partial lines made it 10% slower.
+__device__ __forceinline__ void StOffStream(double* base, uint32_t byte_off, double v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off));
+}
 
 // Largest of two signal speeds; a NaN speed never wins (as a NaN face dt
 // never wins the minimum).
@@ -544,8 +552,8 @@ __global__ __launch_bounds__(kBlock) void SweepPencilKernel(SweepArgs a) {
         const int64_t uoff = ubase + (j - 1) * stride;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
-          StOff(a.out[c] + uoff, toff,
-                Um[c] + dt_dx * (static_cast<double>(Fprev[c]) - static_cast<double>(F[c])));
+          StOffStream(a.out[c] + uoff, toff,
+                      Um[c] + dt_dx * (static_cast<double>(Fprev[c]) - static_cast<double>(F[c])));
         }
       }
       if (a.check && (j < j1 || j1 == nd)) {

@@ -351,7 +351,8 @@ int fvs_patch_stable_dt_noalloc(fvs_handle* h, const double* const* cons,
  * for callers that want the achievable bandwidth of a sweep's traffic):
  * copies ncomp (1..5) device arrays of n doubles (n even, 16-byte aligned),
  * src[c] to dst[c], on the handle's stream; every element is read and written
- * once, 16 bytes per lane and array per access. */
+ * once, 16 bytes per lane and array per access, with nontemporal loads and
+ * stores. */
 int fvs_stream_copy(fvs_handle* h, const double* const* src, double* const* dst,
                     int ncomp, int64_t n);
 
