@@ -1183,6 +1183,13 @@ struct LevelGeom {
   const double* beta[2];    // x: [ngy][ngx + 1], y: [ngy + 1][ngx]
   const double* normals;    // [2][ngy][ngx] (AMReX orientation)
   const double* centroids;  // [2][ngy][ngx]
+  // Per direction, per cell ([ngy][ngx]): -1 for a regular cell (volume
+  // fraction 1 and the four face fractions around it along the direction 1),
+  // -2 for a covered cell, else the cell's slot in the direction's list of
+  // irregular cells, whose two face fluxes the fused sweep leaves in
+  // irr_flux[slot * 8 ..] for KbnIrregularKernel.
+  const int32_t* irr[2];
+  double* irr_flux[2];
   int64_t ngx, ngy;
   __device__ __forceinline__ int64_t cell(int64_t i, int64_t j) const {
     return (j + kG) * ngx + (i + kG);
@@ -1403,6 +1410,44 @@ __device__ __forceinline__ FaceFrac Fractions(double bl, double bm, double br) {
   return f;
 }
 
+// A regular cell away from the boundary: both faces are unshielded
+// (fractions 1, 0, 0), no boundary flux enters, and the general formula of
+// KbnUpdateCell reduces to the plain update with the same result bits.
+__device__ __forceinline__ void KbnRegularUpdate(const LevelState& in, int64_t c,
+                                                 const double* Fl, const double* Fr,
+                                                 double dt_over_dx, double* res) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) res[k] = in.u[k][c] + dt_over_dx * (Fl[k] - Fr[k]);
+  const bool keep = res[0] > 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) res[k] = keep ? res[k] : 0.0;
+}
+
+// Update of the cell from its two face fluxes along DIR (map value of the
+// cell: regular, covered or an irregular slot).  Irregular cells hand their
+// fluxes to KbnIrregularKernel, which writes them after the sweep: the
+// h-box formula inline would hold the sweeps' registers (and occupancy)
+// for the few cells near the boundary.
+template <int DIR>
+__device__ __forceinline__ void KbnSweepUpdate(const LevelState& in, const LevelState& out,
+                                               const LevelGeom& g, int64_t c, const double* Fl,
+                                               const double* Fr, double dt_over_dx) {
+  const int32_t slot = g.irr[DIR][c];
+  if (slot >= 0) {
+    double* f = g.irr_flux[DIR] + 8 * static_cast<int64_t>(slot);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      f[k] = Fl[k];
+      f[4 + k] = Fr[k];
+    }
+    return;
+  }
+  double res[4] = {0.0, 0.0, 0.0, 0.0};
+  if (slot == -1) KbnRegularUpdate(in, c, Fl, Fr, dt_over_dx, res);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
+}
+
 // h-box update (TimeIntegrator, or TimeIntegrator2 when `second`) of the
 // uncovered interior cell (i, j) from the regular fluxes Fl / Fr of its two
 // faces in DIR; the boundary fluxes fb are read only where a shielded
@@ -1421,14 +1466,7 @@ __device__ __forceinline__ void KbnUpdateCell(const LevelState& in, const LevelG
   const double bR = g.beta_at(DIR, i + di, j + dj);
   const double bRR = g.beta_at(DIR, i + 2 * di, j + 2 * dj);
   if (alpha == 1.0 && bLL == 1.0 && bL == 1.0 && bR == 1.0 && bRR == 1.0) {
-    // A regular cell away from the boundary: both faces are unshielded
-    // (fractions 1, 0, 0), no boundary flux enters, and the general formula
-    // below reduces to the plain update with the same result bits.
-#pragma unroll
-    for (int k = 0; k < 4; ++k) res[k] = in.u[k][c] + dt_over_dx * (Fl[k] - Fr[k]);
-    const bool keep = res[0] > 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) res[k] = keep ? res[k] : 0.0;
+    KbnRegularUpdate(in, c, Fl, Fr, dt_over_dx, res);
     return;
   }
   const FaceFrac fl = Fractions(bLL, bL, bR);
@@ -1505,8 +1543,7 @@ template <int LIM, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelState out,
                                                           LevelGeom g, int64_t nx, int64_t ny,
                                                           double dx, const double* dt_ptr,
-                                                          Gas<double> gas, const double* fb,
-                                                          int second,
+                                                          Gas<double> gas,
                                                           unsigned long long* partials,
                                                           const int* halt) {
   using TA = typename Policies<T, FAST>::TA;
@@ -1565,12 +1602,8 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   double Fr[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) Fr[k] = ShiftDown(F[k]);
-  if (wave_ok && lane >= 2 && lane < 2 + kKbnOwned && i >= 0 && i < nx) {
-    double res[4] = {0.0, 0.0, 0.0, 0.0};
-    if (alpha != 0.0) KbnUpdateCell<0>(in, g, fb, i, j, F, Fr, DivRn(dt, dx), second, res);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
-  }
+  if (wave_ok && lane >= 2 && lane < 2 + kKbnOwned && i >= 0 && i < nx)
+    KbnSweepUpdate<0>(in, out, g, c, F, Fr, DivRn(dt, dx));
   const unsigned long long m = BlockMinKey(key);
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
@@ -1584,8 +1617,7 @@ template <int LIM, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelState out,
                                                           LevelGeom g, int64_t nx, int64_t ny,
                                                           double dx, const double* dt_ptr,
-                                                          Gas<double> gas, const double* fb,
-                                                          int second,
+                                                          Gas<double> gas,
                                                           unsigned long long* partials,
                                                           const int* halt) {
   using TA = typename Policies<T, FAST>::TA;
@@ -1636,13 +1668,7 @@ __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelSt
         for (int k = 0; k < 4; ++k) F[k] = static_cast<double>(Ft[k]);
         if (col_ok) key = KeyMin(key, DtKey(HllemStableDt<2, 1>(qm, q0, dx, gas)));
       }
-      if (t - 1 >= j0 && col_ok) {
-        const int64_t c = g.cell(i, t - 1);
-        double res[4] = {0.0, 0.0, 0.0, 0.0};
-        if (g.alpha[c] != 0.0) KbnUpdateCell<1>(in, g, fb, i, t - 1, Fm, F, dt_over_dx, second, res);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
-      }
+      if (t - 1 >= j0 && col_ok) KbnSweepUpdate<1>(in, out, g, g.cell(i, t - 1), Fm, F, dt_over_dx);
 #pragma unroll
       for (int k = 0; k < 4; ++k) Fm[k] = F[k];
     }
@@ -1658,6 +1684,33 @@ __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelSt
   }
   const unsigned long long m = BlockMinKey(key);
   if (threadIdx.x == 0) partials[static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x] = m;
+}
+
+// The h-box update (KbnUpdateCell) of the irregular cells of direction DIR,
+// one thread per listed cell, from the two face fluxes the fused sweep left
+// in irr_flux; runs after that sweep on the same stream.
+template <int DIR>
+__global__ __launch_bounds__(kBlock) void KbnIrregularKernel(LevelState in, LevelState out,
+                                                             LevelGeom g, const int32_t* cells,
+                                                             int n, const double* fb,
+                                                             const double* dt_ptr, double dx,
+                                                             int second, const int* halt) {
+  if (*halt) return;
+  const int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  const int64_t c = cells[q];
+  const int64_t i = c % g.ngx - kG, j = c / g.ngx - kG;
+  const double* f = g.irr_flux[DIR] + 8 * static_cast<int64_t>(q);
+  double Fl[4], Fr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    Fl[k] = f[k];
+    Fr[k] = f[4 + k];
+  }
+  double res[4];
+  KbnUpdateCell<DIR>(in, g, fb, i, j, Fl, Fr, DivRn(*dt_ptr, dx), second, res);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
 }
 
 // Workgroups (= partial minima) of a fused cut-cell sweep.
@@ -1708,6 +1761,10 @@ struct fvs_kbn_level {
   int64_t nblocks;
   Ctl* ctl;
   int cur;
+  int32_t* irr_map[2];    // LevelGeom::irr
+  int32_t* irr_cells[2];  // cell index of each irregular slot
+  double* irr_flux[2];
+  int nirr[2];
 };
 
 namespace {
@@ -1808,6 +1865,53 @@ int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
   L->lg.centroids = gc;
   L->lg.ngx = ngx;
   L->lg.ngy = ngy;
+  // regular / covered / irregular map of the interior cells per direction
+  {
+    std::vector<double> ha(ncg), hbx(nbx), hby(nby);
+    if (hipMemcpyAsync(ha.data(), ga, ncg * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(hbx.data(), gbx, nbx * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(hby.data(), gby, nby * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return fail(FVS_HIP_ERROR);
+    for (int d = 0; d < 2; ++d) {
+      std::vector<int32_t> map(ncg, -2), cells;
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t i = 0; i < nx; ++i) {
+          const int64_t c = (j + kG) * ngx + (i + kG);
+          const double a = ha[c];
+          if (a == 0.0) continue;
+          // face fractions at faces i - 1 .. i + 2 (or j - 1 .. j + 2)
+          bool regular = a == 1.0;
+          for (int64_t f = -1; f <= 2 && regular; ++f)
+            regular = (d == 0 ? hbx[(j + kG) * (ngx + 1) + (i + f + kG)]
+                              : hby[(j + f + kG) * ngx + (i + kG)]) == 1.0;
+          if (regular) {
+            map[c] = -1;
+          } else {
+            map[c] = static_cast<int32_t>(cells.size());
+            cells.push_back(static_cast<int32_t>(c));
+          }
+        }
+      L->nirr[d] = static_cast<int>(cells.size());
+      const size_t ncells = cells.empty() ? 1 : cells.size();
+      if (hipMalloc(&L->irr_map[d], ncg * sizeof(int32_t)) != hipSuccess ||
+          hipMalloc(&L->irr_cells[d], ncells * sizeof(int32_t)) != hipSuccess ||
+          hipMalloc(&L->irr_flux[d], 8 * ncells * sizeof(double)) != hipSuccess)
+        return fail(FVS_HIP_ERROR);
+      if (hipMemcpyAsync(L->irr_map[d], map.data(), ncg * sizeof(int32_t), hipMemcpyHostToDevice,
+                         s) != hipSuccess ||
+          (!cells.empty() &&
+           hipMemcpyAsync(L->irr_cells[d], cells.data(), cells.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s) != hipSuccess) ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return fail(FVS_HIP_ERROR);
+      L->lg.irr[d] = L->irr_map[d];
+      L->lg.irr_flux[d] = L->irr_flux[d];
+    }
+  }
   // list of the interior cut cells
   int32_t* flags = nullptr;
   if (hipMalloc(&flags, nx * ny * sizeof(int32_t)) != hipSuccess) return fail(FVS_HIP_ERROR);
@@ -1875,7 +1979,10 @@ int fvs_kbn_level_destroy(fvs_kbn_level* L) {
   for (int b = 0; b < 3; ++b)
     for (int k = 0; k < 4; ++k)
       if (L->buf[b][k]) (void)hipFree(L->buf[b][k]);
-  void* ps[] = {L->geom, L->cut, L->refs, L->fb, L->F, L->partials, L->ctl};
+  void* ps[] = {L->geom,         L->cut,          L->refs,         L->fb,
+                L->F,            L->partials,     L->ctl,          L->irr_map[0],
+                L->irr_map[1],   L->irr_cells[0], L->irr_cells[1], L->irr_flux[0],
+                L->irr_flux[1]};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   delete L;
@@ -1969,10 +2076,10 @@ static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
 #define FVS_KS(K, LM, LA)                                                                     \
   if (fp32)                                                                                   \
     hipLaunchKernelGGL((K<LM, LA, float, true>), grid, dim3(kBlock), 0, s, in, out, L->lg,    \
-                       L->nx, L->ny, dx, dtp, gas, L->fb, L->integrator, L->partials, halt);  \
+                       L->nx, L->ny, dx, dtp, gas, L->partials, halt);                        \
   else                                                                                        \
     hipLaunchKernelGGL((K<LM, LA, double, false>), grid, dim3(kBlock), 0, s, in, out, L->lg,  \
-                       L->nx, L->ny, dx, dtp, gas, L->fb, L->integrator, L->partials, halt)
+                       L->nx, L->ny, dx, dtp, gas, L->partials, halt)
 #define FVS_KS_LIM(K, LA)                                  \
   switch (h->desc.limiter) {                               \
   case kMinMod: FVS_KS(K, kMinMod, LA); break;             \
@@ -1988,6 +2095,17 @@ static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
 #undef FVS_KS_LIM
 #undef FVS_KS
   FVS_LAUNCH_CHECK();
+  if (L->nirr[dir] > 0) {
+    if (dir == 0)
+      hipLaunchKernelGGL(KbnIrregularKernel<0>, dim3(Blocks(L->nirr[0])), dim3(kBlock), 0, s, in,
+                         out, L->lg, L->irr_cells[0], L->nirr[0], L->fb, dtp, dx, L->integrator,
+                         halt);
+    else
+      hipLaunchKernelGGL(KbnIrregularKernel<1>, dim3(Blocks(L->nirr[1])), dim3(kBlock), 0, s, in,
+                         out, L->lg, L->irr_cells[1], L->nirr[1], L->fb, dtp, dx, L->integrator,
+                         halt);
+    FVS_LAUNCH_CHECK();
+  }
   if (check) {
     if (int rc = LaunchFinalizeCheck(s, L->partials, nb, nullptr, L->ctl)) return rc;
   }
