@@ -1183,12 +1183,9 @@ struct LevelGeom {
   const double* beta[2];    // x: [ngy][ngx + 1], y: [ngy + 1][ngx]
   const double* normals;    // [2][ngy][ngx] (AMReX orientation)
   const double* centroids;  // [2][ngy][ngx]
-  // Per direction, per cell ([ngy][ngx]): -1 for a regular cell (volume
-  // fraction 1 and the four face fractions around it along the direction 1),
-  // -2 for a covered cell, else the cell's slot in the direction's list of
-  // irregular cells, whose two face fluxes the fused sweep leaves in
-  // irr_flux[slot * 8 ..] for KbnIrregularKernel.
-  const int32_t* irr[2];
+  // Per direction, per cell ([ngy][ngx], ghost ring included), what the
+  // fused sweeps need of the geometry in one word (CellCode).
+  const int32_t* code[2];
   double* irr_flux[2];
   int64_t ngx, ngy;
   __device__ __forceinline__ int64_t cell(int64_t i, int64_t j) const {
@@ -1410,42 +1407,64 @@ __device__ __forceinline__ FaceFrac Fractions(double bl, double bm, double br) {
   return f;
 }
 
+// Geometry word of a cell for the fused sweep of one direction:
+//   bit 0: volume fraction > 0 (the reconstruction mask),
+//   bit 1: fraction of the cell's lower face in the direction > 0 (the face
+//          carries a regular flux),
+//   bits 2..: 0 covered (volume fraction 0: the update is the zero state),
+//          1 regular (volume fraction 1 and the four face fractions around
+//          the cell along the direction 1: the plain update), or 2 + the
+//          cell's slot in the direction's list of irregular cells, whose two
+//          face fluxes the sweep leaves in irr_flux[slot * 8 ..] for
+//          KbnIrregularKernel.
+struct CellCode {
+  static constexpr int32_t kCovered = 0, kRegular = 1, kSlot0 = 2;
+  __host__ __device__ static int32_t Make(bool pos, bool face, int32_t kind) {
+    return (pos ? 1 : 0) | (face ? 2 : 0) | (kind << 2);
+  }
+  __device__ static bool Pos(int32_t c) { return (c & 1) != 0; }
+  __device__ static bool Face(int32_t c) { return (c & 2) != 0; }
+  __device__ static int32_t Kind(int32_t c) { return c >> 2; }
+};
+
 // A regular cell away from the boundary: both faces are unshielded
 // (fractions 1, 0, 0), no boundary flux enters, and the general formula of
 // KbnUpdateCell reduces to the plain update with the same result bits.
-__device__ __forceinline__ void KbnRegularUpdate(const LevelState& in, int64_t c,
-                                                 const double* Fl, const double* Fr,
-                                                 double dt_over_dx, double* res) {
+// u: the cell's conservative state.
+__device__ __forceinline__ void KbnRegularUpdate(const double* u, const double* Fl,
+                                                 const double* Fr, double dt_over_dx,
+                                                 double* res) {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) res[k] = in.u[k][c] + dt_over_dx * (Fl[k] - Fr[k]);
+  for (int k = 0; k < 4; ++k) res[k] = u[k] + dt_over_dx * (Fl[k] - Fr[k]);
   const bool keep = res[0] > 0.0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) res[k] = keep ? res[k] : 0.0;
 }
 
-// Update of the cell from its two face fluxes along DIR (map value of the
-// cell: regular, covered or an irregular slot).  Irregular cells hand their
-// fluxes to KbnIrregularKernel, which writes them after the sweep: the
-// h-box formula inline would hold the sweeps' registers (and occupancy)
-// for the few cells near the boundary.
+// Update of a cell from its conservative state u and its two face fluxes
+// along DIR.  Irregular cells hand their fluxes to KbnIrregularKernel, which
+// writes them after the sweep (the h-box formula inline would cost the
+// sweeps registers and occupancy for the few cells near the boundary), and
+// return false; otherwise res holds the update (the zero state for covered
+// cells).
 template <int DIR>
-__device__ __forceinline__ void KbnSweepUpdate(const LevelState& in, const LevelState& out,
-                                               const LevelGeom& g, int64_t c, const double* Fl,
-                                               const double* Fr, double dt_over_dx) {
-  const int32_t slot = g.irr[DIR][c];
-  if (slot >= 0) {
-    double* f = g.irr_flux[DIR] + 8 * static_cast<int64_t>(slot);
+__device__ __forceinline__ bool KbnSweepUpdate(const LevelGeom& g, int32_t code, const double* u,
+                                               const double* Fl, const double* Fr,
+                                               double dt_over_dx, double* res) {
+  const int32_t kind = CellCode::Kind(code);
+  if (kind >= CellCode::kSlot0) {
+    double* f = g.irr_flux[DIR] + 8 * static_cast<int64_t>(kind - CellCode::kSlot0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       f[k] = Fl[k];
       f[4 + k] = Fr[k];
     }
-    return;
+    return false;
   }
-  double res[4] = {0.0, 0.0, 0.0, 0.0};
-  if (slot == -1) KbnRegularUpdate(in, c, Fl, Fr, dt_over_dx, res);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
+  for (int k = 0; k < 4; ++k) res[k] = 0.0;
+  if (kind == CellCode::kRegular) KbnRegularUpdate(u, Fl, Fr, dt_over_dx, res);
+  return true;
 }
 
 // h-box update (TimeIntegrator, or TimeIntegrator2 when `second`) of the
@@ -1466,7 +1485,9 @@ __device__ __forceinline__ void KbnUpdateCell(const LevelState& in, const LevelG
   const double bR = g.beta_at(DIR, i + di, j + dj);
   const double bRR = g.beta_at(DIR, i + 2 * di, j + 2 * dj);
   if (alpha == 1.0 && bLL == 1.0 && bL == 1.0 && bR == 1.0 && bRR == 1.0) {
-    KbnRegularUpdate(in, c, Fl, Fr, dt_over_dx, res);
+    double u[4];
+    LoadCons(in, c, u);
+    KbnRegularUpdate(u, Fl, Fr, dt_over_dx, res);
     return;
   }
   const FaceFrac fl = Fractions(bLL, bL, bR);
@@ -1558,9 +1579,10 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   const int64_t i = (w % nseg) * kKbnOwned + lane - 2;  // -2 .. nx + 61
   const int64_t ic = i < nx + 1 ? i : nx + 1;            // stay inside the ghost ring
   const int64_t c = g.cell(ic, j);
+  const int32_t code = g.code[0][c];
   double q[6];
-  CellComplete(in, g, c, gas, q);
-  const double alpha = g.alpha[c];
+  LoadCons(in, c, q);
+  CompleteMasked<2>(q, CellCode::Pos(code), gas);
   double qm[6], qp[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
@@ -1569,8 +1591,8 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   }
   // Every lane must take part in a DPP shift (an inactive source lane yields
   // the reader's own value): shift first, combine without short-circuit.
-  const double alpha_m = ShiftUp(alpha), alpha_p = ShiftDown(alpha);
-  const bool mask = (alpha_m > 0.0) & (alpha > 0.0) & (alpha_p > 0.0);
+  const int code_m = DppUp(code), code_p = DppDown(code);
+  const bool mask = CellCode::Pos(code_m) & CellCode::Pos(code) & CellCode::Pos(code_p);
   const double dt = *dt_ptr;
   const double lh = DivRn(0.5 * dt, dx);
   T recL[6], recR[6];
@@ -1589,10 +1611,9 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   for (int k = 0; k < 6; ++k) recRm[k] = ShiftUp(recR[k]);
   // left face of the lane's cell: faces 0 .. nx, lanes 2 .. 62
   const bool face_lane = wave_ok && lane >= 2 && lane <= 2 + kKbnOwned && i >= 0 && i <= nx;
-  const double beta = g.beta_at(0, ic, j);
   double F[4] = {0.0, 0.0, 0.0, 0.0};
   unsigned long long key = 0x7ff0000000000000ull;
-  if (face_lane && beta > 0.0) {
+  if (face_lane && CellCode::Face(code)) {
     T Ft[4];
     HllemFlux<2, 0, LARR, false, T, TA>(Ft, recRm, recL, gasT);
 #pragma unroll
@@ -1602,16 +1623,28 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   double Fr[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) Fr[k] = ShiftDown(F[k]);
-  if (wave_ok && lane >= 2 && lane < 2 + kKbnOwned && i >= 0 && i < nx)
-    KbnSweepUpdate<0>(in, out, g, c, F, Fr, DivRn(dt, dx));
+  if (wave_ok && lane >= 2 && lane < 2 + kKbnOwned && i >= 0 && i < nx) {
+    double u[4], res[4];
+    LoadCons(in, c, u);
+    if (KbnSweepUpdate<0>(g, code, u, F, Fr, DivRn(dt, dx), res)) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out.u[k][c] = res[k];
+    }
+  }
   const unsigned long long m = BlockMinKey(key);
   if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
-// The same step in Y: one thread per interior column marching a segment of
-// rows with a register window of three completed cells, the previous
-// cell's right half-step state and the previous face's flux.
-constexpr int kKbnSeg = 16;
+// The same step in Y through an LDS transpose.  A workgroup stages a tile of
+// kYtCols columns by 64 rows (rows j0 - 2 .. j0 + 61, clamped into the ghost
+// ring) with coalesced loads: the conservative state and the cell words.
+// Each wave then takes one column at a time with lane l on row j0 - 2 + l
+// and runs the X kernel's lane-parallel step along it (neighbours by DPP,
+// lanes 2 .. 61 own their rows), writes the updates back into its column of
+// the tile, and the workgroup stores the owned rows with coalesced stores.
+constexpr int kYtCols = 16;   // columns per tile (one 128-byte row segment)
+constexpr int kYtPitch = 68;  // LDS doubles per tile column (64 rows + padding:
+                              // row-wise staging writes are 2-way conflicted at most)
 
 template <int LIM, bool LARR, class T, bool FAST>
 __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelState out,
@@ -1621,69 +1654,103 @@ __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelSt
                                                           unsigned long long* partials,
                                                           const int* halt) {
   using TA = typename Policies<T, FAST>::TA;
+  __shared__ double s_u[4][kYtCols][kYtPitch];
+  __shared__ int32_t s_code[kYtCols][kYtPitch];
   if (*halt) return;
   const Gas<T> gasT = GasAs<T>(gas);
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const bool col_ok = i < nx;
-  const int64_t ic = col_ok ? i : nx - 1;
-  const int64_t j0 = static_cast<int64_t>(blockIdx.y) * kKbnSeg;
-  const int64_t j1 = j0 + kKbnSeg < ny ? j0 + kKbnSeg : ny;
+  // Tiles in x-fastest order; consecutive tiles (which share cache lines at
+  // their edges) go to the same XCD (workgroup b runs on XCD b % 8).
+  const int64_t ncols = (nx + kYtCols - 1) / kYtCols;
+  const int64_t ntiles = ncols * ((ny + kKbnOwned - 1) / kKbnOwned);
+  const int64_t nt8 = ntiles / 8 * 8;
+  const int64_t b = blockIdx.x;
+  const int64_t tile = b < nt8 ? (b % 8) * (nt8 / 8) + b / 8 : b;
+  const int64_t i0 = (tile % ncols) * kYtCols;
+  const int64_t j0 = (tile / ncols) * kKbnOwned;
+  for (int e = threadIdx.x; e < 64 * kYtCols; e += kBlock) {
+    const int r = e / kYtCols, cc = e % kYtCols;
+    const int64_t jr = j0 - 2 + r;
+    const int64_t jj = jr < ny + 1 ? jr : ny + 1;
+    const int64_t ii = i0 + cc < nx ? i0 + cc : nx - 1;
+    const int64_t c = g.cell(ii, jj);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s_u[k][cc][r] = in.u[k][c];
+    s_code[cc][r] = g.code[1][c];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t j = j0 - 2 + lane;  // the lane's row, -2 .. ny + 61
   const double dt = *dt_ptr;
   const double lh = DivRn(0.5 * dt, dx);
   const double dt_over_dx = DivRn(dt, dx);
-  double qm[6], q0[6], qp[6];
-  double am, a0, ap;
-  auto fetch = [&](int64_t jj, double* q, double& a) {
-    const int64_t c = g.cell(ic, jj);
-    CellComplete(in, g, c, gas, q);
-    a = g.alpha[c];
-  };
-  fetch(j0 - 2, qm, am);
-  fetch(j0 - 1, q0, a0);
-  fetch(j0, qp, ap);
-  T recRm[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-  double Fm[4] = {0.0, 0.0, 0.0, 0.0};
+  const bool face_row = lane >= 2 && lane <= 2 + kKbnOwned && j >= 0 && j <= ny;
+  const bool own_row = lane >= 2 && lane < 2 + kKbnOwned && j < ny;
   unsigned long long key = 0x7ff0000000000000ull;
-  for (int64_t t = j0 - 1; t <= j1; ++t) {
-    // q0 is cell t, qm cell t - 1, qp cell t + 1
+#pragma unroll 1
+  for (int cc = threadIdx.x >> 6; cc < kYtCols; cc += kBlock / 64) {
+    const bool col_ok = i0 + cc < nx;
+    const int32_t code = s_code[cc][lane];
+    double u[4], q[6];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = u[k] = s_u[k][cc][lane];
+    CompleteMasked<2>(q, CellCode::Pos(code), gas);
+    double qm[6], qp[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      qm[k] = ShiftUp(q[k]);
+      qp[k] = ShiftDown(q[k]);
+    }
+    const int code_m = DppUp(code), code_p = DppDown(code);
+    const bool mask = CellCode::Pos(code_m) & CellCode::Pos(code) & CellCode::Pos(code_p);
     T recL[6], recR[6];
     {
       T tm[4], t0[4], tp[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
         tm[k] = static_cast<T>(qm[k]);
-        t0[k] = static_cast<T>(q0[k]);
+        t0[k] = static_cast<T>(q[k]);
         tp[k] = static_cast<T>(qp[k]);
       }
-      MaskedRecon<2, 1, LIM, T, TA>(tm, t0, tp, am > 0.0 && a0 > 0.0 && ap > 0.0,
-                                    static_cast<T>(lh), gasT, recL, recR);
+      MaskedRecon<2, 1, LIM, T, TA>(tm, t0, tp, mask, static_cast<T>(lh), gasT, recL, recR);
     }
-    if (t >= j0) {
-      // face t between cells t - 1 and t
-      double F[4] = {0.0, 0.0, 0.0, 0.0};
-      if (g.beta_at(1, ic, t) > 0.0) {
-        T Ft[4];
-        HllemFlux<2, 1, LARR, false, T, TA>(Ft, recRm, recL, gasT);
+    T recRm[6];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) F[k] = static_cast<double>(Ft[k]);
-        if (col_ok) key = KeyMin(key, DtKey(HllemStableDt<2, 1>(qm, q0, dx, gas)));
+    for (int k = 0; k < 6; ++k) recRm[k] = ShiftUp(recR[k]);
+    // lower face of the lane's cell: faces 0 .. ny, lanes 2 .. 62
+    double F[4] = {0.0, 0.0, 0.0, 0.0};
+    if (face_row && CellCode::Face(code)) {
+      T Ft[4];
+      HllemFlux<2, 1, LARR, false, T, TA>(Ft, recRm, recL, gasT);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) F[k] = static_cast<double>(Ft[k]);
+      if (col_ok) key = KeyMin(key, DtKey(HllemStableDt<2, 1>(qm, q, dx, gas)));
+    }
+    double Fr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) Fr[k] = ShiftDown(F[k]);
+    // The column's tile entries belong to this wave alone: the update
+    // replaces the state it was read from.  Irregular cells keep it (the
+    // store below skips them).
+    if (own_row && col_ok) {
+      double res[4];
+      if (KbnSweepUpdate<1>(g, code, u, F, Fr, dt_over_dx, res)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_u[k][cc][lane] = res[k];
       }
-      if (t - 1 >= j0 && col_ok) KbnSweepUpdate<1>(in, out, g, g.cell(i, t - 1), Fm, F, dt_over_dx);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) Fm[k] = F[k];
     }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * kYtCols; e += kBlock) {
+    const int r = e / kYtCols, cc = e % kYtCols;
+    const int64_t jr = j0 - 2 + r, ii = i0 + cc;
+    if (r < 2 || r >= 2 + kKbnOwned || jr >= ny || ii >= nx) continue;
+    if (CellCode::Kind(s_code[cc][r]) >= CellCode::kSlot0) continue;
+    const int64_t c = g.cell(ii, jr);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      recRm[k] = recR[k];
-      qm[k] = q0[k];
-      q0[k] = qp[k];
-    }
-    am = a0;
-    a0 = ap;
-    if (t + 2 <= j1 + 1) fetch(t + 2, qp, ap);
+    for (int k = 0; k < 4; ++k) out.u[k][c] = s_u[k][cc][r];
   }
   const unsigned long long m = BlockMinKey(key);
-  if (threadIdx.x == 0) partials[static_cast<int64_t>(blockIdx.y) * gridDim.x + blockIdx.x] = m;
+  if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
 // The h-box update (KbnUpdateCell) of the irregular cells of direction DIR,
@@ -1719,7 +1786,7 @@ int64_t KbnSweepBlocks(int dir, int64_t nx, int64_t ny) {
     const int64_t waves = ((nx + kKbnOwned - 1) / kKbnOwned) * ny;
     return (waves + kBlock / 64 - 1) / (kBlock / 64);
   }
-  return ((nx + kBlock - 1) / kBlock) * ((ny + kKbnSeg - 1) / kKbnSeg);
+  return ((nx + kYtCols - 1) / kYtCols) * ((ny + kKbnOwned - 1) / kKbnOwned);
 }
 
 // Masked reconstruction of the interior at set_state: covered cells and
@@ -1761,7 +1828,7 @@ struct fvs_kbn_level {
   int64_t nblocks;
   Ctl* ctl;
   int cur;
-  int32_t* irr_map[2];    // LevelGeom::irr
+  int32_t* code[2];       // LevelGeom::code
   int32_t* irr_cells[2];  // cell index of each irregular slot
   double* irr_flux[2];
   int nirr[2];
@@ -1865,7 +1932,8 @@ int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
   L->lg.centroids = gc;
   L->lg.ngx = ngx;
   L->lg.ngy = ngy;
-  // regular / covered / irregular map of the interior cells per direction
+  // geometry words of every cell (CellCode) and the irregular cells per
+  // direction
   {
     std::vector<double> ha(ncg), hbx(nbx), hby(nby);
     if (hipMemcpyAsync(ha.data(), ga, ncg * sizeof(double), hipMemcpyDeviceToHost, s) !=
@@ -1877,38 +1945,45 @@ int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
         hipStreamSynchronize(s) != hipSuccess)
       return fail(FVS_HIP_ERROR);
     for (int d = 0; d < 2; ++d) {
-      std::vector<int32_t> map(ncg, -2), cells;
-      for (int64_t j = 0; j < ny; ++j)
-        for (int64_t i = 0; i < nx; ++i) {
-          const int64_t c = (j + kG) * ngx + (i + kG);
+      // fraction of face f (ghosted index) of the row or column through cell (ci, cj)
+      auto beta = [&](int64_t ci, int64_t cj) {
+        return d == 0 ? hbx[cj * (ngx + 1) + ci] : hby[cj * ngx + ci];
+      };
+      std::vector<int32_t> code(ncg), cells;
+      for (int64_t cj = 0; cj < ngy; ++cj)
+        for (int64_t ci = 0; ci < ngx; ++ci) {
+          const int64_t c = cj * ngx + ci;
           const double a = ha[c];
-          if (a == 0.0) continue;
-          // face fractions at faces i - 1 .. i + 2 (or j - 1 .. j + 2)
-          bool regular = a == 1.0;
-          for (int64_t f = -1; f <= 2 && regular; ++f)
-            regular = (d == 0 ? hbx[(j + kG) * (ngx + 1) + (i + f + kG)]
-                              : hby[(j + f + kG) * ngx + (i + kG)]) == 1.0;
-          if (regular) {
-            map[c] = -1;
-          } else {
-            map[c] = static_cast<int32_t>(cells.size());
-            cells.push_back(static_cast<int32_t>(c));
+          int32_t kind = CellCode::kCovered;
+          const bool interior = ci >= kG && ci < nx + kG && cj >= kG && cj < ny + kG;
+          if (interior && a != 0.0) {
+            // face fractions at faces i - 1 .. i + 2 (or j - 1 .. j + 2)
+            bool regular = a == 1.0;
+            for (int64_t f = -1; f <= 2 && regular; ++f)
+              regular = (d == 0 ? beta(ci + f, cj) : beta(ci, cj + f)) == 1.0;
+            if (regular) {
+              kind = CellCode::kRegular;
+            } else {
+              kind = CellCode::kSlot0 + static_cast<int32_t>(cells.size());
+              cells.push_back(static_cast<int32_t>(c));
+            }
           }
+          code[c] = CellCode::Make(a > 0.0, beta(ci, cj) > 0.0, kind);
         }
       L->nirr[d] = static_cast<int>(cells.size());
       const size_t ncells = cells.empty() ? 1 : cells.size();
-      if (hipMalloc(&L->irr_map[d], ncg * sizeof(int32_t)) != hipSuccess ||
+      if (hipMalloc(&L->code[d], ncg * sizeof(int32_t)) != hipSuccess ||
           hipMalloc(&L->irr_cells[d], ncells * sizeof(int32_t)) != hipSuccess ||
           hipMalloc(&L->irr_flux[d], 8 * ncells * sizeof(double)) != hipSuccess)
         return fail(FVS_HIP_ERROR);
-      if (hipMemcpyAsync(L->irr_map[d], map.data(), ncg * sizeof(int32_t), hipMemcpyHostToDevice,
+      if (hipMemcpyAsync(L->code[d], code.data(), ncg * sizeof(int32_t), hipMemcpyHostToDevice,
                          s) != hipSuccess ||
           (!cells.empty() &&
            hipMemcpyAsync(L->irr_cells[d], cells.data(), cells.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice, s) != hipSuccess) ||
           hipStreamSynchronize(s) != hipSuccess)
         return fail(FVS_HIP_ERROR);
-      L->lg.irr[d] = L->irr_map[d];
+      L->lg.code[d] = L->code[d];
       L->lg.irr_flux[d] = L->irr_flux[d];
     }
   }
@@ -1958,7 +2033,7 @@ int fvs_kbn_level_create(fvs_handle* h, const fvs_patch_desc* patch,
         return fail(FVS_HIP_ERROR);
   L->nblocks = (std::max((nx + 1) * ny, nx * (ny + 1)) + kBlock - 1) / kBlock;
   const int64_t npart = std::max({2 * L->nblocks, KbnSweepBlocks(0, nx, ny), KbnSweepBlocks(1, nx, ny)});
-  if (npart > 0x7fffffff || (ny + kKbnSeg - 1) / kKbnSeg > 65535) {
+  if (npart > 0x7fffffff) {
     SetError("cut-cell patch too large for the sweep grid");
     return fail(FVS_BAD_ARGUMENT);
   }
@@ -1980,8 +2055,8 @@ int fvs_kbn_level_destroy(fvs_kbn_level* L) {
     for (int k = 0; k < 4; ++k)
       if (L->buf[b][k]) (void)hipFree(L->buf[b][k]);
   void* ps[] = {L->geom,         L->cut,          L->refs,         L->fb,
-                L->F,            L->partials,     L->ctl,          L->irr_map[0],
-                L->irr_map[1],   L->irr_cells[0], L->irr_cells[1], L->irr_flux[0],
+                L->F,            L->partials,     L->ctl,          L->code[0],
+                L->code[1],      L->irr_cells[0], L->irr_cells[1], L->irr_flux[0],
                 L->irr_flux[1]};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -2069,9 +2144,7 @@ static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
   const bool larr = h->desc.base == FVS_BASE_HLLEM_LARROUTUROU;
   const double* dtp = &L->ctl->split_dt;
   const int64_t nb = KbnSweepBlocks(dir, L->nx, L->ny);
-  const dim3 grid = dir == 0 ? dim3(static_cast<unsigned>(nb))
-                             : dim3(static_cast<unsigned>((L->nx + kBlock - 1) / kBlock),
-                                    static_cast<unsigned>((L->ny + kKbnSeg - 1) / kKbnSeg));
+  const dim3 grid(static_cast<unsigned>(nb));
   const bool fp32 = h->desc.precision == FVS_FP32_FLUX;
 #define FVS_KS(K, LM, LA)                                                                     \
   if (fp32)                                                                                   \
