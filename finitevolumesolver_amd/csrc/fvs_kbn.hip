@@ -1198,6 +1198,26 @@ struct LevelGeom {
   }
 };
 
+// Geometry word of a cell for the fused sweep of one direction:
+//   bit 0: volume fraction > 0 (the reconstruction mask),
+//   bit 1: fraction of the cell's lower face in the direction > 0 (the face
+//          carries a regular flux),
+//   bits 2..: 0 covered (volume fraction 0: the update is the zero state),
+//          1 regular (volume fraction 1 and the four face fractions around
+//          the cell along the direction 1: the plain update), or 2 + the
+//          cell's slot in the direction's list of irregular cells, whose two
+//          face fluxes the sweep leaves in irr_flux[slot * 8 ..] for
+//          KbnIrregularKernel.
+struct CellCode {
+  static constexpr int32_t kCovered = 0, kRegular = 1, kSlot0 = 2;
+  __host__ __device__ static int32_t Make(bool pos, bool face, int32_t kind) {
+    return (pos ? 1 : 0) | (face ? 2 : 0) | (kind << 2);
+  }
+  __device__ static bool Pos(int32_t c) { return (c & 1) != 0; }
+  __device__ static bool Face(int32_t c) { return (c & 2) != 0; }
+  __device__ static int32_t Kind(int32_t c) { return c >> 2; }
+};
+
 struct LevelState {
   double* u[4];
 };
@@ -1302,17 +1322,26 @@ __global__ __launch_bounds__(kBlock) void LevelDt2Kernel(LevelState st, LevelGeo
     const int64_t j1 = j0 + kDt2Seg < ny + 1 ? j0 + kDt2Seg : ny + 1;
     const bool col_ok = i <= nx;
     const int64_t ic = col_ok ? i : nx;
+    // The cell words carry the volume mask (bit 0 of either direction's)
+    // and the fractions of the cell's lower x and y faces.
     double prev[6];
-    CellComplete(st, g, g.cell(ic, j0 - 1), gas, prev);
+    {
+      const int64_t c = g.cell(ic, j0 - 1);
+      LoadCons(st, c, prev);
+      CompleteMasked<2>(prev, CellCode::Pos(g.code[0][c]), gas);
+    }
     for (int64_t j = j0; j < j1; ++j) {
+      const int64_t c = g.cell(ic, j);
+      const int32_t cx = g.code[0][c], cy = g.code[1][c];
       double cur[6];
-      CellComplete(st, g, g.cell(ic, j), gas, cur);
+      LoadCons(st, c, cur);
+      CompleteMasked<2>(cur, CellCode::Pos(cx), gas);
 #pragma unroll
       for (int k = 0; k < 6; ++k) s_q[k][t] = cur[k];
       __syncthreads();
       if (t >= 1 && col_ok) {
         // x face i between cells i - 1 and i (rows 0 .. ny - 1, faces 0 .. nx)
-        if (j < ny && i >= 0 && g.beta_at(0, i, j) > 0.0) {
+        if (j < ny && i >= 0 && CellCode::Face(cx)) {
           double L[6];
 #pragma unroll
           for (int k = 0; k < 6; ++k) L[k] = s_q[k][t - 1];
@@ -1322,8 +1351,7 @@ __global__ __launch_bounds__(kBlock) void LevelDt2Kernel(LevelState st, LevelGeo
         }
         // y face j between cells (i, j - 1) and (i, j) (columns 0 .. nx - 1)
         if (i >= 0 && i < nx) {
-          const double v = g.beta_at(1, i, j) > 0.0 ? HllemStableDt<2, 1>(prev, cur, dx1, gas)
-                                                    : DBL_MAX;
+          const double v = CellCode::Face(cy) ? HllemStableDt<2, 1>(prev, cur, dx1, gas) : DBL_MAX;
           ky = KeyMin(ky, DtKey(v));
         }
       }
@@ -1406,26 +1434,6 @@ __device__ __forceinline__ FaceFrac Fractions(double bl, double bm, double br) {
   f.sr_rel = DivRn(f.sr, b);
   return f;
 }
-
-// Geometry word of a cell for the fused sweep of one direction:
-//   bit 0: volume fraction > 0 (the reconstruction mask),
-//   bit 1: fraction of the cell's lower face in the direction > 0 (the face
-//          carries a regular flux),
-//   bits 2..: 0 covered (volume fraction 0: the update is the zero state),
-//          1 regular (volume fraction 1 and the four face fractions around
-//          the cell along the direction 1: the plain update), or 2 + the
-//          cell's slot in the direction's list of irregular cells, whose two
-//          face fluxes the sweep leaves in irr_flux[slot * 8 ..] for
-//          KbnIrregularKernel.
-struct CellCode {
-  static constexpr int32_t kCovered = 0, kRegular = 1, kSlot0 = 2;
-  __host__ __device__ static int32_t Make(bool pos, bool face, int32_t kind) {
-    return (pos ? 1 : 0) | (face ? 2 : 0) | (kind << 2);
-  }
-  __device__ static bool Pos(int32_t c) { return (c & 1) != 0; }
-  __device__ static bool Face(int32_t c) { return (c & 2) != 0; }
-  __device__ static int32_t Kind(int32_t c) { return c >> 2; }
-};
 
 // A regular cell away from the boundary: both faces are unshielded
 // (fractions 1, 0, 0), no boundary flux enters, and the general formula of
@@ -2245,10 +2253,12 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
       }
       const int64_t dt2_tiles = ((L->nx + 1 + kBlock - 2) / (kBlock - 1)) *
                                 ((L->ny + 1 + kDt2Seg - 1) / kDt2Seg);
+      int64_t npart = L->nblocks;  // partial minima per direction
       if (dt2_tiles <= L->nblocks) {
-        hipLaunchKernelGGL(LevelDt2Kernel, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
+        npart = dt2_tiles;
+        hipLaunchKernelGGL(LevelDt2Kernel, dim3(unsigned(dt2_tiles)), dim3(kBlock), 0, s, st0,
                            L->lg, L->nx, L->ny, L->patch.dx[0], L->patch.dx[1], gas, L->partials,
-                           L->nblocks, halt);
+                           dt2_tiles, halt);
       } else {  // very thin levels: one pass per direction
         hipLaunchKernelGGL(LevelDtKernel<0>, dim3(unsigned(L->nblocks)), dim3(kBlock), 0, s, st0,
                            L->lg, L->nx, L->ny, L->patch.dx[0], gas, L->partials, halt);
@@ -2256,7 +2266,7 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
                            L->lg, L->nx, L->ny, L->patch.dx[1], gas, L->partials + L->nblocks,
                            halt);
       }
-      rc = LaunchFinalizeCfl(s, L->partials, L->nblocks, 2, nullptr, L->ctl, seq[0], strang);
+      rc = LaunchFinalizeCfl(s, L->partials, npart, 2, nullptr, L->ctl, seq[0], strang);
       if (rc) break;
       if (L->ncut > 0) {
         hipLaunchKernelGGL(LevelRefsKernel, dim3(Blocks(L->ncut)), dim3(kBlock), 0, s, st0, L->lg,
