@@ -1166,8 +1166,10 @@ int fvs_kbn_complete_from_cons(fvs_handle* h, const fvs_view* complete, const fv
 // and masked reconstruction).  The state is stored as conservative
 // variables with two ghost layers; complete states are recomputed from them
 // with the masked reconstruction wherever a kernel needs them (the values
-// the reference stores).  Kernels per split step:
-//   ghost fill (transmissive, along the sweep direction)
+// the reference stores).  The transmissive boundary fill is implicit: a
+// kernel reading a ghost cell's state reads the nearest interior cell's
+// (LevelGeom::state_cell), which is what the fill would have copied there,
+// while the ghost cell's geometry stays its own.  Kernels per split step:
 //   boundary fluxes over the list of cut cells (a dense zero-initialised
 //     array; only cut cells are ever written)
 //   regular fluxes + masked CFL of every face, with per-workgroup minima
@@ -1190,6 +1192,14 @@ struct LevelGeom {
   int64_t ngx, ngy;
   __device__ __forceinline__ int64_t cell(int64_t i, int64_t j) const {
     return (j + kG) * ngx + (i + kG);
+  }
+  // Cell holding the state of cell (i, j) under the transmissive boundary:
+  // (i, j) clamped into the interior.
+  __device__ __forceinline__ int64_t state_cell(int64_t i, int64_t j) const {
+    const int64_t nx = ngx - 2 * kG, ny = ngy - 2 * kG;
+    i = i < 0 ? 0 : (i < nx ? i : nx - 1);
+    j = j < 0 ? 0 : (j < ny ? j : ny - 1);
+    return cell(i, j);
   }
   // face f of direction d at transverse index t (interior coordinates)
   __device__ __forceinline__ double beta_at(int d, int64_t i, int64_t j) const {
@@ -1227,27 +1237,6 @@ __device__ __forceinline__ void LoadCons(const LevelState& s, int64_t c, double*
   for (int k = 0; k < 4; ++k) q[k] = s.u[k][c];
 }
 
-__global__ void GhostFillKernel(LevelState st, int64_t nx, int64_t ny, int64_t ngx, int dir) {
-  // one thread per (ghost layer pair, transverse grown index)
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t nt = dir == 0 ? ny + 2 * kG : nx + 2 * kG;
-  if (q >= nt * 2 * kG) return;
-  const int64_t t = q % nt - kG;
-  const int g = static_cast<int>(q / nt);  // 0..2G-1
-  const int64_t n = dir == 0 ? nx : ny;
-  const int64_t dst = g < kG ? -(g + 1) : n + (g - kG);
-  const int64_t src = g < kG ? 0 : n - 1;
-  int64_t di, dj, si, sj;
-  if (dir == 0) {
-    di = dst, si = src, dj = sj = t;
-  } else {
-    dj = dst, sj = src, di = si = t;
-  }
-  const int64_t cd = (dj + kG) * ngx + (di + kG);
-  const int64_t cs = (sj + kG) * ngx + (si + kG);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) st.u[k][cd] = st.u[k][cs];
-}
 
 __device__ __forceinline__ unsigned long long BlockMinKey(unsigned long long k) {
   __shared__ unsigned long long s[kBlock / 64];
@@ -1259,11 +1248,20 @@ __device__ __forceinline__ unsigned long long BlockMinKey(unsigned long long k) 
   return m;
 }
 
-// Complete state of interior/ghost cell c with the volume mask.
+// Complete state of interior cell c with the volume mask.
 __device__ __forceinline__ void CellComplete(const LevelState& st, const LevelGeom& g, int64_t c,
                                              const Gas<double>& gas, double* q) {
   LoadCons(st, c, q);
   CompleteMasked<2>(q, g.alpha[c] > 0.0, gas);
+}
+
+// Complete state of cell (i, j), a ghost cell included (transmissive
+// state, the cell's own volume mask).
+__device__ __forceinline__ void CellCompleteAt(const LevelState& st, const LevelGeom& g,
+                                               int64_t i, int64_t j, const Gas<double>& gas,
+                                               double* q) {
+  LoadCons(st, g.state_cell(i, j), q);
+  CompleteMasked<2>(q, g.alpha[g.cell(i, j)] > 0.0, gas);
 }
 
 // Masked HLLEM CFL of the faces 0..n of direction DIR (interior rows); the
@@ -1283,8 +1281,8 @@ __global__ __launch_bounds__(kBlock) void LevelDtKernel(LevelState st, LevelGeom
     double dt = DBL_MAX;
     if (g.beta_at(DIR, i, j) > 0.0) {
       double L[6], R[6];
-      CellComplete(st, g, g.cell(i - (DIR == 0), j - (DIR == 1)), gas, L);
-      CellComplete(st, g, g.cell(i, j), gas, R);
+      CellCompleteAt(st, g, i - (DIR == 0), j - (DIR == 1), gas, L);
+      CellCompleteAt(st, g, i, j, gas, R);
       dt = HllemStableDt<2, DIR>(L, R, dx, gas);
     }
     key = DtKey(dt);
@@ -1300,7 +1298,7 @@ __global__ __launch_bounds__(kBlock) void LevelDtKernel(LevelState st, LevelGeom
 // face below with the previous row kept in registers.  Face values are
 // those of LevelDtKernel (DBL_MAX where beta is zero); the per-direction
 // minima go to partials[b] (x) and partials[nblocks + b] (y); blocks beyond
-// the tiles write +inf.  Ghost rows and columns must be filled.
+// the tiles write +inf.
 constexpr int kDt2Seg = 8;
 
 __global__ __launch_bounds__(kBlock) void LevelDt2Kernel(LevelState st, LevelGeom g, int64_t nx,
@@ -1326,15 +1324,14 @@ __global__ __launch_bounds__(kBlock) void LevelDt2Kernel(LevelState st, LevelGeo
     // and the fractions of the cell's lower x and y faces.
     double prev[6];
     {
-      const int64_t c = g.cell(ic, j0 - 1);
-      LoadCons(st, c, prev);
-      CompleteMasked<2>(prev, CellCode::Pos(g.code[0][c]), gas);
+      LoadCons(st, g.state_cell(ic, j0 - 1), prev);
+      CompleteMasked<2>(prev, CellCode::Pos(g.code[0][g.cell(ic, j0 - 1)]), gas);
     }
     for (int64_t j = j0; j < j1; ++j) {
       const int64_t c = g.cell(ic, j);
       const int32_t cx = g.code[0][c], cy = g.code[1][c];
       double cur[6];
-      LoadCons(st, c, cur);
+      LoadCons(st, g.state_cell(ic, j), cur);
       CompleteMasked<2>(cur, CellCode::Pos(cx), gas);
 #pragma unroll
       for (int k = 0; k < 6; ++k) s_q[k][t] = cur[k];
@@ -1553,7 +1550,7 @@ __device__ __forceinline__ void KbnUpdateCell(const LevelState& in, const LevelG
 // consecutive cells of one row, lanes 2..61 own theirs; each lane completes
 // and reconstructs its cell once, neighbours come from DPP lane shifts, the
 // flux of a lane's left face is shifted to its left neighbour as that one's
-// right-face flux.  Ghost columns (two per side) are filled beforehand.
+// right-face flux.  Ghost columns read the transmissive state.
 constexpr int kKbnOwned = 60;
 
 // Flux arithmetic of the fused cut-cell sweeps: T = double with IEEE
@@ -1589,7 +1586,7 @@ __global__ __launch_bounds__(kBlock) void KbnSweepXKernel(LevelState in, LevelSt
   const int64_t c = g.cell(ic, j);
   const int32_t code = g.code[0][c];
   double q[6];
-  LoadCons(in, c, q);
+  LoadCons(in, g.state_cell(ic, j), q);
   CompleteMasked<2>(q, CellCode::Pos(code), gas);
   double qm[6], qp[6];
 #pragma unroll
@@ -1680,10 +1677,10 @@ __global__ __launch_bounds__(kBlock) void KbnSweepYKernel(LevelState in, LevelSt
     const int64_t jr = j0 - 2 + r;
     const int64_t jj = jr < ny + 1 ? jr : ny + 1;
     const int64_t ii = i0 + cc < nx ? i0 + cc : nx - 1;
-    const int64_t c = g.cell(ii, jj);
+    const int64_t cs = g.state_cell(ii, jj);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) s_u[k][cc][r] = in.u[k][c];
-    s_code[cc][r] = g.code[1][c];
+    for (int k = 0; k < 4; ++k) s_u[k][cc][r] = in.u[k][cs];
+    s_code[cc][r] = g.code[1][g.cell(ii, jj)];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -2138,10 +2135,6 @@ static int KbnSweep(fvs_kbn_level* L, int dir, int src, int dst, bool check) {
   gas.gamma = h->desc.gamma;
   gas.gm1inv = 1.0 / (h->desc.gamma - 1.0);
   gas.gm1 = h->desc.gamma - 1.0;
-  const int64_t nt = dir == 0 ? L->ny + 2 * kG : L->nx + 2 * kG;
-  hipLaunchKernelGGL(GhostFillKernel, dim3(Blocks(nt * 2 * kG)), dim3(kBlock), 0, s, in, L->nx,
-                     L->ny, L->ngx, dir);
-  FVS_LAUNCH_CHECK();
   const BoundaryParams bp = BoundaryOf(h->desc);
   if (L->ncut > 0) {
     hipLaunchKernelGGL(LevelBoundaryFluxKernel, dim3(Blocks(L->ncut)), dim3(kBlock), 0, s, in,
@@ -2244,13 +2237,8 @@ int fvs_kbn_level_advance(fvs_kbn_level* L, int max_steps, double final_time, in
         seq[ns++] = dirs[1];
         seq[ns++] = dirs[0];
       }
-      // step-start CFL of both directions (ghosts of each direction filled)
+      // step-start CFL of both directions
       const LevelState st0 = StateOf(L, cur);
-      for (int d = 0; d < 2; ++d) {
-        const int64_t nt = d == 0 ? L->ny + 2 * kG : L->nx + 2 * kG;
-        hipLaunchKernelGGL(GhostFillKernel, dim3(Blocks(nt * 2 * kG)), dim3(kBlock), 0, s, st0,
-                           L->nx, L->ny, L->ngx, d);
-      }
       const int64_t dt2_tiles = ((L->nx + 1 + kBlock - 2) / (kBlock - 1)) *
                                 ((L->ny + 1 + kDt2Seg - 1) / kDt2Seg);
       int64_t npart = L->nblocks;  // partial minima per direction
