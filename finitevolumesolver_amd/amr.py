@@ -221,6 +221,10 @@ class AmrHierarchy:
         L.check(self._lib.fvs_amr_set_cycle(self.a, int(value)))
 
     def level_cfl(self) -> np.ndarray:
+        """Per level, the CFL check value of the last advance's sweeps: the
+        minimum stable dt where it lies below the level's time step (the
+        reference's TimeStepTooLarge; run() retries with it), otherwise a
+        value of at least that time step."""
         out = np.zeros(self.nlevels)
         L.check(self._lib.fvs_amr_level_cfl(self.a, out.ctypes.data))
         return out

@@ -95,7 +95,9 @@ class PatchBatch:
         L.check(self._lib.fvs_batch_replay(self.b, steps))
 
     def cfl(self) -> np.ndarray:
-        """CFL minima of the last replayed step's X and Y sweeps."""
+        """CFL check values of the last replayed step's X and Y sweeps (the
+        minimum stable dt where it lies below the time step, otherwise a
+        value of at least the time step)."""
         out = np.zeros(2)
         L.check(self._lib.fvs_batch_read_cfl(self.b, out.ctypes.data))
         return out

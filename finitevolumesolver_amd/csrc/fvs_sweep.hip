@@ -170,52 +170,6 @@ __device__ __forceinline__ double FaceSpeed(const DtCell<D>& L, const DtCell<D>&
   return A::max(rabs(sL), rabs(sR));
 }
 
-// Sufficient test of a sweep's CFL check (the level driver needs only its
-// outcome while it passes).  With w = sqrt(rho_L) / (sqrt(rho_L) +
-// sqrt(rho_R)) and h = a^2 / (g - 1) + |u|^2 / 2, the Roe sound speed of
-// FaceSpeed satisfies a~^2 = w aL^2 + (1 - w) aR^2 + (g - 1) / 2 w (1 - w)
-// |uR - uL|^2 <= max(aL^2, aR^2) + (g - 1) / 8 |uR - uL|^2, and |u~_n| <=
-// max(|uL_n|, |uR_n|), so the face speed is at most U + sqrt(A^2 + k |du|^2)
-// (U the larger normal speed, A^2 the larger squared sound speed, k = (g -
-// 1) / 8).  A face passes when that bound stays below dx / dt by a relative
-// margin of 2^-20, far above the rounding of either side; then its stable
-// dt, dx / speed, is at least dt and the exact check passes too.  A wave
-// evaluates the exact speeds when any of its faces does not pass the bound,
-// so a failing check reports the exact minimum dt as before.  The bound
-// uses the refined hardware reciprocal in every arithmetic mode.
-template <int D> struct BoundCell {
-  double u[D];
-  double a2;  // squared speed of sound, gamma p / rho
-};
-
-template <int D>
-__device__ __forceinline__ BoundCell<D> MakeBoundCell(const double* q, const Gas<double>& g) {
-  using FA = FastArith<double>;
-  const FA::Inv ir = FA::inv(q[0]);
-  BoundCell<D> b;
-#pragma unroll
-  for (int i = 0; i < D; ++i) b.u[i] = FA::div(q[1 + i], ir);
-  const double e_kin = FA::div(0.5 * SqNormArray<D>(q + 1), ir);
-  const double p = (q[D + 1] - e_kin) * g.gm1;
-  b.a2 = FA::div(g.gamma * p, ir);
-  return b;
-}
-
-template <int D, int DIR>
-__device__ __forceinline__ bool BoundPasses(const BoundCell<D>& L, const BoundCell<D>& R,
-                                            double s_crit, double k) {
-  const double U = __builtin_fmax(rabs(L.u[DIR]), rabs(R.u[DIR]));
-  const double T = s_crit - U;
-  double du2 = 0.0;
-#pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const double d = R.u[i] - L.u[i];
-    du2 += d * d;
-  }
-  const double q = __builtin_fmax(L.a2, R.a2) + k * du2;
-  return T >= 0.0 && L.a2 >= 0.0 && R.a2 >= 0.0 && q <= T * T;
-}
-
 template <int D>
 __device__ __forceinline__ BoundCell<D> ShflUp(const BoundCell<D>& v) {
   BoundCell<D> r;
@@ -225,10 +179,6 @@ __device__ __forceinline__ BoundCell<D> ShflUp(const BoundCell<D>& v) {
   return r;
 }
 
-// dx / dt shrunk by the bound's margin (+inf for dt = 0: every face passes).
-__device__ __forceinline__ double BoundSpeedLimit(double dx, double dt) {
-  return (dx / dt) * (1.0 - 0x1p-20);
-}
 
 template <int D>
 __device__ __forceinline__ DtCell<D> ShflUp(const DtCell<D>& v) {

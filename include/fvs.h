@@ -563,8 +563,12 @@ int fvs_batch_set_dt(fvs_batch* batch, double dt);
 /* Transmissive ghost layers of every patch of pool `which`. */
 int fvs_batch_fill_ghosts(fvs_batch* batch, int which);
 /* One sweep of direction dir from pool src into pool 1 - src (interior
- * cells); the minimum HLLEM stable dt of all patch faces of the sweep
- * (nullable) is written to *dt_device. */
+ * cells); the sweep's CFL check value (nullable) is written to *dt_device:
+ * with the time step 0 the minimum HLLEM stable dt of all patch faces; with
+ * a positive time step that minimum wherever it lies below the time step
+ * (the check fails), and otherwise a value of at least the time step (faces
+ * whose certified speed bound proves their stable dt above it are not
+ * evaluated exactly). */
 int fvs_batch_sweep(fvs_batch* batch, int dir, int src, double* dt_device);
 /* Autotuner: times every candidate tile shape of direction dir over `reps`
  * sweeps and keeps the fastest (shape index, per-shape ms; nullable). */
@@ -615,9 +619,11 @@ int fvs_amr_average_down(fvs_amr* amr, int fine_level);
  * refilled before each sweep; then reflux and average-down onto level l.
  * States live in pool 0 of each level between steps; pool 1 is scratch. */
 int fvs_amr_advance(fvs_amr* amr, double dt);
-/* Per level, the minimum stable dt over the sweeps of the last advance
- * (host, nlevels values; synchronises).  A level's value below its own
- * time step dt / ratio^l is the reference's TimeStepTooLarge. */
+/* Per level, the CFL check value of the sweeps of the last advance (host,
+ * nlevels values; synchronises): the minimum stable dt over the level's
+ * sweeps wherever it lies below the level's own time step dt / ratio^l
+ * (the reference's TimeStepTooLarge, whose retry uses that minimum), and
+ * otherwise a value of at least that time step (see fvs_batch_sweep). */
 int fvs_amr_level_cfl(fvs_amr* amr, double* host);
 /* Stable coarse dt of the current state: min over levels of ratio^l times
  * the level's minimum HLLEM stable dt over both directions
