@@ -23,7 +23,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["fvs_api.hip", "fvs_sweep.hip", "fvs_sweep_fast.hip", "fvs_godunov.hip", "fvs_mlp.hip",
-           "fvs_kbn.hip", "fvs_batch.hip", "fvs_batch_fast.hip", "fvs_amr.hip"]
+           "fvs_kbn.hip", "fvs_batch.hip", "fvs_batch_fast.hip", "fvs_amr.hip",
+           "fvs_level_composed.hip"]
 
 # Translation units that hold only the kernels of the non-IEEE arithmetic
 # modes (FVS_FP64_RCP, FVS_FP32_FLUX): there multiply-adds may be fused,

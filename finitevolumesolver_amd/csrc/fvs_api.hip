@@ -898,6 +898,7 @@ int fvs_newton_stats(fvs_handle* h, uint64_t stats[5], int reset) {
 struct fvs_level {
   fvs_handle* h;
   fvs_patch_desc patch;
+  fvs_composed_level* composed = nullptr;  // methods without a fused sweep
   int64_t n[3];
   int64_t ncell;
   double* block;          // 3 state buffers
@@ -993,15 +994,6 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
   *out = nullptr;
   const int D = h->desc.dim;
   const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;
-  if (godunov && h->desc.reconstruction != FVS_RECON_NONE) {
-    SetError("the fused level driver runs exact Godunov at first order");
-    return FVS_BAD_ARGUMENT;
-  }
-  if (h->desc.base == FVS_BASE_HLL_EINFELDT ||
-      h->desc.reconstruction >= FVS_RECON_MUSCL_HANCOCK_PRIM) {
-    SetError("the fused level driver has no HLL or MusclHancockPrim kernel (patch operators only)");
-    return FVS_BAD_ARGUMENT;
-  }
   int64_t n[3] = {patch->n[0], patch->n[1], D == 3 ? patch->n[2] : 1};
   for (int d = 0; d < D; ++d) {
     if (n[d] < 1 || !(patch->dx[d] > 0)) {
@@ -1020,6 +1012,18 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
   }
   L->h = h;
   L->patch = *patch;
+  if (ComposedLevelNeeded(h->desc)) {
+    // MUSCL-Hancock over exact Godunov, HLL, MusclHancockPrim and
+    // MusclHancockCharacteristic: the level composed from the patch operators
+    for (int d = 0; d < 3; ++d) L->n[d] = n[d];
+    L->ncell = n[0] * n[1] * n[2];
+    if (int rc = ComposedCreate(h, patch, &L->composed)) {
+      delete L;
+      return rc;
+    }
+    *out = L;
+    return FVS_OK;
+  }
   for (int d = 0; d < 3; ++d) L->n[d] = n[d];
   // The launch geometry queries the device's CU count once; do it here, not
   // inside a step capture.
@@ -1084,6 +1088,11 @@ int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch, fvs_level** out
 
 int fvs_level_destroy(fvs_level* L) {
   if (!L) return FVS_OK;
+  if (L->composed) {
+    ComposedDestroy(L->composed);
+    delete L;
+    return FVS_OK;
+  }
   (void)hipStreamSynchronize(L->h->stream);
   (void)hipFree(L->block);
   (void)hipFree(L->ctl);
@@ -1135,6 +1144,10 @@ int fvs_level_kernel_times(fvs_level* L, double* ms, int64_t* count) {
 
 double* fvs_level_state(fvs_level* L, int c) {
   if (!L || c < 0 || c >= L->h->desc.dim + 2) return nullptr;
+  if (L->composed) {
+    SetError("a composed level keeps its state inside a ghosted scratch (fvs_level_get_state)");
+    return nullptr;
+  }
   return L->buf[L->cur][c];
 }
 
@@ -1149,6 +1162,9 @@ int fvs_level_set_state(fvs_level* L, const double* const* comps, double t0) {
       SetError("null component pointer %d", c);
       return FVS_BAD_ARGUMENT;
     }
+  }
+  if (L->composed) return ComposedSetState(L->composed, comps, t0);
+  for (int c = 0; c < D + 2; ++c) {
     FVS_HIP_CHECK(hipMemcpyAsync(L->buf[L->cur][c], comps[c], L->ncell * sizeof(double),
                                  hipMemcpyDeviceToDevice, L->h->stream));
   }
@@ -1171,6 +1187,9 @@ int fvs_level_get_state(fvs_level* L, double* const* comps) {
       SetError("null component pointer %d", c);
       return FVS_BAD_ARGUMENT;
     }
+  }
+  if (L->composed) return ComposedGetState(L->composed, comps);
+  for (int c = 0; c < D + 2; ++c) {
     FVS_HIP_CHECK(hipMemcpyAsync(comps[c], L->buf[L->cur][c], L->ncell * sizeof(double),
                                  hipMemcpyDeviceToDevice, L->h->stream));
   }
@@ -1181,6 +1200,10 @@ int fvs_level_time(fvs_level* L, double* t, int64_t* cycles) {
   if (!L) {
     SetError("null level");
     return FVS_BAD_ARGUMENT;
+  }
+  if (L->composed) {
+    ComposedTime(L->composed, t, cycles);
+    return FVS_OK;
   }
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, L->h->stream));
@@ -1202,6 +1225,7 @@ int fvs_level_hint_max_speed(fvs_level* L, const double speed[3]) {
     }
   }
   // the exact-Godunov passes and the 3D HLLEM pass have a floor
+  if (L->composed) return FVS_OK;
   if (L->h->desc.dim != 3 && L->h->desc.base != FVS_BASE_GODUNOV_EXACT) return FVS_OK;
   Ctl c;
   FVS_HIP_CHECK(hipMemcpyAsync(&c, L->ctl, sizeof c, hipMemcpyDeviceToHost, L->h->stream));
@@ -1391,6 +1415,8 @@ int fvs_level_advance(fvs_level* L, int max_steps, double final_time, int batch,
     SetError("bad argument");
     return FVS_BAD_ARGUMENT;
   }
+  if (L->composed)
+    return ComposedAdvance(L->composed, max_steps, final_time, dt_log, steps_done, failures);
   fvs_handle* h = L->h;
   hipStream_t s = h->stream;
   const bool godunov = h->desc.base == FVS_BASE_GODUNOV_EXACT;

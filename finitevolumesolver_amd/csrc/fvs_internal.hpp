@@ -28,6 +28,10 @@ namespace fvs {
 
 void SetError(const char* fmt, ...);
 
+// Single-patch level of the flux methods without a fused sweep, composed
+// from the patch operators (fvs_level_composed.hip).
+struct fvs_composed_level;
+
 #define FVS_HIP_CHECK(expr)                                                       \
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
@@ -203,5 +207,16 @@ int LaunchCheckValue(hipStream_t s, Ctl* ctl, const double* value);
 int LaunchKeysToDt(hipStream_t s, unsigned long long* keys, int n, double* out);
 int LaunchResetKeys(hipStream_t s, unsigned long long* keys, int n,
                     unsigned int* counters, int nc);
+
+// Composed level (fvs_level_composed.hip): the methods it serves, and the
+// level operations the fvs_level_* entry points forward to it.
+bool ComposedLevelNeeded(const fvs_method_desc& d);
+int ComposedCreate(fvs_handle* h, const fvs_patch_desc* patch, fvs_composed_level** out);
+void ComposedDestroy(fvs_composed_level* C);
+int ComposedSetState(fvs_composed_level* C, const double* const* comps, double t0);
+int ComposedGetState(fvs_composed_level* C, double* const* comps);
+void ComposedTime(const fvs_composed_level* C, double* t, int64_t* cycles);
+int ComposedAdvance(fvs_composed_level* C, int max_steps, double final_time, double* dt_log,
+                    int* steps_done, int* failures);
 
 } // namespace fvs

@@ -249,12 +249,21 @@ typedef struct fvs_patch_desc {
 typedef struct fvs_level fvs_level;
 
 /* Allocates the level's three state buffers and its device control block and
- * uploads nothing.  The state starts zeroed. */
+ * uploads nothing.  The state starts zeroed.  HLLEM (with MUSCL-Hancock or
+ * first order) and first-order exact Godunov run the fused sweep kernels.
+ * The other methods (MUSCL-Hancock over exact Godunov, the reference's
+ * default MusclHancockMethod; the HLL base; MusclHancockPrim;
+ * MusclHancockCharacteristic) get a level composed from the patch operators
+ * above on one ghosted scratch of complete states, in the same sequence and
+ * with the same results as the reference's level integrator; it synchronises
+ * with the host a few times per step, ignores batch, speed hints, graphs and
+ * timing, and has no contiguous state arrays (fvs_level_state returns NULL;
+ * use fvs_level_get_state). */
 int fvs_level_create(fvs_handle* h, const fvs_patch_desc* patch,
                      fvs_level** out);
 int fvs_level_destroy(fvs_level* level);
 /* Device pointer of component c of the current state (valid until the next
- * advance call). */
+ * advance call); NULL for a composed level. */
 double* fvs_level_state(fvs_level* level, int c);
 /* Copies D+2 component arrays (device memory, ncell each) into the current
  * state; resets time and cycle count to (t0, 0). */
