@@ -1,12 +1,14 @@
-"""Parity at the benchmark sizes themselves (two steps each), against the
-oracle run on the host cores:
+"""Parity at the benchmark sizes themselves, against the oracle run on the
+host cores:
 
-- the 3D 256^3 level (MUSCL-Hancock MinMod + HLLEM Larrouturou, Godunov
-  split, CFL 0.8, transmissive) in IEEE FP64: every dt and every state value
+- two steps of the 3D 256^3 level (MUSCL-Hancock MinMod + HLLEM
+  Larrouturou, Godunov split, CFL 0.8, transmissive) in IEEE FP64: every dt and every state value
   equal to the oracle's level driver bit for bit; the refined-reciprocal
   form within the 1e-12 bar per cell (1e-13 for dt);
-- the 2048 x 1024 cut-cell channel (cylinder and wedge, Mach 2 shock, KBN
-  h-box, HLLEM boundary solver): bit for bit.
+- two steps of the 2048 x 1024 cut-cell channel (cylinder and wedge, Mach 2 shock, KBN
+  h-box, HLLEM boundary solver): bit for bit;
+- one step of the 384^3 first-order exact-Godunov level on the smooth
+  Fourier-mode field: dt within 1e-13, states within the 1e-12 bar.
 
 The smaller-size tests cover the edge cases; these pin the production
 shapes (grid sizes, segment and tile counts, the cut-cell lists) directly."""
@@ -92,3 +94,22 @@ def test_2048x1024_cutcell_level_matches_oracle(cuda):
     assert dts == list(ref_dts)
     assert np.array_equal(got, ref)
     _note(f"  bit-exact, dts {dts}")
+
+
+def test_384_cubed_exact_godunov_step_matches_oracle(cuda):
+    import torch
+    from finitevolumesolver_amd.initial_data import fourier_modes_3d
+    cons, dx = fourier_modes_3d(384, seed0=11)
+    fm = fvs.FluxMethod(fvs.PerfectGas(dim=3), base="Godunov", reconstruction="None")
+    _note("384^3 exact Godunov, 1 step")
+    lvl = fvs.SinglePatchLevel(fm, (384, 384, 384), dx, splitting="Godunov", cfl=0.8)
+    lvl.set_state(torch.from_numpy(cons).to(cuda))
+    done, fails, dts = lvl.advance(1, batch=1, return_dts=True)
+    got = lvl.get_state().cpu().numpy()
+    del lvl
+    ref, ref_dts, ref_fails = O.level_run(fm._h.desc, cons, dx, 1)
+    assert done == 1 and fails == ref_fails == 0
+    assert abs(dts[0] - ref_dts[0]) <= 1e-13 * ref_dts[0]
+    ratio = tolerance_ratio(got, ref)
+    _note(f"  dt {dts[0]!r} (oracle {ref_dts[0]!r}), tolerance ratio {ratio:.3f}")
+    assert ratio <= 1.0
