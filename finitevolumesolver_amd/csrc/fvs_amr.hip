@@ -558,12 +558,15 @@ int SetDt(fvs_amr* A, int l, double dt) {
 // Register contributions of fine level f (fine = 1: its faces, taken from its
 // pool src) or of its coarser level (fine = 0, coarse pool src) for the
 // sweep of direction dir.
-int AddRegister(fvs_amr* A, int f, int dir, int fine, int src) {
+// Arguments of the register contributions of register f (between level f
+// and level f - 1) from a sweep in dir of the fine (fine = 1) or the coarse
+// level reading pool src; count 0 when the register has no such entries.
+batch::RegArgs RegisterArgs(fvs_amr* A, int f, int dir, int fine, int src) {
   fvs_amr::Level& LV = A->L[f];
   const int first = LV.group[2 * dir], count = LV.group[2 * dir + 2] - first;
-  if (count == 0) return FVS_OK;
+  batch::RegArgs a{};
+  if (count == 0) return a;
   fvs_batch* b = A->lv[fine ? f : f - 1];
-  batch::RegArgs a;
   a.faces = LV.reg;
   a.first = first;
   a.count = count;
@@ -579,9 +582,7 @@ int AddRegister(fvs_amr* A, int f, int dir, int fine, int src) {
   a.ratio = fine ? A->ratio : 1;
   a.fine = fine;
   a.sign = fine ? 1.0 : -1.0;
-  batch::LaunchRegFlux(A->s, dir, a, A->h->desc.precision);
-  FVS_LAUNCH_CHECK();
-  return FVS_OK;
+  return a;
 }
 
 int Reflux(fvs_amr* A, int f) {
@@ -635,11 +636,13 @@ int AdvanceLevel(fvs_amr* A, int l, int sub_first, std::vector<int>& slot) {
     if (w > 0) {
       if (int rc = ref ? FillPhysical(A, l, src, d) : FillGhosts(A, l, src)) return rc;
     }
-    if (finer) {
-      if (int rc = AddRegister(A, l + 1, d, 0, src)) return rc;
-    }
-    if (l > 0) {
-      if (int rc = AddRegister(A, l, d, 1, src)) return rc;
+    // the sweep's register contributions: to the finer level's register (as
+    // its coarse side) and to this level's own (as its fine side), one launch
+    {
+      const batch::RegArgs rc0 = finer ? RegisterArgs(A, l + 1, d, 0, src) : batch::RegArgs{};
+      const batch::RegArgs rc1 = l > 0 ? RegisterArgs(A, l, d, 1, src) : batch::RegArgs{};
+      batch::LaunchRegFlux(A->s, d, rc0, rc1, A->h->desc.precision);
+      FVS_LAUNCH_CHECK();
     }
     const bool grown = ref && w == 0;
     if ((grown ? A->lv[l]->grid_grown[d] : A->lv[l]->grid[d]) > static_cast<unsigned>(A->pcap)) {

@@ -269,10 +269,9 @@ void LaunchSweepShape(hipStream_t s, int dir, int shape, const BatchArgs& a, int
 // adjacent cells' MUSCL-Hancock half-step states, then HLLEM) from the same
 // pool and time step, so the register holds the fluxes the sweeps applied.
 template <int DIR, int LIM, class T, bool FAST>
-__global__ __launch_bounds__(256) void RegFluxKernel(batch::RegArgs a) {
+__device__ __forceinline__ void RegFluxEntry(const batch::RegArgs& a, int q0) {
   using TA = typename Policies<T, FAST>::TA;
   constexpr int NQ = 6, NC = 4;
-  const int q0 = blockIdx.x * blockDim.x + threadIdx.x;
   if (q0 >= a.count) return;
   const int q = a.first + q0;
   const batch::RegFace F = a.faces[q];
@@ -308,14 +307,28 @@ __global__ __launch_bounds__(256) void RegFluxKernel(batch::RegArgs a) {
   for (int k = 0; k < NC; ++k) a.reg[k * a.nfaces + q] = acc[k];
 }
 
+// Two independent register contributions of one sweep (different
+// registers) in one launch: the first nb0 workgroups serve a0, the rest a1.
+template <int DIR, int LIM, class T, bool FAST>
+__global__ __launch_bounds__(256) void RegFluxKernel(batch::RegArgs a0, batch::RegArgs a1,
+                                                     int nb0) {
+  const int b = blockIdx.x;
+  if (b < nb0) {
+    RegFluxEntry<DIR, LIM, T, FAST>(a0, b * 256 + static_cast<int>(threadIdx.x));
+  } else {
+    RegFluxEntry<DIR, LIM, T, FAST>(a1, (b - nb0) * 256 + static_cast<int>(threadIdx.x));
+  }
+}
+
 template <int DIR, class T, bool FAST>
-void LaunchRegFluxT(hipStream_t s, const batch::RegArgs& a) {
-  const dim3 grid((a.count + 255) / 256);
-  switch (a.limiter) {
-  case kMinMod: hipLaunchKernelGGL((RegFluxKernel<DIR, kMinMod, T, FAST>), grid, dim3(256), 0, s, a); break;
-  case kVanLeer: hipLaunchKernelGGL((RegFluxKernel<DIR, kVanLeer, T, FAST>), grid, dim3(256), 0, s, a); break;
-  case kNoLimiter: hipLaunchKernelGGL((RegFluxKernel<DIR, kNoLimiter, T, FAST>), grid, dim3(256), 0, s, a); break;
-  default: hipLaunchKernelGGL((RegFluxKernel<DIR, kNoGradient, T, FAST>), grid, dim3(256), 0, s, a); break;
+void LaunchRegFluxT(hipStream_t s, const batch::RegArgs& a0, const batch::RegArgs& a1) {
+  const int nb0 = (a0.count + 255) / 256, nb1 = (a1.count + 255) / 256;
+  const dim3 grid(nb0 + nb1);
+  switch (a0.count > 0 ? a0.limiter : a1.limiter) {
+  case kMinMod: hipLaunchKernelGGL((RegFluxKernel<DIR, kMinMod, T, FAST>), grid, dim3(256), 0, s, a0, a1, nb0); break;
+  case kVanLeer: hipLaunchKernelGGL((RegFluxKernel<DIR, kVanLeer, T, FAST>), grid, dim3(256), 0, s, a0, a1, nb0); break;
+  case kNoLimiter: hipLaunchKernelGGL((RegFluxKernel<DIR, kNoLimiter, T, FAST>), grid, dim3(256), 0, s, a0, a1, nb0); break;
+  default: hipLaunchKernelGGL((RegFluxKernel<DIR, kNoGradient, T, FAST>), grid, dim3(256), 0, s, a0, a1, nb0); break;
   }
 }
 
@@ -323,11 +336,12 @@ void LaunchRegFluxT(hipStream_t s, const batch::RegArgs& a) {
 
 #ifdef FVS_BATCH_FAST_TU
 
-void batch::LaunchRegFluxFast(hipStream_t s, int dir, const batch::RegArgs& a, int precision) {
+void batch::LaunchRegFluxFast(hipStream_t s, int dir, const batch::RegArgs& a0,
+                              const batch::RegArgs& a1, int precision) {
   if (precision == FVS_FP32_FLUX) {
-    if (dir == 0) LaunchRegFluxT<0, float, true>(s, a); else LaunchRegFluxT<1, float, true>(s, a);
+    if (dir == 0) LaunchRegFluxT<0, float, true>(s, a0, a1); else LaunchRegFluxT<1, float, true>(s, a0, a1);
   } else {
-    if (dir == 0) LaunchRegFluxT<0, double, true>(s, a); else LaunchRegFluxT<1, double, true>(s, a);
+    if (dir == 0) LaunchRegFluxT<0, double, true>(s, a0, a1); else LaunchRegFluxT<1, double, true>(s, a0, a1);
   }
 }
 
@@ -343,14 +357,15 @@ void batch::LaunchSweepFast(hipStream_t s, int dir, int shape, const BatchArgs& 
 
 #else // FVS_BATCH_FAST_TU
 
-void batch::LaunchRegFlux(hipStream_t s, int dir, const batch::RegArgs& a, int precision) {
-  if (a.count <= 0) return;
+void batch::LaunchRegFlux(hipStream_t s, int dir, const batch::RegArgs& a0,
+                          const batch::RegArgs& a1, int precision) {
+  if (a0.count <= 0 && a1.count <= 0) return;
   if (precision != FVS_FP64) {
-    batch::LaunchRegFluxFast(s, dir, a, precision);
+    batch::LaunchRegFluxFast(s, dir, a0, a1, precision);
   } else if (dir == 0) {
-    LaunchRegFluxT<0, double, false>(s, a);
+    LaunchRegFluxT<0, double, false>(s, a0, a1);
   } else {
-    LaunchRegFluxT<1, double, false>(s, a);
+    LaunchRegFluxT<1, double, false>(s, a0, a1);
   }
 }
 

@@ -66,8 +66,10 @@ struct RegArgs {
   int fine;      // 1: fine faces (fp, fi, fj), 0: coarse face (cp, ci, cj)
   double sign;
 };
-void LaunchRegFlux(hipStream_t s, int dir, const RegArgs& a, int precision);
-void LaunchRegFluxFast(hipStream_t s, int dir, const RegArgs& a, int precision);
+// Register contributions a0 and a1 (count 0: none) in one launch.
+void LaunchRegFlux(hipStream_t s, int dir, const RegArgs& a0, const RegArgs& a1, int precision);
+void LaunchRegFluxFast(hipStream_t s, int dir, const RegArgs& a0, const RegArgs& a1,
+                       int precision);
 
 } // namespace batch
 } // namespace fvs
