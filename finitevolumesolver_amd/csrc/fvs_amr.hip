@@ -555,9 +555,6 @@ int SetDt(fvs_amr* A, int l, double dt) {
   return FVS_OK;
 }
 
-// Register contributions of fine level f (fine = 1: its faces, taken from its
-// pool src) or of its coarser level (fine = 0, coarse pool src) for the
-// sweep of direction dir.
 // Arguments of the register contributions of register f (between level f
 // and level f - 1) from a sweep in dir of the fine (fine = 1) or the coarse
 // level reading pool src; count 0 when the register has no such entries.
