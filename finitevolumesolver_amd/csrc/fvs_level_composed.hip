@@ -7,15 +7,17 @@
 // (include/fub/solver/DimensionalSplitLevelIntegrator.hpp:221-276) composed
 // from the library's patch operators on one device scratch of complete
 // states with 2 x stencil-width ghost layers:
-//   stable dt of every direction (transmissive fill of all ghosts, then
-//     FluxMethod::ComputeStableDt over the interior grown by the ghosts in
-//     that direction), cfl times the minimum, capped at the final time
-//     (RunSimulation's limited_dt, include/fub/RunSimulation.hpp:125-131);
+//   stable dt of every direction (transmissive fill of that direction's
+//     ghosts, then FluxMethod::ComputeStableDt over the interior grown by
+//     the ghosts in that direction), cfl times the minimum, capped at the
+//     final time (RunSimulation's limited_dt, include/fub/RunSimulation.hpp:125-131);
 //   per split direction (MakeSplitDirections order, Strang half steps): the
-//     direction's CFL check (the same stable dt against the split dt;
-//     TimeStepTooLarge repeats the step from a device copy of its start
-//     state with the failing dt), ComputeNumericFluxes over faces 0 .. n,
-//     UpdateConservatively of the interior and CompleteFromCons.
+//     direction's CFL check (the same stable dt against the split dt; the
+//     first split step's is the step-start value of its direction, computed
+//     on the same state; TimeStepTooLarge repeats the step from a device
+//     copy of its start state with the failing dt), ComputeNumericFluxes
+//     over faces 0 .. n, UpdateConservatively of the interior and
+//     CompleteFromCons.
 // Each check reads one double back to the host, so this path synchronises
 // a few times per step; the fused level kernels serve the HLLEM and
 // first-order exact-Godunov methods.
@@ -81,19 +83,20 @@ void Grown(const CL* C, int64_t lo[3], int64_t hi[3]) {
   }
 }
 
-int FillAllGhosts(CL* C) {
+// Transmissive fill of the ghost layers in direction dir: the stable dt and
+// the fluxes of dir read the interior grown in dir only, so the other
+// directions' ghosts (and the corners) are never read.
+int FillGhosts(CL* C, int dir) {
   int64_t glo[3], ghi[3], ilo[3], ihi[3];
   Grown(C, glo, ghi);
   Interior(C, ilo, ihi);
   const fvs_view v = ScratchView(C, glo, ghi, C->D + 4);
-  for (int d = 0; d < C->D; ++d)
-    if (int rc = fvs_transmissive_fill(C->h, &v, ilo, ihi, d)) return rc;
-  return FVS_OK;
+  return fvs_transmissive_fill(C->h, &v, ilo, ihi, dir);
 }
 
 // Stable dt of direction dir on freshly filled ghosts (host value).
 int DirStableDt(CL* C, int dir, double* dt) {
-  if (int rc = FillAllGhosts(C)) return rc;
+  if (int rc = FillGhosts(C, dir)) return rc;
   int64_t lo[3], hi[3];
   Interior(C, lo, hi);
   lo[dir] -= C->gw;
@@ -115,9 +118,16 @@ int NewtonCapHits(CL* C, uint64_t* hits) {
 }
 
 // One split step in dir with the split dt; *level_dt receives the
-// direction's stable dt.
-int SplitStep(CL* C, int dir, double split_dt, double* level_dt) {
-  if (int rc = DirStableDt(C, dir, level_dt)) return rc;
+// direction's stable dt.  With known_dt > 0 the ghosts of dir are filled and
+// known_dt is that stable dt already (the step-start value of the first
+// split direction: the same faces of the same state).
+int SplitStep(CL* C, int dir, double split_dt, double known_dt, double* level_dt) {
+  if (known_dt > 0.0) {
+    if (int rc = FillGhosts(C, dir)) return rc;
+    *level_dt = known_dt;
+  } else if (int rc = DirStableDt(C, dir, level_dt)) {
+    return rc;
+  }
   if (*level_dt < split_dt) return FVS_TIME_STEP_TOO_LARGE;
   const int D = C->D;
   int64_t flo[3], fhi[3];
@@ -285,12 +295,12 @@ int ComposedAdvance(fvs_composed_level* C, int max_steps, double final_time, dou
     FVS_HIP_CHECK(hipMemcpyAsync(C->backup, C->scratch, sbytes, hipMemcpyDeviceToDevice,
                                  C->h->stream));
     double stable = failure_dt;
+    double dir_dt[3] = {0.0, 0.0, 0.0};
     if (!have_failure) {
       stable = 1.7976931348623157e308;
       for (int d = 0; d < D && rc == FVS_OK; ++d) {
-        double v;
-        rc = DirStableDt(C, d, &v);
-        stable = std::min(stable, v);
+        rc = DirStableDt(C, d, &dir_dt[d]);
+        stable = std::min(stable, dir_dt[d]);
       }
       if (rc) break;
     }
@@ -307,9 +317,13 @@ int ComposedAdvance(fvs_composed_level* C, int max_steps, double final_time, dou
       for (int i = 0; i < D; ++i) seq.push_back({dirs[i], limited});
     }
     bool failed = false;
-    for (const auto& s : seq) {
+    for (size_t i = 0; i < seq.size(); ++i) {
+      const auto& s = seq[i];
+      // The first split step starts from the state the step-start stable
+      // dts were computed on: its check is that direction's value.
+      const double known = (i == 0 && !have_failure) ? dir_dt[s.first] : 0.0;
       double level_dt;
-      const int r = SplitStep(C, s.first, s.second, &level_dt);
+      const int r = SplitStep(C, s.first, s.second, known, &level_dt);
       if (r == FVS_TIME_STEP_TOO_LARGE) {
         failed = true;
         failure_dt = level_dt;
